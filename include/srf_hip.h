@@ -1,0 +1,120 @@
+/*
+ * srf_hip.h -- C-ABI of the MI355X (gfx950) kernels for Simple-RF's rendering hot path.
+ *
+ * The reference (NagabhushanSN95/Simple-RF) has no FFI: its hot path is PyTorch ops
+ * called from the model methods cited below.  Each entry point here replaces the
+ * reference function named in its comment, with the same argument meaning; the
+ * Python drop-in (simple-rf_amd/srf_hip, simple-rf_amd/models) binds them with ctypes.
+ *
+ * Conventions
+ *   - Return value: SRF_OK (0) or an SRF_E* code.  Nothing throws across the ABI.
+ *   - Every pointer is a device pointer owned by the caller; the library never
+ *     allocates.  Tensors are contiguous row-major float32 unless stated otherwise.
+ *   - `stream` is a hipStream_t (passed as void*); all work is stream-ordered, there
+ *     are no host synchronisations, so every entry point is hipGraph-capturable.
+ *   - Entry points are re-entrant (no global mutable state).
+ */
+#ifndef SRF_HIP_H_
+#define SRF_HIP_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SRF_ABI_VERSION 1
+
+enum srf_status {
+  SRF_OK = 0,
+  SRF_EHIP = 1,        /* a HIP launch/runtime error */
+  SRF_EINVAL = 2,      /* bad size, null required pointer, unsupported option */
+  SRF_EUNSUPPORTED = 3 /* shape outside what the specialised kernels support */
+};
+
+/* ABI version of the loaded library (SRF_ABI_VERSION). */
+int srf_abi_version(void);
+
+/* Human-readable text for a status code. */
+const char* srf_status_string(int status);
+
+/* ------------------------------------------------------------------ sampling (a) */
+
+/* Coarse depths z[n_rays, n_samples].
+ * Replaces SimpleNeRF.get_z_vals_coarse (src/models/SimpleNeRF17.py:330-358) and the NDC
+ * branch of SimpleTensoRF.get_z_vals_coarse (src/models/SimpleTensoRF09.py:364-381).
+ * t = torch.linspace(0,1,n_samples); z = near(1-t)+far t (lindisp: 1/(1/near(1-t)+1/far t));
+ * if u != NULL (training, perturb=True) z = lower + (upper-lower) u with u[n_rays,n_samples]
+ * taking the place of the CPU torch.rand draw at :355. */
+int srf_stratified_samples(int n_rays, int n_samples, float near, float far, int lindisp,
+                           const float* u, float* z, void* stream);
+
+/* Inverse-CDF sampling of n_fine depths per ray.
+ * Replaces SimpleNeRF.sample_pdf (src/models/SimpleNeRF17.py:384-417; identical copy at
+ * src/models/SimpleTensoRF09.py:427-460).
+ *   bins    [n_rays, n_bins]        (z_vals_mid)
+ *   weights [n_rays, n_bins-1]      (weights[..., 1:-1]); ignored when cdf_in != NULL
+ *   cdf_in  [n_rays, n_bins] or NULL: a precomputed CDF, bypassing :386-390
+ *   u       [n_rays, n_fine] or NULL: NULL is det=True (u = torch.linspace(0,1,n_fine))
+ * Outputs: samples [n_rays, n_fine]; inds [n_rays, n_fine] int64 or NULL
+ * (= torch.searchsorted(cdf, u, right=True)); cdf_out [n_rays, n_bins] or NULL.
+ * n_bins <= 1024. */
+int srf_sample_pdf(int n_rays, int n_bins, int n_fine, const float* bins, const float* weights,
+                   const float* cdf_in, const float* u, float* samples, int64_t* inds,
+                   float* cdf_out, void* stream);
+
+/* Hierarchical depths: sample_pdf on the coarse weights followed by the sorted merge
+ * torch.sort(torch.cat([z_coarse, z_fine], -1), -1).
+ * Replaces SimpleNeRF.get_z_vals_fine (src/models/SimpleNeRF17.py:360-371; TensoRF copy
+ * src/models/SimpleTensoRF09.py:403-414).
+ *   z_coarse [n_rays, n_coarse], weights_coarse [n_rays, n_coarse], u [n_rays, n_fine] or NULL.
+ * Outputs: z_merged [n_rays, n_coarse + n_fine] (sorted); z_fine [n_rays, n_fine] or NULL.
+ * n_coarse + n_fine <= 1024. */
+int srf_fine_z_vals(int n_rays, int n_coarse, int n_fine, const float* z_coarse,
+                    const float* weights_coarse, const float* u, float* z_merged, float* z_fine,
+                    void* stream);
+
+/* ----------------------------------------------------------------- compositing (d) */
+
+/* Alpha compositing of one ray per wavefront with a log-space FP32 transmittance scan.
+ * Replaces SimpleNeRF.volume_rendering + convert_depth_from_ndc
+ * (src/models/SimpleNeRF17.py:486-558) and LowRankTensor.get_volume_rendering_weights +
+ * volume_render (src/models/SimpleTensoRF09.py:767-819; distance_scale at :787).
+ *   Dense layout when ray_offsets == NULL: sample k of ray r at r*n_samples + k.
+ *   Ragged layout: ray r owns samples [ray_offsets[r], ray_offsets[r+1]) (int32 CSR).
+ *   sigma [P], rgb [P,3], z [P] (NDC depths when ndc != 0, metric otherwise);
+ *   rays_o, rays_d [n_rays,3] (world, used for the metric depth and, when !ndc, delta);
+ *   rays_d_ndc [n_rays,3] (NDC delta norm, required when ndc != 0).
+ *   distance_scale multiplies sigma*delta (1 for NeRF, 25 for TensoRF).
+ * Outputs (per ray): rgb_map [n_rays,3], acc, depth, depth_var, and when ndc != 0
+ * depth_ndc, depth_var_ndc (may be NULL otherwise); (per sample, each may be NULL):
+ * alpha, visibility, weights [P]. */
+int srf_composite_fwd(int n_rays, int n_samples, const int32_t* ray_offsets, const float* sigma,
+                      const float* rgb, const float* z, const float* rays_o, const float* rays_d,
+                      const float* rays_d_ndc, int ndc, int white_bkgd, float distance_scale,
+                      float* rgb_map, float* acc, float* depth, float* depth_var, float* depth_ndc,
+                      float* depth_var_ndc, float* alpha, float* visibility, float* weights,
+                      void* stream);
+
+/* Backward of srf_composite_fwd: a forward scan for the chunk transmittances followed by
+ * a reverse (suffix) scan, recomputing alpha/T from sigma and z instead of storing them.
+ *   acc, depth, depth_ndc: the forward's per-ray outputs (depth_ndc may be NULL if !ndc).
+ *   Upstream gradients, each may be NULL (= zero): g_rgb [n_rays,3], g_acc, g_depth,
+ *   g_depth_var, g_depth_ndc, g_depth_var_ndc [n_rays], g_alpha, g_visibility, g_weights [P].
+ * Outputs: d_sigma [P], d_rgb [P,3] (overwritten). */
+int srf_composite_bwd(int n_rays, int n_samples, const int32_t* ray_offsets, const float* sigma,
+                      const float* rgb, const float* z, const float* rays_o, const float* rays_d,
+                      const float* rays_d_ndc, int ndc, int white_bkgd, float distance_scale,
+                      const float* acc, const float* depth, const float* depth_ndc,
+                      const float* g_rgb, const float* g_acc, const float* g_depth,
+                      const float* g_depth_var, const float* g_depth_ndc,
+                      const float* g_depth_var_ndc, const float* g_alpha,
+                      const float* g_visibility, const float* g_weights, float* d_sigma,
+                      float* d_rgb, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SRF_HIP_H_ */

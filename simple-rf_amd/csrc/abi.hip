@@ -1,0 +1,65 @@
+// Library-level entry points: version, status strings and the MFMA lane-layout self-test.
+#include "common.h"
+#include "mfma.h"
+
+namespace srf {
+namespace {
+
+// One wave computes D = A B for a single 16x16 tile with the operand layouts of mfma.h
+// (slot (g, i) = logical k index 8 g + i for 16-bit types, g for fp32).
+template <int P>
+__global__ __launch_bounds__(64) void selftest_mfma_kernel(const float* __restrict__ a,
+                                                           const float* __restrict__ b,
+                                                           float* __restrict__ d) {
+  const int l = threadIdx.x;
+  const int g = l / 16, j = l % 16;
+  f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+  if constexpr (P == kF32) {
+    acc = mfma32(a[j * 4 + g], b[g * 16 + j], acc);
+  } else {
+    u32x4 av, bv;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int k = 8 * g + 2 * q;
+      av[q] = pack2<P>(a[j * 32 + k], a[j * 32 + k + 1]);
+      bv[q] = pack2<P>(b[k * 16 + j], b[(k + 1) * 16 + j]);
+    }
+    acc = mfma16<P>(av, bv, acc);
+  }
+#pragma unroll
+  for (int r = 0; r < 4; ++r) d[(4 * g + r) * 16 + j] = acc[r];
+}
+
+}  // namespace
+}  // namespace srf
+
+using namespace srf;
+
+extern "C" int srf_abi_version(void) { return SRF_ABI_VERSION; }
+
+extern "C" const char* srf_status_string(int status) {
+  switch (status) {
+    case SRF_OK: return "ok";
+    case SRF_EHIP: return "HIP runtime or launch error";
+    case SRF_EINVAL: return "invalid argument (size, null pointer or option)";
+    case SRF_EUNSUPPORTED: return "shape outside what the specialised kernels support";
+    default: return "unknown status";
+  }
+}
+
+extern "C" int srf_selftest_mfma(int prec, const float* a, const float* b, float* d, void* stream) {
+  if (!a || !b || !d) return SRF_EINVAL;
+  switch (prec) {
+    case kF32:
+      hipLaunchKernelGGL(selftest_mfma_kernel<kF32>, dim3(1), dim3(64), 0, as_stream(stream), a, b, d);
+      break;
+    case kBF16:
+      hipLaunchKernelGGL(selftest_mfma_kernel<kBF16>, dim3(1), dim3(64), 0, as_stream(stream), a, b, d);
+      break;
+    case kF16:
+      hipLaunchKernelGGL(selftest_mfma_kernel<kF16>, dim3(1), dim3(64), 0, as_stream(stream), a, b, d);
+      break;
+    default: return SRF_EINVAL;
+  }
+  return launch_status();
+}

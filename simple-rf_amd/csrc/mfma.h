@@ -1,0 +1,83 @@
+// MFMA building blocks for gfx950 (CDNA4), 16x16 output tiles.
+//
+// Lane layout used throughout this library (64-lane wave, g = lane / 16, j = lane % 16):
+//   C/D tile (16x16 fp32, 4 VGPRs):  D[4g + r][j] in register r            (r = 0..3)
+//   A operand (16 x K):              A[j][slot (g, i)]
+//   B operand (K x 16):              B[slot (g, i)][j]
+// where a "slot" (g, i) is one of the K reduction positions held by lane group g; A and B
+// hold the same K positions in the same registers, so any fixed map from slots to logical
+// reduction indices gives the same product.  The kernels choose that map so that a D tile
+// produced by one layer can be fed to the next layer as its B operand without moving data:
+//   16-bit types (16x16x32, 8 values per lane): slot (g, i) of k-step s is logical feature
+//     32 s + 4 g + i          for i < 4   (D tile 2s,   register i)
+//     32 s + 16 + 4 g + i - 4 for i >= 4  (D tile 2s+1, register i-4)
+//   fp32 (16x16x4, 1 value per lane): k-step s = 4 t + r uses lane group g's value
+//     feature 16 t + 4 g + r (D tile t, register r).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace srf {
+
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+
+enum Prec { kF32 = 0, kBF16 = 1, kF16 = 2 };
+
+// Round-to-nearest-even float -> bf16 bits (NaN kept quiet).
+__device__ __forceinline__ uint32_t f32_to_bf16_bits(float f) {
+  uint32_t u = __float_as_uint(f);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return ((u >> 16) | 0x40u) & 0xffffu;
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return u >> 16;
+}
+
+__device__ __forceinline__ float bf16_bits_to_f32(uint32_t b) { return __uint_as_float(b << 16); }
+
+__device__ __forceinline__ uint32_t f32_to_f16_bits(float f) {
+  _Float16 h = (_Float16)f;
+  return (uint32_t)__builtin_bit_cast(uint16_t, h);
+}
+
+__device__ __forceinline__ float f16_bits_to_f32(uint32_t b) {
+  return (float)__builtin_bit_cast(_Float16, (uint16_t)(b & 0xffffu));
+}
+
+template <int P>
+__device__ __forceinline__ uint32_t pack2(float lo, float hi) {
+  if constexpr (P == kBF16) return f32_to_bf16_bits(lo) | (f32_to_bf16_bits(hi) << 16);
+  else return f32_to_f16_bits(lo) | (f32_to_f16_bits(hi) << 16);
+}
+
+template <int P>
+__device__ __forceinline__ float unpack_lo(uint32_t v) {
+  if constexpr (P == kBF16) return bf16_bits_to_f32(v & 0xffffu);
+  else return f16_bits_to_f32(v & 0xffffu);
+}
+
+template <int P>
+__device__ __forceinline__ float unpack_hi(uint32_t v) {
+  if constexpr (P == kBF16) return bf16_bits_to_f32(v >> 16);
+  else return f16_bits_to_f32(v >> 16);
+}
+
+// One 16x16x32 MFMA on 16-bit operands held as 4 packed dwords.
+template <int P>
+__device__ __forceinline__ f32x4 mfma16(const u32x4& a, const u32x4& b, const f32x4& c) {
+  if constexpr (P == kBF16)
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a),
+                                                   __builtin_bit_cast(bf16x8, b), c, 0, 0, 0);
+  else
+    return __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(f16x8, a),
+                                                  __builtin_bit_cast(f16x8, b), c, 0, 0, 0);
+}
+
+__device__ __forceinline__ f32x4 mfma32(float a, float b, const f32x4& c) {
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+
+}  // namespace srf

@@ -1,0 +1,52 @@
+"""Ray generation and NDC warping (host side of the drop-in, PyTorch ops on the GPU).
+
+These produce the inputs of the hot path exactly as the reference does
+(src/utils/CommonUtils04.py:73-149).  They are small per-ray computations kept in
+PyTorch; fusing them into the sampler is the next step after the hot path.
+"""
+from __future__ import annotations
+
+import torch
+
+
+def get_rays_tr(pixel_id: torch.Tensor, intrinsic: torch.Tensor, extrinsics: torch.Tensor, mip_nerf_used: bool,
+                nerf_synthetic: bool = False):
+    """src/utils/CommonUtils04.py:73-95."""
+    if nerf_synthetic:
+        p = torch.eye(4).to(extrinsics)
+        p[1, 1] = -1
+        p[2, 2] = -1
+        extrinsics = torch.matmul(torch.matmul(p, extrinsics), p)
+    pixel_id = pixel_id.float()
+    x, y = pixel_id[:, 1], pixel_id[:, 2]
+    if mip_nerf_used:
+        x = x + 0.5
+        y = y + 0.5
+    points_homo = torch.stack([x, y, torch.ones_like(x)], dim=1)
+    dirs = (torch.linalg.inv(intrinsic) @ points_homo[:, :, None])[:, :, 0]
+    dirs[:, 1:] *= -1
+    if nerf_synthetic:
+        dirs[:, 1:] *= -1
+    rays_d = torch.sum(dirs[:, None, :] * extrinsics[:, :3, :3], dim=-1)
+    rays_o = extrinsics[:, :3, -1]
+    return rays_o, rays_d
+
+
+def get_ndc_rays_tr(rays_o, rays_d, resolution, intrinsic, near):
+    """src/utils/CommonUtils04.py:120-138."""
+    h, w = resolution
+    fx, fy = intrinsic[:, 0, 0], intrinsic[:, 1, 1]
+    t = -(near + rays_o[:, 2]) / rays_d[:, 2]
+    rays_o = rays_o + t[:, None] * rays_d
+    o0 = -1. / (w / (2. * fx)) * rays_o[:, 0] / rays_o[:, 2]
+    o1 = -1. / (h / (2. * fy)) * rays_o[:, 1] / rays_o[:, 2]
+    o2 = 1. + 2. * near / rays_o[:, 2]
+    d0 = -1. / (w / (2. * fx)) * (rays_d[:, 0] / rays_d[:, 2] - rays_o[:, 0] / rays_o[:, 2])
+    d1 = -1. / (h / (2. * fy)) * (rays_d[:, 1] / rays_d[:, 2] - rays_o[:, 1] / rays_o[:, 2])
+    d2 = -2. * near / rays_o[:, 2]
+    return torch.stack([o0, o1, o2], -1), torch.stack([d0, d1, d2], -1)
+
+
+def get_view_dirs_tr(rays_d):
+    """src/utils/CommonUtils04.py:147-149."""
+    return rays_d / torch.linalg.norm(rays_d, ord=2, dim=-1, keepdim=True)

@@ -1,0 +1,150 @@
+"""GPU parity of the compositing kernels (forward + hand-written backward) against the oracle."""
+import pytest
+import torch
+
+import oracle.nerf as on
+from srf_hip import compositing, synthetic
+
+pytestmark = pytest.mark.gpu
+
+OUT_KEYS = ['rgb', 'acc', 'depth', 'depth_var', 'depth_ndc', 'depth_var_ndc', 'alpha', 'visibility', 'weights']
+
+
+def make_case(n_rays, n_samples, seed, sigma_scale=5.0, dtype=torch.float32):
+    r = synthetic.rays(n_rays, seed=seed)
+    g = torch.Generator().manual_seed(seed + 100)
+    u = torch.rand(n_rays, n_samples, generator=g)
+    z = on.stratified_z_vals(n_rays, n_samples, 0., 1., u=u)
+    sigma = torch.relu(torch.randn(n_rays, n_samples, generator=g)) * sigma_scale
+    sigma[::5] *= 40.0                                   # nearly opaque rays
+    sigma[1::9] = 0.0                                    # empty rays
+    rgb = torch.sigmoid(torch.randn(n_rays, n_samples, 3, generator=g))
+    return dict(sigma=sigma.to(dtype), rgb=rgb.to(dtype), z=z.to(dtype), rays_o=r['rays_o'].to(dtype),
+                rays_d=r['rays_d'].to(dtype), rays_d_ndc=r['rays_d_ndc'].to(dtype))
+
+
+def oracle_out(c, ndc=True, white=False, ds=None):
+    if ndc:
+        return on.volume_rendering(c['sigma'], c['rgb'], z_vals_ndc=c['z'], rays_o=c['rays_o'], rays_d=c['rays_d'],
+                                   rays_d_ndc=c['rays_d_ndc'], ndc=True, white_bkgd=white, distance_scale=ds)
+    return on.volume_rendering(c['sigma'], c['rgb'], z_vals=c['z'], rays_d=c['rays_d'], ndc=False,
+                               white_bkgd=white, distance_scale=ds)
+
+
+def hip_out(c, dev, ndc=True, white=False, ds=1.0, req_grad=False):
+    sig = c['sigma'].to(dev).requires_grad_(req_grad)
+    rgb = c['rgb'].to(dev).requires_grad_(req_grad)
+    out = compositing.composite(sig, rgb, c['z'].to(dev), c['rays_o'].to(dev), c['rays_d'].to(dev),
+                                c['rays_d_ndc'].to(dev) if ndc else None, ndc=ndc, white_bkgd=white,
+                                distance_scale=ds)
+    return out, sig, rgb
+
+
+TOL = dict(rgb=1e-5, acc=1e-5, depth_ndc=1e-5, alpha=1e-6, visibility=1e-5, weights=1e-5)
+
+
+def check_fwd(out, ref, keys):
+    for k in keys:
+        a, b = out[k].detach().cpu(), ref[k]
+        if k in ('depth', 'depth_var', 'depth_var_ndc'):
+            # metric depth grows like 1/(1-z_ndc); compare with a relative floor
+            assert torch.allclose(a, b, atol=1e-5, rtol=2e-5), (k, (a - b).abs().max())
+        else:
+            assert (a - b).abs().max().item() <= TOL[k], (k, (a - b).abs().max().item())
+
+
+@pytest.mark.parametrize('n_samples', [64, 192, 443, 7])
+@pytest.mark.parametrize('white', [False, True])
+def test_forward_ndc_matches_oracle(cuda, n_samples, white):
+    c = make_case(1000, n_samples, seed=n_samples)
+    out, _, _ = hip_out(c, cuda, white=white)
+    check_fwd(out, oracle_out(c, white=white), OUT_KEYS)
+
+
+def test_forward_tensorf_distance_scale(cuda):
+    c = make_case(800, 443, seed=3, sigma_scale=0.3)
+    out, _, _ = hip_out(c, cuda, ds=25.0)
+    check_fwd(out, oracle_out(c, ds=25.0), OUT_KEYS)
+
+
+def test_forward_metric_z(cuda):
+    c = make_case(600, 64, seed=9)
+    c['z'] = 2.0 + 4.0 * c['z']
+    out, _, _ = hip_out(c, cuda, ndc=False)
+    check_fwd(out, oracle_out(c, ndc=False), ['rgb', 'acc', 'depth', 'depth_var', 'alpha', 'visibility', 'weights'])
+
+
+def test_forward_vs_fp64(cuda):
+    """Error of the fp32 kernel against the fp64 oracle stays at fp32 rounding level."""
+    c = make_case(2000, 192, seed=11)
+    out, _, _ = hip_out(c, cuda)
+    ref = oracle_out({k: v.double() for k, v in c.items()})
+    assert (out['rgb'].cpu().double() - ref['rgb']).abs().max() < 2e-6
+    assert (out['depth_ndc'].cpu().double() - ref['depth_ndc']).abs().max() < 2e-6
+
+
+@pytest.mark.parametrize('ndc,white,ds', [(True, False, 1.0), (True, True, 1.0), (True, False, 25.0),
+                                          (False, False, 1.0)])
+def test_backward_matches_oracle_autograd(cuda, ndc, white, ds):
+    n_rays, n_samples = 512, 96
+    c = make_case(n_rays, n_samples, seed=21, sigma_scale=(0.2 if ds > 1 else 3.0))
+    if not ndc:
+        c['z'] = 2.0 + 4.0 * c['z']
+    g = torch.Generator().manual_seed(5)
+    keys = ['rgb', 'acc', 'depth', 'depth_var', 'alpha', 'visibility', 'weights'] + (
+        ['depth_ndc', 'depth_var_ndc'] if ndc else [])
+    # reference gradients in float64
+    c64 = {k: v.double() for k, v in c.items()}
+    s64 = c64['sigma'].clone().requires_grad_(True)
+    r64 = c64['rgb'].clone().requires_grad_(True)
+    ref = oracle_out(dict(c64, sigma=s64, rgb=r64), ndc=ndc, white=white, ds=None if ds == 1.0 else ds)
+    grads = {k: torch.randn(ref[k].shape, generator=g, dtype=torch.float64) for k in keys}
+    grads['depth'] = grads['depth'] * 0.01
+    grads['depth_var'] = grads['depth_var'] * 0.001
+    loss = sum((ref[k] * grads[k]).sum() for k in keys)
+    ds64, dr64 = torch.autograd.grad(loss, [s64, r64])
+    out, sig, rgb = hip_out(c, cuda, ndc=ndc, white=white, ds=ds, req_grad=True)
+    loss_h = sum((out[k] * grads[k].float().to(cuda)).sum() for k in keys)
+    dsg, drg = torch.autograd.grad(loss_h, [sig, rgb])
+    rel = lambda a, b: ((a.cpu().double() - b).norm() / b.norm().clamp_min(1e-30)).item()
+    assert rel(dsg, ds64) < 1e-4, rel(dsg, ds64)
+    assert rel(drg, dr64) < 1e-5, rel(drg, dr64)
+
+
+def test_backward_only_rgb_loss(cuda):
+    """The common training case: MSE on rgb only (all other upstream grads None)."""
+    c = make_case(256, 64, seed=2)
+    out, sig, rgb = hip_out(c, cuda, req_grad=True)
+    target = torch.rand(256, 3, generator=torch.Generator().manual_seed(0)).to(cuda)
+    torch.mean((out['rgb'] - target) ** 2).backward()
+    s64 = c['sigma'].double().requires_grad_(True)
+    r64 = c['rgb'].double().requires_grad_(True)
+    ref = oracle_out(dict({k: v.double() for k, v in c.items()}, sigma=s64, rgb=r64))
+    torch.mean((ref['rgb'] - target.cpu().double()) ** 2).backward()
+    assert ((sig.grad.cpu().double() - s64.grad).norm() / s64.grad.norm()).item() < 1e-4
+    assert ((rgb.grad.cpu().double() - r64.grad).norm() / r64.grad.norm()).item() < 1e-5
+
+
+def test_ragged_matches_dense(cuda):
+    """Ragged (CSR) rays give the dense result of each ray truncated to its own length."""
+    n_rays = 300
+    g = torch.Generator().manual_seed(4)
+    counts = torch.randint(1, 200, (n_rays,), generator=g)
+    counts[5] = 0
+    off = torch.zeros(n_rays + 1, dtype=torch.int32)
+    off[1:] = torch.cumsum(counts, 0).int()
+    c = make_case(n_rays, 200, seed=4)
+    sig = torch.cat([c['sigma'][r, :counts[r]] for r in range(n_rays)])
+    rgb = torch.cat([c['rgb'][r, :counts[r]] for r in range(n_rays)])
+    z = torch.cat([c['z'][r, :counts[r]] for r in range(n_rays)])
+    out = compositing.composite(sig.to(cuda), rgb.to(cuda), z.to(cuda), c['rays_o'].to(cuda), c['rays_d'].to(cuda),
+                                c['rays_d_ndc'].to(cuda), ray_offsets=off.to(cuda), max_samples=int(counts.max()))
+    for r in [0, 1, 5, 17, 299]:
+        n = int(counts[r])
+        if n == 0:
+            assert out['acc'][r].item() == 0.0
+            continue
+        sub = {k: v[r:r + 1, :n] if k in ('sigma', 'rgb', 'z') else v[r:r + 1] for k, v in c.items()}
+        ref = oracle_out(sub)
+        assert abs(out['rgb'][r].cpu() - ref['rgb'][0]).max() < 1e-5
+        assert abs(out['depth_ndc'][r].cpu() - ref['depth_ndc'][0]).max() < 1e-5
