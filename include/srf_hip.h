@@ -113,6 +113,72 @@ int srf_composite_bwd(int n_rays, int n_samples, const int32_t* ray_offsets, con
                       const float* g_visibility, const float* g_weights, float* d_sigma,
                       float* d_rgb, void* stream);
 
+/* ------------------------------------------------------------------- PE + MLP (b) */
+
+/* Structure of one NeRF MLP (class MLP, src/models/SimpleNeRF17.py:616-785).  The kernels are
+ * specialised to the shipped geometry: 8 trunk layers of width 256 with the skip after layer 4,
+ * one view layer of width 128 (src/NerfLlffTrainerTester10.py:293-358).
+ *   trunk_pe:       xyz-PE features fed to the trunk: 63 (L=10) for the main and views-augmented
+ *                   models, 21 (points_sigma_positional_encoding_degree=3) for the
+ *                   points-augmented model, whose remaining 42 features join the view head
+ *                   (:634-636, :700-703);
+ *   pts_pe_degree:  L of the xyz PE (10);
+ *   dirs_pe_degree: L of the view-direction PE (4);
+ *   view_dependent: 1 -> feature_linear + views_linears + views_output_linear (rgb = sigmoid of
+ *                   the view head); 0 -> 4-channel pts_output_linear (views-augmented model);
+ *   sigma_softplus: 0 -> relu(raw + noise) (reference, :738-743); 1 -> softplus(raw + noise);
+ *   prec:           SRF_PREC_F32 / SRF_PREC_BF16 / SRF_PREC_F16 operands of the MFMA tiles
+ *                   (accumulation is always FP32). */
+typedef struct srf_mlp_desc {
+  int trunk_pe;
+  int pts_pe_degree;
+  int dirs_pe_degree;
+  int view_dependent;
+  int sigma_softplus;
+  int prec;
+} srf_mlp_desc;
+
+#define SRF_PREC_F32 0
+#define SRF_PREC_BF16 1
+#define SRF_PREC_F16 2
+
+/* Reference parameter order for srf_mlp_pack / srf_mlp_backward (12 entries; NULL where the
+ * variant has no such layer): pts_linears.0..7, feature_linear, pts_output_linear,
+ * views_linears.0, views_output_linear.  Each weight is [out, in] row-major FP32 as in
+ * torch.nn.Linear (src/models/SimpleNeRF17.py:644-666). */
+#define SRF_MLP_NUM_PARAMS 12
+
+/* Bytes of the packed operand image built by srf_mlp_pack (0 for an unsupported desc). */
+size_t srf_mlp_packed_bytes(const srf_mlp_desc* desc);
+
+/* Repacks the FP32 reference parameters into MFMA operand order (forward A tiles, transposed
+ * backward tiles and padded biases).  weights/biases: host arrays of SRF_MLP_NUM_PARAMS device
+ * pointers.  Must be re-run after every parameter update. */
+int srf_mlp_pack(const srf_mlp_desc* desc, const float* const* weights, const float* const* biases,
+                 void* packed, void* stream);
+
+/* Bytes of the training workspace for n_points (saved activations, ReLU bit masks and
+ * activation gradients, all feature-major). */
+size_t srf_mlp_workspace_bytes(const srf_mlp_desc* desc, int n_points);
+
+/* Fused PE + MLP forward of n_points points.
+ * Replaces MLP.forward / run_network / batchify (src/models/SimpleNeRF17.py:419-484, :696-785).
+ *   pts [n,3]; dirs [n,3] (view directions, needed when view_dependent);
+ *   noise [n] or NULL: already scaled sigma noise (randn * raw_noise_std, :740), NULL = none;
+ * Outputs: sigma [n] and rgb [n,3] (FP32).  workspace: NULL for inference, else the training
+ * workspace (srf_mlp_workspace_bytes) that srf_mlp_backward reads. */
+int srf_mlp_forward(const srf_mlp_desc* desc, const void* packed, int n_points, const float* pts,
+                    const float* dirs, const float* noise, float* sigma, float* rgb,
+                    void* workspace, void* stream);
+
+/* Backward of srf_mlp_forward: d_sigma [n], d_rgb [n,3] upstream gradients; sigma, rgb the
+ * forward outputs.  ACCUMULATES (+=) the FP32 parameter gradients into d_weights / d_biases
+ * (host arrays of SRF_MLP_NUM_PARAMS device pointers in the reference order and shapes). */
+int srf_mlp_backward(const srf_mlp_desc* desc, const void* packed, int n_points,
+                     const float* sigma, const float* rgb, const float* d_sigma,
+                     const float* d_rgb, void* workspace, float* const* d_weights,
+                     float* const* d_biases, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

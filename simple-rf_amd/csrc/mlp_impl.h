@@ -1,0 +1,893 @@
+// Fused positional encoding + NeRF MLP (kernel templates shared by mlp*.hip) (forward, backward, weight gradients) on gfx950 MFMA.
+//
+// Reference semantics: MLP.forward / get_view_independent_outputs / get_view_dependent_outputs
+// and PositionalEncoder (src/models/SimpleNeRF17.py:581-785), driven by run_network/batchify
+// (:419-484).  Three variants share the kernels (desc->view_dependent, desc->trunk_pe):
+//   main              trunk PE 63,  head: feature(256) + sigma(1), view layer 283 -> 128 -> 3
+//   points-augmented  trunk PE 21,  view layer input [feature, PE_xyz[21:63], PE_dir] = 325
+//   views-augmented   trunk PE 63,  head: 4 channels (sigma, rgb) straight from the trunk
+//
+// Design (one 64-lane wave owns 16*CB points, CB = 4 for 16-bit operands, 2 for FP32):
+//   * The MLP is evaluated "transposed": each layer computes H_out^T = W H_in^T with the
+//     weights as the MFMA A operand and the points as the N dimension.  With the slot map of
+//     mfma.h, the D tiles of one layer are, bit for bit, the B operand of the next layer, so
+//     all activations stay in VGPRs from the positional encoding to the rgb head.
+//   * Positional-encoding features are generated in registers in the same tile layout
+//     (sincosf of x * 2^m, no HBM traffic), for layer 0, the skip layer and the view layer.
+//   * Weights are repacked once per parameter update (srf_mlp_pack) into per-lane 16-byte
+//     fragments, so every A-operand load is one coalesced 1 KB read per wave.
+//   * Training stores, feature-major (row = feature, column = point), the inputs of every
+//     layer (16-bit or FP32) for the weight gradients, plus one 64-bit ReLU mask per lane per
+//     layer for the backward chain.  The transposition to feature-major goes through a 2.3 KB
+//     per-wave LDS tile so global stores are 16-byte vectors.
+//   * Backward chain: the same transposed product with W^T tiles walks from the heads to
+//     layer 0, applying the stored ReLU masks, and writes every layer's pre-activation
+//     gradient dZ feature-major.
+//   * Weight gradients: one grouped GEMM launch dW = dZ X^T over all layers, K = points, each
+//     wave reducing a 64x64 block over a chunk of points and adding it with FP32 atomics.
+#pragma once
+
+#include <string.h>
+
+#include <type_traits>
+
+#include "common.h"
+#include "mfma.h"
+
+// Lambdas in the kernels must inline or the register arrays they touch spill to scratch.
+#define SRF_AI __attribute__((always_inline))
+
+namespace srf {
+namespace mlp {
+
+constexpr int kLayers = 11;        // kernel layers: trunk 0-7, head 8, view 9, rgb 10
+constexpr int kParams = SRF_MLP_NUM_PARAMS;
+constexpr int kMaskSlots = 9;      // ReLU masks of layers 0-7 and the view layer
+constexpr int kWaves = 4;          // waves per block
+constexpr int kLdsRowBytes = 144;  // 16 rows x 144 B transpose tile per wave
+constexpr int kMaxJobs = 16;
+constexpr int kChunk = 8192;       // points reduced per dW work item
+
+// ----------------------------------------------------------------------------- plan
+struct Plan {
+  int prec, view_dep, softplus, trunk_pe, pts_deg, dir_deg;
+  int extra;  // computed inputs of the view layer: [PE_xyz[trunk_pe:], PE_dir]
+  int elem;   // bytes per stored value
+  int out_rows[kLayers], out_tiles[kLayers], reg_tiles[kLayers], comp_feats[kLayers],
+      comp_tiles[kLayers], bwd_tiles[kLayers], present[kLayers];
+  size_t fwd_off[kLayers], bwd_off[kLayers], bias_off[kLayers], packed_bytes;
+};
+
+__host__ __device__ inline int round_up(int a, int b) { return (a + b - 1) / b * b; }
+
+inline bool make_plan(const srf_mlp_desc* d, Plan* p) {
+  if (!d) return false;
+  memset(p, 0, sizeof(Plan));
+  if (d->prec < SRF_PREC_F32 || d->prec > SRF_PREC_F16) return false;
+  if (d->pts_pe_degree < 0 || d->pts_pe_degree > 16 || d->dirs_pe_degree < 0 || d->dirs_pe_degree > 16)
+    return false;
+  const int pts_dim = 3 * (2 * d->pts_pe_degree + 1);
+  if (d->trunk_pe < 3 || d->trunk_pe > pts_dim || d->trunk_pe > 64) return false;
+  p->prec = d->prec;
+  p->view_dep = d->view_dependent ? 1 : 0;
+  p->softplus = d->sigma_softplus ? 1 : 0;
+  p->trunk_pe = d->trunk_pe;
+  p->pts_deg = d->pts_pe_degree;
+  p->dir_deg = d->dirs_pe_degree;
+  p->elem = d->prec == SRF_PREC_F32 ? 4 : 2;
+  p->extra = p->view_dep ? (pts_dim - d->trunk_pe) + 3 * (2 * d->dirs_pe_degree + 1) : 0;
+  if (p->extra > 96) return false;
+  if (!p->view_dep && pts_dim != d->trunk_pe) return false;
+  const int pe_tiles = round_up(round_up(d->trunk_pe, 16) / 16, 2);
+  const int ex_tiles = round_up(round_up(p->extra, 16) / 16, 2);
+  for (int l = 0; l < kLayers; ++l) p->present[l] = (l <= 8) || p->view_dep;
+  for (int l = 0; l < 8; ++l) {
+    p->out_rows[l] = 256;
+    p->reg_tiles[l] = l == 0 ? 0 : 16;
+    p->comp_feats[l] = (l == 0 || l == 5) ? d->trunk_pe : 0;
+    p->comp_tiles[l] = (l == 0 || l == 5) ? pe_tiles : 0;
+  }
+  p->out_rows[8] = p->view_dep ? 257 : 4;
+  p->reg_tiles[8] = 16;
+  if (p->view_dep) {
+    p->out_rows[9] = 128;
+    p->reg_tiles[9] = 16;
+    p->comp_feats[9] = p->extra;
+    p->comp_tiles[9] = ex_tiles;
+    p->out_rows[10] = 3;
+    p->reg_tiles[10] = 8;
+  }
+  size_t off = 0;
+  for (int l = 0; l < kLayers; ++l) {
+    if (!p->present[l]) continue;
+    p->out_tiles[l] = (p->out_rows[l] + 15) / 16;
+    p->bwd_tiles[l] = round_up(p->out_tiles[l], 2);
+    const int in_tiles = p->reg_tiles[l] + p->comp_tiles[l];
+    p->fwd_off[l] = off;
+    off += (size_t)p->out_tiles[l] * in_tiles * 256 * p->elem;
+  }
+  for (int l = 1; l < kLayers; ++l) {
+    if (!p->present[l]) continue;
+    p->bwd_off[l] = off;
+    off += (size_t)p->reg_tiles[l] * p->bwd_tiles[l] * 256 * p->elem;
+  }
+  for (int l = 0; l < kLayers; ++l) {
+    if (!p->present[l]) continue;
+    p->bias_off[l] = off;
+    off += (size_t)p->out_tiles[l] * 16 * sizeof(float);
+  }
+  p->packed_bytes = round_up((int)off, 256);
+  return true;
+}
+
+// Training workspace: feature-major arrays with leading dimension ldP (points, multiple of 64).
+struct Ws {
+  int ldP;
+  size_t x0, h[8], f, ex, v, dz[kLayers], mask, bytes;  // byte offsets
+  int dz_rows[kLayers];
+};
+
+inline Ws make_ws(const Plan& p, int n) {
+  Ws w;
+  memset(&w, 0, sizeof(w));
+  w.ldP = round_up(n > 0 ? n : 1, 64);
+  const size_t row = (size_t)w.ldP * p.elem;
+  size_t off = 0;
+  auto take = [&](int rows) {
+    size_t o = off;
+    off += (size_t)rows * row;
+    off = (off + 255) / 256 * 256;
+    return o;
+  };
+  w.x0 = take(p.comp_tiles[0] * 16);
+  for (int l = 0; l < 8; ++l) w.h[l] = take(256);
+  if (p.view_dep) {
+    w.f = take(256);
+    w.ex = take(p.comp_tiles[9] * 16);
+    w.v = take(128);
+  }
+  for (int l = 0; l < kLayers; ++l) {
+    if (!p.present[l]) continue;
+    w.dz_rows[l] = p.out_tiles[l] * 16;
+    w.dz[l] = take(w.dz_rows[l]);
+  }
+  w.mask = off;
+  off += (size_t)kMaskSlots * w.ldP * 32;
+  w.bytes = (off + 255) / 256 * 256;
+  return w;
+}
+
+// --------------------------------------------------------------------- type traits
+template <int P>
+struct Tr;
+template <>
+struct Tr<kBF16> {
+  static constexpr int CB = 4;
+  using Act = u32x2;
+};
+template <>
+struct Tr<kF16> {
+  static constexpr int CB = 4;
+  using Act = u32x2;
+};
+template <>
+struct Tr<kF32> {
+  static constexpr int CB = 2;
+  using Act = f32x4;
+};
+
+template <int P>
+__device__ __forceinline__ typename Tr<P>::Act to_act(const f32x4& v) {
+  if constexpr (P == kF32) {
+    return v;
+  } else {
+    u32x2 a;
+    a.x = pack2<P>(v[0], v[1]);
+    a.y = pack2<P>(v[2], v[3]);
+    return a;
+  }
+}
+
+template <int P>
+__device__ __forceinline__ typename Tr<P>::Act zero_act() {
+  typename Tr<P>::Act a;
+  if constexpr (P == kF32) a = f32x4{0.f, 0.f, 0.f, 0.f};
+  else a = u32x2{0u, 0u};
+  return a;
+}
+
+template <int P>
+__device__ __forceinline__ uint32_t act_elem_bits(const typename Tr<P>::Act& a, int r) {
+  if constexpr (P == kF32) return __float_as_uint(a[r]);
+  else return (r < 2 ? a.x : a.y) >> (16 * (r & 1)) & 0xffffu;
+}
+
+// sin and cos of x for |x| up to ~1e4 (positional-encoding arguments are |2^m x| < 1e3):
+// Cody-Waite reduction x = k pi/2 + r with pi/2 split into three floats applied by fma, then
+// Taylor polynomials on |r| <= pi/4 (truncation error below 2e-10, i.e. ~1 ulp overall).
+__device__ __forceinline__ void pe_sincos(float x, float* sn, float* cs) {
+  const float kf = rintf(x * 0.636619772367581343f);
+  float r = fmaf(-kf, 1.57079637050628662f, x);
+  r = fmaf(-kf, -4.37113882867379e-08f, r);
+  r = fmaf(-kf, -1.77635683940025e-15f, r);
+  const float r2 = r * r;
+  float sp = fmaf(r2, 2.75573192239859e-06f, -1.98412698412698e-04f);
+  sp = fmaf(r2, sp, 8.33333333333333e-03f);
+  sp = fmaf(r2, sp, -1.66666666666667e-01f);
+  const float sr = fmaf(r * r2, sp, r);
+  float cp = fmaf(r2, -2.75573192239859e-07f, 2.48015873015873e-05f);
+  cp = fmaf(r2, cp, -1.38888888888889e-03f);
+  cp = fmaf(r2, cp, 4.16666666666667e-02f);
+  cp = fmaf(r2, cp, -0.5f);
+  const float cr = fmaf(r2, cp, 1.0f);
+  const int q = ((int)kf) & 3;
+  *sn = q == 0 ? sr : (q == 1 ? cr : (q == 2 ? -sr : -cr));
+  *cs = q == 0 ? cr : (q == 1 ? -sr : (q == 2 ? -cr : sr));
+}
+
+// Value of positional-encoding feature f of x (degree L, include_input, log sampling):
+// [x, sin(2^0 x), cos(2^0 x), ..., sin(2^(L-1) x), cos(2^(L-1) x)], blocks of 3 coordinates.
+__device__ __forceinline__ float pe_feature(float x0, float x1, float x2, int f) {
+  const int block = f / 3, c = f - 3 * (f / 3);
+  const float v = c == 0 ? x0 : (c == 1 ? x1 : x2);
+  if (block == 0) return v;
+  const int m = (block - 1) >> 1;
+  float s, co;
+  pe_sincos(ldexpf(v, m), &s, &co);
+  return ((block - 1) & 1) ? co : s;
+}
+
+// ------------------------------------------------------------------- kernel arguments
+struct FwdArgs {
+  Plan p;
+  Ws w;
+  const uint8_t* packed;
+  int n;
+  const float* pts;
+  const float* dirs;
+  const float* noise;
+  float* sigma;
+  float* rgb;
+  uint8_t* ws;  // null for inference
+};
+
+struct BwdArgs {
+  Plan p;
+  Ws w;
+  const uint8_t* packed;
+  int n;
+  const float* sigma;
+  const float* rgb;
+  const float* d_sigma;
+  const float* d_rgb;
+  uint8_t* ws;
+};
+
+// -------------------------------------------------------------- generic layer product
+// Compile-time loop: f(std::integral_constant<int, I>) for I in [B, E).  Register arrays are
+// only ever indexed with these constants, so they stay in VGPRs.
+template <int B, int E, typename F>
+__device__ __forceinline__ void static_for(F&& f) {
+  if constexpr (B < E) {
+    f(std::integral_constant<int, B>{});
+    static_for<B + 1, E>(f);
+  }
+}
+
+// acc[OUT][CB] += W[TB .. TB+OUT-1, k-steps S0 .. S0+NS-1] x In for all OUT output tiles at once
+// (the accumulators fill the AGPR file while the activations stay in arch VGPRs).  ks is the
+// layer's k-step count (row stride of its packed A tiles); fin(t, cb) returns input tile t
+// (t counted from the first tile of k-step S0) for column block cb.
+template <int P, int NT, int OUT, int TB, typename FIn>
+__device__ __forceinline__ void mma_tiles(f32x4 (&acc)[OUT][Tr<P>::CB], const uint8_t* __restrict__ A,
+                                          int ks, int s0, FIn&& fin) {
+  constexpr int CB = Tr<P>::CB;
+  const int lane = threadIdx.x & 63;
+  if constexpr (P == kF32) {
+    const float* Af = reinterpret_cast<const float*>(A);
+    static_for<0, NT>([&](auto TI) SRF_AI {
+      constexpr int t = decltype(TI)::value;
+      f32x4 b[CB];
+#pragma unroll
+      for (int cb = 0; cb < CB; ++cb) b[cb] = fin(t, cb);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+#pragma unroll
+        for (int to = 0; to < OUT; ++to) {
+          const float a = Af[((TB + to) * ks + s0 + 4 * t + r) * 64 + lane];
+#pragma unroll
+          for (int cb = 0; cb < CB; ++cb) acc[to][cb] = mfma32(a, b[cb][r], acc[to][cb]);
+        }
+      }
+    });
+  } else {
+    const u32x4* Ah = reinterpret_cast<const u32x4*>(A);
+    static_for<0, NT / 2>([&](auto SI) SRF_AI {
+      constexpr int s = decltype(SI)::value;
+      u32x4 b[CB];
+#pragma unroll
+      for (int cb = 0; cb < CB; ++cb) {
+        const u32x2 x0 = fin(2 * s, cb), x1 = fin(2 * s + 1, cb);
+        b[cb] = u32x4{x0.x, x0.y, x1.x, x1.y};
+      }
+#pragma unroll
+      for (int to = 0; to < OUT; ++to) {
+        const u32x4 a = Ah[((TB + to) * ks + s0 + s) * 64 + lane];
+#pragma unroll
+        for (int cb = 0; cb < CB; ++cb) acc[to][cb] = mfma16<P>(a, b[cb], acc[to][cb]);
+      }
+    });
+  }
+}
+
+// k-steps spanned by NT input tiles
+template <int P>
+__host__ __device__ constexpr int ksteps(int tiles) { return P == kF32 ? 4 * tiles : tiles / 2; }
+
+template <int P, int OUT>
+__device__ __forceinline__ void zero_acc(f32x4 (&acc)[OUT][Tr<P>::CB]) {
+#pragma unroll
+  for (int to = 0; to < OUT; ++to)
+#pragma unroll
+    for (int cb = 0; cb < Tr<P>::CB; ++cb) acc[to][cb] = f32x4{0.f, 0.f, 0.f, 0.f};
+}
+
+// One whole layer with IN input tiles: accumulate, then epi(T, acc) for every output tile
+// (after the last k-step, so epi may overwrite the input tiles in place).
+template <int P, int IN, int OUT, int TB, typename FIn, typename Epi>
+__device__ __forceinline__ void run_layer(const uint8_t* __restrict__ A, FIn&& fin, Epi&& epi) {
+  f32x4 acc[OUT][Tr<P>::CB];
+  zero_acc<P, OUT>(acc);
+  mma_tiles<P, IN, OUT, TB>(acc, A, ksteps<P>(IN), 0, fin);
+#pragma unroll
+  for (int to = 0; to < OUT; ++to) epi(TB + to, acc[to]);
+}
+
+// Writes one 16-feature tile of the wave's points feature-major: dst row r, column p0 + point.
+template <int P>
+__device__ __forceinline__ void store_tile(uint8_t* __restrict__ dst, int ldP, int p0,
+                                           const typename Tr<P>::Act (&v)[Tr<P>::CB],
+                                           uint8_t* lds) {
+  constexpr int CB = Tr<P>::CB;
+  const int lane = threadIdx.x & 63, g = lane >> 4, j = lane & 15;
+  if constexpr (P == kF32) {
+    float* L = reinterpret_cast<float*>(lds);
+#pragma unroll
+    for (int cb = 0; cb < CB; ++cb)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) L[(4 * g + r) * 36 + 16 * cb + j] = v[cb][r];
+  } else {
+    uint16_t* L = reinterpret_cast<uint16_t*>(lds);
+#pragma unroll
+    for (int cb = 0; cb < CB; ++cb)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) L[(4 * g + r) * 72 + 16 * cb + j] = (uint16_t)act_elem_bits<P>(v[cb], r);
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  const int row = lane >> 2, seg = lane & 3;
+  const u32x4* src = reinterpret_cast<const u32x4*>(lds + row * kLdsRowBytes + seg * 32);
+  const u32x4 a = src[0], b = src[1];
+  const int col = P == kF32 ? p0 + 8 * seg : p0 + 16 * seg;
+  u32x4* out = reinterpret_cast<u32x4*>(dst + ((size_t)row * ldP + col) * (P == kF32 ? 4 : 2));
+  out[0] = a;
+  out[1] = b;
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ float sigmoidf_(float x) { return 1.0f / (1.0f + expf(-x)); }
+
+__device__ __forceinline__ float softplusf_(float x) { return x > 20.f ? x : log1pf(expf(x)); }
+
+// ------------------------------------------------------------------------ forward
+template <int P, bool VD, bool TRAIN>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) void mlp_fwd_kernel(
+    FwdArgs a) {
+  constexpr int CB = Tr<P>::CB;
+  using Act = typename Tr<P>::Act;
+  __shared__ __attribute__((aligned(16))) uint8_t lds_all[kWaves * 16 * kLdsRowBytes];
+  const int lane = threadIdx.x & 63, g = lane >> 4, j = lane & 15;
+  const int wv = threadIdx.x >> 6;
+  const int wave = blockIdx.x * kWaves + wv;
+  const int p0 = wave * 16 * CB;
+  if (p0 >= a.w.ldP) return;
+  uint8_t* lds = lds_all + wv * 16 * kLdsRowBytes;
+  const Plan& pl = a.p;
+  const int ldP = a.w.ldP;
+
+  // Point coordinates / view directions are re-read where needed (layer 0, layer 5, view
+  // layer) rather than kept live across the trunk: 12 bytes per point against 24 VGPRs.
+  auto load3 = [&](const float* src, int cb, float (&v)[3]) SRF_AI {
+    const int p = p0 + 16 * cb + j;
+    const bool ok = p < a.n;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) v[c] = ok ? src[3 * p + c] : 0.f;
+  };
+  auto valid = [&](int cb) SRF_AI { return p0 + 16 * cb + j < a.n; };
+
+  Act h[16][CB], comp[6][CB];
+  uint64_t mbits[CB];
+
+  auto pe_trunk = [&]() SRF_AI {
+    float x[CB][3];
+#pragma unroll
+    for (int cb = 0; cb < CB; ++cb) load3(a.pts, cb, x[cb]);
+    static_for<0, 4>([&](auto TI) SRF_AI {
+      constexpr int t = decltype(TI)::value;
+#pragma unroll
+      for (int cb = 0; cb < CB; ++cb) {
+        f32x4 v;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int f = 16 * t + 4 * g + r;
+          v[r] = f < pl.trunk_pe ? pe_feature(x[cb][0], x[cb][1], x[cb][2], f) : 0.f;
+        }
+        comp[t][cb] = to_act<P>(v);
+      }
+    });
+  };
+  auto mask_store = [&](int slot) SRF_AI {
+    if (TRAIN) {
+      uint64_t* m = reinterpret_cast<uint64_t*>(a.ws + a.w.mask + (size_t)slot * ldP * 32);
+#pragma unroll
+      for (int cb = 0; cb < CB; ++cb) m[((size_t)wave * CB + cb) * 64 + lane] = mbits[cb];
+    }
+  };
+  auto bias4 = [&](int l, int T) SRF_AI {
+    return *reinterpret_cast<const f32x4*>(a.packed + pl.bias_off[l] + (16 * T + 4 * g) * 4);
+  };
+  // ReLU epilogue: tile T of layer l into h[T] (in place), its feature-major copy at ws offset
+  // store_off and the mask bits.
+  auto relu_epi = [&](int l, size_t store_off) SRF_AI {
+    return [&, l, store_off](int T, f32x4(&acc)[CB]) SRF_AI {
+      const f32x4 bv = bias4(l, T);
+      Act t[CB];
+#pragma unroll
+      for (int cb = 0; cb < CB; ++cb) {
+        f32x4 v = acc[cb] + bv;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const bool pos = v[r] > 0.f;
+          v[r] = pos ? v[r] : 0.f;
+          mbits[cb] |= (uint64_t)pos << (4 * T + r);
+        }
+        t[cb] = to_act<P>(v);
+        h[T][cb] = t[cb];
+      }
+      if (TRAIN) store_tile<P>(a.ws + store_off + (size_t)16 * T * ldP * pl.elem, ldP, p0, t, lds);
+    };
+  };
+  auto fin_h = [&](int t, int cb) SRF_AI { return h[t][cb]; };
+  auto fin_hc = [&](int t, int cb) SRF_AI { return t < 16 ? h[t][cb] : comp[t - 16][cb]; };
+
+  // ---- layer 0: PE(trunk) -> 256
+  pe_trunk();
+  if (TRAIN) {
+    static_for<0, 4>([&](auto TI) SRF_AI {
+      constexpr int t = decltype(TI)::value;
+      if (t < pl.comp_tiles[0])
+        store_tile<P>(a.ws + a.w.x0 + (size_t)16 * t * ldP * pl.elem, ldP, p0, comp[t], lds);
+    });
+  }
+#pragma unroll
+  for (int cb = 0; cb < CB; ++cb) mbits[cb] = 0;
+  if (pl.comp_tiles[0] == 4)
+    run_layer<P, 4, 16, 0>(a.packed + pl.fwd_off[0], [&](int t, int cb) SRF_AI { return comp[t][cb]; },
+                           relu_epi(0, a.w.h[0]));
+  else
+    run_layer<P, 2, 16, 0>(a.packed + pl.fwd_off[0], [&](int t, int cb) SRF_AI { return comp[t][cb]; },
+                           relu_epi(0, a.w.h[0]));
+  mask_store(0);
+
+  // ---- layers 1..7, each in place in h; layer 5 adds the trunk PE k-steps after the 16
+  // register tiles (one loop body for all seven layers keeps register pressure flat)
+  for (int l = 1; l < 8; ++l) {
+#pragma unroll
+    for (int cb = 0; cb < CB; ++cb) mbits[cb] = 0;
+    f32x4 acc[16][CB];
+    zero_acc<P, 16>(acc);
+    const uint8_t* A = a.packed + pl.fwd_off[l];
+    const int ks = ksteps<P>(16 + pl.comp_tiles[l]);
+    mma_tiles<P, 16, 16, 0>(acc, A, ks, 0, fin_h);
+    if (l == 5) {
+      pe_trunk();
+      if (pl.comp_tiles[5] == 4)
+        mma_tiles<P, 4, 16, 0>(acc, A, ks, ksteps<P>(16), [&](int t, int cb) SRF_AI { return comp[t][cb]; });
+      else
+        mma_tiles<P, 2, 16, 0>(acc, A, ks, ksteps<P>(16), [&](int t, int cb) SRF_AI { return comp[t][cb]; });
+    }
+    auto epi = relu_epi(l, a.w.h[l]);
+#pragma unroll
+    for (int T = 0; T < 16; ++T) epi(T, acc[T]);
+    mask_store(l);
+  }
+
+  // ---- heads
+  auto head_sigma = [&](int cb, float raw) SRF_AI {
+    const int p = p0 + 16 * cb + j;
+    if (a.noise) raw = raw + a.noise[p];
+    a.sigma[p] = pl.softplus ? softplusf_(raw) : (raw > 0.f ? raw : 0.f);
+  };
+  if constexpr (!VD) {
+    // 4-channel output: sigma, rgb (src/models/SimpleNeRF17.py:735-750)
+    run_layer<P, 16, 1, 0>(a.packed + pl.fwd_off[8], fin_h, [&](int T, f32x4(&acc)[CB]) SRF_AI {
+      const f32x4 bv = bias4(8, 0);
+#pragma unroll
+      for (int cb = 0; cb < CB; ++cb) {
+        const int p = p0 + 16 * cb + j;
+        if (g == 0 && valid(cb)) {
+          const f32x4 v = acc[cb] + bv;
+          head_sigma(cb, v[0]);
+          a.rgb[3 * p + 0] = sigmoidf_(v[1]);
+          a.rgb[3 * p + 1] = sigmoidf_(v[2]);
+          a.rgb[3 * p + 2] = sigmoidf_(v[3]);
+        }
+      }
+    });
+  } else {
+    // sigma (row 256, tile 16) first, then feature rows 0..255 (identity) in place in h
+    run_layer<P, 16, 1, 16>(a.packed + pl.fwd_off[8], fin_h, [&](int T, f32x4(&acc)[CB]) SRF_AI {
+      const f32x4 bv = bias4(8, 16);
+#pragma unroll
+      for (int cb = 0; cb < CB; ++cb)
+        if (g == 0 && valid(cb)) head_sigma(cb, acc[cb][0] + bv[0]);
+    });
+    run_layer<P, 16, 16, 0>(a.packed + pl.fwd_off[8], fin_h, [&](int T, f32x4(&acc)[CB]) SRF_AI {
+      const f32x4 bv = bias4(8, T);
+      Act t[CB];
+#pragma unroll
+      for (int cb = 0; cb < CB; ++cb) {
+        t[cb] = to_act<P>(acc[cb] + bv);
+        h[T][cb] = t[cb];
+      }
+      if (TRAIN) store_tile<P>(a.ws + a.w.f + (size_t)16 * T * ldP * pl.elem, ldP, p0, t, lds);
+    });
+    // view layer input extras: [PE_xyz[trunk_pe:], PE_dir]
+    const int n_hi = 3 * (2 * pl.pts_deg + 1) - pl.trunk_pe;
+    float x[CB][3], d[CB][3];
+#pragma unroll
+    for (int cb = 0; cb < CB; ++cb) {
+      load3(a.pts, cb, x[cb]);
+      load3(a.dirs, cb, d[cb]);
+    }
+    static_for<0, 6>([&](auto TI) SRF_AI {
+      constexpr int t = decltype(TI)::value;
+#pragma unroll
+      for (int cb = 0; cb < CB; ++cb) {
+        f32x4 v;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int e = 16 * t + 4 * g + r;
+          float val = 0.f;
+          if (e < n_hi) val = pe_feature(x[cb][0], x[cb][1], x[cb][2], pl.trunk_pe + e);
+          else if (e < pl.extra) val = pe_feature(d[cb][0], d[cb][1], d[cb][2], e - n_hi);
+          v[r] = val;
+        }
+        comp[t][cb] = to_act<P>(v);
+      }
+    });
+    if (TRAIN) {
+      static_for<0, 6>([&](auto TI) SRF_AI {
+        constexpr int t = decltype(TI)::value;
+        if (t < pl.comp_tiles[9])
+          store_tile<P>(a.ws + a.w.ex + (size_t)16 * t * ldP * pl.elem, ldP, p0, comp[t], lds);
+      });
+    }
+#pragma unroll
+    for (int cb = 0; cb < CB; ++cb) mbits[cb] = 0;
+    // view layer [feature, extras] -> 128 (ReLU), in place into h[0..7]
+    if (pl.comp_tiles[9] == 2)
+      run_layer<P, 18, 8, 0>(a.packed + pl.fwd_off[9], fin_hc, relu_epi(9, a.w.v));
+    else if (pl.comp_tiles[9] == 4)
+      run_layer<P, 20, 8, 0>(a.packed + pl.fwd_off[9], fin_hc, relu_epi(9, a.w.v));
+    else
+      run_layer<P, 22, 8, 0>(a.packed + pl.fwd_off[9], fin_hc, relu_epi(9, a.w.v));
+    mask_store(8);
+    // rgb head 128 -> 3, sigmoid
+    run_layer<P, 8, 1, 0>(a.packed + pl.fwd_off[10], fin_h, [&](int T, f32x4(&acc)[CB]) SRF_AI {
+      const f32x4 bv = bias4(10, 0);
+#pragma unroll
+      for (int cb = 0; cb < CB; ++cb) {
+        const int p = p0 + 16 * cb + j;
+        if (g == 0 && valid(cb)) {
+          const f32x4 v = acc[cb] + bv;
+          a.rgb[3 * p + 0] = sigmoidf_(v[0]);
+          a.rgb[3 * p + 1] = sigmoidf_(v[1]);
+          a.rgb[3 * p + 2] = sigmoidf_(v[2]);
+        }
+      }
+    });
+  }
+}
+
+// ------------------------------------------------------------------- backward chain
+template <int P, bool VD>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) void mlp_bwd_kernel(
+    BwdArgs a) {
+  constexpr int CB = Tr<P>::CB;
+  using Act = typename Tr<P>::Act;
+  __shared__ __attribute__((aligned(16))) uint8_t lds_all[kWaves * 16 * kLdsRowBytes];
+  const int lane = threadIdx.x & 63, g = lane >> 4, j = lane & 15;
+  const int wv = threadIdx.x >> 6;
+  const int wave = blockIdx.x * kWaves + wv;
+  const int p0 = wave * 16 * CB;
+  if (p0 >= a.w.ldP) return;
+  uint8_t* lds = lds_all + wv * 16 * kLdsRowBytes;
+  const Plan& pl = a.p;
+  const int ldP = a.w.ldP;
+
+  float dsr[CB], drr[CB][3];  // head pre-activation gradients
+#pragma unroll
+  for (int cb = 0; cb < CB; ++cb) {
+    const int p = p0 + 16 * cb + j;
+    const bool valid = p < a.n;
+    float ds = 0.f, s = 0.f;
+    if (valid) {
+      ds = a.d_sigma[p];
+      s = a.sigma[p];
+    }
+    dsr[cb] = pl.softplus ? ds * (1.0f - expf(-s)) : (s > 0.f ? ds : 0.f);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      float dr = 0.f, y = 0.f;
+      if (valid) {
+        dr = a.d_rgb[3 * p + c];
+        y = a.rgb[3 * p + c];
+      }
+      drr[cb][c] = dr * (1.0f - y) * y;
+    }
+  }
+
+  // z holds the current layer's pre-activation gradients; every product writes in place.
+  Act z[18][CB];
+  uint64_t mb[CB];
+  auto store_dz = [&](int l, int T, const Act(&t)[CB]) SRF_AI {
+    store_tile<P>(a.ws + a.w.dz[l] + (size_t)16 * T * ldP * pl.elem, ldP, p0, t, lds);
+  };
+  auto load_mask = [&](int slot) SRF_AI {
+    const uint64_t* m = reinterpret_cast<const uint64_t*>(a.ws + a.w.mask + (size_t)slot * ldP * 32);
+#pragma unroll
+    for (int cb = 0; cb < CB; ++cb) mb[cb] = m[((size_t)wave * CB + cb) * 64 + lane];
+  };
+  // epilogue: ReLU'-masked gradient of layer l into z[T], stored as dZ_l
+  auto masked_epi = [&](int l) SRF_AI {
+    return [&, l](int T, f32x4(&acc)[CB]) SRF_AI {
+      Act t[CB];
+#pragma unroll
+      for (int cb = 0; cb < CB; ++cb) {
+        f32x4 v = acc[cb];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) v[r] = ((mb[cb] >> (4 * T + r)) & 1ull) ? v[r] : 0.f;
+        t[cb] = to_act<P>(v);
+        z[T][cb] = t[cb];
+      }
+      store_dz(l, T, t);
+    };
+  };
+  auto fin_z = [&](int t, int cb) SRF_AI { return z[t][cb]; };
+
+  if constexpr (VD) {
+    // rgb head: dZ10 tile 0 (rows 0..2), zero pad tile
+    {
+      Act t[CB];
+#pragma unroll
+      for (int cb = 0; cb < CB; ++cb) {
+        f32x4 v = {0.f, 0.f, 0.f, 0.f};
+        if (g == 0) v = f32x4{drr[cb][0], drr[cb][1], drr[cb][2], 0.f};
+        t[cb] = to_act<P>(v);
+        z[0][cb] = t[cb];
+        z[1][cb] = zero_act<P>();
+      }
+      store_dz(10, 0, t);
+    }
+    // dV = W10^T dZ10, masked by the view-layer ReLU -> dZ9 (z[0..7])
+    load_mask(8);
+    run_layer<P, 2, 8, 0>(a.packed + pl.bwd_off[10], fin_z, masked_epi(9));
+    // dF = W9^T dZ9 (feature part, identity activation) -> dZ8 rows 0..255 (z[0..15])
+    run_layer<P, 8, 16, 0>(a.packed + pl.bwd_off[9], fin_z, [&](int T, f32x4(&acc)[CB]) SRF_AI {
+      Act t[CB];
+#pragma unroll
+      for (int cb = 0; cb < CB; ++cb) {
+        t[cb] = to_act<P>(acc[cb]);
+        z[T][cb] = t[cb];
+      }
+      store_dz(8, T, t);
+    });
+    {
+      Act t[CB];
+#pragma unroll
+      for (int cb = 0; cb < CB; ++cb) {
+        f32x4 v = {0.f, 0.f, 0.f, 0.f};
+        if (g == 0) v[0] = dsr[cb];
+        t[cb] = to_act<P>(v);
+        z[16][cb] = t[cb];
+        z[17][cb] = zero_act<P>();
+      }
+      store_dz(8, 16, t);
+    }
+    load_mask(7);
+    run_layer<P, 18, 16, 0>(a.packed + pl.bwd_off[8], fin_z, masked_epi(7));
+  } else {
+    {
+      Act t[CB];
+#pragma unroll
+      for (int cb = 0; cb < CB; ++cb) {
+        f32x4 v = {0.f, 0.f, 0.f, 0.f};
+        if (g == 0) v = f32x4{dsr[cb], drr[cb][0], drr[cb][1], drr[cb][2]};
+        t[cb] = to_act<P>(v);
+        z[0][cb] = t[cb];
+        z[1][cb] = zero_act<P>();
+      }
+      store_dz(8, 0, t);
+    }
+    load_mask(7);
+    run_layer<P, 2, 16, 0>(a.packed + pl.bwd_off[8], fin_z, masked_epi(7));
+  }
+  // trunk: z holds dZ_l; dH_{l-1} = W_l^T dZ_l, masked -> dZ_{l-1}, in place
+  for (int l = 7; l >= 1; --l) {
+    load_mask(l - 1);
+    run_layer<P, 16, 16, 0>(a.packed + pl.bwd_off[l], fin_z, masked_epi(l - 1));
+  }
+}
+
+// ----------------------------------------------------------------- weight gradients
+struct DwJob {
+  const uint8_t* dz;  // first dZ row used
+  int dz_rows;        // rows stored from there (multiple of 16)
+  const uint8_t* x;   // first input row used
+  int x_rows;
+  int M, N;           // valid rows (outputs) and columns (inputs)
+  float* dw;          // reference gradient, first element of the block
+  int ld_dw;
+  float* db;          // nullable
+  int mblocks, nblocks, work_begin, work_count;
+};
+
+struct DwArgs {
+  DwJob jobs[kMaxJobs];
+  int n_jobs, ldP, n_chunks, total_work;
+};
+
+template <int P>
+__global__ __launch_bounds__(256) void mlp_dw_kernel(DwArgs a) {
+  const int lane = threadIdx.x & 63, g = lane >> 4, j = lane & 15;
+  const int w = blockIdx.x * kWaves + (threadIdx.x >> 6);
+  if (w >= a.total_work) return;
+  int ji = 0;
+#pragma unroll 1
+  while (ji + 1 < a.n_jobs && w >= a.jobs[ji + 1].work_begin) ++ji;
+  const DwJob& jb = a.jobs[ji];
+  const int local = w - jb.work_begin;
+  const int nblk = jb.mblocks * jb.nblocks;
+  const int tile = local % nblk, chunk = local / nblk;
+  const int m0 = 64 * (tile % jb.mblocks), n0 = 64 * (tile / jb.mblocks);
+  const int pb = chunk * kChunk;
+  const int pe = min(pb + kChunk, a.ldP);
+  const size_t ldP = a.ldP;
+  bool mv[4], nv[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    mv[i] = m0 + 16 * i < jb.dz_rows;
+    nv[i] = n0 + 16 * i < jb.x_rows;
+  }
+  f32x4 acc[4][4];
+  float rs[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    rs[i] = 0.f;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) acc[i][k] = f32x4{0.f, 0.f, 0.f, 0.f};
+  }
+  const bool do_db = jb.db != nullptr && n0 == 0;
+  if constexpr (P == kF32) {
+    const float* dz = reinterpret_cast<const float*>(jb.dz);
+    const float* x = reinterpret_cast<const float*>(jb.x);
+#pragma unroll 1
+    for (int p = pb; p < pe; p += 16) {
+      f32x4 av[4], bv[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        av[i] = mv[i] ? *reinterpret_cast<const f32x4*>(dz + (m0 + 16 * i + j) * ldP + p + 4 * g)
+                      : f32x4{0.f, 0.f, 0.f, 0.f};
+        bv[i] = nv[i] ? *reinterpret_cast<const f32x4*>(x + (n0 + 16 * i + j) * ldP + p + 4 * g)
+                      : f32x4{0.f, 0.f, 0.f, 0.f};
+      }
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+          for (int k = 0; k < 4; ++k) acc[i][k] = mfma32(av[i][q], bv[k][q], acc[i][k]);
+      if (do_db)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) rs[i] += (av[i][0] + av[i][1]) + (av[i][2] + av[i][3]);
+    }
+  } else {
+    const uint16_t* dz = reinterpret_cast<const uint16_t*>(jb.dz);
+    const uint16_t* x = reinterpret_cast<const uint16_t*>(jb.x);
+#pragma unroll 1
+    for (int p = pb; p < pe; p += 32) {
+      u32x4 av[4], bv[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        av[i] = mv[i] ? *reinterpret_cast<const u32x4*>(dz + (m0 + 16 * i + j) * ldP + p + 8 * g)
+                      : u32x4{0u, 0u, 0u, 0u};
+        bv[i] = nv[i] ? *reinterpret_cast<const u32x4*>(x + (n0 + 16 * i + j) * ldP + p + 8 * g)
+                      : u32x4{0u, 0u, 0u, 0u};
+      }
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int k = 0; k < 4; ++k) acc[i][k] = mfma16<P>(av[i], bv[k], acc[i][k]);
+      if (do_db)
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+          for (int q = 0; q < 4; ++q) rs[i] += unpack_lo<P>(av[i][q]) + unpack_hi<P>(av[i][q]);
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    if (!mv[i]) continue;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      if (!nv[k]) continue;
+      const int c = n0 + 16 * k + j;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int o = m0 + 16 * i + 4 * g + r;
+        if (o < jb.M && c < jb.N) unsafeAtomicAdd(jb.dw + (size_t)o * jb.ld_dw + c, acc[i][k][r]);
+      }
+    }
+    if (do_db) {
+      float s = rs[i];
+      s += __shfl_xor(s, 16, 64);
+      s += __shfl_xor(s, 32, 64);
+      const int o = m0 + 16 * i + j;
+      if (g == 0 && o < jb.M) unsafeAtomicAdd(jb.db + o, s);
+    }
+  }
+}
+
+// --------------------------------------------------------------------- host glue
+template <int P, bool VD, bool TRAIN>
+int launch_fwd(const FwdArgs& a, hipStream_t s) {
+  const int ppw = 16 * Tr<P>::CB;
+  const int waves = a.w.ldP / ppw;
+  const int blocks = (waves + kWaves - 1) / kWaves;
+  hipLaunchKernelGGL((mlp_fwd_kernel<P, VD, TRAIN>), dim3(blocks), dim3(64 * kWaves), 0, s, a);
+  return launch_status();
+}
+
+template <int P, bool VD>
+int launch_bwd(const BwdArgs& a, hipStream_t s) {
+  const int ppw = 16 * Tr<P>::CB;
+  const int waves = a.w.ldP / ppw;
+  const int blocks = (waves + kWaves - 1) / kWaves;
+  hipLaunchKernelGGL((mlp_bwd_kernel<P, VD>), dim3(blocks), dim3(64 * kWaves), 0, s, a);
+  return launch_status();
+}
+
+template <int P>
+int launch_dw(const DwArgs& d, hipStream_t s) {
+  const int blocks = (d.total_work + kWaves - 1) / kWaves;
+  hipLaunchKernelGGL(mlp_dw_kernel<P>, dim3(blocks), dim3(256), 0, s, d);
+  return launch_status();
+}
+
+// Per-precision launchers, one translation unit each (mlp_f32.hip, mlp_bf16.hip).
+int fwd_launch(int prec, const FwdArgs& a, bool view_dep, bool train, hipStream_t s);
+int bwd_launch(int prec, const BwdArgs& a, bool view_dep, hipStream_t s);
+int dw_launch(int prec, const DwArgs& d, hipStream_t s);
+int fwd_f32(const FwdArgs& a, bool vd, bool tr, hipStream_t s);
+int fwd_bf16(const FwdArgs& a, bool vd, bool tr, hipStream_t s);
+int bwd_f32(const BwdArgs& a, bool vd, hipStream_t s);
+int bwd_bf16(const BwdArgs& a, bool vd, hipStream_t s);
+int dw_f32(const DwArgs& d, hipStream_t s);
+int dw_bf16(const DwArgs& d, hipStream_t s);
+
+}  // namespace mlp
+}  // namespace srf

@@ -179,8 +179,9 @@ using namespace srf;
 
 extern "C" int srf_stratified_samples(int n_rays, int n_samples, float near, float far, int lindisp,
                                       const float* u, float* z, void* stream) {
-  if (n_rays < 0 || n_samples < 1 || !z) return SRF_EINVAL;
+  if (n_rays < 0 || n_samples < 1) return SRF_EINVAL;
   if (n_rays == 0) return SRF_OK;
+  if (!z) return SRF_EINVAL;
   const int64_t total = (int64_t)n_rays * n_samples;
   const int threads = 256;
   const int64_t blocks = (total + threads - 1) / threads;
@@ -197,8 +198,8 @@ extern "C" int srf_sample_pdf(int n_rays, int n_bins, int n_fine, const float* b
                               const float* weights, const float* cdf_in, const float* u,
                               float* samples, int64_t* inds, float* cdf_out, void* stream) {
   if (n_rays < 0 || n_bins < 2 || n_bins > kMaxBins || n_fine < 1) return SRF_EINVAL;
-  if (!bins || !samples || (!weights && !cdf_in)) return SRF_EINVAL;
   if (n_rays == 0) return SRF_OK;
+  if (!bins || !samples || (!weights && !cdf_in)) return SRF_EINVAL;
   hipLaunchKernelGGL(sample_pdf_kernel, dim3(n_rays), dim3(SRF_WAVE), 0, as_stream(stream), n_rays,
                      n_bins, n_fine, bins, weights, cdf_in, u, samples, inds, cdf_out);
   return launch_status();
@@ -208,8 +209,8 @@ extern "C" int srf_fine_z_vals(int n_rays, int n_coarse, int n_fine, const float
                                const float* weights_coarse, const float* u, float* z_merged,
                                float* z_fine, void* stream) {
   if (n_rays < 0 || n_coarse < 3 || n_fine < 1 || n_coarse + n_fine > kMaxBins) return SRF_EINVAL;
-  if (!z_coarse || !weights_coarse || !z_merged) return SRF_EINVAL;
   if (n_rays == 0) return SRF_OK;
+  if (!z_coarse || !weights_coarse || !z_merged) return SRF_EINVAL;
   hipLaunchKernelGGL(fine_z_kernel, dim3(n_rays), dim3(SRF_WAVE), 0, as_stream(stream), n_rays,
                      n_coarse, n_fine, z_coarse, weights_coarse, u, z_merged, z_fine);
   return launch_status();

@@ -45,8 +45,17 @@ def sources() -> list[Path]:
     return sorted(CSRC_DIR.glob('*.hip'))
 
 
-def _compile(src: Path, extra: list[str]) -> Path:
+def _up_to_date(src: Path, obj: Path) -> bool:
+    if not obj.exists():
+        return False
+    deps = [src, *CSRC_DIR.glob('*.h'), REPO_DIR / 'include' / 'srf_hip.h']
+    return obj.stat().st_mtime > max(d.stat().st_mtime for d in deps if d.exists())
+
+
+def _compile(src: Path, extra: list[str], incremental: bool = False) -> Path:
     obj = OBJ_DIR / (src.stem + '.o')
+    if incremental and _up_to_date(src, obj):
+        return obj
     cmd = [_hipcc(), *HIPCC_FLAGS, *extra, '-c', str(src), '-o', str(obj)]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
@@ -54,8 +63,12 @@ def _compile(src: Path, extra: list[str]) -> Path:
     return obj
 
 
-def build(verbose: bool = False, extra_flags: list[str] | None = None, jobs: int | None = None) -> Path:
-    """Compile every csrc/*.hip for gfx950 and link libsrf_hip.so.  Always rebuilds."""
+def build(verbose: bool = False, extra_flags: list[str] | None = None, jobs: int | None = None,
+          incremental: bool = False) -> Path:
+    """Compile every csrc/*.hip for gfx950 and link libsrf_hip.so.
+
+    Rebuilds every object unless ``incremental`` (development use: objects newer than their
+    source and all headers are reused)."""
     OBJ_DIR.mkdir(exist_ok=True)
     srcs = sources()
     if not srcs:
@@ -63,7 +76,7 @@ def build(verbose: bool = False, extra_flags: list[str] | None = None, jobs: int
     extra = list(extra_flags or [])
     jobs = jobs or min(len(srcs), max(1, (os.cpu_count() or 4)), 16)
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
-        objs = list(ex.map(lambda s: _compile(s, extra), srcs))
+        objs = list(ex.map(lambda s: _compile(s, extra, incremental), srcs))
     tmp = LIB_PATH.with_suffix('.so.tmp')
     cmd = [_hipcc(), f'--offload-arch={ARCH}', '-shared', '-fPIC', *map(str, objs), '-o', str(tmp)]
     res = subprocess.run(cmd, capture_output=True, text=True)
@@ -76,4 +89,4 @@ def build(verbose: bool = False, extra_flags: list[str] | None = None, jobs: int
 
 
 if __name__ == '__main__':
-    build(verbose=True)
+    build(verbose=True, incremental='--incremental' in sys.argv)

@@ -43,12 +43,22 @@ def hip_out(c, dev, ndc=True, white=False, ds=1.0, req_grad=False):
 TOL = dict(rgb=1e-5, acc=1e-5, depth_ndc=1e-5, alpha=1e-6, visibility=1e-5, weights=1e-5)
 
 
-def check_fwd(out, ref, keys):
+def check_fwd(out, ref, keys, ref64=None):
     for k in keys:
         a, b = out[k].detach().cpu(), ref[k]
-        if k in ('depth', 'depth_var', 'depth_var_ndc'):
+        if k in ('depth_var', 'depth_var_ndc') and ref64 is not None:
+            # second moments of metric depths up to ~1e3 carry summation-order rounding of a
+            # few 1e-4 in FP32: the kernel must be as close to the FP64 value as the FP32 oracle
+            t = ref64[k]
+            ek = ((a.double() - t).abs() / t.abs().clamp_min(1e-5)).max().item()
+            eo = ((b.double() - t).abs() / t.abs().clamp_min(1e-5)).max().item()
+            assert ek <= 2 * eo + 1e-5, (k, ek, eo)
+        elif k == 'depth':
             # metric depth grows like 1/(1-z_ndc); compare with a relative floor
             assert torch.allclose(a, b, atol=1e-5, rtol=2e-5), (k, (a - b).abs().max())
+        elif k in ('depth_var', 'depth_var_ndc'):
+            # second moments of metric depths up to ~1e3: summation-order rounding is relative
+            assert torch.allclose(a, b, atol=1e-5, rtol=3e-4), (k, ((a - b).abs() / b.abs().clamp_min(1e-5)).max())
         else:
             assert (a - b).abs().max().item() <= TOL[k], (k, (a - b).abs().max().item())
 
@@ -58,20 +68,21 @@ def check_fwd(out, ref, keys):
 def test_forward_ndc_matches_oracle(cuda, n_samples, white):
     c = make_case(1000, n_samples, seed=n_samples)
     out, _, _ = hip_out(c, cuda, white=white)
-    check_fwd(out, oracle_out(c, white=white), OUT_KEYS)
+    check_fwd(out, oracle_out(c, white=white), OUT_KEYS, oracle_out({k: v.double() for k, v in c.items()}, white=white))
 
 
 def test_forward_tensorf_distance_scale(cuda):
     c = make_case(800, 443, seed=3, sigma_scale=0.3)
     out, _, _ = hip_out(c, cuda, ds=25.0)
-    check_fwd(out, oracle_out(c, ds=25.0), OUT_KEYS)
+    check_fwd(out, oracle_out(c, ds=25.0), OUT_KEYS, oracle_out({k: v.double() for k, v in c.items()}, ds=25.0))
 
 
 def test_forward_metric_z(cuda):
     c = make_case(600, 64, seed=9)
     c['z'] = 2.0 + 4.0 * c['z']
     out, _, _ = hip_out(c, cuda, ndc=False)
-    check_fwd(out, oracle_out(c, ndc=False), ['rgb', 'acc', 'depth', 'depth_var', 'alpha', 'visibility', 'weights'])
+    check_fwd(out, oracle_out(c, ndc=False), ['rgb', 'acc', 'depth', 'depth_var', 'alpha', 'visibility', 'weights'],
+              oracle_out({k: v.double() for k, v in c.items()}, ndc=False))
 
 
 def test_forward_vs_fp64(cuda):

@@ -57,21 +57,25 @@ def test_sample_pdf_given_cdf_bit_exact(cuda, det):
 
 
 def test_sample_pdf_internal_cdf(cuda):
+    """With its own CDF the kernel is as close to the FP64 result as the FP32 oracle is.
+    (Bit-exactness is defined given the same CDF, tested above; here torch's float
+    normaliser sum and the kernel's may differ by an ulp, which narrow CDF steps amplify.)"""
     n_rays, n_bins, n_fine = 2048, 63, 128
     g = torch.Generator().manual_seed(4)
     bins = torch.sort(torch.rand(n_rays, n_bins, generator=g), -1)[0]
     w = _coarse_weights(n_rays, n_bins - 1, 5)
     u = torch.rand(n_rays, n_fine, generator=g)
     ref_s, ref_i, ref_cdf = on.sample_pdf(bins, w, n_fine, u=u)
+    s64, _, _ = on.sample_pdf(bins.double(), w.double(), n_fine, u=u.double())
     s, i, cdf = sampling.sample_pdf(bins.to(cuda), w.to(cuda), n_fine, u=u.to(cuda), return_inds=True,
                                     return_cdf=True)
     # cumsum accumulates in double as torch's CPU kernel does; the normaliser may differ by 1 ulp
     assert (cdf.cpu() - ref_cdf).abs().max().item() <= 5e-7
-    mism = (i.cpu() != ref_i)
-    assert mism.float().mean().item() < 1e-3       # only u within an ulp of a cdf entry may flip
-    ok = ~mism
-    # a 1-ulp cdf difference is amplified by (bin width / cdf step) inside narrow cdf steps
-    assert torch.allclose(s.cpu()[ok], ref_s[ok], atol=5e-5, rtol=0)
+    assert (i.cpu() != ref_i).float().mean().item() < 1e-3   # only u within an ulp of a cdf entry
+    err_k = (s.cpu().double() - s64).abs()
+    err_o = (ref_s.double() - s64).abs()
+    assert err_k.max() <= 2 * err_o.max() + 1e-6
+    assert err_k.mean() <= 2 * err_o.mean() + 1e-8
 
 
 @pytest.mark.parametrize('det', [True, False])
@@ -83,15 +87,23 @@ def test_fine_z_vals(cuda, det, n_coarse, n_fine):
     w = _coarse_weights(n_rays, n_coarse, 8)
     u = None if det else torch.rand(n_rays, n_fine, generator=g)
     ref_m, ref_f = on.fine_z_vals(z, w, n_fine, u=u)
+    _, f64 = on.fine_z_vals(z.double(), w.double(), n_fine, u=None if u is None else u.double())
     m, f = sampling.fine_z_vals(z.to(cuda), w.to(cuda), n_fine, u=None if u is None else u.to(cuda),
                                 return_fine=True)
     m, f = m.cpu(), f.cpu()
     # sortedness and multiset are exact by construction
     assert bool((m[:, 1:] >= m[:, :-1]).all())
     assert torch.equal(torch.sort(torch.cat([z, f], -1), -1)[0], m)
-    # the fine samples follow the oracle up to the 1-ulp CDF normaliser difference
-    assert torch.allclose(f, ref_f, atol=5e-5, rtol=0)
-    assert torch.allclose(m, ref_m, atol=5e-5, rtol=0)
+    # the fine samples are as close to FP64 as the FP32 oracle's (see test_sample_pdf_internal_cdf)
+    err_k = (f.double() - f64).abs()
+    err_o = (ref_f.double() - f64).abs()
+    assert err_k.max() <= 2 * err_o.max() + 1e-6
+    assert err_k.mean() <= 2 * err_o.mean() + 1e-8
+    # ... and given the oracle's CDF the merge reproduces torch.sort(cat) of the oracle bit for bit
+    mids = .5 * (z[..., 1:] + z[..., :-1])
+    cdf = on.pdf_cdf(w[..., 1:-1])
+    fs = sampling.sample_pdf(mids.to(cuda), None, n_fine, u=None if u is None else u.to(cuda), cdf=cdf.to(cuda))
+    assert torch.equal(fs.cpu(), ref_f)
 
 
 def test_fine_z_vals_given_same_samples_is_sort(cuda):
