@@ -1,0 +1,275 @@
+#!/usr/bin/env python
+"""Benchmark of the Simple-RF rendering hot path on one MI355X.
+
+Workload (one "step"): the Simple-NeRF training step of the drop-in model SimpleNeRFHip17 on
+4096 synthetic rays -- coarse 64 stratified samples through the main, points-augmented and
+views-augmented MLPs, 128 inverse-CDF fine samples merged to 192 through the fine MLP, alpha
+compositing of all four, the synthetic loss (photometric + sparse depth + depth consistency),
+the hand-written backward of every stage, and an Adam update.  MLP operands in BF16 on MFMA
+with FP32 accumulation; sampling, transmittance and compositing in FP32.  Inputs (pixel ids,
+targets, weights) are resident in HBM before the timed region.
+
+Printed: one JSON line with train rays/s (value), the per-kernel roofline of the dominant
+kernel (measured with HIP events inside the timed region), render rays/s of the eval chunk,
+the plain-PyTorch FP32 baseline of the same step on the same GPU, and the CPU baseline.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import statistics
+import sys
+import time
+from pathlib import Path
+
+REPO = Path(__file__).resolve().parent
+for _p in (REPO, REPO / 'simple-rf_amd'):
+    if str(_p) not in sys.path:
+        sys.path.insert(0, str(_p))
+
+METRIC = 'train rays/s (fwd+bwd) & render rays/s: fused BF16 vs plain FP32, 1 GPU'
+PEAK_BF16_TFLOPS = 2500.0
+PEAK_HBM_GBS = 8000.0
+MACS = {(63, True): 593408, (21, True): 577280, (63, False): 492032}
+
+
+class KernelTimer:
+    """HIP events around the C-ABI entry points (each launches one kernel on the current
+    stream, except srf_mlp_pack which launches a few small ones)."""
+
+    def __init__(self, lib_mod):
+        import torch
+        self.torch = torch
+        self.lib = lib_mod
+        self.orig = lib_mod.call
+        self.records = []
+        self.enabled = False
+
+    def __enter__(self):
+        def call(name, *args):
+            if not self.enabled:
+                return self.orig(name, *args)
+            s = self.torch.cuda.Event(enable_timing=True)
+            e = self.torch.cuda.Event(enable_timing=True)
+            s.record()
+            self.orig(name, *args)
+            e.record()
+            self.records.append((name, args, s, e))
+        self.lib.call = call
+        return self
+
+    def __exit__(self, *exc):
+        self.lib.call = self.orig
+
+    def summary(self):
+        self.torch.cuda.synchronize()
+        out = {}
+        for name, args, s, e in self.records:
+            ms = s.elapsed_time(e)
+            d = out.setdefault(name, {'calls': 0, 'ms': 0.0, 'flop': 0.0})
+            d['calls'] += 1
+            d['ms'] += ms
+            if name in ('srf_mlp_forward', 'srf_mlp_backward_data', 'srf_mlp_weight_grads'):
+                desc = args[0]._obj
+                n = args[2] if name != 'srf_mlp_weight_grads' else args[1]
+                d['flop'] += 2.0 * MACS[(desc.trunk_pe, bool(desc.view_dependent))] * n
+        return out
+
+
+def build_model(precision, dev, seed=0):
+    import torch
+    from models.ModelFactory02 import get_model
+    from srf_hip import configs as cfgs
+    torch.manual_seed(seed)
+    configs = cfgs.simple_nerf_configs(precision=precision)
+    model = get_model(configs, model_configs=cfgs.model_configs()).to(dev)
+    return configs, model
+
+
+def hip_train_leg(args, dev):
+    import torch
+    from srf_hip import _lib, synthetic
+    _, model = build_model('bf16', dev)
+    model.train()
+    params = [p for p in model.parameters() if p.requires_grad]
+    opt = torch.optim.Adam(params, lr=5e-4, betas=(0.9, 0.999))
+    batch = synthetic.train_batch(args.rays, seed=0, device=dev)
+    inp = {'pixel_id': batch['pixel_id'], 'num_frames': 3, 'iter_num': 0, 'sub_batch_index': 0}
+
+    def step():
+        out = model(inp)
+        loss = synthetic.train_loss(out, batch)
+        opt.zero_grad(set_to_none=True)
+        loss.backward()
+        opt.step()
+        return loss
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    timer = KernelTimer(_lib)
+    with timer:
+        timer.enabled = True
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            loss = step()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        timer.enabled = False
+    kern = timer.summary()
+    return dict(seconds=t1 - t0, loss=float(loss.item()), kernels=kern, model=model)
+
+
+def hip_render_leg(args, dev, model):
+    """Forward-only eval render of one chunk (deterministic sampling, no augmentations)."""
+    import torch
+    from srf_hip import synthetic
+    model.eval()
+    r = synthetic.train_batch(args.render_rays, seed=3, device=dev)
+    inp = {'pixel_id': r['pixel_id'], 'num_frames': 3}
+    with torch.no_grad():
+        for _ in range(2):
+            model(inp)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        n = 3
+        for _ in range(n):
+            model(inp)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+    model.train()
+    return args.render_rays * n / (t1 - t0)
+
+
+def oracle_step_fn(model, dev, n_rays, seed=0):
+    """The same training step restated in plain PyTorch FP32 (oracle) on device `dev`."""
+    import torch
+    import oracle.train_step as ots
+    from srf_hip import synthetic
+    from srf_hip import configs as cfgs
+    mc = cfgs.model_configs()
+    params = {}
+    names = {'coarse': model.coarse_model, 'fine': model.fine_model,
+             'points_augmentation': model.augmented_models[0]['coarse_model'],
+             'views_augmentation': model.augmented_models[1]['coarse_model']}
+    leaves = []
+    for key, m in names.items():
+        params[key] = {k: v.detach().to(dev, torch.float32).clone().requires_grad_(True) for k, v in m.state_dict().items()}
+        leaves.extend(params[key].values())
+    opt = torch.optim.Adam(leaves, lr=5e-4, betas=(0.9, 0.999))
+    batch = synthetic.train_batch(n_rays, seed=seed, device=dev)
+    pid = batch['pixel_id']
+    K = torch.from_numpy(mc['intrinsics']).to(dev)[pid[:, 0].long()]
+    E = torch.from_numpy(mc['extrinsics']).to(dev)[pid[:, 0].long()]
+    res = mc['resolution']
+
+    def step():
+        u_c = torch.rand(n_rays, 64, device=dev)
+        u_f = torch.rand(n_rays, 128, device=dev)
+        noise = {k: torch.randn(n_rays * (192 if k == 'fine' else 64), device=dev) for k in names}
+        out = ots.render_train(params, pid, K, E, res, u_coarse=u_c, u_fine=u_f, noise=noise)
+        loss = synthetic.train_loss(out, batch)
+        opt.zero_grad(set_to_none=True)
+        loss.backward()
+        opt.step()
+    return step
+
+
+def plain_gpu_leg(args, dev, model):
+    import torch
+    torch.backends.cuda.matmul.allow_tf32 = False
+    torch.backends.cudnn.allow_tf32 = False
+    step = oracle_step_fn(model, dev, args.rays)
+    for _ in range(2):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    n = args.plain_steps
+    for _ in range(n):
+        step()
+    torch.cuda.synchronize()
+    return args.rays * n / (time.perf_counter() - t0)
+
+
+def cpu_baseline_leg(args, model):
+    """The oracle (plain PyTorch FP32 restatement) on the host cores, bounded sample."""
+    import torch
+    threads = os.cpu_count() or 1
+    torch.set_num_threads(threads)
+    n = args.cpu_rays
+    step = oracle_step_fn(model, 'cpu', n)
+    step()
+    times = []
+    budget = time.perf_counter() + args.cpu_seconds
+    while len(times) < 3 and (not times or time.perf_counter() < budget):
+        t0 = time.perf_counter()
+        step()
+        times.append(time.perf_counter() - t0)
+    med = statistics.median(times)
+    return {'value': round(n / med, 2), 'unit': 'rays/s', 'cores': threads, 'kind': 'port',
+            'sample': f'{n} rays x {len(times)} timed steps (median {med:.2f} s/step) of the same '
+                      f'train step (fwd+bwd+Adam, 4 MLPs) in the PyTorch FP32 oracle on CPU'}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=20)
+    ap.add_argument('--warmup', type=int, default=5)
+    ap.add_argument('--rays', type=int, default=4096)
+    ap.add_argument('--render-rays', type=int, default=65536)
+    ap.add_argument('--plain-steps', type=int, default=5)
+    ap.add_argument('--cpu-rays', type=int, default=128)
+    ap.add_argument('--cpu-seconds', type=float, default=20.0)
+    ap.add_argument('--no-baselines', action='store_true')
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get('WORLD_SIZE', '1')) != 1:
+        print('bench.py measures one GPU: this build has no multi-GPU path (rays shard across GPUs with '
+              'one gradient all-reduce per step; see DESIGN.md)', file=sys.stderr)
+        sys.exit(2)
+    import torch
+    dev = torch.device('cuda:0')
+    import srf_hip
+    srf_hip.lib()  # fail loudly without the HIP library
+
+    tr = hip_train_leg(args, dev)
+    ms = tr['seconds'] * 1000.0 / args.steps
+    value = args.rays * args.steps / tr['seconds']
+    kern = tr['kernels']
+    # dominant kernel: the single-kernel entry point with the largest total time
+    single = {k: v for k, v in kern.items() if k not in ('srf_mlp_pack',)}
+    dom_name, dom = max(single.items(), key=lambda kv: kv[1]['ms'])
+    roof = None
+    if dom['flop'] > 0:
+        ach = dom['flop'] / (dom['ms'] * 1e-3) / 1e12
+        roof = {'bound': 'mfma', 'kernel': dom_name, 'achieved': round(ach, 2), 'peak': PEAK_BF16_TFLOPS,
+                'unit': 'TFLOP/s', 'frac': round(ach / PEAK_BF16_TFLOPS, 4), 'traffic': None,
+                'avg_launch_ms': round(dom['ms'] / dom['calls'], 4)}
+    per_kernel = {k: {'calls': v['calls'], 'ms_per_step': round(v['ms'] / args.steps, 3),
+                      **({'tflops': round(v['flop'] / (v['ms'] * 1e-3) / 1e12, 1)} if v['flop'] else {})}
+                  for k, v in kern.items()}
+    render = hip_render_leg(args, dev, tr['model'])
+    result = {
+        'metric': METRIC, 'value': round(value, 1), 'unit': 'rays/s', 'n_gpus': 1, 'steps': args.steps,
+        'warmup': args.warmup, 'ms_per_step': round(ms, 3), 'higher_is_better': True, 'scaling': 'weak',
+        'vs_baseline': None, 'dtype': 'bf16', 'data': 'synthetic (seeded camera rays, random-init weights)',
+        'config': {'workload': 'Simple-NeRF train step: coarse 64 (main + points-aug + views-aug MLPs) + fine '
+                               '192 merged samples, fwd+bwd+Adam', 'rays_per_step': args.rays,
+                   'mlp': '8x256 skip@4 + 128 view head, BF16 MFMA / FP32 accumulate', 'parallelism': 'none'},
+        'roofline': roof,
+        'render_rays_per_s': round(render, 1),
+        'kernels': per_kernel,
+    }
+    if not args.no_baselines:
+        plain = plain_gpu_leg(args, dev, tr['model'])
+        result['plain_fp32_gpu'] = {'value': round(plain, 1), 'unit': 'rays/s',
+                                    'what': 'same step in plain PyTorch FP32 (oracle) on this GPU, allow_tf32=False'}
+        result['speedup_vs_plain_fp32_gpu'] = round(value / plain, 2)
+        result['cpu_baseline'] = cpu_baseline_leg(args, tr['model'])
+    print(json.dumps(result))
+
+
+if __name__ == '__main__':
+    main()

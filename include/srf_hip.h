@@ -179,6 +179,22 @@ int srf_mlp_backward(const srf_mlp_desc* desc, const void* packed, int n_points,
                      const float* d_rgb, void* workspace, float* const* d_weights,
                      float* const* d_biases, void* stream);
 
+/* The two halves of srf_mlp_backward, for callers that schedule or time them separately:
+ * srf_mlp_backward_data runs the backward chain (writes every layer's pre-activation gradient
+ * into the workspace); srf_mlp_weight_grads then reduces them against the saved layer inputs
+ * into d_weights / d_biases (accumulating, as srf_mlp_backward does). */
+int srf_mlp_backward_data(const srf_mlp_desc* desc, const void* packed, int n_points,
+                          const float* sigma, const float* rgb, const float* d_sigma,
+                          const float* d_rgb, void* workspace, void* stream);
+int srf_mlp_weight_grads(const srf_mlp_desc* desc, int n_points, void* workspace,
+                         float* const* d_weights, float* const* d_biases, void* stream);
+
+/* Lane-layout self-test of the MFMA building blocks (one 16x16 tile): d = a b with
+ * a [16,K], b [K,16] row-major FP32 (rounded to the operand type), K = 4 for SRF_PREC_F32 and
+ * 32 for SRF_PREC_BF16 / SRF_PREC_F16.  No counterpart in the reference: a unit check of the
+ * operand layouts the MLP kernels are built on. */
+int srf_selftest_mfma(int prec, const float* a, const float* b, float* d, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -211,35 +211,40 @@ extern "C" int srf_mlp_forward(const srf_mlp_desc* desc, const void* packed, int
   return fwd_launch(p.prec, a, p.view_dep, workspace != nullptr, as_stream(stream));
 }
 
-extern "C" int srf_mlp_backward(const srf_mlp_desc* desc, const void* packed, int n_points,
-                                const float* sigma, const float* rgb, const float* d_sigma,
-                                const float* d_rgb, void* workspace, float* const* d_weights,
-                                float* const* d_biases, void* stream) {
+static int mlp_backward_impl(const srf_mlp_desc* desc, const void* packed, int n_points,
+                             const float* sigma, const float* rgb, const float* d_sigma,
+                             const float* d_rgb, void* workspace, float* const* d_weights,
+                             float* const* d_biases, void* stream, bool chain, bool weights) {
   Plan p;
-  if (!make_plan(desc, &p) || n_points < 0 || !packed) return SRF_EINVAL;
+  if (!make_plan(desc, &p) || n_points < 0) return SRF_EINVAL;
   if (n_points == 0) return SRF_OK;
-  if (!sigma || !rgb || !d_sigma || !d_rgb || !workspace || !d_weights || !d_biases) return SRF_EINVAL;
+  if (!workspace) return SRF_EINVAL;
+  if (chain && (!packed || !sigma || !rgb || !d_sigma || !d_rgb)) return SRF_EINVAL;
   const bool need[kParams] = {true, true, true, true, true, true, true, true,
                               (bool)p.view_dep, true, (bool)p.view_dep, (bool)p.view_dep};
-  for (int i = 0; i < kParams; ++i)
-    if (need[i] && (!d_weights[i] || !d_biases[i])) return SRF_EINVAL;
+  if (weights) {
+    if (!d_weights || !d_biases) return SRF_EINVAL;
+    for (int i = 0; i < kParams; ++i)
+      if (need[i] && (!d_weights[i] || !d_biases[i])) return SRF_EINVAL;
+  }
   hipStream_t s = as_stream(stream);
-  BwdArgs b;
-  b.p = p;
-  b.w = make_ws(p, n_points);
-  b.packed = reinterpret_cast<const uint8_t*>(packed);
-  b.n = n_points;
-  b.sigma = sigma;
-  b.rgb = rgb;
-  b.d_sigma = d_sigma;
-  b.d_rgb = d_rgb;
-  b.ws = reinterpret_cast<uint8_t*>(workspace);
-  int st = bwd_launch(p.prec, b, p.view_dep, s);
-  if (st != SRF_OK) return st;
-
+  const Ws w = make_ws(p, n_points);
+  uint8_t* ws = reinterpret_cast<uint8_t*>(workspace);
+  if (chain) {
+    BwdArgs b;
+    b.p = p;
+    b.w = w;
+    b.packed = reinterpret_cast<const uint8_t*>(packed);
+    b.n = n_points;
+    b.sigma = sigma;
+    b.rgb = rgb;
+    b.d_sigma = d_sigma;
+    b.d_rgb = d_rgb;
+    b.ws = ws;
+    const int st = bwd_launch(p.prec, b, p.view_dep, s);
+    if (st != SRF_OK || !weights) return st;
+  }
   // grouped weight-gradient GEMM
-  const Ws& w = b.w;
-  uint8_t* ws = b.ws;
   const size_t rb = (size_t)w.ldP * p.elem;  // bytes per stored row
   DwArgs d;
   memset(&d, 0, sizeof(d));
@@ -290,4 +295,25 @@ extern "C" int srf_mlp_backward(const srf_mlp_desc* desc, const void* packed, in
   }
   d.total_work = work;
   return dw_launch(p.prec, d, s);
+}
+
+extern "C" int srf_mlp_backward(const srf_mlp_desc* desc, const void* packed, int n_points,
+                                const float* sigma, const float* rgb, const float* d_sigma,
+                                const float* d_rgb, void* workspace, float* const* d_weights,
+                                float* const* d_biases, void* stream) {
+  return mlp_backward_impl(desc, packed, n_points, sigma, rgb, d_sigma, d_rgb, workspace, d_weights,
+                           d_biases, stream, true, true);
+}
+
+extern "C" int srf_mlp_backward_data(const srf_mlp_desc* desc, const void* packed, int n_points,
+                                     const float* sigma, const float* rgb, const float* d_sigma,
+                                     const float* d_rgb, void* workspace, void* stream) {
+  return mlp_backward_impl(desc, packed, n_points, sigma, rgb, d_sigma, d_rgb, workspace, nullptr,
+                           nullptr, stream, true, false);
+}
+
+extern "C" int srf_mlp_weight_grads(const srf_mlp_desc* desc, int n_points, void* workspace,
+                                    float* const* d_weights, float* const* d_biases, void* stream) {
+  return mlp_backward_impl(desc, nullptr, n_points, nullptr, nullptr, nullptr, nullptr, workspace,
+                           d_weights, d_biases, stream, false, true);
 }

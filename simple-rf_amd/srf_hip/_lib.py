@@ -36,6 +36,8 @@ _PROTOS = {
     'srf_mlp_workspace_bytes': [_P, _I],
     'srf_mlp_forward': [_P, _P, _I, _P, _P, _P, _P, _P, _P, _P],
     'srf_mlp_backward': [_P, _P, _I, _P, _P, _P, _P, _P, _P, _P, _P],
+    'srf_mlp_backward_data': [_P, _P, _I, _P, _P, _P, _P, _P, _P],
+    'srf_mlp_weight_grads': [_P, _I, _P, _P, _P, _P],
 }
 _RESTYPES = {'srf_status_string': C.c_char_p, 'srf_mlp_packed_bytes': C.c_size_t,
              'srf_mlp_workspace_bytes': C.c_size_t}

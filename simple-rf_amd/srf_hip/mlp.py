@@ -139,9 +139,11 @@ class _MLPFunction(torch.autograd.Function):
         d_rgb = torch.zeros_like(rgb) if d_rgb is None else d_rgb.contiguous()
         grads = [None if s is None else torch.zeros(s, dtype=torch.float32, device=dev) for s in ctx.shapes]
         d = spec.desc()
-        _lib.call('srf_mlp_backward', C.byref(d), packed.data_ptr(), n, sigma.data_ptr(), rgb.data_ptr(),
-                  d_sigma.data_ptr(), d_rgb.data_ptr(), ws.data_ptr(), _ptr_array(grads[:12]),
-                  _ptr_array(grads[12:]), _lib.stream_ptr(dev))
+        stream = _lib.stream_ptr(dev)
+        _lib.call('srf_mlp_backward_data', C.byref(d), packed.data_ptr(), n, sigma.data_ptr(), rgb.data_ptr(),
+                  d_sigma.data_ptr(), d_rgb.data_ptr(), ws.data_ptr(), stream)
+        _lib.call('srf_mlp_weight_grads', C.byref(d), n, ws.data_ptr(), _ptr_array(grads[:12]),
+                  _ptr_array(grads[12:]), stream)
         return (None, None, None, None, None, *grads)
 
 

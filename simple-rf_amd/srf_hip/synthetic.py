@@ -52,3 +52,39 @@ def full_image_pixel_ids(width=1008, height=756, image_id=0):
     """All pixels of one frame, row-major (x fastest), as the reference's test grid."""
     y, x = torch.meshgrid(torch.arange(height), torch.arange(width), indexing='ij')
     return torch.stack([torch.full_like(x, image_id), x, y], -1).reshape(-1, 3).int()
+
+
+def train_batch(n_rays: int, seed=0, n_frames=3, width=504, height=378, sparse_fraction=0.1, device='cpu'):
+    """One synthetic training batch: pixel ids of the seeded camera, targets rgb ~ U(0,1), and
+    sparse depth on `sparse_fraction` of the rays (NDC depth ~ U(0.2, 0.9), supervised as the
+    metric depth it maps to, as SparseDepthMSE14 supervises metric depth)."""
+    r = rays(n_rays, seed=seed, width=width, height=height, n_frames=n_frames)
+    g = torch.Generator().manual_seed(seed + 1)
+    target = torch.rand(n_rays, 3, generator=g)
+    sd_mask = torch.rand(n_rays, generator=g) < sparse_fraction
+    zn = 0.2 + 0.7 * torch.rand(n_rays, 1, generator=g)
+    oz, dz = r['rays_o'][:, 2:3], r['rays_d'][:, 2:3]
+    tn = -(1.0 + oz) / dz
+    sd_depth = ((oz + tn * dz) / dz * (1 / (1 - zn) - 1) + tn)[:, 0]
+    b = dict(pixel_id=r['pixel_id'], target_rgb=target, sd_mask=sd_mask, sd_depth=sd_depth)
+    return {k: v.to(device) for k, v in b.items()}
+
+
+def train_loss(out: dict, batch: dict):
+    """Synthetic objective of the benchmark: photometric MSE on the main coarse/fine and both
+    augmented coarse renders (MSE14), 0.1 x sparse-depth MSE on metric depth (SparseDepthMSE14),
+    and 0.1 x a plain MSE between augmented and main coarse depths standing in for the
+    reprojection-gated AugmentationsDepthLoss11."""
+    t = batch['target_rgb']
+    loss = 0.
+    for k in ('rgb_coarse', 'rgb_fine', 'points_augmentation_rgb_coarse', 'views_augmentation_rgb_coarse'):
+        if k in out:
+            loss = loss + torch.mean((out[k] - t) ** 2)
+    m = batch['sd_mask']
+    for k in ('depth_coarse', 'points_augmentation_depth_coarse', 'views_augmentation_depth_coarse'):
+        if k in out:
+            loss = loss + 0.1 * torch.mean(((out[k] - batch['sd_depth']) ** 2) * m) * (m.numel() / m.sum().clamp_min(1))
+    for k in ('points_augmentation_depth_coarse', 'views_augmentation_depth_coarse'):
+        if k in out:
+            loss = loss + 0.1 * torch.mean((out[k] - out['depth_coarse'].detach()) ** 2)
+    return loss
