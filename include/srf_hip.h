@@ -195,6 +195,11 @@ int srf_mlp_weight_grads(const srf_mlp_desc* desc, int n_points, void* workspace
  * operand layouts the MLP kernels are built on. */
 int srf_selftest_mfma(int prec, const float* a, const float* b, float* d, void* stream);
 
+/* Self-test of the direct global -> LDS fragment copy the MLP kernels stream weights with:
+ * src and dst are 8 KB; dst fragment f (1 KB) = src fragment 7 - f.  No counterpart in the
+ * reference. */
+int srf_selftest_lds_dma(const void* src, void* dst, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

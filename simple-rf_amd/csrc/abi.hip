@@ -30,10 +30,33 @@ __global__ __launch_bounds__(64) void selftest_mfma_kernel(const float* __restri
   for (int r = 0; r < 4; ++r) d[(4 * g + r) * 16 + j] = acc[r];
 }
 
+// Four waves each copy two 1 KB fragments into LDS with lds_dma_1k; the block then writes LDS
+// back out in reversed fragment order, so dst fragment f = src fragment 7 - f.
+__global__ __launch_bounds__(256) void selftest_lds_dma_kernel(const uint8_t* __restrict__ src,
+                                                               uint8_t* __restrict__ dst) {
+  __shared__ __attribute__((aligned(16))) uint8_t buf[8 * 1024];
+  const int w = threadIdx.x >> 6;
+  lds_dma_1k(src + (2 * w) * 1024, buf + (2 * w) * 1024);
+  lds_dma_1k(src + (2 * w + 1) * 1024, buf + (2 * w + 1) * 1024);
+  wait_vmem();
+  __syncthreads();
+  for (int i = threadIdx.x; i < 8 * 64; i += 256) {
+    const int f = i / 64, l = i % 64;
+    *reinterpret_cast<u32x4*>(dst + (7 - f) * 1024 + 16 * l) = *reinterpret_cast<const u32x4*>(buf + f * 1024 + 16 * l);
+  }
+}
+
 }  // namespace
 }  // namespace srf
 
 using namespace srf;
+
+extern "C" int srf_selftest_lds_dma(const void* src, void* dst, void* stream) {
+  if (!src || !dst) return SRF_EINVAL;
+  hipLaunchKernelGGL(selftest_lds_dma_kernel, dim3(1), dim3(256), 0, as_stream(stream),
+                     reinterpret_cast<const uint8_t*>(src), reinterpret_cast<uint8_t*>(dst));
+  return launch_status();
+}
 
 extern "C" int srf_abi_version(void) { return SRF_ABI_VERSION; }
 

@@ -80,4 +80,16 @@ __device__ __forceinline__ f32x4 mfma32(float a, float b, const f32x4& c) {
   return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
 }
 
+// Direct global -> LDS copy of one 1 KB fragment by a whole wave (gfx950
+// global_load_lds_dwordx4): lane l's 16 bytes at src + 16 l land at lds_base + 16 l.  src is
+// the fragment base, lds_base must be wave-uniform.  Completion is tracked by vmcnt.
+__device__ __forceinline__ void lds_dma_1k(const uint8_t* src, uint8_t* lds_base) {
+  const int lane = threadIdx.x & 63;
+  __builtin_amdgcn_global_load_lds((const void*)(src + 16 * lane),
+                                   (__attribute__((address_space(3))) void*)lds_base, 16, 0, 0);
+}
+
+// Wait for every outstanding vector-memory operation of this wave (LDS DMA included).
+__device__ __forceinline__ void wait_vmem() { __builtin_amdgcn_s_waitcnt(0); }
+
 }  // namespace srf

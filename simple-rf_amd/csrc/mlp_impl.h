@@ -13,9 +13,12 @@
 //     mfma.h, the D tiles of one layer are, bit for bit, the B operand of the next layer, so
 //     all activations stay in VGPRs from the positional encoding to the rgb head.
 //   * Positional-encoding features are generated in registers in the same tile layout
-//     (sincosf of x * 2^m, no HBM traffic), for layer 0, the skip layer and the view layer.
+//     (Cody-Waite sin/cos of x * 2^m, no HBM traffic) for layer 0 and the view layer; the skip
+//     layer re-reads layer 0's PE tiles from LDS.
+//   * Weights stream through double-buffered LDS, copied global -> LDS once per block
+//     (gfx950 global_load_lds_dwordx4) and shared by its four waves.
 //   * Weights are repacked once per parameter update (srf_mlp_pack) into per-lane 16-byte
-//     fragments, so every A-operand load is one coalesced 1 KB read per wave.
+//     fragments: one 1 KB fragment per (output tile, k-unit).
 //   * Training stores, feature-major (row = feature, column = point), the inputs of every
 //     layer (16-bit or FP32) for the weight gradients, plus one 64-bit ReLU mask per lane per
 //     layer for the backward chain.  The transposition to feature-major goes through a 2.3 KB
@@ -120,7 +123,7 @@ inline bool make_plan(const srf_mlp_desc* d, Plan* p) {
   return true;
 }
 
-// Training workspace: feature-major arrays with leading dimension ldP (points, multiple of 64).
+// Training workspace: feature-major arrays with leading dimension ldP (points, multiple of 256).
 struct Ws {
   int ldP;
   size_t x0, h[8], f, ex, v, dz[kLayers], mask, bytes;  // byte offsets
@@ -130,7 +133,7 @@ struct Ws {
 inline Ws make_ws(const Plan& p, int n) {
   Ws w;
   memset(&w, 0, sizeof(w));
-  w.ldP = round_up(n > 0 ? n : 1, 64);
+  w.ldP = round_up(n > 0 ? n : 1, 256);  // whole blocks of 4 waves (64 or 32 points each)
   const size_t row = (size_t)w.ldP * p.elem;
   size_t off = 0;
   auto take = [&](int rows) {
@@ -274,55 +277,112 @@ __device__ __forceinline__ void static_for(F&& f) {
   }
 }
 
-// acc[OUT][CB] += W[TB .. TB+OUT-1, k-steps S0 .. S0+NS-1] x In for all OUT output tiles at once
-// (the accumulators fill the AGPR file while the activations stay in arch VGPRs).  ks is the
-// layer's k-step count (row stride of its packed A tiles); fin(t, cb) returns input tile t
-// (t counted from the first tile of k-step S0) for column block cb.
+// Weights are streamed through LDS: a "k-unit" is the 1 KB of packed A that one output tile
+// needs for 32 reduction features (16-bit operands, one k-step) or 16 features (FP32, four
+// k-steps).  Chunks of kKC k-units for all output tiles are copied global -> LDS by the four
+// waves of the block (lds_dma_1k), double-buffered so the copy of chunk c+1 overlaps the MFMAs
+// of chunk c; every wave then reads the fragments it needs from LDS.  This divides the weight
+// traffic out of L2 by the number of waves per block and hides its latency.
+constexpr int kKC = 2;
+constexpr int kStageBytes = 16 * kKC * 1024;
+
+template <int P>
+__host__ __device__ constexpr int units(int tiles) { return P == kF32 ? tiles : tiles / 2; }
+
+// acc[OUT][CB] += W[TB .. TB+OUT-1, k-units u0 .. u0+NT/TPU-1] x In for all OUT output tiles at
+// once (the accumulators fill the AGPR file while the activations stay in arch VGPRs).  upr is
+// the number of k-units per packed row of the layer; fin(t, cb) returns input tile t (counted
+// from the first tile of k-unit u0) for column block cb.  Must be reached by all waves of the
+// block (it synchronises them).
 template <int P, int NT, int OUT, int TB, typename FIn>
 __device__ __forceinline__ void mma_tiles(f32x4 (&acc)[OUT][Tr<P>::CB], const uint8_t* __restrict__ A,
-                                          int ks, int s0, FIn&& fin) {
+                                          int upr, int u0, FIn&& fin, uint8_t* stage) {
   constexpr int CB = Tr<P>::CB;
+  constexpr int TPU = P == kF32 ? 1 : 2;
+  constexpr int NU = NT / TPU;
+  constexpr int NCH = (NU + kKC - 1) / kKC;
+  static_assert(OUT <= 16, "stage buffer holds 16 output tiles");
+  // wave index and every fragment address are wave-uniform: keep them in SGPRs
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
-  if constexpr (P == kF32) {
-    const float* Af = reinterpret_cast<const float*>(A);
-    static_for<0, NT>([&](auto TI) SRF_AI {
-      constexpr int t = decltype(TI)::value;
-      f32x4 b[CB];
+  auto dma_chunk = [&](auto CI, uint8_t* buf) SRF_AI {
+    constexpr int c = decltype(CI)::value;
+    constexpr int kn = (NU - c * kKC) < kKC ? (NU - c * kKC) : kKC;
 #pragma unroll
-      for (int cb = 0; cb < CB; ++cb) b[cb] = fin(t, cb);
+    for (int i = 0; i < (OUT * kn + kWaves - 1) / kWaves; ++i) {
+      const int f = w + kWaves * i;
+      if (f < OUT * kn) {
+        const int to = f / kn, kc = f - (f / kn) * kn;
+        const int frag = __builtin_amdgcn_readfirstlane((TB + to) * upr + u0 + c * kKC + kc);
+        const int dst = __builtin_amdgcn_readfirstlane((to * kKC + kc) * 1024);
+        lds_dma_1k(A + (size_t)frag * 1024, buf + dst);
+      }
+    }
+  };
+  dma_chunk(std::integral_constant<int, 0>{}, stage);
+  wait_vmem();
+  __syncthreads();
+  static_for<0, NCH>([&](auto CI) SRF_AI {
+    constexpr int c = decltype(CI)::value;
+    const uint8_t* cur = stage + (c & 1) * kStageBytes;
+    if constexpr (c + 1 < NCH) dma_chunk(std::integral_constant<int, c + 1>{}, stage + ((c + 1) & 1) * kStageBytes);
+    constexpr int kn = (NU - c * kKC) < kKC ? (NU - c * kKC) : kKC;
+    static_for<0, kn>([&](auto KI) SRF_AI {
+      constexpr int kc = decltype(KI)::value;
+      constexpr int u = c * kKC + kc;
+      if constexpr (P == kF32) {
+        f32x4 b[CB];
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
+        for (int cb = 0; cb < CB; ++cb) b[cb] = fin(u, cb);
 #pragma unroll
-        for (int to = 0; to < OUT; ++to) {
-          const float a = Af[((TB + to) * ks + s0 + 4 * t + r) * 64 + lane];
+        for (int r = 0; r < 4; ++r) {
 #pragma unroll
-          for (int cb = 0; cb < CB; ++cb) acc[to][cb] = mfma32(a, b[cb][r], acc[to][cb]);
+          for (int to = 0; to < OUT; ++to) {
+            const float a = *reinterpret_cast<const float*>(cur + (to * kKC + kc) * 1024 + (r * 64 + lane) * 4);
+#pragma unroll
+            for (int cb = 0; cb < CB; ++cb) acc[to][cb] = mfma32(a, b[cb][r], acc[to][cb]);
+          }
         }
+      } else {
+        u32x4 b[CB];
+#pragma unroll
+        for (int cb = 0; cb < CB; ++cb) {
+          const u32x2 x0 = fin(2 * u, cb), x1 = fin(2 * u + 1, cb);
+          b[cb] = u32x4{x0.x, x0.y, x1.x, x1.y};
+        }
+        // Output tiles in groups of kG: the A fragments of group g+1 are read from LDS while
+        // the MFMAs of group g run; scheduling barriers keep the compiler from hoisting every
+        // fragment read of the chunk (which would exhaust the arch VGPRs).
+        constexpr int kG = 4;
+        constexpr int NG = (OUT + kG - 1) / kG;
+        auto read_a = [&](auto GI, u32x4 (&a)[kG]) SRF_AI {
+          constexpr int gi = decltype(GI)::value;
+#pragma unroll
+          for (int q = 0; q < kG; ++q)
+            if (gi * kG + q < OUT)
+              a[q] = *reinterpret_cast<const u32x4*>(cur + ((gi * kG + q) * kKC + kc) * 1024 + lane * 16);
+        };
+        u32x4 a_cur[kG], a_nxt[kG];
+        read_a(std::integral_constant<int, 0>{}, a_cur);
+        static_for<0, NG>([&](auto GI) SRF_AI {
+          constexpr int gi = decltype(GI)::value;
+          if constexpr (gi + 1 < NG) read_a(std::integral_constant<int, gi + 1>{}, a_nxt);
+#pragma unroll
+          for (int q = 0; q < kG; ++q)
+            if (gi * kG + q < OUT)
+#pragma unroll
+              for (int cb = 0; cb < CB; ++cb)
+                acc[gi * kG + q][cb] = mfma16<P>(a_cur[q], b[cb], acc[gi * kG + q][cb]);
+          __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+          for (int q = 0; q < kG; ++q) a_cur[q] = a_nxt[q];
+        });
       }
     });
-  } else {
-    const u32x4* Ah = reinterpret_cast<const u32x4*>(A);
-    static_for<0, NT / 2>([&](auto SI) SRF_AI {
-      constexpr int s = decltype(SI)::value;
-      u32x4 b[CB];
-#pragma unroll
-      for (int cb = 0; cb < CB; ++cb) {
-        const u32x2 x0 = fin(2 * s, cb), x1 = fin(2 * s + 1, cb);
-        b[cb] = u32x4{x0.x, x0.y, x1.x, x1.y};
-      }
-#pragma unroll
-      for (int to = 0; to < OUT; ++to) {
-        const u32x4 a = Ah[((TB + to) * ks + s0 + s) * 64 + lane];
-#pragma unroll
-        for (int cb = 0; cb < CB; ++cb) acc[to][cb] = mfma16<P>(a, b[cb], acc[to][cb]);
-      }
-    });
-  }
+    wait_vmem();
+    __syncthreads();
+  });
 }
-
-// k-steps spanned by NT input tiles
-template <int P>
-__host__ __device__ constexpr int ksteps(int tiles) { return P == kF32 ? 4 * tiles : tiles / 2; }
 
 template <int P, int OUT>
 __device__ __forceinline__ void zero_acc(f32x4 (&acc)[OUT][Tr<P>::CB]) {
@@ -333,12 +393,12 @@ __device__ __forceinline__ void zero_acc(f32x4 (&acc)[OUT][Tr<P>::CB]) {
 }
 
 // One whole layer with IN input tiles: accumulate, then epi(T, acc) for every output tile
-// (after the last k-step, so epi may overwrite the input tiles in place).
+// (after the last k-unit, so epi may overwrite the input tiles in place).
 template <int P, int IN, int OUT, int TB, typename FIn, typename Epi>
-__device__ __forceinline__ void run_layer(const uint8_t* __restrict__ A, FIn&& fin, Epi&& epi) {
+__device__ __forceinline__ void run_layer(const uint8_t* __restrict__ A, FIn&& fin, Epi&& epi, uint8_t* stage) {
   f32x4 acc[OUT][Tr<P>::CB];
   zero_acc<P, OUT>(acc);
-  mma_tiles<P, IN, OUT, TB>(acc, A, ksteps<P>(IN), 0, fin);
+  mma_tiles<P, IN, OUT, TB>(acc, A, units<P>(IN), 0, fin, stage);
 #pragma unroll
   for (int to = 0; to < OUT; ++to) epi(TB + to, acc[to]);
 }
@@ -389,11 +449,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
   constexpr int CB = Tr<P>::CB;
   using Act = typename Tr<P>::Act;
   __shared__ __attribute__((aligned(16))) uint8_t lds_all[kWaves * 16 * kLdsRowBytes];
+  __shared__ __attribute__((aligned(16))) uint8_t stage[2 * kStageBytes];
+  // computed input features of each wave's points (trunk PE for layers 0 and 5, then the view
+  // layer's [PE_xyz[trunk_pe:], PE_dir]) as B-operand tiles: 6 tiles x CB column blocks x 64
+  // lanes.  Each lane reads back only its own entries, so no barrier is needed.
+  __shared__ __attribute__((aligned(16))) Act pe_lds[kWaves][6][CB][64];
   const int lane = threadIdx.x & 63, g = lane >> 4, j = lane & 15;
   const int wv = threadIdx.x >> 6;
   const int wave = blockIdx.x * kWaves + wv;
-  const int p0 = wave * 16 * CB;
-  if (p0 >= a.w.ldP) return;
+  const int p0 = wave * 16 * CB;  // the grid covers ldP exactly: every wave has points
   uint8_t* lds = lds_all + wv * 16 * kLdsRowBytes;
   const Plan& pl = a.p;
   const int ldP = a.w.ldP;
@@ -408,27 +472,34 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
   };
   auto valid = [&](int cb) SRF_AI { return p0 + 16 * cb + j < a.n; };
 
-  Act h[16][CB], comp[6][CB];
+  Act h[16][CB];
   uint64_t mbits[CB];
 
+  // trunk PE tiles -> pe_lds (and, training, the feature-major X0 copy for the weight grads)
   auto pe_trunk = [&]() SRF_AI {
     float x[CB][3];
 #pragma unroll
     for (int cb = 0; cb < CB; ++cb) load3(a.pts, cb, x[cb]);
     static_for<0, 4>([&](auto TI) SRF_AI {
       constexpr int t = decltype(TI)::value;
+      if (t < pl.comp_tiles[0]) {
+        Act tl[CB];
 #pragma unroll
-      for (int cb = 0; cb < CB; ++cb) {
-        f32x4 v;
+        for (int cb = 0; cb < CB; ++cb) {
+          f32x4 v;
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int f = 16 * t + 4 * g + r;
-          v[r] = f < pl.trunk_pe ? pe_feature(x[cb][0], x[cb][1], x[cb][2], f) : 0.f;
+          for (int r = 0; r < 4; ++r) {
+            const int f = 16 * t + 4 * g + r;
+            v[r] = f < pl.trunk_pe ? pe_feature(x[cb][0], x[cb][1], x[cb][2], f) : 0.f;
+          }
+          tl[cb] = to_act<P>(v);
+          pe_lds[wv][t][cb][lane] = tl[cb];
         }
-        comp[t][cb] = to_act<P>(v);
+        if (TRAIN) store_tile<P>(a.ws + a.w.x0 + (size_t)16 * t * ldP * pl.elem, ldP, p0, tl, lds);
       }
     });
   };
+  auto fin_pe = [&](int t, int cb) SRF_AI { return pe_lds[wv][t][cb][lane]; };
   auto mask_store = [&](int slot) SRF_AI {
     if (TRAIN) {
       uint64_t* m = reinterpret_cast<uint64_t*>(a.ws + a.w.mask + (size_t)slot * ldP * 32);
@@ -461,25 +532,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
     };
   };
   auto fin_h = [&](int t, int cb) SRF_AI { return h[t][cb]; };
-  auto fin_hc = [&](int t, int cb) SRF_AI { return t < 16 ? h[t][cb] : comp[t - 16][cb]; };
+  auto fin_hc = [&](int t, int cb) SRF_AI { return t < 16 ? h[t][cb] : pe_lds[wv][t - 16][cb][lane]; };
 
   // ---- layer 0: PE(trunk) -> 256
   pe_trunk();
-  if (TRAIN) {
-    static_for<0, 4>([&](auto TI) SRF_AI {
-      constexpr int t = decltype(TI)::value;
-      if (t < pl.comp_tiles[0])
-        store_tile<P>(a.ws + a.w.x0 + (size_t)16 * t * ldP * pl.elem, ldP, p0, comp[t], lds);
-    });
-  }
 #pragma unroll
   for (int cb = 0; cb < CB; ++cb) mbits[cb] = 0;
   if (pl.comp_tiles[0] == 4)
-    run_layer<P, 4, 16, 0>(a.packed + pl.fwd_off[0], [&](int t, int cb) SRF_AI { return comp[t][cb]; },
-                           relu_epi(0, a.w.h[0]));
+    run_layer<P, 4, 16, 0>(a.packed + pl.fwd_off[0], fin_pe, relu_epi(0, a.w.h[0]), stage);
   else
-    run_layer<P, 2, 16, 0>(a.packed + pl.fwd_off[0], [&](int t, int cb) SRF_AI { return comp[t][cb]; },
-                           relu_epi(0, a.w.h[0]));
+    run_layer<P, 2, 16, 0>(a.packed + pl.fwd_off[0], fin_pe, relu_epi(0, a.w.h[0]), stage);
   mask_store(0);
 
   // ---- layers 1..7, each in place in h; layer 5 adds the trunk PE k-steps after the 16
@@ -490,14 +552,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
     f32x4 acc[16][CB];
     zero_acc<P, 16>(acc);
     const uint8_t* A = a.packed + pl.fwd_off[l];
-    const int ks = ksteps<P>(16 + pl.comp_tiles[l]);
-    mma_tiles<P, 16, 16, 0>(acc, A, ks, 0, fin_h);
+    const int upr = units<P>(16 + pl.comp_tiles[l]);
+    mma_tiles<P, 16, 16, 0>(acc, A, upr, 0, fin_h, stage);
     if (l == 5) {
-      pe_trunk();
       if (pl.comp_tiles[5] == 4)
-        mma_tiles<P, 4, 16, 0>(acc, A, ks, ksteps<P>(16), [&](int t, int cb) SRF_AI { return comp[t][cb]; });
+        mma_tiles<P, 4, 16, 0>(acc, A, upr, units<P>(16), fin_pe, stage);
       else
-        mma_tiles<P, 2, 16, 0>(acc, A, ks, ksteps<P>(16), [&](int t, int cb) SRF_AI { return comp[t][cb]; });
+        mma_tiles<P, 2, 16, 0>(acc, A, upr, units<P>(16), fin_pe, stage);
     }
     auto epi = relu_epi(l, a.w.h[l]);
 #pragma unroll
@@ -526,7 +587,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
           a.rgb[3 * p + 2] = sigmoidf_(v[3]);
         }
       }
-    });
+    }, stage);
   } else {
     // sigma (row 256, tile 16) first, then feature rows 0..255 (identity) in place in h
     run_layer<P, 16, 1, 16>(a.packed + pl.fwd_off[8], fin_h, [&](int T, f32x4(&acc)[CB]) SRF_AI {
@@ -534,7 +595,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
 #pragma unroll
       for (int cb = 0; cb < CB; ++cb)
         if (g == 0 && valid(cb)) head_sigma(cb, acc[cb][0] + bv[0]);
-    });
+    }, stage);
     run_layer<P, 16, 16, 0>(a.packed + pl.fwd_off[8], fin_h, [&](int T, f32x4(&acc)[CB]) SRF_AI {
       const f32x4 bv = bias4(8, T);
       Act t[CB];
@@ -544,7 +605,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
         h[T][cb] = t[cb];
       }
       if (TRAIN) store_tile<P>(a.ws + a.w.f + (size_t)16 * T * ldP * pl.elem, ldP, p0, t, lds);
-    });
+    }, stage);
     // view layer input extras: [PE_xyz[trunk_pe:], PE_dir]
     const int n_hi = 3 * (2 * pl.pts_deg + 1) - pl.trunk_pe;
     float x[CB][3], d[CB][3];
@@ -555,36 +616,34 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
     }
     static_for<0, 6>([&](auto TI) SRF_AI {
       constexpr int t = decltype(TI)::value;
+      if (t < pl.comp_tiles[9]) {
+        Act tl[CB];
 #pragma unroll
-      for (int cb = 0; cb < CB; ++cb) {
-        f32x4 v;
+        for (int cb = 0; cb < CB; ++cb) {
+          f32x4 v;
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int e = 16 * t + 4 * g + r;
-          float val = 0.f;
-          if (e < n_hi) val = pe_feature(x[cb][0], x[cb][1], x[cb][2], pl.trunk_pe + e);
-          else if (e < pl.extra) val = pe_feature(d[cb][0], d[cb][1], d[cb][2], e - n_hi);
-          v[r] = val;
+          for (int r = 0; r < 4; ++r) {
+            const int e = 16 * t + 4 * g + r;
+            float val = 0.f;
+            if (e < n_hi) val = pe_feature(x[cb][0], x[cb][1], x[cb][2], pl.trunk_pe + e);
+            else if (e < pl.extra) val = pe_feature(d[cb][0], d[cb][1], d[cb][2], e - n_hi);
+            v[r] = val;
+          }
+          tl[cb] = to_act<P>(v);
+          pe_lds[wv][t][cb][lane] = tl[cb];
         }
-        comp[t][cb] = to_act<P>(v);
+        if (TRAIN) store_tile<P>(a.ws + a.w.ex + (size_t)16 * t * ldP * pl.elem, ldP, p0, tl, lds);
       }
     });
-    if (TRAIN) {
-      static_for<0, 6>([&](auto TI) SRF_AI {
-        constexpr int t = decltype(TI)::value;
-        if (t < pl.comp_tiles[9])
-          store_tile<P>(a.ws + a.w.ex + (size_t)16 * t * ldP * pl.elem, ldP, p0, comp[t], lds);
-      });
-    }
 #pragma unroll
     for (int cb = 0; cb < CB; ++cb) mbits[cb] = 0;
     // view layer [feature, extras] -> 128 (ReLU), in place into h[0..7]
     if (pl.comp_tiles[9] == 2)
-      run_layer<P, 18, 8, 0>(a.packed + pl.fwd_off[9], fin_hc, relu_epi(9, a.w.v));
+      run_layer<P, 18, 8, 0>(a.packed + pl.fwd_off[9], fin_hc, relu_epi(9, a.w.v), stage);
     else if (pl.comp_tiles[9] == 4)
-      run_layer<P, 20, 8, 0>(a.packed + pl.fwd_off[9], fin_hc, relu_epi(9, a.w.v));
+      run_layer<P, 20, 8, 0>(a.packed + pl.fwd_off[9], fin_hc, relu_epi(9, a.w.v), stage);
     else
-      run_layer<P, 22, 8, 0>(a.packed + pl.fwd_off[9], fin_hc, relu_epi(9, a.w.v));
+      run_layer<P, 22, 8, 0>(a.packed + pl.fwd_off[9], fin_hc, relu_epi(9, a.w.v), stage);
     mask_store(8);
     // rgb head 128 -> 3, sigmoid
     run_layer<P, 8, 1, 0>(a.packed + pl.fwd_off[10], fin_h, [&](int T, f32x4(&acc)[CB]) SRF_AI {
@@ -599,7 +658,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
           a.rgb[3 * p + 2] = sigmoidf_(v[2]);
         }
       }
-    });
+    }, stage);
   }
 }
 
@@ -610,11 +669,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
   constexpr int CB = Tr<P>::CB;
   using Act = typename Tr<P>::Act;
   __shared__ __attribute__((aligned(16))) uint8_t lds_all[kWaves * 16 * kLdsRowBytes];
+  __shared__ __attribute__((aligned(16))) uint8_t stage[2 * kStageBytes];
   const int lane = threadIdx.x & 63, g = lane >> 4, j = lane & 15;
   const int wv = threadIdx.x >> 6;
   const int wave = blockIdx.x * kWaves + wv;
-  const int p0 = wave * 16 * CB;
-  if (p0 >= a.w.ldP) return;
+  const int p0 = wave * 16 * CB;  // the grid covers ldP exactly: every wave has points
   uint8_t* lds = lds_all + wv * 16 * kLdsRowBytes;
   const Plan& pl = a.p;
   const int ldP = a.w.ldP;
@@ -685,7 +744,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
     }
     // dV = W10^T dZ10, masked by the view-layer ReLU -> dZ9 (z[0..7])
     load_mask(8);
-    run_layer<P, 2, 8, 0>(a.packed + pl.bwd_off[10], fin_z, masked_epi(9));
+    run_layer<P, 2, 8, 0>(a.packed + pl.bwd_off[10], fin_z, masked_epi(9), stage);
     // dF = W9^T dZ9 (feature part, identity activation) -> dZ8 rows 0..255 (z[0..15])
     run_layer<P, 8, 16, 0>(a.packed + pl.bwd_off[9], fin_z, [&](int T, f32x4(&acc)[CB]) SRF_AI {
       Act t[CB];
@@ -695,7 +754,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
         z[T][cb] = t[cb];
       }
       store_dz(8, T, t);
-    });
+    }, stage);
     {
       Act t[CB];
 #pragma unroll
@@ -709,7 +768,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
       store_dz(8, 16, t);
     }
     load_mask(7);
-    run_layer<P, 18, 16, 0>(a.packed + pl.bwd_off[8], fin_z, masked_epi(7));
+    run_layer<P, 18, 16, 0>(a.packed + pl.bwd_off[8], fin_z, masked_epi(7), stage);
   } else {
     {
       Act t[CB];
@@ -724,12 +783,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
       store_dz(8, 0, t);
     }
     load_mask(7);
-    run_layer<P, 2, 16, 0>(a.packed + pl.bwd_off[8], fin_z, masked_epi(7));
+    run_layer<P, 2, 16, 0>(a.packed + pl.bwd_off[8], fin_z, masked_epi(7), stage);
   }
   // trunk: z holds dZ_l; dH_{l-1} = W_l^T dZ_l, masked -> dZ_{l-1}, in place
   for (int l = 7; l >= 1; --l) {
     load_mask(l - 1);
-    run_layer<P, 16, 16, 0>(a.packed + pl.bwd_off[l], fin_z, masked_epi(l - 1));
+    run_layer<P, 16, 16, 0>(a.packed + pl.bwd_off[l], fin_z, masked_epi(l - 1), stage);
   }
 }
 

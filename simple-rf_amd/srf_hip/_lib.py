@@ -22,6 +22,7 @@ _PROTOS = {
     'srf_abi_version': [],
     'srf_status_string': [_I],
     'srf_selftest_mfma': [_I, _P, _P, _P, _P],
+    'srf_selftest_lds_dma': [_P, _P, _P],
     'srf_stratified_samples': [_I, _I, _F, _F, _I, _P, _P, _P],
     'srf_sample_pdf': [_I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P],
     'srf_fine_z_vals': [_I, _I, _I, _P, _P, _P, _P, _P, _P],

@@ -21,3 +21,12 @@ def test_mfma_tile_layout(cuda, prec, k, dtype):
     torch.cuda.synchronize()
     ref = a.double() @ b.double()
     assert torch.allclose(d.cpu().double(), ref, atol=1e-4, rtol=1e-5), (d.cpu() - ref).abs().max()
+
+
+def test_lds_dma_fragment_copy(cuda):
+    src = torch.randint(0, 256, (8 * 1024,), dtype=torch.uint8, generator=torch.Generator().manual_seed(0))
+    dst = torch.zeros(8 * 1024, dtype=torch.uint8, device=cuda)
+    s = src.to(cuda)
+    _lib.call('srf_selftest_lds_dma', s.data_ptr(), dst.data_ptr(), _lib.stream_ptr(cuda))
+    torch.cuda.synchronize()
+    assert torch.equal(dst.cpu().view(8, 1024), src.view(8, 1024).flip(0))
