@@ -47,10 +47,17 @@ __device__ __forceinline__ float f16_bits_to_f32(uint32_t b) {
   return (float)__builtin_bit_cast(_Float16, (uint16_t)(b & 0xffffu));
 }
 
+typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
+
+// Two floats -> packed 16-bit pair (round to nearest even; gfx950 v_cvt_pk_bf16_f32 for BF16).
 template <int P>
 __device__ __forceinline__ uint32_t pack2(float lo, float hi) {
-  if constexpr (P == kBF16) return f32_to_bf16_bits(lo) | (f32_to_bf16_bits(hi) << 16);
-  else return f32_to_f16_bits(lo) | (f32_to_f16_bits(hi) << 16);
+  if constexpr (P == kBF16) {
+    const bf16x2_t v = {(__bf16)lo, (__bf16)hi};
+    return __builtin_bit_cast(uint32_t, v);
+  } else {
+    return f32_to_f16_bits(lo) | (f32_to_f16_bits(hi) << 16);
+  }
 }
 
 template <int P>

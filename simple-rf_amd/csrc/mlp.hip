@@ -263,8 +263,8 @@ static int mlp_backward_impl(const srf_mlp_desc* desc, const void* packed, int n
     jb.dw = dw;
     jb.ld_dw = ld;
     jb.db = db;
-    jb.mblocks = (M + 63) / 64;
-    jb.nblocks = (N + 63) / 64;
+    jb.mblocks = (M + 127) / 128;
+    jb.nblocks = (N + 127) / 128;
   };
   const int D = p.trunk_pe, pe_rows = p.comp_tiles[0] * 16;
   job(w.dz[0], 0, 256, w.x0, pe_rows, 256, D, d_weights[0], D, d_biases[0]);

@@ -49,7 +49,7 @@ constexpr int kMaskSlots = 9;      // ReLU masks of layers 0-7 and the view laye
 constexpr int kWaves = 4;          // waves per block
 constexpr int kLdsRowBytes = 144;  // 16 rows x 144 B transpose tile per wave
 constexpr int kMaxJobs = 16;
-constexpr int kChunk = 8192;       // points reduced per dW work item
+constexpr int kChunk = 8192;       // points reduced per dW work item (multiple of 256)
 
 // ----------------------------------------------------------------------------- plan
 struct Plan {
@@ -392,12 +392,28 @@ __device__ __forceinline__ void zero_acc(f32x4 (&acc)[OUT][Tr<P>::CB]) {
     for (int cb = 0; cb < Tr<P>::CB; ++cb) acc[to][cb] = f32x4{0.f, 0.f, 0.f, 0.f};
 }
 
-// One whole layer with IN input tiles: accumulate, then epi(T, acc) for every output tile
-// (after the last k-unit, so epi may overwrite the input tiles in place).
+// Accumulators start at the bias (rows TB.. of the layer's padded FP32 bias): the bias then
+// lives in the accumulator registers instead of being loaded in the epilogue.
+template <int P, int OUT, int TB>
+__device__ __forceinline__ void bias_acc(f32x4 (&acc)[OUT][Tr<P>::CB], const uint8_t* __restrict__ bias) {
+  const int g = (threadIdx.x & 63) >> 4;
+#pragma unroll
+  for (int to = 0; to < OUT; ++to) {
+    const f32x4 b = *reinterpret_cast<const f32x4*>(bias + (16 * (TB + to) + 4 * g) * 4);
+#pragma unroll
+    for (int cb = 0; cb < Tr<P>::CB; ++cb) acc[to][cb] = b;
+  }
+}
+
+// One whole layer with IN input tiles: accumulate (from the bias if `bias` is non-null), then
+// epi(T, acc) for every output tile (after the last k-unit, so epi may overwrite the input
+// tiles in place).
 template <int P, int IN, int OUT, int TB, typename FIn, typename Epi>
-__device__ __forceinline__ void run_layer(const uint8_t* __restrict__ A, FIn&& fin, Epi&& epi, uint8_t* stage) {
+__device__ __forceinline__ void run_layer(const uint8_t* __restrict__ A, FIn&& fin, Epi&& epi, uint8_t* stage,
+                                          const uint8_t* __restrict__ bias = nullptr) {
   f32x4 acc[OUT][Tr<P>::CB];
-  zero_acc<P, OUT>(acc);
+  if (bias) bias_acc<P, OUT, TB>(acc, bias);
+  else zero_acc<P, OUT>(acc);
   mma_tiles<P, IN, OUT, TB>(acc, A, units<P>(IN), 0, fin, stage);
 #pragma unroll
   for (int to = 0; to < OUT; ++to) epi(TB + to, acc[to]);
@@ -473,7 +489,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
   auto valid = [&](int cb) SRF_AI { return p0 + 16 * cb + j < a.n; };
 
   Act h[16][CB];
-  uint64_t mbits[CB];
+  uint32_t mbits[CB][2];  // ReLU mask bits 4T+r of tiles 0-7 (word 0) and 8-15 (word 1)
 
   // trunk PE tiles -> pe_lds (and, training, the feature-major X0 copy for the weight grads)
   auto pe_trunk = [&]() SRF_AI {
@@ -504,26 +520,24 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
     if (TRAIN) {
       uint64_t* m = reinterpret_cast<uint64_t*>(a.ws + a.w.mask + (size_t)slot * ldP * 32);
 #pragma unroll
-      for (int cb = 0; cb < CB; ++cb) m[((size_t)wave * CB + cb) * 64 + lane] = mbits[cb];
+      for (int cb = 0; cb < CB; ++cb)
+        m[((size_t)wave * CB + cb) * 64 + lane] = (uint64_t)mbits[cb][0] | ((uint64_t)mbits[cb][1] << 32);
     }
   };
-  auto bias4 = [&](int l, int T) SRF_AI {
-    return *reinterpret_cast<const f32x4*>(a.packed + pl.bias_off[l] + (16 * T + 4 * g) * 4);
-  };
+  auto bias_of = [&](int l) SRF_AI { return a.packed + pl.bias_off[l]; };
   // ReLU epilogue: tile T of layer l into h[T] (in place), its feature-major copy at ws offset
   // store_off and the mask bits.
   auto relu_epi = [&](int l, size_t store_off) SRF_AI {
     return [&, l, store_off](int T, f32x4(&acc)[CB]) SRF_AI {
-      const f32x4 bv = bias4(l, T);
       Act t[CB];
 #pragma unroll
       for (int cb = 0; cb < CB; ++cb) {
-        f32x4 v = acc[cb] + bv;
+        f32x4 v = acc[cb];  // bias already in the accumulator
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
           const bool pos = v[r] > 0.f;
           v[r] = pos ? v[r] : 0.f;
-          mbits[cb] |= (uint64_t)pos << (4 * T + r);
+          mbits[cb][T >> 3] |= pos ? (1u << (4 * (T & 7) + r)) : 0u;
         }
         t[cb] = to_act<P>(v);
         h[T][cb] = t[cb];
@@ -537,20 +551,20 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
   // ---- layer 0: PE(trunk) -> 256
   pe_trunk();
 #pragma unroll
-  for (int cb = 0; cb < CB; ++cb) mbits[cb] = 0;
+  for (int cb = 0; cb < CB; ++cb) mbits[cb][0] = mbits[cb][1] = 0;
   if (pl.comp_tiles[0] == 4)
-    run_layer<P, 4, 16, 0>(a.packed + pl.fwd_off[0], fin_pe, relu_epi(0, a.w.h[0]), stage);
+    run_layer<P, 4, 16, 0>(a.packed + pl.fwd_off[0], fin_pe, relu_epi(0, a.w.h[0]), stage, bias_of(0));
   else
-    run_layer<P, 2, 16, 0>(a.packed + pl.fwd_off[0], fin_pe, relu_epi(0, a.w.h[0]), stage);
+    run_layer<P, 2, 16, 0>(a.packed + pl.fwd_off[0], fin_pe, relu_epi(0, a.w.h[0]), stage, bias_of(0));
   mask_store(0);
 
   // ---- layers 1..7, each in place in h; layer 5 adds the trunk PE k-steps after the 16
   // register tiles (one loop body for all seven layers keeps register pressure flat)
   for (int l = 1; l < 8; ++l) {
 #pragma unroll
-    for (int cb = 0; cb < CB; ++cb) mbits[cb] = 0;
+    for (int cb = 0; cb < CB; ++cb) mbits[cb][0] = mbits[cb][1] = 0;
     f32x4 acc[16][CB];
-    zero_acc<P, 16>(acc);
+    bias_acc<P, 16, 0>(acc, bias_of(l));
     const uint8_t* A = a.packed + pl.fwd_off[l];
     const int upr = units<P>(16 + pl.comp_tiles[l]);
     mma_tiles<P, 16, 16, 0>(acc, A, upr, 0, fin_h, stage);
@@ -575,37 +589,34 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
   if constexpr (!VD) {
     // 4-channel output: sigma, rgb (src/models/SimpleNeRF17.py:735-750)
     run_layer<P, 16, 1, 0>(a.packed + pl.fwd_off[8], fin_h, [&](int T, f32x4(&acc)[CB]) SRF_AI {
-      const f32x4 bv = bias4(8, 0);
 #pragma unroll
       for (int cb = 0; cb < CB; ++cb) {
         const int p = p0 + 16 * cb + j;
         if (g == 0 && valid(cb)) {
-          const f32x4 v = acc[cb] + bv;
+          const f32x4 v = acc[cb];
           head_sigma(cb, v[0]);
           a.rgb[3 * p + 0] = sigmoidf_(v[1]);
           a.rgb[3 * p + 1] = sigmoidf_(v[2]);
           a.rgb[3 * p + 2] = sigmoidf_(v[3]);
         }
       }
-    }, stage);
+    }, stage, bias_of(8));
   } else {
     // sigma (row 256, tile 16) first, then feature rows 0..255 (identity) in place in h
     run_layer<P, 16, 1, 16>(a.packed + pl.fwd_off[8], fin_h, [&](int T, f32x4(&acc)[CB]) SRF_AI {
-      const f32x4 bv = bias4(8, 16);
 #pragma unroll
       for (int cb = 0; cb < CB; ++cb)
-        if (g == 0 && valid(cb)) head_sigma(cb, acc[cb][0] + bv[0]);
-    }, stage);
+        if (g == 0 && valid(cb)) head_sigma(cb, acc[cb][0]);
+    }, stage, bias_of(8));
     run_layer<P, 16, 16, 0>(a.packed + pl.fwd_off[8], fin_h, [&](int T, f32x4(&acc)[CB]) SRF_AI {
-      const f32x4 bv = bias4(8, T);
       Act t[CB];
 #pragma unroll
       for (int cb = 0; cb < CB; ++cb) {
-        t[cb] = to_act<P>(acc[cb] + bv);
+        t[cb] = to_act<P>(acc[cb]);
         h[T][cb] = t[cb];
       }
       if (TRAIN) store_tile<P>(a.ws + a.w.f + (size_t)16 * T * ldP * pl.elem, ldP, p0, t, lds);
-    }, stage);
+    }, stage, bias_of(8));
     // view layer input extras: [PE_xyz[trunk_pe:], PE_dir]
     const int n_hi = 3 * (2 * pl.pts_deg + 1) - pl.trunk_pe;
     float x[CB][3], d[CB][3];
@@ -636,29 +647,28 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
       }
     });
 #pragma unroll
-    for (int cb = 0; cb < CB; ++cb) mbits[cb] = 0;
+    for (int cb = 0; cb < CB; ++cb) mbits[cb][0] = mbits[cb][1] = 0;
     // view layer [feature, extras] -> 128 (ReLU), in place into h[0..7]
     if (pl.comp_tiles[9] == 2)
-      run_layer<P, 18, 8, 0>(a.packed + pl.fwd_off[9], fin_hc, relu_epi(9, a.w.v), stage);
+      run_layer<P, 18, 8, 0>(a.packed + pl.fwd_off[9], fin_hc, relu_epi(9, a.w.v), stage, bias_of(9));
     else if (pl.comp_tiles[9] == 4)
-      run_layer<P, 20, 8, 0>(a.packed + pl.fwd_off[9], fin_hc, relu_epi(9, a.w.v), stage);
+      run_layer<P, 20, 8, 0>(a.packed + pl.fwd_off[9], fin_hc, relu_epi(9, a.w.v), stage, bias_of(9));
     else
-      run_layer<P, 22, 8, 0>(a.packed + pl.fwd_off[9], fin_hc, relu_epi(9, a.w.v), stage);
+      run_layer<P, 22, 8, 0>(a.packed + pl.fwd_off[9], fin_hc, relu_epi(9, a.w.v), stage, bias_of(9));
     mask_store(8);
     // rgb head 128 -> 3, sigmoid
     run_layer<P, 8, 1, 0>(a.packed + pl.fwd_off[10], fin_h, [&](int T, f32x4(&acc)[CB]) SRF_AI {
-      const f32x4 bv = bias4(10, 0);
 #pragma unroll
       for (int cb = 0; cb < CB; ++cb) {
         const int p = p0 + 16 * cb + j;
         if (g == 0 && valid(cb)) {
-          const f32x4 v = acc[cb] + bv;
+          const f32x4 v = acc[cb];
           a.rgb[3 * p + 0] = sigmoidf_(v[0]);
           a.rgb[3 * p + 1] = sigmoidf_(v[1]);
           a.rgb[3 * p + 2] = sigmoidf_(v[2]);
         }
       }
-    }, stage);
+    }, stage, bias_of(10));
   }
 }
 
@@ -702,14 +712,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
 
   // z holds the current layer's pre-activation gradients; every product writes in place.
   Act z[18][CB];
-  uint64_t mb[CB];
+  uint32_t mb[CB][2];
   auto store_dz = [&](int l, int T, const Act(&t)[CB]) SRF_AI {
     store_tile<P>(a.ws + a.w.dz[l] + (size_t)16 * T * ldP * pl.elem, ldP, p0, t, lds);
   };
   auto load_mask = [&](int slot) SRF_AI {
     const uint64_t* m = reinterpret_cast<const uint64_t*>(a.ws + a.w.mask + (size_t)slot * ldP * 32);
 #pragma unroll
-    for (int cb = 0; cb < CB; ++cb) mb[cb] = m[((size_t)wave * CB + cb) * 64 + lane];
+    for (int cb = 0; cb < CB; ++cb) {
+      const uint64_t v = m[((size_t)wave * CB + cb) * 64 + lane];
+      mb[cb][0] = (uint32_t)v;
+      mb[cb][1] = (uint32_t)(v >> 32);
+    }
   };
   // epilogue: ReLU'-masked gradient of layer l into z[T], stored as dZ_l
   auto masked_epi = [&](int l) SRF_AI {
@@ -719,7 +733,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
       for (int cb = 0; cb < CB; ++cb) {
         f32x4 v = acc[cb];
 #pragma unroll
-        for (int r = 0; r < 4; ++r) v[r] = ((mb[cb] >> (4 * T + r)) & 1ull) ? v[r] : 0.f;
+        for (int r = 0; r < 4; ++r) v[r] = (mb[cb][T >> 3] & (1u << (4 * (T & 7) + r))) ? v[r] : 0.f;
         t[cb] = to_act<P>(v);
         z[T][cb] = t[cb];
       }
@@ -810,28 +824,58 @@ struct DwArgs {
   int n_jobs, ldP, n_chunks, total_work;
 };
 
+// One block reduces a 128 x 128 tile of dW = dZ X^T over a chunk of points.  Per k-unit
+// (32 points for 16-bit operands, 16 for FP32) the 8 dZ-row and 8 X-row fragments of the tile
+// (1 KB each, gathered from the feature-major arrays by per-lane addresses) are copied
+// global -> LDS by the four waves, two k-units per double-buffered stage; each wave then runs
+// the 16 MFMAs of its 64 x 64 quadrant from LDS.  Partial sums are added with FP32 atomics.
+constexpr int kDwStage = 2 * 16 * 1024;  // two k-units x (8 A + 8 B) fragments
+
 template <int P>
 __global__ __launch_bounds__(256) void mlp_dw_kernel(DwArgs a) {
+  __shared__ __attribute__((aligned(16))) uint8_t stage[2 * kDwStage];
   const int lane = threadIdx.x & 63, g = lane >> 4, j = lane & 15;
-  const int w = blockIdx.x * kWaves + (threadIdx.x >> 6);
-  if (w >= a.total_work) return;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int item = blockIdx.x;
   int ji = 0;
 #pragma unroll 1
-  while (ji + 1 < a.n_jobs && w >= a.jobs[ji + 1].work_begin) ++ji;
+  while (ji + 1 < a.n_jobs && item >= a.jobs[ji + 1].work_begin) ++ji;
   const DwJob& jb = a.jobs[ji];
-  const int local = w - jb.work_begin;
+  const int local = item - jb.work_begin;
   const int nblk = jb.mblocks * jb.nblocks;
   const int tile = local % nblk, chunk = local / nblk;
-  const int m0 = 64 * (tile % jb.mblocks), n0 = 64 * (tile / jb.mblocks);
+  const int m0 = 128 * (tile % jb.mblocks), n0 = 128 * (tile / jb.mblocks);
   const int pb = chunk * kChunk;
   const int pe = min(pb + kChunk, a.ldP);
   const size_t ldP = a.ldP;
+  constexpr int ELEM = P == kF32 ? 4 : 2;
+  constexpr int KPTS = P == kF32 ? 16 : 32;  // points per k-unit
+  // this wave's quadrant: row tiles 4*(w>>1)+i, column tiles 4*(w&1)+k
+  const int qm = 4 * (w >> 1), qn = 4 * (w & 1);
   bool mv[4], nv[4];
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
-    mv[i] = m0 + 16 * i < jb.dz_rows;
-    nv[i] = n0 + 16 * i < jb.x_rows;
+    mv[i] = m0 + 16 * (qm + i) < jb.dz_rows;
+    nv[i] = n0 + 16 * (qn + i) < jb.x_rows;
   }
+  // DMA of the 16 fragments of k-unit at point p into buf: fragment f < 8 is dZ row tile f,
+  // f >= 8 is X row tile f - 8; wave w copies f = w, w+4, w+8, w+12 (if the tile exists).
+  auto dma_unit = [&](int p, uint8_t* buf) SRF_AI {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int f = w + 4 * q;
+      const bool is_a = f < 8;
+      const int rt = is_a ? f : f - 8;
+      const int row0 = (is_a ? m0 : n0) + 16 * rt;
+      if (row0 < (is_a ? jb.dz_rows : jb.x_rows)) {
+        const uint8_t* base = is_a ? jb.dz : jb.x;
+        const int pg = P == kF32 ? 4 * g : 8 * g;
+        const uint8_t* src = base + (((size_t)(row0 + j)) * ldP + p + pg) * ELEM;
+        __builtin_amdgcn_global_load_lds((const void*)src,
+                                         (__attribute__((address_space(3))) void*)(buf + f * 1024), 16, 0, 0);
+      }
+    }
+  };
   f32x4 acc[4][4];
   float rs[4];
 #pragma unroll
@@ -840,53 +884,61 @@ __global__ __launch_bounds__(256) void mlp_dw_kernel(DwArgs a) {
 #pragma unroll
     for (int k = 0; k < 4; ++k) acc[i][k] = f32x4{0.f, 0.f, 0.f, 0.f};
   }
-  const bool do_db = jb.db != nullptr && n0 == 0;
-  if constexpr (P == kF32) {
-    const float* dz = reinterpret_cast<const float*>(jb.dz);
-    const float* x = reinterpret_cast<const float*>(jb.x);
+  const bool do_db = jb.db != nullptr && n0 == 0 && qn == 0;
+  const int n_units = (pe - pb) / KPTS;  // even: chunks and ldP are multiples of 256 points
+  dma_unit(pb, stage);
+  dma_unit(pb + KPTS, stage + 16 * 1024);
+  wait_vmem();
+  __syncthreads();
 #pragma unroll 1
-    for (int p = pb; p < pe; p += 16) {
-      f32x4 av[4], bv[4];
+  for (int u = 0; u < n_units; u += 2) {
+    const uint8_t* cur = stage + ((u >> 1) & 1) * kDwStage;
+    if (u + 2 < n_units) {
+      uint8_t* nxt = stage + (((u >> 1) + 1) & 1) * kDwStage;
+      dma_unit(pb + (u + 2) * KPTS, nxt);
+      dma_unit(pb + (u + 3) * KPTS, nxt + 16 * 1024);
+    }
 #pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        av[i] = mv[i] ? *reinterpret_cast<const f32x4*>(dz + (m0 + 16 * i + j) * ldP + p + 4 * g)
-                      : f32x4{0.f, 0.f, 0.f, 0.f};
-        bv[i] = nv[i] ? *reinterpret_cast<const f32x4*>(x + (n0 + 16 * i + j) * ldP + p + 4 * g)
-                      : f32x4{0.f, 0.f, 0.f, 0.f};
-      }
+    for (int h = 0; h < 2; ++h) {
+      const uint8_t* fr = cur + h * 16 * 1024;
+      if constexpr (P == kF32) {
+        f32x4 av[4], bv[4];
 #pragma unroll
-      for (int q = 0; q < 4; ++q)
+        for (int i = 0; i < 4; ++i) {
+          av[i] = *reinterpret_cast<const f32x4*>(fr + (qm + i) * 1024 + lane * 16);
+          bv[i] = *reinterpret_cast<const f32x4*>(fr + (8 + qn + i) * 1024 + lane * 16);
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+#pragma unroll
+          for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+              if (mv[i] && nv[k]) acc[i][k] = mfma32(av[i][q], bv[k][q], acc[i][k]);
+        if (do_db)
+#pragma unroll
+          for (int i = 0; i < 4; ++i) rs[i] += (av[i][0] + av[i][1]) + (av[i][2] + av[i][3]);
+      } else {
+        u32x4 av[4], bv[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          av[i] = *reinterpret_cast<const u32x4*>(fr + (qm + i) * 1024 + lane * 16);
+          bv[i] = *reinterpret_cast<const u32x4*>(fr + (8 + qn + i) * 1024 + lane * 16);
+        }
 #pragma unroll
         for (int i = 0; i < 4; ++i)
 #pragma unroll
-          for (int k = 0; k < 4; ++k) acc[i][k] = mfma32(av[i][q], bv[k][q], acc[i][k]);
-      if (do_db)
+          for (int k = 0; k < 4; ++k)
+            if (mv[i] && nv[k]) acc[i][k] = mfma16<P>(av[i], bv[k], acc[i][k]);
+        if (do_db)
 #pragma unroll
-        for (int i = 0; i < 4; ++i) rs[i] += (av[i][0] + av[i][1]) + (av[i][2] + av[i][3]);
-    }
-  } else {
-    const uint16_t* dz = reinterpret_cast<const uint16_t*>(jb.dz);
-    const uint16_t* x = reinterpret_cast<const uint16_t*>(jb.x);
-#pragma unroll 1
-    for (int p = pb; p < pe; p += 32) {
-      u32x4 av[4], bv[4];
+          for (int i = 0; i < 4; ++i)
 #pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        av[i] = mv[i] ? *reinterpret_cast<const u32x4*>(dz + (m0 + 16 * i + j) * ldP + p + 8 * g)
-                      : u32x4{0u, 0u, 0u, 0u};
-        bv[i] = nv[i] ? *reinterpret_cast<const u32x4*>(x + (n0 + 16 * i + j) * ldP + p + 8 * g)
-                      : u32x4{0u, 0u, 0u, 0u};
+            for (int q = 0; q < 4; ++q) rs[i] += unpack_lo<P>(av[i][q]) + unpack_hi<P>(av[i][q]);
       }
-#pragma unroll
-      for (int i = 0; i < 4; ++i)
-#pragma unroll
-        for (int k = 0; k < 4; ++k) acc[i][k] = mfma16<P>(av[i], bv[k], acc[i][k]);
-      if (do_db)
-#pragma unroll
-        for (int i = 0; i < 4; ++i)
-#pragma unroll
-          for (int q = 0; q < 4; ++q) rs[i] += unpack_lo<P>(av[i][q]) + unpack_hi<P>(av[i][q]);
     }
+    wait_vmem();
+    __syncthreads();
   }
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
@@ -894,19 +946,19 @@ __global__ __launch_bounds__(256) void mlp_dw_kernel(DwArgs a) {
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
       if (!nv[k]) continue;
-      const int c = n0 + 16 * k + j;
+      const int c = n0 + 16 * (qn + k) + j;
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int o = m0 + 16 * i + 4 * g + r;
+        const int o = m0 + 16 * (qm + i) + 4 * g + r;
         if (o < jb.M && c < jb.N) unsafeAtomicAdd(jb.dw + (size_t)o * jb.ld_dw + c, acc[i][k][r]);
       }
     }
     if (do_db) {
-      float s = rs[i];
-      s += __shfl_xor(s, 16, 64);
-      s += __shfl_xor(s, 32, 64);
-      const int o = m0 + 16 * i + j;
-      if (g == 0 && o < jb.M) unsafeAtomicAdd(jb.db + o, s);
+      float sr = rs[i];
+      sr += __shfl_xor(sr, 16, 64);
+      sr += __shfl_xor(sr, 32, 64);
+      const int o = m0 + 16 * (qm + i) + j;
+      if (g == 0 && o < jb.M) unsafeAtomicAdd(jb.db + o, sr);
     }
   }
 }
@@ -932,8 +984,7 @@ int launch_bwd(const BwdArgs& a, hipStream_t s) {
 
 template <int P>
 int launch_dw(const DwArgs& d, hipStream_t s) {
-  const int blocks = (d.total_work + kWaves - 1) / kWaves;
-  hipLaunchKernelGGL(mlp_dw_kernel<P>, dim3(blocks), dim3(256), 0, s, d);
+  hipLaunchKernelGGL(mlp_dw_kernel<P>, dim3(d.total_work), dim3(256), 0, s, d);
   return launch_status();
 }
 
