@@ -32,6 +32,15 @@ METRIC = 'train rays/s (fwd+bwd) & render rays/s: fused BF16 vs plain FP32, 1 GP
 PEAK_BF16_TFLOPS = 2500.0
 PEAK_HBM_GBS = 8000.0
 MACS = {(63, True): 593408, (21, True): 577280, (63, False): 492032}
+# HBM bytes per launch of the dominant kernel from the rocprofv3 FETCH_SIZE / WRITE_SIZE passes
+# (scaled by a plain copy of known size) over this same train step.
+TRAFFIC_PROFILE = REPO / 'profiles' / 'mlp_fwd_train_step_4096rays.json'
+
+
+def profiled_traffic(kernel: str, rays: int):
+    if kernel != 'srf_mlp_forward' or rays != 4096 or not TRAFFIC_PROFILE.exists():
+        return None
+    return json.loads(TRAFFIC_PROFILE.read_text())['hbm_bytes_per_launch']
 
 
 class KernelTimer:
@@ -224,6 +233,9 @@ def main():
     ap.add_argument('--cpu-rays', type=int, default=128)
     ap.add_argument('--cpu-seconds', type=float, default=20.0)
     ap.add_argument('--no-baselines', action='store_true')
+    ap.add_argument('--copy-calibration-mb', type=int, default=0,
+                    help='after the timed legs, run 3 device copies of this many MiB (a plain copy kernel '
+                         'of known size for scaling rocprof FETCH_SIZE / WRITE_SIZE)')
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get('WORLD_SIZE', '1')) != 1:
         print('bench.py measures one GPU: this build has no multi-GPU path (rays shard across GPUs with '
@@ -245,12 +257,20 @@ def main():
     if dom['flop'] > 0:
         ach = dom['flop'] / (dom['ms'] * 1e-3) / 1e12
         roof = {'bound': 'mfma', 'kernel': dom_name, 'achieved': round(ach, 2), 'peak': PEAK_BF16_TFLOPS,
-                'unit': 'TFLOP/s', 'frac': round(ach / PEAK_BF16_TFLOPS, 4), 'traffic': None,
+                'unit': 'TFLOP/s', 'frac': round(ach / PEAK_BF16_TFLOPS, 4),
+                'traffic': profiled_traffic(dom_name, args.rays),
                 'avg_launch_ms': round(dom['ms'] / dom['calls'], 4)}
     per_kernel = {k: {'calls': v['calls'], 'ms_per_step': round(v['ms'] / args.steps, 3),
                       **({'tflops': round(v['flop'] / (v['ms'] * 1e-3) / 1e12, 1)} if v['flop'] else {})}
                   for k, v in kern.items()}
     render = hip_render_leg(args, dev, tr['model'])
+    if args.copy_calibration_mb:
+        n = args.copy_calibration_mb * (1 << 20) // 4
+        src = torch.rand(n, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
     result = {
         'metric': METRIC, 'value': round(value, 1), 'unit': 'rays/s', 'n_gpus': 1, 'steps': args.steps,
         'warmup': args.warmup, 'ms_per_step': round(ms, 3), 'higher_is_better': True, 'scaling': 'weak',

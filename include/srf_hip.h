@@ -87,6 +87,8 @@ int srf_fine_z_vals(int n_rays, int n_coarse, int n_fine, const float* z_coarse,
  *   rays_o, rays_d [n_rays,3] (world, used for the metric depth and, when !ndc, delta);
  *   rays_d_ndc [n_rays,3] (NDC delta norm, required when ndc != 0).
  *   distance_scale multiplies sigma*delta (1 for NeRF, 25 for TensoRF).
+ *   rgb may be NULL (TensoRF composites colours afterwards with srf_volume_render_fwd);
+ *   rgb_map is then not written and may be NULL too.
  * Outputs (per ray): rgb_map [n_rays,3], acc, depth, depth_var, and when ndc != 0
  * depth_ndc, depth_var_ndc (may be NULL otherwise); (per sample, each may be NULL):
  * alpha, visibility, weights [P]. */
@@ -102,7 +104,8 @@ int srf_composite_fwd(int n_rays, int n_samples, const int32_t* ray_offsets, con
  *   acc, depth, depth_ndc: the forward's per-ray outputs (depth_ndc may be NULL if !ndc).
  *   Upstream gradients, each may be NULL (= zero): g_rgb [n_rays,3], g_acc, g_depth,
  *   g_depth_var, g_depth_ndc, g_depth_var_ndc [n_rays], g_alpha, g_visibility, g_weights [P].
- * Outputs: d_sigma [P], d_rgb [P,3] (overwritten). */
+ * Outputs: d_sigma [P], d_rgb [P,3] (overwritten; d_rgb may be NULL, and rgb NULL means
+ * zero colours as in the forward). */
 int srf_composite_bwd(int n_rays, int n_samples, const int32_t* ray_offsets, const float* sigma,
                       const float* rgb, const float* z, const float* rays_o, const float* rays_d,
                       const float* rays_d_ndc, int ndc, int white_bkgd, float distance_scale,
@@ -188,6 +191,107 @@ int srf_mlp_backward_data(const srf_mlp_desc* desc, const void* packed, int n_po
                           const float* d_rgb, void* workspace, void* stream);
 int srf_mlp_weight_grads(const srf_mlp_desc* desc, int n_points, void* workspace,
                          float* const* d_weights, float* const* d_biases, void* stream);
+
+/* ------------------------------------------------------- TensoRF VM factors (c) */
+
+/* Density predictors of LowRankTensor.get_density_predictor (src/models/SimpleTensoRF09.py:671-679). */
+#define SRF_DENSITY_RELU 0
+#define SRF_DENSITY_SOFTPLUS 1 /* softplus(x + offset), beta 1, threshold 20 */
+
+/* The vector-matrix factors of one VmDecomposedTensor field (density or appearance;
+ * src/models/SimpleTensoRF09.py:1126-1165).  Plane p is the reference's matrices_*[p] of shape
+ * [1, n_comp[p], plane_h[p], plane_w[p]] and line p its vectors_*[p] of shape
+ * [1, n_comp[p], line_len[p], 1], stored texel-major: plane p as [plane_h][plane_w][n_comp]
+ * and line p as [line_len][n_comp] (the component axis innermost, 16-byte aligned bases).
+ * Plane p is sampled at (x = pts[a0], y = pts[a1]) with (a0, a1) = (0,1), (0,2), (1,2) and
+ * line p at pts[2 - p] (matrix_axes / vector_axes at :1131-1132).  n_comp[p] % 4 == 0 and
+ * sum n_comp <= 256. */
+typedef struct srf_vm_factors {
+  int n_comp[3];
+  int plane_h[3];
+  int plane_w[3];
+  int line_len[3];
+  const float* planes[3];
+  const float* lines[3];
+} srf_vm_factors;
+
+/* Occupancy volume of AlphaGridMask (src/models/SimpleTensoRF09.py:1323-1349): float
+ * [depth][height][width] (z, y, x), with its own bounding box {lo xyz, hi xyz} (host floats). */
+typedef struct srf_alpha_grid {
+  const float* volume; /* NULL = no alpha mask */
+  int depth, height, width;
+  float bbox[6];
+} srf_alpha_grid;
+
+/* Validity, normalisation and density of n_points samples: valid = pts inside bbox (host
+ * floats {lo xyz, hi xyz}) and, with an alpha grid, its trilinear occupancy > 0; the density
+ * feature sum_p sum_c plane_p * line_p on valid samples, then the predictor; invalid samples
+ * get sigma 0.  Replaces LowRankTensor.forward up to the density (src/models/SimpleTensoRF09.py:
+ * 701-721), normalize_points (:763-765), AlphaGridMask.sample_alpha (:1342-1345) and
+ * VmDecomposedTensor.get_volume_density (:1214-1239).
+ * pts [n,3]; outputs sigma [n]; feat [n] (pre-activation, 0 when invalid) and valid [n]
+ * (uint8) may be NULL and are what srf_vm_density_bwd needs. */
+int srf_vm_density_fwd(const srf_vm_factors* factors, const float* bbox,
+                       const srf_alpha_grid* alpha, int64_t n_points, const float* pts,
+                       int predictor, float offset, float* sigma, float* feat, uint8_t* valid,
+                       void* stream);
+
+/* Backward of srf_vm_density_fwd into the factor gradients d_planes[3] / d_lines[3] (same
+ * texel-major layout as the factors; accumulated with FP32 atomics, so the caller zeroes them
+ * first).  Coordinates carry no gradient (the reference detaches them, :1224-1228). */
+int srf_vm_density_bwd(const srf_vm_factors* factors, float* const* d_planes,
+                       float* const* d_lines, const float* bbox, int64_t n_points,
+                       const float* pts, const uint8_t* valid, const float* feat, int predictor,
+                       float offset, const float* d_sigma, void* stream);
+
+/* Appearance features of n_rows samples: features[j, :] = cat_p(plane_p * line_p) at sample
+ * idx[j] (idx NULL: sample j), [n_rows, sum n_comp] row-major.  Replaces the gather part of
+ * VmDecomposedTensor.get_color (src/models/SimpleTensoRF09.py:1241-1263, before
+ * basis_matrix_color). */
+int srf_vm_appearance_fwd(const srf_vm_factors* factors, const float* bbox, int64_t n_rows,
+                          const int32_t* idx, const float* pts, float* features, void* stream);
+
+/* Backward of srf_vm_appearance_fwd: d_features [n_rows, sum n_comp] scattered into
+ * d_planes / d_lines with FP32 atomics (accumulating). */
+int srf_vm_appearance_bwd(const srf_vm_factors* factors, float* const* d_planes,
+                          float* const* d_lines, const float* bbox, int64_t n_rows,
+                          const int32_t* idx, const float* pts, const float* d_features,
+                          void* stream);
+
+/* Trilinear occupancy value of each point (AlphaGridMask.sample_alpha,
+ * src/models/SimpleTensoRF09.py:1342-1345): out [n_points]. */
+int srf_alpha_mask_sample(const srf_alpha_grid* alpha, int64_t n_points, const float* pts,
+                          float* out, void* stream);
+
+/* Indices (ascending, int32) of the entries with values[i] > threshold, and their number in
+ * *count (device int32); no host synchronisation.  Replaces the surface test
+ * weights > ray_marching_weight_threshold and the boolean indexing that follows it
+ * (src/models/SimpleTensoRF09.py:726, :1249).  idx must hold n entries; workspace holds
+ * srf_compact_workspace_bytes(n) bytes. */
+size_t srf_compact_workspace_bytes(int64_t n);
+int srf_threshold_compact(int64_t n, const float* values, float threshold, int32_t* idx,
+                          int32_t* count, void* workspace, void* stream);
+
+/* out[r, c] = sum_k weights[k] * values[k, c] over the samples k of ray r (dense or ragged,
+ * as srf_composite_fwd).  Replaces LowRankTensor.volume_render (src/models/SimpleTensoRF09.py:
+ * 814-817). */
+int srf_volume_render_fwd(int n_rays, int n_samples, const int32_t* ray_offsets,
+                          const float* weights, const float* values, int n_channels, float* out,
+                          void* stream);
+
+/* Backward of srf_volume_render_fwd: g_weights[k] = sum_c g_out[r, c] values[k, c] and
+ * g_values[k, c] = weights[k] g_out[r, c] (both overwritten; either may be NULL). */
+int srf_volume_render_bwd(int n_rays, int n_samples, const int32_t* ray_offsets,
+                          const float* weights, const float* values, int n_channels,
+                          const float* g_out, float* g_weights, float* g_values, void* stream);
+
+/* Depths of the non-NDC TensoRF sampler (src/models/SimpleTensoRF09.py:382-400): the ray's
+ * entry into bbox (host floats), t_min = max_k min((hi_k - o_k) / v_k, (lo_k - o_k) / v_k)
+ * with v = d (d == 0 -> 1e-6), clamped to [near, far]; z[r, k] = t_min + step * (k + u[r])
+ * (u NULL: k).  rays_o, rays_d [n_rays,3]; z [n_rays, n_samples]. */
+int srf_aabb_samples(int n_rays, int n_samples, const float* rays_o, const float* rays_d,
+                     const float* bbox, float near, float far, float step, const float* u,
+                     float* z, void* stream);
 
 /* Lane-layout self-test of the MFMA building blocks (one 16x16 tile): d = a b with
  * a [16,K], b [K,16] row-major FP32 (rounded to the operand type), K = 4 for SRF_PREC_F32 and

@@ -133,6 +133,22 @@ def color_mlp_forward(params: dict, features: torch.Tensor, view_dirs: torch.Ten
     return torch.sigmoid(F.linear(h, params['mlp.4.weight'], params['mlp.4.bias']))
 
 
+def aabb_z_vals(rays_o: torch.Tensor, rays_d: torch.Tensor, bounding_box: torch.Tensor, near: float, far: float,
+                step: float, n_samples: int, u: torch.Tensor | None = None) -> torch.Tensor:
+    """Non-NDC coarse depths, SimpleTensoRF09.py:382-400: ray entry into the box, clamped to
+    [near, far], plus step * (arange(S) + u) with one u per ray."""
+    vec = torch.where(rays_d == 0, torch.full_like(rays_d, 1e-6), rays_d)
+    rate_a = (bounding_box[1] - rays_o) / vec
+    rate_b = (bounding_box[0] - rays_o) / vec
+    t_min = torch.minimum(rate_a, rate_b).amax(-1).clamp(min=near, max=far)
+    rng = torch.arange(n_samples, dtype=rays_o.dtype)[None]
+    if u is not None:
+        rng = rng.repeat(rays_d.shape[0], 1)
+        rng = rng + u.to(rays_o.dtype)[:, None]
+    st = torch.tensor(step, dtype=rays_o.dtype) * rng
+    return t_min[..., None] + st
+
+
 def tensorf_z_vals_ndc(n_rays: int, n_samples: int, near: float, far: float, u=None, dtype=torch.float32):
     """NDC coarse depths of SimpleTensoRF09.py:364-381 (same construction as NeRF)."""
     from .nerf import stratified_z_vals

@@ -126,9 +126,11 @@ __global__ __launch_bounds__(256) void composite_fwd_kernel(
       if (vis_out) vis_out[i] = T;
       if (w_out) w_out[i] = w;
       s_acc += w;
-      s_r += __fmul_rn(w, rgb[3 * i + 0]);
-      s_g += __fmul_rn(w, rgb[3 * i + 1]);
-      s_b += __fmul_rn(w, rgb[3 * i + 2]);
+      if (rgb) {
+        s_r += __fmul_rn(w, rgb[3 * i + 0]);
+        s_g += __fmul_rn(w, rgb[3 * i + 1]);
+        s_b += __fmul_rn(w, rgb[3 * i + 2]);
+      }
       if (ndc) {
         s_zn += __fmul_rn(w, zk);
         s_zm += __fmul_rn(w, metric_depth(rc, zk));
@@ -180,9 +182,11 @@ __global__ __launch_bounds__(256) void composite_fwd_kernel(
   s_vn = wave_sum(s_vn);
   if (lane == 0) {
     const float bg = white_bkgd ? __fsub_rn(1.0f, s_acc) : 0.0f;
-    rgb_map[3 * r + 0] = white_bkgd ? __fadd_rn(s_r, bg) : s_r;
-    rgb_map[3 * r + 1] = white_bkgd ? __fadd_rn(s_g, bg) : s_g;
-    rgb_map[3 * r + 2] = white_bkgd ? __fadd_rn(s_b, bg) : s_b;
+    if (rgb_map) {
+      rgb_map[3 * r + 0] = white_bkgd ? __fadd_rn(s_r, bg) : s_r;
+      rgb_map[3 * r + 1] = white_bkgd ? __fadd_rn(s_g, bg) : s_g;
+      rgb_map[3 * r + 2] = white_bkgd ? __fadd_rn(s_b, bg) : s_b;
+    }
     acc_out[r] = s_acc;
     depth_out[r] = D;
     depth_var_out[r] = s_v;
@@ -262,9 +266,11 @@ __global__ __launch_bounds__(256) void composite_bwd_kernel(
       sig = sigma[i];
       zk = z[i];
       zk1 = (k + 1 < sp.count) ? z[i + 1] : rc.inf_depth;
-      c0 = rgb[3 * i + 0];
-      c1 = rgb[3 * i + 1];
-      c2 = rgb[3 * i + 2];
+      if (rgb) {
+        c0 = rgb[3 * i + 0];
+        c1 = rgb[3 * i + 1];
+        c2 = rgb[3 * i + 2];
+      }
     }
     const SampleTerms t = sample_terms(sig, zk, zk1, rc.dnorm, ds);
     const float lq = valid ? t.lq : 0.f;
@@ -290,9 +296,59 @@ __global__ __launch_bounds__(256) void composite_bwd_kernel(
     if (valid) {
       const float dalpha = (g_alpha ? g_alpha[i] : 0.f) + g * T;
       d_sigma[i] = t.delta * ds * t.a * (dalpha - S / t.q);
-      d_rgb[3 * i + 0] = w * gC0;
-      d_rgb[3 * i + 1] = w * gC1;
-      d_rgb[3 * i + 2] = w * gC2;
+      if (d_rgb) {
+        d_rgb[3 * i + 0] = w * gC0;
+        d_rgb[3 * i + 1] = w * gC1;
+        d_rgb[3 * i + 2] = w * gC2;
+      }
+    }
+  }
+}
+
+// Weighted sums of per-sample values along each ray (LowRankTensor.volume_render,
+// src/models/SimpleTensoRF09.py:814-817): one wave per ray, values read channel-interleaved.
+__global__ __launch_bounds__(256) void volume_render_fwd_kernel(int n_rays, int n_samples,
+                                                                const int32_t* __restrict__ off,
+                                                                const float* __restrict__ w,
+                                                                const float* __restrict__ v, int C,
+                                                                float* __restrict__ out) {
+  const int r = blockIdx.x * kWavesPerBlock + threadIdx.x / SRF_WAVE;
+  if (r >= n_rays) return;
+  const int lane = lane_id();
+  const RaySpan sp = ray_span(r, n_samples, off);
+  for (int c = 0; c < C; ++c) {
+    float s = 0.f;
+    for (int k = lane; k < sp.count; k += SRF_WAVE) {
+      const int64_t i = sp.begin + k;
+      s += __fmul_rn(w[i], v[i * C + c]);
+    }
+    s = wave_sum(s);
+    if (lane == 0) out[(int64_t)r * C + c] = s;
+  }
+}
+
+__global__ __launch_bounds__(256) void volume_render_bwd_kernel(int n_rays, int n_samples,
+                                                                const int32_t* __restrict__ off,
+                                                                const float* __restrict__ w,
+                                                                const float* __restrict__ v, int C,
+                                                                const float* __restrict__ g_out,
+                                                                float* __restrict__ g_w,
+                                                                float* __restrict__ g_v) {
+  const int r = blockIdx.x * kWavesPerBlock + threadIdx.x / SRF_WAVE;
+  if (r >= n_rays) return;
+  const int lane = lane_id();
+  const RaySpan sp = ray_span(r, n_samples, off);
+  const float* go = g_out + (int64_t)r * C;
+  for (int k = lane; k < sp.count; k += SRF_WAVE) {
+    const int64_t i = sp.begin + k;
+    if (g_w) {
+      float s = 0.f;
+      for (int c = 0; c < C; ++c) s += __fmul_rn(go[c], v[i * C + c]);
+      g_w[i] = s;
+    }
+    if (g_v) {
+      const float wi = w[i];
+      for (int c = 0; c < C; ++c) g_v[i * C + c] = __fmul_rn(wi, go[c]);
     }
   }
 }
@@ -311,7 +367,7 @@ extern "C" int srf_composite_fwd(int n_rays, int n_samples, const int32_t* ray_o
                                  float* weights, void* stream) {
   if (n_rays < 0 || n_samples < 0) return SRF_EINVAL;
   if (n_rays == 0) return SRF_OK;
-  if (!sigma || !rgb || !z || !rgb_map || !acc || !depth || !depth_var) return SRF_EINVAL;
+  if (!sigma || !z || !acc || !depth || !depth_var) return SRF_EINVAL;
   if (ndc && (!rays_o || !rays_d || !rays_d_ndc)) return SRF_EINVAL;
   if (!ndc && !rays_d) return SRF_EINVAL;
   const int blocks = (n_rays + kWavesPerBlock - 1) / kWavesPerBlock;
@@ -335,7 +391,7 @@ extern "C" int srf_composite_bwd(int n_rays, int n_samples, const int32_t* ray_o
   if (n_rays < 0 || n_samples < 0) return SRF_EINVAL;
   if (n_rays == 0) return SRF_OK;
   if (!ray_offsets && n_samples > kMaxChunks * SRF_WAVE) return SRF_EUNSUPPORTED;
-  if (!sigma || !rgb || !z || !acc || !depth || !d_sigma || !d_rgb) return SRF_EINVAL;
+  if (!sigma || !z || !acc || !depth || !d_sigma) return SRF_EINVAL;
   if (ndc && (!rays_o || !rays_d || !rays_d_ndc)) return SRF_EINVAL;
   if (!ndc && !rays_d) return SRF_EINVAL;
   // Ragged rays: the caller guarantees max count <= n_samples (the documented max).
@@ -346,5 +402,31 @@ extern "C" int srf_composite_bwd(int n_rays, int n_samples, const int32_t* ray_o
                      rays_d, rays_d_ndc, ndc, white_bkgd, distance_scale, acc, depth, depth_ndc,
                      g_rgb, g_acc, g_depth, g_depth_var, g_depth_ndc, g_depth_var_ndc, g_alpha,
                      g_visibility, g_weights, d_sigma, d_rgb);
+  return launch_status();
+}
+
+extern "C" int srf_volume_render_fwd(int n_rays, int n_samples, const int32_t* ray_offsets,
+                                     const float* weights, const float* values, int n_channels,
+                                     float* out, void* stream) {
+  if (n_rays < 0 || n_channels < 1 || (!ray_offsets && n_samples < 1)) return SRF_EINVAL;
+  if (n_rays == 0) return SRF_OK;
+  if (!weights || !values || !out) return SRF_EINVAL;
+  const int blocks = (n_rays + kWavesPerBlock - 1) / kWavesPerBlock;
+  hipLaunchKernelGGL(volume_render_fwd_kernel, dim3(blocks), dim3(kWavesPerBlock * SRF_WAVE), 0,
+                     as_stream(stream), n_rays, n_samples, ray_offsets, weights, values, n_channels, out);
+  return launch_status();
+}
+
+extern "C" int srf_volume_render_bwd(int n_rays, int n_samples, const int32_t* ray_offsets,
+                                     const float* weights, const float* values, int n_channels,
+                                     const float* g_out, float* g_weights, float* g_values,
+                                     void* stream) {
+  if (n_rays < 0 || n_channels < 1 || (!ray_offsets && n_samples < 1)) return SRF_EINVAL;
+  if (n_rays == 0) return SRF_OK;
+  if (!g_out || (g_weights && !values) || (g_values && !weights)) return SRF_EINVAL;
+  const int blocks = (n_rays + kWavesPerBlock - 1) / kWavesPerBlock;
+  hipLaunchKernelGGL(volume_render_bwd_kernel, dim3(blocks), dim3(kWavesPerBlock * SRF_WAVE), 0,
+                     as_stream(stream), n_rays, n_samples, ray_offsets, weights, values, n_channels, g_out,
+                     g_weights, g_values);
   return launch_status();
 }

@@ -172,6 +172,34 @@ __global__ __launch_bounds__(64) void fine_z_kernel(int n_rays, int n_coarse, in
   }
 }
 
+// Non-NDC TensoRF depths (SimpleTensoRF09.py:382-400): the AABB entry of each ray plus a
+// fixed step.  One thread per (ray, sample); the per-ray entry is recomputed per thread
+// (6 divisions) rather than staged, which keeps the kernel a single pass.
+struct Box {
+  float lo[3], hi[3];
+};
+
+__global__ __launch_bounds__(256) void aabb_kernel(int n_rays, int n_samples, const float* __restrict__ o,
+                                                   const float* __restrict__ d, Box box, float near,
+                                                   float far, float step, const float* __restrict__ u,
+                                                   float* __restrict__ z) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (int64_t)n_rays * n_samples) return;
+  const int r = (int)(idx / n_samples), k = (int)(idx % n_samples);
+  float tmin = -__builtin_inff();
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const float da = d[3 * r + a], oa = o[3 * r + a];
+    const float v = da == 0.f ? 1e-6f : da;
+    const float ra = __fdiv_rn(__fsub_rn(box.hi[a], oa), v);
+    const float rb = __fdiv_rn(__fsub_rn(box.lo[a], oa), v);
+    tmin = fmaxf(tmin, fminf(ra, rb));
+  }
+  tmin = fminf(fmaxf(tmin, near), far);
+  const float rng = u ? __fadd_rn((float)k, u[r]) : (float)k;
+  z[idx] = __fadd_rn(tmin, __fmul_rn(step, rng));
+}
+
 }  // namespace
 }  // namespace srf
 
@@ -213,5 +241,24 @@ extern "C" int srf_fine_z_vals(int n_rays, int n_coarse, int n_fine, const float
   if (!z_coarse || !weights_coarse || !z_merged) return SRF_EINVAL;
   hipLaunchKernelGGL(fine_z_kernel, dim3(n_rays), dim3(SRF_WAVE), 0, as_stream(stream), n_rays,
                      n_coarse, n_fine, z_coarse, weights_coarse, u, z_merged, z_fine);
+  return launch_status();
+}
+
+extern "C" int srf_aabb_samples(int n_rays, int n_samples, const float* rays_o, const float* rays_d,
+                                const float* bbox, float near, float far, float step, const float* u,
+                                float* z, void* stream) {
+  if (n_rays < 0 || n_samples < 1 || !bbox) return SRF_EINVAL;
+  if (n_rays == 0) return SRF_OK;
+  if (!rays_o || !rays_d || !z) return SRF_EINVAL;
+  Box b;
+  for (int a = 0; a < 3; ++a) {
+    b.lo[a] = bbox[a];
+    b.hi[a] = bbox[3 + a];
+  }
+  const int64_t total = (int64_t)n_rays * n_samples;
+  const int64_t blocks = (total + 255) / 256;
+  if (blocks > 0x7fffffff) return SRF_EUNSUPPORTED;
+  hipLaunchKernelGGL(aabb_kernel, dim3((unsigned)blocks), dim3(256), 0, as_stream(stream), n_rays, n_samples,
+                     rays_o, rays_d, b, near, far, step, u, z);
   return launch_status();
 }

@@ -1,0 +1,648 @@
+// TensoRF vector-matrix (VM) factor sampling on gfx950.
+//
+// Reference semantics: LowRankTensor.forward validity + normalize_points
+// (src/models/SimpleTensoRF09.py:701-765), AlphaGridMask.sample_alpha (:1342-1349),
+// VmDecomposedTensor.get_volume_density / get_color before the basis matrix (:1214-1263),
+// the density predictors (:671-679) and the backward of torch.nn.functional.grid_sample
+// (bilinear, align_corners=True, zero padding) that autograd runs for them.
+//
+// Layout.  The factors live texel-major in HBM: plane p is [H_p][W_p][c_p] and line p is
+// [L_p][c_p] (the reference's [1, c, H, W] / [1, c, L, 1] tensors viewed with the component
+// axis innermost).  A bilinear tap then reads c_p contiguous floats.
+//
+// Mapping.  The channels of the three planes are cut into 4-channel quads (c_p % 4 == 0) and
+// concatenated, Q quads in all.  A group of G lanes (G = next power of two >= Q) owns one
+// sample; lane j of the group handles quads j, j + G, ...  Every tap of a group is thus one
+// contiguous 16 B x (quads of that plane) read, appearance rows are written as coalesced
+// 16 B stores, and the density sum over channels is a butterfly inside the group.
+//
+// Arithmetic follows grid_sample's op order: source index ((x + 1) / 2) * (size - 1), corner
+// weights as products of the opposite distances, taps accumulated nw, ne, sw, se; the file is
+// compiled with -ffp-contract=off so no product is fused into a sum.
+#include <string.h>
+
+#include "common.h"
+
+namespace srf {
+namespace {
+
+constexpr int kMaxQuads = 64;  // at most 256 components over the three planes
+constexpr int kPlanes = 3;
+
+// matrix_axes = [[0,1],[0,2],[1,2]] and vector_axes = [2,1,0] (SimpleTensoRF09.py:1131-1132):
+// plane p is sampled at (x = pts[kA0[p]], y = pts[kA1[p]]), line p at y = pts[kV[p]].
+__device__ __forceinline__ int axis_a0(int p) { return p == 2 ? 1 : 0; }
+__device__ __forceinline__ int axis_a1(int p) { return p == 0 ? 1 : 2; }
+__device__ __forceinline__ int axis_v(int p) { return 2 - p; }
+
+struct VmArgs {
+  const float* plane[kPlanes];
+  const float* line[kPlanes];
+  float* dplane[kPlanes];
+  float* dline[kPlanes];
+  int ph[kPlanes], pw[kPlanes], ll[kPlanes], comp[kPlanes], qoff[kPlanes];
+  int n_quads, total_comp;
+  float lo[3], hi[3], size[3];
+};
+
+struct AlphaArgs {
+  const float* vol;  // [D][H][W] or null
+  int d, h, w;
+  float lo[3], hi[3], size[3];
+};
+
+// Normalised coordinate of pts in the box: ((p - lo) / size) * 2 - 1.
+__device__ __forceinline__ float normalize_coord(float p, float lo, float size) {
+  return __fsub_rn(__fmul_rn(__fdiv_rn(__fsub_rn(p, lo), size), 2.0f), 1.0f);
+}
+
+// grid_sample's align_corners=True unnormalisation.
+__device__ __forceinline__ float source_index(float x, int size) {
+  return __fmul_rn(__fmul_rn(__fadd_rn(x, 1.0f), 0.5f), (float)(size - 1));
+}
+
+// The four bilinear corners of one plane sample; idx < 0 marks a corner outside the plane.
+struct PlaneTaps {
+  int idx[4];
+  float w[4];
+};
+
+__device__ __forceinline__ PlaneTaps plane_taps(float x, float y, int W, int H) {
+  const float ix = source_index(x, W), iy = source_index(y, H);
+  const float x0 = floorf(ix), y0 = floorf(iy);
+  const float x1 = __fadd_rn(x0, 1.0f), y1 = __fadd_rn(y0, 1.0f);
+  PlaneTaps t;
+  t.w[0] = __fmul_rn(__fsub_rn(x1, ix), __fsub_rn(y1, iy));  // nw
+  t.w[1] = __fmul_rn(__fsub_rn(ix, x0), __fsub_rn(y1, iy));  // ne
+  t.w[2] = __fmul_rn(__fsub_rn(x1, ix), __fsub_rn(iy, y0));  // sw
+  t.w[3] = __fmul_rn(__fsub_rn(ix, x0), __fsub_rn(iy, y0));  // se
+  const int xi = (int)x0, yi = (int)y0;
+  const bool xin0 = xi >= 0 && xi < W, xin1 = xi + 1 >= 0 && xi + 1 < W;
+  const bool yin0 = yi >= 0 && yi < H, yin1 = yi + 1 >= 0 && yi + 1 < H;
+  t.idx[0] = (xin0 && yin0) ? yi * W + xi : -1;
+  t.idx[1] = (xin1 && yin0) ? yi * W + xi + 1 : -1;
+  t.idx[2] = (xin0 && yin1) ? (yi + 1) * W + xi : -1;
+  t.idx[3] = (xin1 && yin1) ? (yi + 1) * W + xi + 1 : -1;
+  return t;
+}
+
+// Line sample: grid_sample on [1, c, L, 1] at (x = 0, y): the x weight is exactly 1 and the
+// x + 1 column is outside, so only the two y corners remain.
+struct LineTaps {
+  int idx[2];
+  float w[2];
+};
+
+__device__ __forceinline__ LineTaps line_taps(float y, int L) {
+  const float iy = source_index(y, L);
+  const float y0 = floorf(iy);
+  const float y1 = __fadd_rn(y0, 1.0f);
+  LineTaps t;
+  t.w[0] = __fsub_rn(y1, iy);
+  t.w[1] = __fsub_rn(iy, y0);
+  const int yi = (int)y0;
+  t.idx[0] = (yi >= 0 && yi < L) ? yi : -1;
+  t.idx[1] = (yi + 1 >= 0 && yi + 1 < L) ? yi + 1 : -1;
+  return t;
+}
+
+__device__ __forceinline__ float4 ld4(const float* p) { return *reinterpret_cast<const float4*>(p); }
+
+__device__ __forceinline__ float4 madd4(float4 acc, float4 v, float w) {
+  acc.x = __fadd_rn(acc.x, __fmul_rn(v.x, w));
+  acc.y = __fadd_rn(acc.y, __fmul_rn(v.y, w));
+  acc.z = __fadd_rn(acc.z, __fmul_rn(v.z, w));
+  acc.w = __fadd_rn(acc.w, __fmul_rn(v.w, w));
+  return acc;
+}
+
+__device__ __forceinline__ float4 mul4(float4 a, float4 b) {
+  return make_float4(__fmul_rn(a.x, b.x), __fmul_rn(a.y, b.y), __fmul_rn(a.z, b.z), __fmul_rn(a.w, b.w));
+}
+
+__device__ __forceinline__ float sum4(float4 v) {
+  return __fadd_rn(__fadd_rn(__fadd_rn(v.x, v.y), v.z), v.w);
+}
+
+// Plane index of quad q given the quad offsets of planes 1 and 2 (wave-uniform scalars).
+__device__ __forceinline__ int quad_plane(const VmArgs& a, int q) {
+  return (q >= a.qoff[1]) + (q >= a.qoff[2]);
+}
+
+template <typename T>
+__device__ __forceinline__ T sel3(int p, T v0, T v1, T v2) {
+  return p == 0 ? v0 : (p == 1 ? v1 : v2);
+}
+
+// Everything one lane needs to read or scatter its quad of one sample.
+struct QuadSite {
+  int p, c;  // plane, first component within the plane
+  PlaneTaps pt;
+  LineTaps lt;
+};
+
+__device__ __forceinline__ QuadSite quad_site(const VmArgs& a, int q, const float pn[3]) {
+  QuadSite s;
+  s.p = quad_plane(a, q);
+  s.c = 4 * (q - sel3(s.p, a.qoff[0], a.qoff[1], a.qoff[2]));
+  const float x = sel3(s.p, pn[axis_a0(0)], pn[axis_a0(1)], pn[axis_a0(2)]);
+  const float y = sel3(s.p, pn[axis_a1(0)], pn[axis_a1(1)], pn[axis_a1(2)]);
+  const float v = sel3(s.p, pn[axis_v(0)], pn[axis_v(1)], pn[axis_v(2)]);
+  s.pt = plane_taps(x, y, sel3(s.p, a.pw[0], a.pw[1], a.pw[2]), sel3(s.p, a.ph[0], a.ph[1], a.ph[2]));
+  s.lt = line_taps(v, sel3(s.p, a.ll[0], a.ll[1], a.ll[2]));
+  return s;
+}
+
+// Bilinear plane value and linear line value of the 4 components of a quad.
+__device__ __forceinline__ void quad_values(const VmArgs& a, const QuadSite& s, float4& pv, float4& lv) {
+  const int C = sel3(s.p, a.comp[0], a.comp[1], a.comp[2]);
+  const float* P = sel3(s.p, a.plane[0], a.plane[1], a.plane[2]);
+  const float* L = sel3(s.p, a.line[0], a.line[1], a.line[2]);
+  pv = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+  for (int t = 0; t < 4; ++t)
+    if (s.pt.idx[t] >= 0) pv = madd4(pv, ld4(P + (int64_t)s.pt.idx[t] * C + s.c), s.pt.w[t]);
+  lv = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+  for (int t = 0; t < 2; ++t)
+    if (s.lt.idx[t] >= 0) lv = madd4(lv, ld4(L + (int64_t)s.lt.idx[t] * C + s.c), s.lt.w[t]);
+}
+
+__device__ __forceinline__ void atomic_add4(float* p, float4 g, float w) {
+  unsafeAtomicAdd(p + 0, __fmul_rn(w, g.x));
+  unsafeAtomicAdd(p + 1, __fmul_rn(w, g.y));
+  unsafeAtomicAdd(p + 2, __fmul_rn(w, g.z));
+  unsafeAtomicAdd(p + 3, __fmul_rn(w, g.w));
+}
+
+// Scatter of the plane gradient gp and line gradient gl of one quad into the factor grads:
+// each corner receives (corner weight) x (gradient), as grid_sample's backward does.
+__device__ __forceinline__ void quad_scatter(const VmArgs& a, const QuadSite& s, float4 gp, float4 gl) {
+  const int C = sel3(s.p, a.comp[0], a.comp[1], a.comp[2]);
+  float* dP = sel3(s.p, a.dplane[0], a.dplane[1], a.dplane[2]);
+  float* dL = sel3(s.p, a.dline[0], a.dline[1], a.dline[2]);
+#pragma unroll
+  for (int t = 0; t < 4; ++t)
+    if (s.pt.idx[t] >= 0) atomic_add4(dP + (int64_t)s.pt.idx[t] * C + s.c, gp, s.pt.w[t]);
+#pragma unroll
+  for (int t = 0; t < 2; ++t)
+    if (s.lt.idx[t] >= 0) atomic_add4(dL + (int64_t)s.lt.idx[t] * C + s.c, gl, s.lt.w[t]);
+}
+
+// Trilinear grid_sample of the alpha volume (align_corners=True, zero padding); corners
+// accumulated z0 (y0x0, y0x1, y1x0, y1x1) then z1, weights ((wx * wy) * wz).
+__device__ float alpha_value(const AlphaArgs& m, const float p[3]) {
+  float n[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) n[k] = normalize_coord(p[k], m.lo[k], m.size[k]);
+  const float ix = source_index(n[0], m.w), iy = source_index(n[1], m.h), iz = source_index(n[2], m.d);
+  const float x0 = floorf(ix), y0 = floorf(iy), z0 = floorf(iz);
+  const float x1 = __fadd_rn(x0, 1.f), y1 = __fadd_rn(y0, 1.f), z1 = __fadd_rn(z0, 1.f);
+  const float wx[2] = {__fsub_rn(x1, ix), __fsub_rn(ix, x0)};
+  const float wy[2] = {__fsub_rn(y1, iy), __fsub_rn(iy, y0)};
+  const float wz[2] = {__fsub_rn(z1, iz), __fsub_rn(iz, z0)};
+  const int xi = (int)x0, yi = (int)y0, zi = (int)z0;
+  float out = 0.f;
+#pragma unroll
+  for (int dz = 0; dz < 2; ++dz)
+#pragma unroll
+    for (int dy = 0; dy < 2; ++dy)
+#pragma unroll
+      for (int dx = 0; dx < 2; ++dx) {
+        const int X = xi + dx, Y = yi + dy, Z = zi + dz;
+        if (X >= 0 && X < m.w && Y >= 0 && Y < m.h && Z >= 0 && Z < m.d) {
+          const float w = __fmul_rn(__fmul_rn(wx[dx], wy[dy]), wz[dz]);
+          out = __fadd_rn(out, __fmul_rn(m.vol[((int64_t)Z * m.h + Y) * m.w + X], w));
+        }
+      }
+  return out;
+}
+
+__device__ __forceinline__ bool in_box(const float p[3], const float lo[3], const float hi[3]) {
+  return lo[0] <= p[0] && p[0] <= hi[0] && lo[1] <= p[1] && p[1] <= hi[1] && lo[2] <= p[2] && p[2] <= hi[2];
+}
+
+template <int G>
+__device__ __forceinline__ float group_sum(float v) {
+#pragma unroll
+  for (int d = G / 2; d >= 1; d >>= 1) v = __fadd_rn(v, __shfl_xor(v, d, SRF_WAVE));
+  return v;
+}
+
+// Density predictors (SimpleTensoRF09.py:671-679): ReLU, or softplus(x + offset) with
+// torch's default beta = 1 and threshold = 20.
+__device__ __forceinline__ float predict_density(float f, int kind, float offset) {
+  if (kind == SRF_DENSITY_RELU) return f > 0.f ? f : 0.f;
+  const float x = __fadd_rn(f, offset);
+  return x > 20.f ? x : log1pf(expf(x));
+}
+
+__device__ __forceinline__ float predict_density_grad(float f, int kind, float offset) {
+  if (kind == SRF_DENSITY_RELU) return f > 0.f ? 1.f : 0.f;
+  const float x = __fadd_rn(f, offset);
+  if (x > 20.f) return 1.f;
+  const float e = expf(x);
+  return e / __fadd_rn(e, 1.f);
+}
+
+struct GroupPos {
+  int64_t first, stride;  // first sample slot of this lane's group, stride between passes
+  int ql;                 // lane within the group
+};
+
+template <int G>
+__device__ __forceinline__ GroupPos group_pos() {
+  const int lane = threadIdx.x & (SRF_WAVE - 1);
+  const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / SRF_WAVE;
+  const int64_t n_waves = (int64_t)gridDim.x * blockDim.x / SRF_WAVE;
+  GroupPos g;
+  g.first = wave * (SRF_WAVE / G) + lane / G;
+  g.stride = n_waves * (SRF_WAVE / G);
+  g.ql = lane % G;
+  return g;
+}
+
+// --------------------------------------------------------------------------- kernels
+
+template <int G>
+__global__ __launch_bounds__(256) void density_fwd_kernel(VmArgs a, AlphaArgs m, int64_t n,
+                                                          const float* __restrict__ pts, int kind,
+                                                          float offset, float* __restrict__ sigma,
+                                                          float* __restrict__ feat_out,
+                                                          uint8_t* __restrict__ valid_out) {
+  const GroupPos gp = group_pos<G>();
+  // all lanes of a group share i, so the group stays converged for its butterfly
+  for (int64_t i = gp.first; i < n; i += gp.stride) {
+    const float p[3] = {pts[3 * i + 0], pts[3 * i + 1], pts[3 * i + 2]};
+    bool valid = in_box(p, a.lo, a.hi);
+    if (valid && m.vol) valid = alpha_value(m, p) > 0.f;
+    float pn[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) pn[k] = normalize_coord(p[k], a.lo[k], a.size[k]);
+    float s[kPlanes] = {0.f, 0.f, 0.f};
+    if (valid) {
+      for (int q = gp.ql; q < a.n_quads; q += G) {
+        const QuadSite site = quad_site(a, q, pn);
+        float4 pv, lv;
+        quad_values(a, site, pv, lv);
+        const float d = sum4(mul4(pv, lv));
+        s[0] = site.p == 0 ? __fadd_rn(s[0], d) : s[0];
+        s[1] = site.p == 1 ? __fadd_rn(s[1], d) : s[1];
+        s[2] = site.p == 2 ? __fadd_rn(s[2], d) : s[2];
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < kPlanes; ++k) s[k] = group_sum<G>(s[k]);
+    if (gp.ql == 0) {
+      const float f = __fadd_rn(__fadd_rn(s[0], s[1]), s[2]);
+      sigma[i] = valid ? predict_density(f, kind, offset) : 0.f;
+      if (feat_out) feat_out[i] = valid ? f : 0.f;
+      if (valid_out) valid_out[i] = valid ? 1 : 0;
+    }
+  }
+}
+
+// d_feat = d_sigma * predictor'(feat) on valid samples; planes receive d_feat * line and
+// lines d_feat * plane, per component and corner.
+template <int G>
+__global__ __launch_bounds__(256) void density_bwd_kernel(VmArgs a, int64_t n, const float* __restrict__ pts,
+                                                          const uint8_t* __restrict__ valid,
+                                                          const float* __restrict__ feat, int kind,
+                                                          float offset, const float* __restrict__ d_sigma) {
+  const GroupPos gp = group_pos<G>();
+  for (int64_t i = gp.first; i < n; i += gp.stride) {
+    if (!valid[i]) continue;
+    const float g = __fmul_rn(d_sigma[i], predict_density_grad(feat[i], kind, offset));
+    if (g == 0.f) continue;
+    float pn[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) pn[k] = normalize_coord(pts[3 * i + k], a.lo[k], a.size[k]);
+    for (int q = gp.ql; q < a.n_quads; q += G) {
+      const QuadSite site = quad_site(a, q, pn);
+      float4 pv, lv;
+      quad_values(a, site, pv, lv);
+      const float4 gg = make_float4(g, g, g, g);
+      quad_scatter(a, site, mul4(gg, lv), mul4(gg, pv));
+    }
+  }
+}
+
+// Appearance features of the listed samples: out[j, c] = plane_c * line_c for the sample
+// idx[j] (or j when idx is null), components concatenated over the planes.
+template <int G>
+__global__ __launch_bounds__(256) void appearance_fwd_kernel(VmArgs a, int64_t n, const int32_t* __restrict__ idx,
+                                                             const float* __restrict__ pts,
+                                                             float* __restrict__ out) {
+  const GroupPos gp = group_pos<G>();
+  for (int64_t j = gp.first; j < n; j += gp.stride) {
+    const int64_t i = idx ? (int64_t)idx[j] : j;
+    float pn[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) pn[k] = normalize_coord(pts[3 * i + k], a.lo[k], a.size[k]);
+    for (int q = gp.ql; q < a.n_quads; q += G) {
+      const QuadSite site = quad_site(a, q, pn);
+      float4 pv, lv;
+      quad_values(a, site, pv, lv);
+      *reinterpret_cast<float4*>(out + j * a.total_comp + 4 * q) = mul4(pv, lv);
+    }
+  }
+}
+
+template <int G>
+__global__ __launch_bounds__(256) void appearance_bwd_kernel(VmArgs a, int64_t n, const int32_t* __restrict__ idx,
+                                                             const float* __restrict__ pts,
+                                                             const float* __restrict__ d_out) {
+  const GroupPos gp = group_pos<G>();
+  for (int64_t j = gp.first; j < n; j += gp.stride) {
+    const int64_t i = idx ? (int64_t)idx[j] : j;
+    float pn[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) pn[k] = normalize_coord(pts[3 * i + k], a.lo[k], a.size[k]);
+    for (int q = gp.ql; q < a.n_quads; q += G) {
+      const float4 g = ld4(d_out + j * a.total_comp + 4 * q);
+      const QuadSite site = quad_site(a, q, pn);
+      float4 pv, lv;
+      quad_values(a, site, pv, lv);
+      quad_scatter(a, site, mul4(g, lv), mul4(g, pv));
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void alpha_sample_kernel(AlphaArgs m, int64_t n, const float* __restrict__ pts,
+                                                           float* __restrict__ out) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const float p[3] = {pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]};
+    out[i] = alpha_value(m, p);
+  }
+}
+
+// ------------------------------------------------------------------- threshold compaction
+// Three stream-ordered passes, no host synchronisation: per-tile counts, one exclusive scan
+// of the tile counts (which also writes the total), then an ordered scatter with wave
+// ballots.  Tiles are kCompactTile consecutive entries; indices come out ascending, as
+// boolean-mask indexing returns them.
+constexpr int kCompactTile = 1024;
+
+__global__ __launch_bounds__(256) void compact_count_kernel(int64_t n, const float* __restrict__ v, float thr,
+                                                            int32_t* __restrict__ tile_count) {
+  __shared__ int s_count;
+  if (threadIdx.x == 0) s_count = 0;
+  __syncthreads();
+  const int64_t base = (int64_t)blockIdx.x * kCompactTile;
+  int c = 0;
+#pragma unroll
+  for (int it = 0; it < kCompactTile / 256; ++it) {
+    const int64_t i = base + it * 256 + threadIdx.x;
+    const bool f = i < n && v[i] > thr;
+    c += __popcll(__ballot(f));
+  }
+  if ((threadIdx.x & (SRF_WAVE - 1)) == 0) atomicAdd(&s_count, c);
+  __syncthreads();
+  if (threadIdx.x == 0) tile_count[blockIdx.x] = s_count;
+}
+
+// One block: tile counts -> exclusive tile offsets (in place); *count = total.
+__global__ __launch_bounds__(1024) void compact_scan_kernel(int64_t n_tiles, int32_t* __restrict__ tile,
+                                                            int32_t* __restrict__ count) {
+  __shared__ int32_t part[1024];
+  const int t = threadIdx.x;
+  const int64_t per = (n_tiles + 1023) / 1024;
+  const int64_t b0 = t * per;
+  const int64_t b1 = b0 + per < n_tiles ? b0 + per : n_tiles;
+  int32_t s = 0;
+  for (int64_t b = b0; b < b1; ++b) s += tile[b];
+  part[t] = s;
+  __syncthreads();
+  for (int d = 1; d < 1024; d <<= 1) {
+    const int32_t o = t >= d ? part[t - d] : 0;
+    __syncthreads();
+    part[t] += o;
+    __syncthreads();
+  }
+  int32_t run = t > 0 ? part[t - 1] : 0;
+  for (int64_t b = b0; b < b1; ++b) {
+    const int32_t c = tile[b];
+    tile[b] = run;
+    run += c;
+  }
+  if (t == 1023) *count = part[1023];
+}
+
+__global__ __launch_bounds__(256) void compact_scatter_kernel(int64_t n, const float* __restrict__ v, float thr,
+                                                              const int32_t* __restrict__ tile_off,
+                                                              int32_t* __restrict__ idx) {
+  __shared__ int s_wave[4];
+  const int w = threadIdx.x / SRF_WAVE, lane = threadIdx.x & (SRF_WAVE - 1);
+  int out = tile_off[blockIdx.x];
+  const int64_t base = (int64_t)blockIdx.x * kCompactTile;
+  for (int it = 0; it < kCompactTile / 256; ++it) {
+    const int64_t i = base + it * 256 + threadIdx.x;
+    const bool f = i < n && v[i] > thr;
+    const uint64_t m = __ballot(f);
+    const int within = __popcll(m & ((1ull << lane) - 1ull));
+    if (lane == 0) s_wave[w] = __popcll(m);
+    __syncthreads();
+    int off = out;
+    for (int j = 0; j < w; ++j) off += s_wave[j];
+    if (f) idx[off + within] = (int32_t)i;
+    out += s_wave[0] + s_wave[1] + s_wave[2] + s_wave[3];
+    __syncthreads();
+  }
+}
+
+// ------------------------------------------------------------------------------ host side
+
+int next_pow2(int q) {
+  int g = 1;
+  while (g < q) g <<= 1;
+  return g;
+}
+
+void set_box(float lo[3], float hi[3], float size[3], const float* box) {
+  for (int k = 0; k < 3; ++k) {
+    lo[k] = box[k];
+    hi[k] = box[3 + k];
+    size[k] = box[3 + k] - box[k];  // float subtraction, as the reference's bounding_box_size
+  }
+}
+
+int make_vm_args(const srf_vm_factors* f, const float* bbox, VmArgs* a, bool need_values) {
+  if (!f || !bbox) return SRF_EINVAL;
+  memset(a, 0, sizeof(*a));
+  int q = 0;
+  for (int p = 0; p < kPlanes; ++p) {
+    if (f->n_comp[p] < 0 || f->n_comp[p] % 4 != 0 || f->plane_h[p] < 1 || f->plane_w[p] < 1 ||
+        f->line_len[p] < 1)
+      return SRF_EINVAL;
+    if (need_values && f->n_comp[p] > 0 && (!f->planes[p] || !f->lines[p])) return SRF_EINVAL;
+    // taps are read as float4: the factor bases must be 16-byte aligned
+    if ((reinterpret_cast<uintptr_t>(f->planes[p]) | reinterpret_cast<uintptr_t>(f->lines[p])) & 15)
+      return SRF_EINVAL;
+    a->plane[p] = f->planes[p];
+    a->line[p] = f->lines[p];
+    a->ph[p] = f->plane_h[p];
+    a->pw[p] = f->plane_w[p];
+    a->ll[p] = f->line_len[p];
+    a->comp[p] = f->n_comp[p];
+    a->qoff[p] = q;
+    q += f->n_comp[p] / 4;
+  }
+  if (q < 1) return SRF_EINVAL;
+  if (q > kMaxQuads) return SRF_EUNSUPPORTED;
+  a->n_quads = q;
+  a->total_comp = 4 * q;
+  set_box(a->lo, a->hi, a->size, bbox);
+  return SRF_OK;
+}
+
+int make_alpha_args(const srf_alpha_grid* g, AlphaArgs* m) {
+  memset(m, 0, sizeof(*m));
+  if (!g || !g->volume) return SRF_OK;
+  if (g->depth < 1 || g->height < 1 || g->width < 1) return SRF_EINVAL;
+  m->vol = g->volume;
+  m->d = g->depth;
+  m->h = g->height;
+  m->w = g->width;
+  set_box(m->lo, m->hi, m->size, g->bbox);
+  return SRF_OK;
+}
+
+unsigned grid_for(int64_t n, int G) {
+  const int64_t per_block = (int64_t)(256 / SRF_WAVE) * (SRF_WAVE / G);
+  int64_t blocks = (n + per_block - 1) / per_block;
+  // 256 CUs x 8 resident blocks of 256 threads: beyond that the grid-stride loop takes over
+  const int64_t cap = 256 * 8 * 4;
+  if (blocks > cap) blocks = cap;
+  return (unsigned)(blocks < 1 ? 1 : blocks);
+}
+
+#define SRF_VM_DISPATCH(G_, KERNEL, ...)                                                   \
+  switch (G_) {                                                                           \
+    case 1: hipLaunchKernelGGL(KERNEL<1>, __VA_ARGS__); break;                            \
+    case 2: hipLaunchKernelGGL(KERNEL<2>, __VA_ARGS__); break;                            \
+    case 4: hipLaunchKernelGGL(KERNEL<4>, __VA_ARGS__); break;                            \
+    case 8: hipLaunchKernelGGL(KERNEL<8>, __VA_ARGS__); break;                            \
+    case 16: hipLaunchKernelGGL(KERNEL<16>, __VA_ARGS__); break;                          \
+    case 32: hipLaunchKernelGGL(KERNEL<32>, __VA_ARGS__); break;                          \
+    default: hipLaunchKernelGGL(KERNEL<64>, __VA_ARGS__); break;                          \
+  }
+
+}  // namespace
+}  // namespace srf
+
+using namespace srf;
+
+extern "C" int srf_vm_density_fwd(const srf_vm_factors* factors, const float* bbox,
+                                  const srf_alpha_grid* alpha, int64_t n_points, const float* pts,
+                                  int predictor, float offset, float* sigma, float* feat,
+                                  uint8_t* valid, void* stream) {
+  if (n_points < 0 || (predictor != SRF_DENSITY_RELU && predictor != SRF_DENSITY_SOFTPLUS)) return SRF_EINVAL;
+  if (n_points == 0) return SRF_OK;
+  VmArgs a;
+  int st = make_vm_args(factors, bbox, &a, true);
+  if (st != SRF_OK) return st;
+  AlphaArgs m;
+  if ((st = make_alpha_args(alpha, &m)) != SRF_OK) return st;
+  if (!pts || !sigma) return SRF_EINVAL;
+  const int G = next_pow2(a.n_quads);
+  SRF_VM_DISPATCH(G, density_fwd_kernel, dim3(grid_for(n_points, G)), dim3(256), 0, as_stream(stream), a, m,
+                  n_points, pts, predictor, offset, sigma, feat, valid);
+  return launch_status();
+}
+
+extern "C" int srf_vm_density_bwd(const srf_vm_factors* factors, float* const* d_planes,
+                                  float* const* d_lines, const float* bbox, int64_t n_points,
+                                  const float* pts, const uint8_t* valid, const float* feat,
+                                  int predictor, float offset, const float* d_sigma, void* stream) {
+  if (n_points < 0 || (predictor != SRF_DENSITY_RELU && predictor != SRF_DENSITY_SOFTPLUS)) return SRF_EINVAL;
+  if (n_points == 0) return SRF_OK;
+  VmArgs a;
+  int st = make_vm_args(factors, bbox, &a, true);
+  if (st != SRF_OK) return st;
+  if (!d_planes || !d_lines || !pts || !valid || !feat || !d_sigma) return SRF_EINVAL;
+  for (int p = 0; p < kPlanes; ++p) {
+    if (a.comp[p] > 0 && (!d_planes[p] || !d_lines[p])) return SRF_EINVAL;
+    a.dplane[p] = d_planes[p];
+    a.dline[p] = d_lines[p];
+  }
+  const int G = next_pow2(a.n_quads);
+  SRF_VM_DISPATCH(G, density_bwd_kernel, dim3(grid_for(n_points, G)), dim3(256), 0, as_stream(stream), a,
+                  n_points, pts, valid, feat, predictor, offset, d_sigma);
+  return launch_status();
+}
+
+extern "C" int srf_vm_appearance_fwd(const srf_vm_factors* factors, const float* bbox, int64_t n_rows,
+                                     const int32_t* idx, const float* pts, float* features,
+                                     void* stream) {
+  if (n_rows < 0) return SRF_EINVAL;
+  if (n_rows == 0) return SRF_OK;
+  VmArgs a;
+  int st = make_vm_args(factors, bbox, &a, true);
+  if (st != SRF_OK) return st;
+  if (!pts || !features) return SRF_EINVAL;
+  const int G = next_pow2(a.n_quads);
+  SRF_VM_DISPATCH(G, appearance_fwd_kernel, dim3(grid_for(n_rows, G)), dim3(256), 0, as_stream(stream), a,
+                  n_rows, idx, pts, features);
+  return launch_status();
+}
+
+extern "C" int srf_vm_appearance_bwd(const srf_vm_factors* factors, float* const* d_planes,
+                                     float* const* d_lines, const float* bbox, int64_t n_rows,
+                                     const int32_t* idx, const float* pts, const float* d_features,
+                                     void* stream) {
+  if (n_rows < 0) return SRF_EINVAL;
+  if (n_rows == 0) return SRF_OK;
+  VmArgs a;
+  int st = make_vm_args(factors, bbox, &a, true);
+  if (st != SRF_OK) return st;
+  if (!d_planes || !d_lines || !pts || !d_features) return SRF_EINVAL;
+  for (int p = 0; p < kPlanes; ++p) {
+    if (a.comp[p] > 0 && (!d_planes[p] || !d_lines[p])) return SRF_EINVAL;
+    a.dplane[p] = d_planes[p];
+    a.dline[p] = d_lines[p];
+  }
+  const int G = next_pow2(a.n_quads);
+  SRF_VM_DISPATCH(G, appearance_bwd_kernel, dim3(grid_for(n_rows, G)), dim3(256), 0, as_stream(stream), a,
+                  n_rows, idx, pts, d_features);
+  return launch_status();
+}
+
+extern "C" int srf_alpha_mask_sample(const srf_alpha_grid* alpha, int64_t n_points, const float* pts,
+                                     float* out, void* stream) {
+  if (n_points < 0 || !alpha || !alpha->volume) return SRF_EINVAL;
+  if (n_points == 0) return SRF_OK;
+  AlphaArgs m;
+  int st = make_alpha_args(alpha, &m);
+  if (st != SRF_OK) return st;
+  if (!pts || !out) return SRF_EINVAL;
+  int64_t blocks = (n_points + 255) / 256;
+  if (blocks > 8192) blocks = 8192;
+  hipLaunchKernelGGL(alpha_sample_kernel, dim3((unsigned)blocks), dim3(256), 0, as_stream(stream), m, n_points,
+                     pts, out);
+  return launch_status();
+}
+
+extern "C" size_t srf_compact_workspace_bytes(int64_t n) {
+  if (n < 0) return 0;
+  const int64_t tiles = (n + kCompactTile - 1) / kCompactTile;
+  return (size_t)((tiles * 4 + 255) / 256 * 256 + 256);
+}
+
+extern "C" int srf_threshold_compact(int64_t n, const float* values, float threshold, int32_t* idx,
+                                     int32_t* count, void* workspace, void* stream) {
+  if (n < 0 || n > 0x7fffffffLL || !count) return SRF_EINVAL;
+  if (n > 0 && (!values || !idx || !workspace)) return SRF_EINVAL;
+  const int64_t tiles = (n + kCompactTile - 1) / kCompactTile;
+  int32_t* tile = reinterpret_cast<int32_t*>(workspace);
+  if (tiles > 0) {
+    hipLaunchKernelGGL(compact_count_kernel, dim3((unsigned)tiles), dim3(256), 0, as_stream(stream), n, values,
+                       threshold, tile);
+  }
+  // the scan also writes *count (0 for an empty input), so no memset is needed
+  hipLaunchKernelGGL(compact_scan_kernel, dim3(1), dim3(1024), 0, as_stream(stream), tiles, tile, count);
+  if (tiles > 0) {
+    hipLaunchKernelGGL(compact_scatter_kernel, dim3((unsigned)tiles), dim3(256), 0, as_stream(stream), n, values,
+                       threshold, tile, idx);
+  }
+  return launch_status();
+}

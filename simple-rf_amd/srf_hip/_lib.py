@@ -15,7 +15,20 @@ ABI_VERSION = 1
 
 _P = C.c_void_p
 _I = C.c_int
+_L = C.c_int64
 _F = C.c_float
+
+
+class VmFactors(C.Structure):
+    """struct srf_vm_factors."""
+    _fields_ = [('n_comp', C.c_int * 3), ('plane_h', C.c_int * 3), ('plane_w', C.c_int * 3),
+                ('line_len', C.c_int * 3), ('planes', C.c_void_p * 3), ('lines', C.c_void_p * 3)]
+
+
+class AlphaGrid(C.Structure):
+    """struct srf_alpha_grid."""
+    _fields_ = [('volume', C.c_void_p), ('depth', C.c_int), ('height', C.c_int), ('width', C.c_int),
+                ('bbox', C.c_float * 6)]
 
 # name -> argtypes (restype is int unless listed in _RESTYPES)
 _PROTOS = {
@@ -39,9 +52,19 @@ _PROTOS = {
     'srf_mlp_backward': [_P, _P, _I, _P, _P, _P, _P, _P, _P, _P, _P],
     'srf_mlp_backward_data': [_P, _P, _I, _P, _P, _P, _P, _P, _P],
     'srf_mlp_weight_grads': [_P, _I, _P, _P, _P, _P],
+    'srf_vm_density_fwd': [_P, _P, _P, _L, _P, _I, _F, _P, _P, _P, _P],
+    'srf_vm_density_bwd': [_P, _P, _P, _P, _L, _P, _P, _P, _I, _F, _P, _P],
+    'srf_vm_appearance_fwd': [_P, _P, _L, _P, _P, _P, _P],
+    'srf_vm_appearance_bwd': [_P, _P, _P, _P, _L, _P, _P, _P, _P],
+    'srf_alpha_mask_sample': [_P, _L, _P, _P, _P],
+    'srf_compact_workspace_bytes': [_L],
+    'srf_threshold_compact': [_L, _P, _F, _P, _P, _P, _P],
+    'srf_volume_render_fwd': [_I, _I, _P, _P, _P, _I, _P, _P],
+    'srf_volume_render_bwd': [_I, _I, _P, _P, _P, _I, _P, _P, _P, _P],
+    'srf_aabb_samples': [_I, _I, _P, _P, _P, _F, _F, _F, _P, _P, _P],
 }
 _RESTYPES = {'srf_status_string': C.c_char_p, 'srf_mlp_packed_bytes': C.c_size_t,
-             'srf_mlp_workspace_bytes': C.c_size_t}
+             'srf_mlp_workspace_bytes': C.c_size_t, 'srf_compact_workspace_bytes': C.c_size_t}
 
 _lib = None
 

@@ -22,7 +22,7 @@ class _Composite(torch.autograd.Function):
         P = sigma.shape[0]
         dev = sigma.device
         f = dict(dtype=torch.float32, device=dev)
-        rgb_map = torch.empty((n_rays, 3), **f)
+        rgb_map = torch.empty((n_rays, 3), **f) if rgb is not None else torch.zeros((n_rays, 3), **f)
         acc, depth, depth_var = (torch.empty((n_rays,), **f) for _ in range(3))
         depth_ndc = torch.empty((n_rays,), **f) if ndc else None
         depth_var_ndc = torch.empty((n_rays,), **f) if ndc else None
@@ -34,6 +34,8 @@ class _Composite(torch.autograd.Function):
                   _lib.stream_ptr(dev))
         ctx.save_for_backward(sigma, rgb, z, rays_o, rays_d, rays_d_ndc, ray_offsets, acc, depth, depth_ndc)
         ctx.cfg = (n_rays, n_samples, ndc, white_bkgd, distance_scale)
+        if rgb is None:
+            ctx.mark_non_differentiable(rgb_map)
         if not ndc:
             depth_ndc = torch.zeros((n_rays,), **f)
             depth_var_ndc = torch.zeros((n_rays,), **f)
@@ -46,7 +48,7 @@ class _Composite(torch.autograd.Function):
         n_rays, n_samples, ndc, white_bkgd, distance_scale = ctx.cfg
         c = lambda t: None if t is None else t.contiguous()
         d_sigma = torch.empty_like(sigma)
-        d_rgb = torch.empty_like(rgb)
+        d_rgb = torch.empty_like(rgb) if rgb is not None and ctx.needs_input_grad[1] else None
         _lib.call('srf_composite_bwd', n_rays, n_samples, _lib.ptr(ray_offsets), _lib.ptr(sigma), _lib.ptr(rgb),
                   _lib.ptr(z), _lib.ptr(rays_o), _lib.ptr(rays_d), _lib.ptr(rays_d_ndc), int(ndc), int(white_bkgd),
                   float(distance_scale), _lib.ptr(acc), _lib.ptr(depth), _lib.ptr(depth_ndc),
@@ -62,7 +64,8 @@ def composite(sigma: torch.Tensor, rgb: torch.Tensor, z: torch.Tensor, rays_o: t
               distance_scale: float = 1.0, ray_offsets: torch.Tensor | None = None, max_samples: int | None = None):
     """Volume rendering of dense [N,S] (or ragged, with ``ray_offsets`` [N+1] int32) samples.
 
-    sigma [N,S] (or [N,S,1]), rgb [N,S,3], z [N,S] (NDC depths if ndc else metric).
+    sigma [N,S] (or [N,S,1]), rgb [N,S,3] or None (weights and depths only; the returned rgb
+    is then zeros), z [N,S] (NDC depths if ndc else metric).
     Returns the reference's output dict: rgb, acc, depth, depth_var, (depth_ndc, depth_var_ndc),
     alpha, visibility, weights.
     """
@@ -73,7 +76,7 @@ def composite(sigma: torch.Tensor, rgb: torch.Tensor, z: torch.Tensor, rays_o: t
         n_samples = sigma.shape[1]
         shape_s = sigma.shape
         sigma_f = dev_tensor(sigma, 'sigma', (n_rays, n_samples)).reshape(-1)
-        rgb_f = dev_tensor(rgb, 'rgb', (n_rays, n_samples, 3)).reshape(-1, 3)
+        rgb_f = None if rgb is None else dev_tensor(rgb, 'rgb', (n_rays, n_samples, 3)).reshape(-1, 3)
         z_f = dev_tensor(z, 'z', (n_rays, n_samples)).reshape(-1)
     else:
         ray_offsets = dev_tensor(ray_offsets, 'ray_offsets', (n_rays + 1,), torch.int32)
@@ -83,7 +86,7 @@ def composite(sigma: torch.Tensor, rgb: torch.Tensor, z: torch.Tensor, rays_o: t
         shape_s = sigma.shape
         sigma_f = dev_tensor(sigma, 'sigma').reshape(-1)
         P = sigma_f.shape[0]
-        rgb_f = dev_tensor(rgb, 'rgb', (P, 3))
+        rgb_f = None if rgb is None else dev_tensor(rgb, 'rgb', (P, 3))
         z_f = dev_tensor(z, 'z', (P,))
     rays_o = dev_tensor(rays_o, 'rays_o', (n_rays, 3))
     rays_d = dev_tensor(rays_d, 'rays_d', (n_rays, 3))
@@ -98,3 +101,44 @@ def composite(sigma: torch.Tensor, rgb: torch.Tensor, z: torch.Tensor, rays_o: t
     if not ndc:
         del res['depth_ndc'], res['depth_var_ndc']
     return res
+
+
+class _VolumeRender(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, weights, values, ray_offsets, n_rays, n_samples):
+        C = values.shape[1]
+        out = torch.empty((n_rays, C), dtype=torch.float32, device=values.device)
+        _lib.call('srf_volume_render_fwd', n_rays, n_samples, _lib.ptr(ray_offsets), _lib.ptr(weights),
+                  _lib.ptr(values), C, _lib.ptr(out), _lib.stream_ptr(values.device))
+        ctx.save_for_backward(weights, values, ray_offsets)
+        ctx.cfg = (n_rays, n_samples)
+        return out
+
+    @staticmethod
+    def backward(ctx, g_out):
+        weights, values, ray_offsets = ctx.saved_tensors
+        n_rays, n_samples = ctx.cfg
+        g_w = torch.empty_like(weights) if ctx.needs_input_grad[0] else None
+        g_v = torch.empty_like(values) if ctx.needs_input_grad[1] else None
+        if g_w is not None or g_v is not None:
+            _lib.call('srf_volume_render_bwd', n_rays, n_samples, _lib.ptr(ray_offsets), _lib.ptr(weights),
+                      _lib.ptr(values), values.shape[1], _lib.ptr(g_out.contiguous()), _lib.ptr(g_w),
+                      _lib.ptr(g_v), _lib.stream_ptr(values.device))
+        return g_w, g_v, None, None, None
+
+
+def volume_render(values: torch.Tensor, weights: torch.Tensor, ray_offsets: torch.Tensor | None = None):
+    """sum_k weights[..., k] * values[..., k, :] (LowRankTensor.volume_render,
+    src/models/SimpleTensoRF09.py:814-817).  values [N,S,C], weights [N,S] (dense) or, with
+    ray_offsets [N+1] int32, values [P,C] and weights [P]."""
+    if ray_offsets is None:
+        n_rays, n_samples, C = values.shape
+        w = dev_tensor(weights, 'weights', (n_rays, n_samples)).reshape(-1)
+        v = dev_tensor(values, 'values').reshape(-1, C)
+    else:
+        n_rays = ray_offsets.shape[0] - 1
+        n_samples = 0
+        ray_offsets = dev_tensor(ray_offsets, 'ray_offsets', (n_rays + 1,), torch.int32)
+        v = dev_tensor(values, 'values')
+        w = dev_tensor(weights, 'weights', (v.shape[0],))
+    return _VolumeRender.apply(w, v, ray_offsets, n_rays, n_samples)

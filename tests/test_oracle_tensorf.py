@@ -1,0 +1,124 @@
+"""CPU pins of the TensoRF oracle (oracle/tensorf.py) against closed forms and library routines.
+
+The reference ships no golden vectors for this path; these checks fix the axis assignment, the
+align_corners=True sampling and the sampler against what the mathematics determines.
+"""
+import math
+
+import pytest
+import torch
+import torch.nn.functional as F
+
+import oracle.tensorf as ot
+
+
+def rand_factors(comps, res, seed, dtype=torch.float64):
+    """Planes [1,c,res[a1],res[a0]] and lines [1,c,res[v],1] as created at SimpleTensoRF09.py:1154-1165."""
+    g = torch.Generator().manual_seed(seed)
+    planes, lines = [], []
+    for p, c in enumerate(comps):
+        a0, a1 = ot.MATRIX_AXES[p]
+        v = ot.VECTOR_AXES[p]
+        planes.append(0.1 * torch.randn(1, c, res[a1], res[a0], generator=g, dtype=dtype))
+        lines.append(0.1 * torch.randn(1, c, res[v], 1, generator=g, dtype=dtype))
+    return planes, lines
+
+
+def test_texel_centres_read_stored_values():
+    """At a texel centre the density feature is sum_p sum_c M_p[c, y, x] V_p[c, z] exactly, with
+    (x, y) = (pts[a0], pts[a1]) indexing (W, H) of plane p and pts[v] indexing line p."""
+    res = [5, 7, 9]
+    planes, lines = rand_factors([3, 2, 4], res, 0)
+    idx = torch.tensor([[1, 2, 3], [4, 6, 8], [0, 0, 0]])
+    pts_n = torch.stack([2 * idx[:, k].double() / (res[k] - 1) - 1 for k in range(3)], -1)
+    got = ot.vm_density_features(planes, lines, pts_n)
+    want = torch.zeros(3, dtype=torch.float64)
+    for p in range(3):
+        a0, a1 = ot.MATRIX_AXES[p]
+        v = ot.VECTOR_AXES[p]
+        for i in range(3):
+            want[i] += (planes[p][0, :, idx[i, a1], idx[i, a0]] * lines[p][0, :, idx[i, v], 0]).sum()
+    assert torch.allclose(got, want, rtol=0, atol=1e-14)
+
+
+def test_separable_linear_factors_interpolate_exactly():
+    """Factors that are linear in each coordinate are reproduced exactly by (bi)linear sampling,
+    so the feature of a rank-1 field is the product of the three linear functions."""
+    res = [6, 8, 10]
+    lin = [lambda t, a=a: 0.5 + 0.25 * a + 0.3 * t for a in range(3)]  # functions of normalised coords
+    planes, lines = [], []
+    for p in range(3):
+        a0, a1 = ot.MATRIX_AXES[p]
+        v = ot.VECTOR_AXES[p]
+        xs = torch.linspace(-1, 1, res[a0], dtype=torch.float64)
+        ys = torch.linspace(-1, 1, res[a1], dtype=torch.float64)
+        zs = torch.linspace(-1, 1, res[v], dtype=torch.float64)
+        planes.append((lin[a1](ys)[:, None] * lin[a0](xs)[None, :])[None, None])
+        lines.append(lin[v](zs)[None, None, :, None])
+    g = torch.Generator().manual_seed(3)
+    pts_n = torch.rand(200, 3, generator=g, dtype=torch.float64) * 2 - 1
+    got = ot.vm_density_features(planes, lines, pts_n)
+    want = 3 * lin[0](pts_n[:, 0]) * lin[1](pts_n[:, 1]) * lin[2](pts_n[:, 2])
+    assert torch.allclose(got, want, rtol=0, atol=1e-12)
+
+
+def test_grid_sample_matches_closed_form_bilinear():
+    g = torch.Generator().manual_seed(4)
+    img = torch.randn(3, 11, 7, generator=g, dtype=torch.float64)
+    x = torch.rand(500, generator=g, dtype=torch.float64) * 2.2 - 1.1   # includes the zero-padded border
+    y = torch.rand(500, generator=g, dtype=torch.float64) * 2.2 - 1.1
+    ref = F.grid_sample(img[None], torch.stack([x, y], -1).view(1, -1, 1, 2), align_corners=True).view(3, -1)
+    assert torch.allclose(ot.bilinear_align_corners(img, x, y), ref, rtol=0, atol=1e-13)
+
+
+def test_appearance_features_are_channelwise_products():
+    res = [5, 6, 7]
+    planes, lines = rand_factors([4, 8, 4], res, 5)
+    g = torch.Generator().manual_seed(6)
+    pts_n = torch.rand(64, 3, generator=g, dtype=torch.float64) * 2 - 1
+    app = ot.vm_appearance_features(planes, lines, pts_n)
+    assert app.shape == (64, 16)
+    # summing each plane's block of channels gives that plane's density contribution
+    dens = ot.vm_density_features(planes, lines, pts_n)
+    assert torch.allclose(app.sum(-1), dens, rtol=0, atol=1e-14)
+
+
+def test_alpha_mask_trilinear_at_voxel_centres_and_rank1():
+    g = torch.Generator().manual_seed(7)
+    vol = (torch.rand(4, 5, 6, generator=g) > 0.5).double()
+    bbox = torch.tensor([[-1.0, -2.0, 0.0], [1.0, 2.0, 3.0]], dtype=torch.float64)
+    # voxel (z, y, x) centre in world coordinates
+    zyx = torch.tensor([[1, 2, 3], [3, 4, 5], [0, 0, 0]])
+    pts = torch.stack([bbox[0, k] + (bbox[1, k] - bbox[0, k]) * zyx[:, 2 - k].double() / (vol.shape[2 - k] - 1)
+                       for k in range(3)], -1)
+    got = ot.alpha_mask_sample(vol, bbox, pts)
+    assert torch.equal(got, vol[zyx[:, 0], zyx[:, 1], zyx[:, 2]])
+    # a trilinear (rank-1 affine) volume is reproduced exactly anywhere inside
+    zs, ys, xs = (torch.linspace(0, 1, n, dtype=torch.float64) for n in vol.shape)
+    lin = (1 + zs)[:, None, None] * (2 + ys)[None, :, None] * (3 - xs)[None, None, :]
+    q = bbox[0] + (bbox[1] - bbox[0]) * torch.rand(100, 3, generator=g, dtype=torch.float64)
+    t = (q - bbox[0]) / (bbox[1] - bbox[0])
+    want = (1 + t[:, 2]) * (2 + t[:, 1]) * (3 - t[:, 0])
+    assert torch.allclose(ot.alpha_mask_sample(lin, bbox, q), want, rtol=0, atol=1e-12)
+
+
+def test_aabb_entry_closed_form():
+    bbox = torch.tensor([[-1.0, -1.0, -1.0], [1.0, 1.0, 1.0]])
+    o = torch.tensor([[0.0, 0.0, -5.0], [3.0, 0.5, 0.0], [0.0, 0.0, 0.0]])
+    d = torch.tensor([[0.0, 0.0, 1.0], [-1.0, 0.0, 0.0], [0.0, 0.0, 1.0]])
+    z = ot.aabb_z_vals(o, d, bbox, near=0.1, far=6.0, step=0.25, n_samples=4)
+    # ray 0 enters at t = 4; ray 1 at t = 2 (the zero components of d become 1e-6 and give huge
+    # negative or positive rates that the max over axes ignores); ray 2 starts inside -> near
+    assert torch.allclose(z[:, 0], torch.tensor([4.0, 2.0, 0.1]))
+    assert torch.allclose(z[0], torch.tensor([4.0, 4.25, 4.5, 4.75]))
+    u = torch.tensor([0.5, 0.0, 0.25])
+    zu = ot.aabb_z_vals(o, d, bbox, near=0.1, far=6.0, step=0.25, n_samples=4, u=u)
+    assert torch.allclose(zu - z, (0.25 * u)[:, None].expand(3, 4))
+
+
+def test_num_samples_rule():
+    """compute_num_samples = min(max, round(|res| / 0.5)): a 128^3 grid gives round(443.4) = 443."""
+    res = torch.tensor([128, 128, 128])
+    assert ot.compute_num_samples(res, 0.5, 1000) == 443
+    assert ot.compute_num_samples(res, 0.5, 300) == 300
+    assert round(math.sqrt(3) * 128 / 0.5) == 443
