@@ -1,0 +1,120 @@
+"""End-to-end parity of the drop-in SimpleTensoRFHip model with the oracle TensoRF render (GPU)."""
+import pytest
+import torch
+
+import oracle.tensorf as ot
+from models.ModelFactory02 import get_model
+from srf_hip import configs as cfgs, synthetic
+
+pytestmark = pytest.mark.gpu
+
+THR = 1e-4  # ray_marching_weight_threshold
+
+
+def _model(cuda, alpha=False, white=True):
+    torch.manual_seed(0)
+    c = cfgs.simple_tensorf_configs(white_bkgd=white, perturb=False, coarse=dict(num_voxels_initial=32 ** 3),
+                                    augmented=dict(num_voxels_initial=16 ** 3))
+    mc = cfgs.model_configs()
+    model = get_model(c, model_configs=mc).to(cuda)
+    if alpha:
+        from models.SimpleTensoRFHip09 import AlphaGridMask
+        g = torch.Generator().manual_seed(1)
+        vol = (torch.rand(24, 24, 24, generator=g) < 0.6).float()
+        model.coarse_model.alpha_mask = AlphaGridMask(vol, model.coarse_model.bounding_box.cpu()).to(cuda)
+    return model, mc
+
+
+def _oracle_params(tm, dtype=torch.float64, grad=False):
+    cast = lambda t: t.detach().cpu().to(dtype).contiguous().requires_grad_(grad)
+    return {'bbox': tm.bounding_box.detach().cpu().to(dtype),
+            'planes_density': [cast(p) for p in tm.matrices_density], 'lines_density': [cast(p) for p in tm.vectors_density],
+            'planes_color': [cast(p) for p in tm.matrices_color], 'lines_color': [cast(p) for p in tm.vectors_color],
+            'basis': cast(tm.basis_matrix_color.weight),
+            'color_mlp': {k: cast(v) for k, v in tm.color_predictor.state_dict().items()}}
+
+
+def _oracle_render(out, prm, tm, dtype=torch.float64, alpha=None):
+    o = out['rays_o_ndc'].detach().cpu().to(dtype)
+    d = out['rays_d_ndc'].detach().cpu().to(dtype)
+    z = out['z_vals_coarse'].detach().cpu().to(dtype)
+    pts = o[:, None, :] + d[:, None, :] * z[..., None]
+    return ot.tensorf_render(prm, pts, z, out['rays_o'].detach().cpu().to(dtype), out['rays_d'].detach().cpu().to(dtype),
+                             d, out['view_dirs'].detach().cpu().to(dtype), alpha_volume=alpha, distance_scale=25.0,
+                             weight_threshold=THR, white_bkgd=True)
+
+
+@pytest.mark.parametrize('alpha', [False, True])
+def test_render_matches_oracle(cuda, alpha):
+    model, mc = _model(cuda, alpha=alpha)
+    model.eval()
+    batch = synthetic.train_batch(256, seed=3, device=cuda)
+    with torch.no_grad():
+        out = model({'pixel_id': batch['pixel_id'], 'num_frames': 3}, retraw=True)
+    tm = model.coarse_model
+    av = None
+    if alpha:
+        av = tm.alpha_mask.alpha_volume.detach().cpu().double().reshape(24, 24, 24)
+    ref = _oracle_render(out, _oracle_params(tm), tm, alpha=av)
+    ref32 = _oracle_render(out, _oracle_params(tm, torch.float32), tm, dtype=torch.float32,
+                           alpha=None if av is None else av.float())
+    # the colour of a sample is evaluated only where its weight exceeds the threshold; a ray with a
+    # weight within rounding distance of it may legitimately switch, so such rays are excluded
+    w64 = ref['weights']
+    near = ((w64 - THR).abs() < 1e-9).any(-1)
+    assert near.sum() <= 2
+    keep = ~near
+    for k in ('rgb', 'acc', 'depth', 'depth_ndc', 'weights'):
+        a = out[f'{k}_coarse'].detach().cpu().double()[keep]
+        b = ref[k].detach()[keep]
+        # metric depth (1 / (1 - z_ndc) scaled, values up to ~20 here) inherits the FP32 rounding of
+        # the transmittance product over ~114 samples; its bound is the FP32 oracle's own error
+        # against FP64 (x4) plus 1e-5 relative.  Every other output is held to 1e-5 absolute.
+        if k == 'depth':
+            e32 = (ref32[k].detach().double()[keep] - b).abs()
+            tol = 1e-5 * b.abs().clamp_min(1.0) + 4 * e32
+        else:
+            tol = 1e-5
+        assert ((a - b).abs() < tol).all(), (k, (a - b).abs().max().item())
+    a = out['raw_sigma_coarse'].cpu().double()[keep]
+    assert (a - ref['raw_sigma'].detach()[keep]).abs().max().item() < 1e-5
+
+
+def test_train_grads_match_oracle(cuda):
+    model, mc = _model(cuda)
+    model.eval()   # deterministic sampling; gradients still flow
+    batch = synthetic.train_batch(128, seed=4, device=cuda)
+    out = model({'pixel_id': batch['pixel_id'], 'num_frames': 3}, retraw=True)
+    g = torch.Generator().manual_seed(5)
+    g_rgb = torch.randn(128, 3, generator=g)
+    g_d = torch.randn(128, generator=g)
+    loss = (out['rgb_coarse'] * g_rgb.to(cuda)).sum() + (out['depth_ndc_coarse'] * g_d.to(cuda)).sum()
+    loss.backward()
+    tm = model.coarse_model
+    prm = _oracle_params(tm, grad=True)
+    ref = _oracle_render(out, prm, tm)
+    assert not ((ref['weights'] - THR).abs() < 1e-9).any()
+    ((ref['rgb'] * g_rgb.double()).sum() + (ref['depth_ndc'] * g_d.double()).sum()).backward()
+    pairs = [(tm.matrices_density, prm['planes_density']), (tm.vectors_density, prm['lines_density']),
+             (tm.matrices_color, prm['planes_color']), (tm.vectors_color, prm['lines_color'])]
+    for ours, theirs in pairs:
+        for a, b in zip(ours, theirs):
+            err = (a.grad.cpu().double() - b.grad).abs().max() / b.grad.abs().max().clamp_min(1e-30)
+            assert err < 1e-3, err
+    for a, b in [(tm.basis_matrix_color.weight, prm['basis'])] + \
+                [(dict(tm.color_predictor.named_parameters())[k], v) for k, v in prm['color_mlp'].items()]:
+        err = (a.grad.cpu().double() - b.grad).abs().max() / b.grad.abs().max().clamp_min(1e-30)
+        assert err < 1e-3, err
+
+
+def test_train_step_with_augmentation_runs(cuda):
+    model, mc = _model(cuda, white=False)
+    model.train()
+    batch = synthetic.train_batch(512, seed=6, device=cuda)
+    out = model({'pixel_id': batch['pixel_id'], 'num_frames': 3, 'iter_num': 1, 'sub_batch_index': 0})
+    for k in ('rgb_coarse', 'depth_coarse', 'points_augmentation_rgb_coarse', 'points_augmentation_depth_coarse'):
+        assert k in out and torch.isfinite(out[k]).all(), k
+    loss = ((out['rgb_coarse'] - batch['target_rgb']) ** 2).mean() + \
+        ((out['depth_coarse'] - out['points_augmentation_depth_coarse'].detach()) ** 2).mean()
+    loss.backward()
+    assert all(p.grad is not None and torch.isfinite(p.grad).all() for p in model.coarse_model.matrices_density)

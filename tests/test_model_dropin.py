@@ -35,3 +35,64 @@ def test_unsupported_geometry_is_rejected():
     c['model']['coarse_model']['points_net_width'] = 128
     with pytest.raises(NotImplementedError):
         get_model(c, model_configs=cfgs.model_configs())
+
+
+def _tensorf_expected_shapes(prefix, comps_d, comps_c, res, feat=27, units=128, in_dim=30):
+    """Parameter shapes VmDecomposedTensor creates (src/models/SimpleTensoRF09.py:1144-1165, 1389-1391)."""
+    axes_m, axes_v = [[0, 1], [0, 2], [1, 2]], [2, 1, 0]
+    out = {}
+    for kind, comps in (('density', comps_d), ('color', comps_c)):
+        for i in range(3):
+            a0, a1 = axes_m[i]
+            out[f'{prefix}matrices_{kind}.{i}'] = (1, comps[i], res[a1], res[a0])
+            out[f'{prefix}vectors_{kind}.{i}'] = (1, comps[i], res[axes_v[i]], 1)
+    out[f'{prefix}basis_matrix_color.weight'] = (feat, sum(comps_c))
+    out[f'{prefix}color_predictor.mlp.0.weight'] = (units, in_dim)
+    out[f'{prefix}color_predictor.mlp.2.weight'] = (units, units)
+    out[f'{prefix}color_predictor.mlp.4.weight'] = (3, units)
+    return out
+
+
+def test_tensorf_dropin_state_dict_matches_reference_layout():
+    import oracle.tensorf as ot
+    c = cfgs.simple_tensorf_configs(coarse=dict(num_voxels_initial=32 ** 3), augmented=dict(num_voxels_initial=16 ** 3))
+    model = get_model(c, model_configs=cfgs.model_configs())
+    sd = model.state_dict()
+    tc = c['model']['coarse_model']
+    res = ot.compute_resolution_in_voxels(tc['num_voxels_initial'], torch.tensor(tc['bounding_box'])).tolist()
+    assert model.coarse_model.resolution.tolist() == res
+    assert int(model.coarse_model.num_samples) == ot.compute_num_samples(torch.tensor(res), 0.5, 1e6)
+    exp = _tensorf_expected_shapes('coarse_model.', [16, 4, 4], [48, 12, 12], res)
+    ac = c['model']['augmentations'][0]['coarse_model']
+    ares = ot.compute_resolution_in_voxels(ac['num_voxels_initial'], torch.tensor(ac['bounding_box'])).tolist()
+    exp.update(_tensorf_expected_shapes('augmented_models_nn.0.coarse_model.', [4, 4, 4], [48, 12, 12], ares))
+    for k, shape in exp.items():
+        assert tuple(sd[k].shape) == shape, k
+    for k in ('bounding_box', 'resolution', 'num_samples', 'bounding_box_size', 'voxel_length', 'step_size'):
+        assert f'coarse_model.{k}' in sd
+    # factors are stored texel-major (component axis innermost) under the reference's shape
+    p = model.coarse_model.matrices_density[0]
+    assert p.permute(0, 2, 3, 1).is_contiguous()
+
+
+def test_tensorf_checkpoint_round_trip_keeps_values_and_layout():
+    c = cfgs.simple_tensorf_configs(augmentations=False, coarse=dict(num_voxels_initial=16 ** 3))
+    a = get_model(c, model_configs=cfgs.model_configs())
+    b = get_model(c, model_configs=cfgs.model_configs())
+    # a checkpoint in the reference's contiguous layout loads into the texel-major parameters
+    sd = {k: v.contiguous() for k, v in a.state_dict().items()}
+    b.load_state_dict(sd)
+    for k, v in a.state_dict().items():
+        assert torch.equal(b.state_dict()[k], v), k
+    assert b.coarse_model.matrices_color[1].permute(0, 2, 3, 1).is_contiguous()
+
+
+def test_tensorf_unsupported_options_raise():
+    c = cfgs.simple_tensorf_configs(augmentations=False, coarse=dict(decomposition_type='CandecompParafac'))
+    with pytest.raises(NotImplementedError):
+        get_model(c, model_configs=cfgs.model_configs())
+    c = cfgs.simple_tensorf_configs(augmentations=False, coarse=dict(num_voxels_initial=16 ** 3))
+    m = get_model(c, model_configs=cfgs.model_configs())
+    m.train()
+    with pytest.raises(NotImplementedError):
+        m.coarse_model.run_model_modifications(2500)
