@@ -34,13 +34,14 @@ PEAK_HBM_GBS = 8000.0
 MACS = {(63, True): 593408, (21, True): 577280, (63, False): 492032}
 # HBM bytes per launch of the dominant kernel from the rocprofv3 FETCH_SIZE / WRITE_SIZE passes
 # (scaled by a plain copy of known size) over this same train step.
-TRAFFIC_PROFILE = REPO / 'profiles' / 'mlp_fwd_train_step_4096rays.json'
+TRAFFIC_PROFILE = REPO / 'profiles' / 'mlp_train_step_4096rays.json'
 
 
 def profiled_traffic(kernel: str, rays: int):
-    if kernel != 'srf_mlp_forward' or rays != 4096 or not TRAFFIC_PROFILE.exists():
+    if rays != 4096 or not TRAFFIC_PROFILE.exists():
         return None
-    return json.loads(TRAFFIC_PROFILE.read_text())['hbm_bytes_per_launch']
+    entry = json.loads(TRAFFIC_PROFILE.read_text())['kernels'].get(kernel)
+    return None if entry is None else entry['hbm_bytes_per_launch']
 
 
 class KernelTimer:
@@ -152,6 +153,165 @@ def hip_render_leg(args, dev, model):
     return args.render_rays * n / (t1 - t0)
 
 
+PEAK_HBM_BYTES = PEAK_HBM_GBS * 1e9
+
+
+def tensorf_model(dev, seed=0):
+    """Simple-TensoRF VM model at 300^2: density 3 x 16 components, appearance 3 x 48 (basis 144 -> 27,
+    colour MLP on features PE L=2 + view dir: 138 -> 128 -> 128 -> 3), 443 samples per ray, a random
+    64^3 occupancy mask with 40 % occupied voxels, and a low-rank points-augmented model (4 density
+    components at 150^2).  Factors N(0, 0.1) as the reference initialises them."""
+    import torch
+    from models.ModelFactory02 import get_model
+    from models.SimpleTensoRFHip09 import AlphaGridMask
+    from srf_hip import configs as cfgs
+    torch.manual_seed(seed)
+    box = ((-1.0, -1.0, -1.0), (1.0, 1.0, 1.0))
+    c = cfgs.simple_tensorf_configs(
+        perturb=True,
+        coarse=dict(num_components_density=(16, 16, 16), num_components_color=(48, 48, 48), bounding_box=box,
+                    num_voxels_initial=300 ** 3 + 4, num_samples_max=443, features_positional_encoding_degree=2),
+        augmented=dict(num_components_density=(4, 4, 4), bounding_box=box, num_voxels_initial=150 ** 3 + 4,
+                       num_samples_max=443))
+    model = get_model(c, model_configs=cfgs.model_configs()).to(dev)
+    g = torch.Generator().manual_seed(seed + 7)
+    vol = (torch.rand(64, 64, 64, generator=g) < 0.4).float()
+    model.coarse_model.alpha_mask = AlphaGridMask(vol, model.coarse_model.bounding_box.cpu()).to(dev)
+    return c, model
+
+
+def tensorf_train_leg(args, dev):
+    """TensoRF train step (fwd + hand-written bwd + Adam) on args.tensorf_rays rays."""
+    import torch
+    from srf_hip import _lib, synthetic
+    _, model = tensorf_model(dev)
+    model.train()
+    groups = model.get_trainable_parameters({'lr_initial_tensor': 2e-2, 'lr_initial_network': 1e-3})
+    opt = torch.optim.Adam(groups, betas=(0.9, 0.99))
+    batch = synthetic.train_batch(args.tensorf_rays, seed=5, device=dev)
+    inp = {'pixel_id': batch['pixel_id'], 'num_frames': 3, 'iter_num': 1, 'sub_batch_index': 0}
+
+    def step():
+        out = model(inp)
+        loss = synthetic.train_loss(out, batch)
+        opt.zero_grad(set_to_none=True)
+        loss.backward()
+        opt.step()
+        return out
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    timer = KernelTimer(_lib)
+    with timer:
+        timer.enabled = True
+        t0 = time.perf_counter()
+        for _ in range(args.tensorf_steps):
+            out = step()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        timer.enabled = False
+    kern = timer.summary()
+    n = args.tensorf_steps
+    return {'value': round(args.tensorf_rays * n / (t1 - t0), 1), 'unit': 'rays/s',
+            'ms_per_step': round((t1 - t0) * 1e3 / n, 3),
+            'workload': f'{args.tensorf_rays} rays x 443 samples, VM 300^2 density 3x16 / appearance 3x48, '
+                        '64^3 alpha mask (40 % occupied), low-rank augmented model, fwd+bwd+Adam, FP32',
+            'kernels': {k: {'calls': v['calls'], 'ms_per_step': round(v['ms'] / n, 3)} for k, v in kern.items()},
+            'model': model, 'out': out}
+
+
+def _events_time(fn, reps):
+    """Average GPU time of fn() over reps back-to-back calls, bracketed by HIP events on the
+    current stream (the stream every entry point is launched on)."""
+    import torch
+    fn()
+    s = torch.cuda.Event(enable_timing=True)
+    e = torch.cuda.Event(enable_timing=True)
+    s.record()
+    for _ in range(reps):
+        fn()
+    e.record()
+    torch.cuda.synchronize()
+    return s.elapsed_time(e) / reps * 1e-3
+
+
+def hbm_leg(dev, tmodel, out, reps=10):
+    """HBM rooflines of the memory-bound kernels on the TensoRF workload: every kernel is timed
+    alone (HIP events around `reps` back-to-back launches) and priced with its algorithmic bytes.
+
+    Algorithmic bytes (a non-caching implementation must move them once):
+      density fwd:    n (12 B pts + 9 B sigma/feat/valid out) + n_box 32 B (8 alpha taps)
+                      + n_valid sum(c) 24 B (4 plane + 2 line taps of 4 B per component)
+      density bwd:    n 1 B (valid) + n_valid (12 + 8 B + sum(c) 24 B tap reads + sum(c) 48 B atomic RMW)
+      appearance fwd: m (4 B idx + 12 B pts + sum(c) 24 B taps + sum(c) 4 B out)
+      appearance bwd: m (4 + 12 B + sum(c) 4 B d_out + sum(c) 24 B taps + sum(c) 48 B atomic RMW)
+      composite fwd:  n (8 B sigma/z + 12 B alpha/vis/weights out) + rays 60 B
+      composite bwd:  n (16 B sigma/z/g_w/d_sigma) + rays 72 B
+      compaction:     n 8 B (two passes over weights) + m 4 B
+    """
+    import torch
+    from srf_hip import compositing, tensorf
+    tm = tmodel.coarse_model
+    o, d = out['rays_o_ndc'].detach(), out['rays_d_ndc'].detach()
+    z = out['z_vals_coarse'].detach()
+    pts = (o[:, None, :] + d[:, None, :] * z[..., None]).reshape(-1, 3).contiguous()
+    n = pts.shape[0]
+    hp = tm.host_params()
+    bbox = hp['bbox']
+    P, L = list(tm.matrices_density), list(tm.vectors_density)
+    Pc, Lc = list(tm.matrices_color), list(tm.vectors_color)
+    cd = sum(p.shape[1] for p in P)
+    cc = sum(p.shape[1] for p in Pc)
+    mask = tm.alpha_mask.kernel_mask()
+    res = {}
+    with torch.no_grad():
+        sig, valid = tensorf.vm_density(pts, P, L, bbox, alpha=mask)
+        lo, hi = torch.tensor(bbox[:3], device=dev), torch.tensor(bbox[3:], device=dev)
+        n_box = int(((pts >= lo) & (pts <= hi)).all(-1).sum())
+        n_valid = int(valid.sum())
+        t = _events_time(lambda: tensorf.vm_density(pts, P, L, bbox, alpha=mask), reps)
+        res['vm_density_fwd'] = (t, n * 21 + n_box * 32 + n_valid * cd * 24, f'{n} samples, {n_valid} valid, c={cd}')
+        cw = compositing.composite(sig.view(z.shape), None, z, out['rays_o'].detach(), out['rays_d'].detach(), d,
+                                   ndc=True, distance_scale=25.0)
+        w = cw['weights'].contiguous()
+        t = _events_time(lambda: compositing.composite(sig.view(z.shape), None, z, out['rays_o'].detach(),
+                                                       out['rays_d'].detach(), d, ndc=True, distance_scale=25.0), reps)
+        res['composite_fwd'] = (t, n * 20 + z.shape[0] * 60, f'{z.shape[0]} rays x {z.shape[1]}')
+        idx, count = tensorf.threshold_compact(w, 1e-4)
+        m = int(count.item())
+        t = _events_time(lambda: tensorf.threshold_compact(w, 1e-4), reps)
+        res['threshold_compact'] = (t, n * 8 + m * 4, f'{n} weights, {m} above 1e-4')
+        sidx = idx[:m]
+        t = _events_time(lambda: tensorf.vm_appearance(pts, Pc, Lc, bbox, idx=sidx, n_rows=m), reps)
+        res['vm_appearance_fwd'] = (t, m * (16 + cc * 28), f'{m} surface samples, c={cc}')
+    # backwards: factor grads accumulate with atomics (values irrelevant here)
+    Pg = [p.detach().requires_grad_(True) for p in P]
+    Lg = [p.detach().requires_grad_(True) for p in L]
+    sig, valid = tensorf.vm_density(pts, Pg, Lg, bbox, alpha=mask)
+    gs = torch.randn_like(sig)
+    t = _events_time(lambda: torch.autograd.grad(sig, Pg + Lg, gs, retain_graph=True), reps)
+    res['vm_density_bwd'] = (t, n + n_valid * (20 + cd * 72), f'{n_valid} valid samples, c={cd}')
+    Pcg = [p.detach().requires_grad_(True) for p in Pc]
+    Lcg = [p.detach().requires_grad_(True) for p in Lc]
+    app = tensorf.vm_appearance(pts, Pcg, Lcg, bbox, idx=sidx, n_rows=m)
+    ga = torch.randn_like(app)
+    t = _events_time(lambda: torch.autograd.grad(app, Pcg + Lcg, ga, retain_graph=True), reps)
+    res['vm_appearance_bwd'] = (t, m * (16 + cc * 76), f'{m} surface samples, c={cc}')
+    sg = sig.detach().view(z.shape).requires_grad_(True)
+    cw = compositing.composite(sg, None, z, out['rays_o'].detach(), out['rays_d'].detach(), d, ndc=True,
+                               distance_scale=25.0)
+    gw = torch.randn_like(cw['weights'])
+    t = _events_time(lambda: torch.autograd.grad(cw['weights'], sg, gw, retain_graph=True), reps)
+    res['composite_bwd'] = (t, n * 16 + z.shape[0] * 72, f'{z.shape[0]} rays x {z.shape[1]}')
+    table = {}
+    for k, (sec, byts, what) in res.items():
+        gbs = byts / sec / 1e9
+        table[k] = {'us': round(sec * 1e6, 1), 'algorithmic_bytes': int(byts), 'GB/s': round(gbs, 1),
+                    'frac_hbm': round(gbs / PEAK_HBM_GBS, 4), 'shape': what}
+    return table
+
+
 def oracle_step_fn(model, dev, n_rays, seed=0):
     """The same training step restated in plain PyTorch FP32 (oracle) on device `dev`."""
     import torch
@@ -233,6 +393,9 @@ def main():
     ap.add_argument('--cpu-rays', type=int, default=128)
     ap.add_argument('--cpu-seconds', type=float, default=20.0)
     ap.add_argument('--no-baselines', action='store_true')
+    ap.add_argument('--tensorf-rays', type=int, default=8192)
+    ap.add_argument('--tensorf-steps', type=int, default=5)
+    ap.add_argument('--no-tensorf', action='store_true')
     ap.add_argument('--copy-calibration-mb', type=int, default=0,
                     help='after the timed legs, run 3 device copies of this many MiB (a plain copy kernel '
                          'of known size for scaling rocprof FETCH_SIZE / WRITE_SIZE)')
@@ -282,6 +445,11 @@ def main():
         'render_rays_per_s': round(render, 1),
         'kernels': per_kernel,
     }
+    if not args.no_tensorf:
+        tf = tensorf_train_leg(args, dev)
+        result['tensorf_train'] = {k: v for k, v in tf.items() if k not in ('model', 'out')}
+        result['hbm_kernels'] = hbm_leg(dev, tf['model'], tf['out'])
+        del tf
     if not args.no_baselines:
         plain = plain_gpu_leg(args, dev, tr['model'])
         result['plain_fp32_gpu'] = {'value': round(plain, 1), 'unit': 'rays/s',

@@ -168,27 +168,6 @@ __device__ __forceinline__ void quad_values(const VmArgs& a, const QuadSite& s, 
     if (s.lt.idx[t] >= 0) lv = madd4(lv, ld4(L + (int64_t)s.lt.idx[t] * C + s.c), s.lt.w[t]);
 }
 
-__device__ __forceinline__ void atomic_add4(float* p, float4 g, float w) {
-  unsafeAtomicAdd(p + 0, __fmul_rn(w, g.x));
-  unsafeAtomicAdd(p + 1, __fmul_rn(w, g.y));
-  unsafeAtomicAdd(p + 2, __fmul_rn(w, g.z));
-  unsafeAtomicAdd(p + 3, __fmul_rn(w, g.w));
-}
-
-// Scatter of the plane gradient gp and line gradient gl of one quad into the factor grads:
-// each corner receives (corner weight) x (gradient), as grid_sample's backward does.
-__device__ __forceinline__ void quad_scatter(const VmArgs& a, const QuadSite& s, float4 gp, float4 gl) {
-  const int C = sel3(s.p, a.comp[0], a.comp[1], a.comp[2]);
-  float* dP = sel3(s.p, a.dplane[0], a.dplane[1], a.dplane[2]);
-  float* dL = sel3(s.p, a.dline[0], a.dline[1], a.dline[2]);
-#pragma unroll
-  for (int t = 0; t < 4; ++t)
-    if (s.pt.idx[t] >= 0) atomic_add4(dP + (int64_t)s.pt.idx[t] * C + s.c, gp, s.pt.w[t]);
-#pragma unroll
-  for (int t = 0; t < 2; ++t)
-    if (s.lt.idx[t] >= 0) atomic_add4(dL + (int64_t)s.lt.idx[t] * C + s.c, gl, s.lt.w[t]);
-}
-
 // Trilinear grid_sample of the alpha volume (align_corners=True, zero padding); corners
 // accumulated z0 (y0x0, y0x1, y1x0, y1x1) then z1, weights ((wx * wy) * wz).
 __device__ float alpha_value(const AlphaArgs& m, const float p[3]) {
@@ -302,27 +281,117 @@ __global__ __launch_bounds__(256) void density_fwd_kernel(VmArgs a, AlphaArgs m,
   }
 }
 
+// Gradient of one lane's quad summed over a run of samples that share the same corners, written
+// to HBM with atomics only when the corners change.  Consecutive samples of a ray mostly share
+// them (the step is half a voxel, and in NDC a ray's x, y barely move), so the atomics to the
+// most contended texels drop by up to the run length.
+constexpr int kRun = 32;  // samples walked by one lane group
+
+struct RunAcc {
+  int pidx[4], lidx[2];
+  float4 pg[4], lg[2];
+};
+
+__device__ __forceinline__ void run_init(RunAcc& r) {
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    r.pidx[t] = -1;
+    r.pg[t] = make_float4(0.f, 0.f, 0.f, 0.f);
+  }
+#pragma unroll
+  for (int t = 0; t < 2; ++t) {
+    r.lidx[t] = -1;
+    r.lg[t] = make_float4(0.f, 0.f, 0.f, 0.f);
+  }
+}
+
+__device__ __forceinline__ void atomic_add4v(float* p, float4 v) {
+  unsafeAtomicAdd(p + 0, v.x);
+  unsafeAtomicAdd(p + 1, v.y);
+  unsafeAtomicAdd(p + 2, v.z);
+  unsafeAtomicAdd(p + 3, v.w);
+}
+
+__device__ __forceinline__ void run_flush_plane(const VmArgs& a, int p, int c, RunAcc& r) {
+  const int C = sel3(p, a.comp[0], a.comp[1], a.comp[2]);
+  float* dP = sel3(p, a.dplane[0], a.dplane[1], a.dplane[2]);
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    if (r.pidx[t] >= 0) atomic_add4v(dP + (int64_t)r.pidx[t] * C + c, r.pg[t]);
+    r.pg[t] = make_float4(0.f, 0.f, 0.f, 0.f);
+  }
+}
+
+__device__ __forceinline__ void run_flush_line(const VmArgs& a, int p, int c, RunAcc& r) {
+  const int C = sel3(p, a.comp[0], a.comp[1], a.comp[2]);
+  float* dL = sel3(p, a.dline[0], a.dline[1], a.dline[2]);
+#pragma unroll
+  for (int t = 0; t < 2; ++t) {
+    if (r.lidx[t] >= 0) atomic_add4v(dL + (int64_t)r.lidx[t] * C + c, r.lg[t]);
+    r.lg[t] = make_float4(0.f, 0.f, 0.f, 0.f);
+  }
+}
+
+// Adds one sample's plane gradient gp and line gradient gl (before the corner weights).
+__device__ __forceinline__ void run_add(const VmArgs& a, const QuadSite& s, RunAcc& r, float4 gp, float4 gl) {
+  if (s.pt.idx[0] != r.pidx[0] || s.pt.idx[1] != r.pidx[1] || s.pt.idx[2] != r.pidx[2] ||
+      s.pt.idx[3] != r.pidx[3]) {
+    run_flush_plane(a, s.p, s.c, r);
+#pragma unroll
+    for (int t = 0; t < 4; ++t) r.pidx[t] = s.pt.idx[t];
+  }
+  if (s.lt.idx[0] != r.lidx[0] || s.lt.idx[1] != r.lidx[1]) {
+    run_flush_line(a, s.p, s.c, r);
+    r.lidx[0] = s.lt.idx[0];
+    r.lidx[1] = s.lt.idx[1];
+  }
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    const float w = s.pt.w[t];
+    r.pg[t] = make_float4(__fadd_rn(r.pg[t].x, __fmul_rn(w, gp.x)), __fadd_rn(r.pg[t].y, __fmul_rn(w, gp.y)),
+                          __fadd_rn(r.pg[t].z, __fmul_rn(w, gp.z)), __fadd_rn(r.pg[t].w, __fmul_rn(w, gp.w)));
+  }
+#pragma unroll
+  for (int t = 0; t < 2; ++t) {
+    const float w = s.lt.w[t];
+    r.lg[t] = make_float4(__fadd_rn(r.lg[t].x, __fmul_rn(w, gl.x)), __fadd_rn(r.lg[t].y, __fmul_rn(w, gl.y)),
+                          __fadd_rn(r.lg[t].z, __fmul_rn(w, gl.z)), __fadd_rn(r.lg[t].w, __fmul_rn(w, gl.w)));
+  }
+}
+
 // d_feat = d_sigma * predictor'(feat) on valid samples; planes receive d_feat * line and
-// lines d_feat * plane, per component and corner.
+// lines d_feat * plane, per component and corner.  A lane group walks kRun consecutive samples.
 template <int G>
 __global__ __launch_bounds__(256) void density_bwd_kernel(VmArgs a, int64_t n, const float* __restrict__ pts,
                                                           const uint8_t* __restrict__ valid,
                                                           const float* __restrict__ feat, int kind,
                                                           float offset, const float* __restrict__ d_sigma) {
   const GroupPos gp = group_pos<G>();
-  for (int64_t i = gp.first; i < n; i += gp.stride) {
-    if (!valid[i]) continue;
-    const float g = __fmul_rn(d_sigma[i], predict_density_grad(feat[i], kind, offset));
-    if (g == 0.f) continue;
-    float pn[3];
-#pragma unroll
-    for (int k = 0; k < 3; ++k) pn[k] = normalize_coord(pts[3 * i + k], a.lo[k], a.size[k]);
+  const int64_t n_runs = (n + kRun - 1) / kRun;
+  for (int64_t run = gp.first; run < n_runs; run += gp.stride) {
+    const int64_t i0 = run * kRun;
+    const int64_t i1 = i0 + kRun < n ? i0 + kRun : n;
     for (int q = gp.ql; q < a.n_quads; q += G) {
-      const QuadSite site = quad_site(a, q, pn);
-      float4 pv, lv;
-      quad_values(a, site, pv, lv);
-      const float4 gg = make_float4(g, g, g, g);
-      quad_scatter(a, site, mul4(gg, lv), mul4(gg, pv));
+      RunAcc r;
+      run_init(r);
+      int p = 0, c = 0;
+      for (int64_t i = i0; i < i1; ++i) {
+        if (!valid[i]) continue;
+        const float g = __fmul_rn(d_sigma[i], predict_density_grad(feat[i], kind, offset));
+        if (g == 0.f) continue;
+        float pn[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) pn[k] = normalize_coord(pts[3 * i + k], a.lo[k], a.size[k]);
+        const QuadSite site = quad_site(a, q, pn);
+        float4 pv, lv;
+        quad_values(a, site, pv, lv);
+        const float4 gg = make_float4(g, g, g, g);
+        run_add(a, site, r, mul4(gg, lv), mul4(gg, pv));
+        p = site.p;
+        c = site.c;
+      }
+      run_flush_plane(a, p, c, r);
+      run_flush_line(a, p, c, r);
     }
   }
 }
@@ -353,17 +422,29 @@ __global__ __launch_bounds__(256) void appearance_bwd_kernel(VmArgs a, int64_t n
                                                              const float* __restrict__ pts,
                                                              const float* __restrict__ d_out) {
   const GroupPos gp = group_pos<G>();
-  for (int64_t j = gp.first; j < n; j += gp.stride) {
-    const int64_t i = idx ? (int64_t)idx[j] : j;
-    float pn[3];
-#pragma unroll
-    for (int k = 0; k < 3; ++k) pn[k] = normalize_coord(pts[3 * i + k], a.lo[k], a.size[k]);
+  const int64_t n_runs = (n + kRun - 1) / kRun;
+  for (int64_t run = gp.first; run < n_runs; run += gp.stride) {
+    const int64_t j0 = run * kRun;
+    const int64_t j1 = j0 + kRun < n ? j0 + kRun : n;
     for (int q = gp.ql; q < a.n_quads; q += G) {
-      const float4 g = ld4(d_out + j * a.total_comp + 4 * q);
-      const QuadSite site = quad_site(a, q, pn);
-      float4 pv, lv;
-      quad_values(a, site, pv, lv);
-      quad_scatter(a, site, mul4(g, lv), mul4(g, pv));
+      RunAcc r;
+      run_init(r);
+      int p = 0, c = 0;
+      for (int64_t j = j0; j < j1; ++j) {
+        const int64_t i = idx ? (int64_t)idx[j] : j;
+        float pn[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) pn[k] = normalize_coord(pts[3 * i + k], a.lo[k], a.size[k]);
+        const float4 g = ld4(d_out + j * a.total_comp + 4 * q);
+        const QuadSite site = quad_site(a, q, pn);
+        float4 pv, lv;
+        quad_values(a, site, pv, lv);
+        run_add(a, site, r, mul4(g, lv), mul4(g, pv));
+        p = site.p;
+        c = site.c;
+      }
+      run_flush_plane(a, p, c, r);
+      run_flush_line(a, p, c, r);
     }
   }
 }
@@ -566,7 +647,8 @@ extern "C" int srf_vm_density_bwd(const srf_vm_factors* factors, float* const* d
     a.dline[p] = d_lines[p];
   }
   const int G = next_pow2(a.n_quads);
-  SRF_VM_DISPATCH(G, density_bwd_kernel, dim3(grid_for(n_points, G)), dim3(256), 0, as_stream(stream), a,
+  SRF_VM_DISPATCH(G, density_bwd_kernel, dim3(grid_for((n_points + kRun - 1) / kRun, G)), dim3(256), 0,
+                  as_stream(stream), a,
                   n_points, pts, valid, feat, predictor, offset, d_sigma);
   return launch_status();
 }
@@ -602,7 +684,8 @@ extern "C" int srf_vm_appearance_bwd(const srf_vm_factors* factors, float* const
     a.dline[p] = d_lines[p];
   }
   const int G = next_pow2(a.n_quads);
-  SRF_VM_DISPATCH(G, appearance_bwd_kernel, dim3(grid_for(n_rows, G)), dim3(256), 0, as_stream(stream), a,
+  SRF_VM_DISPATCH(G, appearance_bwd_kernel, dim3(grid_for((n_rows + kRun - 1) / kRun, G)), dim3(256), 0,
+                  as_stream(stream), a,
                   n_rows, idx, pts, d_features);
   return launch_status();
 }
