@@ -1,5 +1,7 @@
 // C-ABI of the fused PE + MLP kernels: planning, weight packing and dispatch.
 // Kernel design notes are at the top of mlp_impl.h.
+#include <stdlib.h>
+
 #include "mlp_impl.h"
 
 namespace srf {
@@ -249,7 +251,7 @@ static int mlp_backward_impl(const srf_mlp_desc* desc, const void* packed, int n
   DwArgs d;
   memset(&d, 0, sizeof(d));
   d.ldP = w.ldP;
-  d.n_chunks = (w.ldP + kChunk - 1) / kChunk;
+
   int nj = 0;
   auto job = [&](size_t dz_off, int dz_row0, int dz_rows, size_t x_off, int x_rows, int M, int N, float* dw,
                  int ld, float* db) {
@@ -263,8 +265,8 @@ static int mlp_backward_impl(const srf_mlp_desc* desc, const void* packed, int n
     jb.dw = dw;
     jb.ld_dw = ld;
     jb.db = db;
-    jb.mblocks = (M + 127) / 128;
-    jb.nblocks = (N + 127) / 128;
+    jb.mblocks = (M + kDwTile - 1) / kDwTile;
+    jb.nblocks = (N + kDwTile - 1) / kDwTile;
   };
   const int D = p.trunk_pe, pe_rows = p.comp_tiles[0] * 16;
   job(w.dz[0], 0, 256, w.x0, pe_rows, 256, D, d_weights[0], D, d_biases[0]);
@@ -287,6 +289,15 @@ static int mlp_backward_impl(const srf_mlp_desc* desc, const void* packed, int n
     job(w.dz[8], 0, 16, w.h[7], 256, 4, 256, d_weights[9], 256, d_biases[9]);
   }
   d.n_jobs = nj;
+  // points per work item: about 1024 blocks over all jobs, whole multiples of 256 points (so every
+  // item holds an even number of k-units), between 2048 and 16384 points
+  long long target = ((long long)w.ldP * nj / 1024 + 255) / 256 * 256;
+  if (const char* env = getenv("SRF_DW_CHUNK")) {  // tuning override (points, rounded to 256)
+    const long long v = atoll(env);
+    if (v > 0) target = (v + 255) / 256 * 256;
+  }
+  d.chunk = (int)(target < 2048 ? 2048 : (target > 262144 ? 262144 : target));
+  d.n_chunks = (w.ldP + d.chunk - 1) / d.chunk;
   int work = 0;
   for (int i = 0; i < nj; ++i) {
     d.jobs[i].work_begin = work;

@@ -26,8 +26,9 @@
 //   * Backward chain: the same transposed product with W^T tiles walks from the heads to
 //     layer 0, applying the stored ReLU masks, and writes every layer's pre-activation
 //     gradient dZ feature-major.
-//   * Weight gradients: one grouped GEMM launch dW = dZ X^T over all layers, K = points, each
-//     wave reducing a 64x64 block over a chunk of points and adding it with FP32 atomics.
+//   * Weight gradients: one grouped GEMM launch dW = dZ X^T over all layers, K = points; a block
+//     of eight waves reduces a whole 256 x 256 layer over a chunk of points (each dZ and X row
+//     read once) and adds it with FP32 atomics.
 #pragma once
 
 #include <string.h>
@@ -49,7 +50,6 @@ constexpr int kMaskSlots = 9;      // ReLU masks of layers 0-7 and the view laye
 constexpr int kWaves = 4;          // waves per block
 constexpr int kLdsRowBytes = 144;  // 16 rows x 144 B transpose tile per wave
 constexpr int kMaxJobs = 16;
-constexpr int kChunk = 8192;       // points reduced per dW work item (multiple of 256)
 
 // ----------------------------------------------------------------------------- plan
 struct Plan {
@@ -821,19 +821,32 @@ struct DwJob {
 
 struct DwArgs {
   DwJob jobs[kMaxJobs];
-  int n_jobs, ldP, n_chunks, total_work;
+  int n_jobs, ldP, chunk, n_chunks, total_work;
 };
 
-// One block reduces a 128 x 128 tile of dW = dZ X^T over a chunk of points.  Per k-unit
-// (32 points for 16-bit operands, 16 for FP32) the 8 dZ-row and 8 X-row fragments of the tile
-// (1 KB each, gathered from the feature-major arrays by per-lane addresses) are copied
-// global -> LDS by the four waves, two k-units per double-buffered stage; each wave then runs
-// the 16 MFMAs of its 64 x 64 quadrant from LDS.  Partial sums are added with FP32 atomics.
-constexpr int kDwStage = 2 * 16 * 1024;  // two k-units x (8 A + 8 B) fragments
+// One block of 8 waves reduces a 256 x 256 tile of dW = dZ X^T (a whole layer of the MLP) over a
+// chunk of points, so every dZ and X row of the chunk is read from HBM once; the reduction over
+// points streams at the HBM rate (a 256 x 256 layer does 128 FLOP per byte of dZ and X, below
+// the MFMA:HBM balance of the chip).  Per k-unit (32 points for 16-bit operands, 16 for FP32)
+// the 16 dZ-row and 16 X-row fragments of the tile (1 KB each, gathered from the feature-major
+// arrays by per-lane addresses) are copied global -> LDS by the eight waves into a ring of four
+// k-unit slots, three k-units ahead of the one being multiplied (96 KB in flight per CU to cover
+// the HBM latency with a single resident block); wave w runs the MFMAs of its 64 x 128 sub-tile
+// (row tiles 4 (w >> 1) .. +3, column tiles 8 (w & 1) .. +7) from LDS.  Partial sums are added
+// to dW with FP32 atomics; the bias gradient is the row sum of dZ.
+constexpr int kDwTile = 256;
+constexpr int kDwWaves = 8;
+constexpr int kDwFrags = 32;              // 16 dZ + 16 X fragments per k-unit
+constexpr int kDwSlot = kDwFrags * 1024;  // one k-unit
+constexpr int kDwSlots = 4;               // ring depth (prefetch distance kDwSlots - 1)
+
+// s_waitcnt immediate (gfx9 encoding) that waits for at most n outstanding vector-memory
+// operations and leaves expcnt / lgkmcnt unconstrained.
+constexpr int vmcnt_only(int n) { return 0x0F70 | (n & 0xF) | ((n >> 4) << 14); }
 
 template <int P>
-__global__ __launch_bounds__(256) void mlp_dw_kernel(DwArgs a) {
-  __shared__ __attribute__((aligned(16))) uint8_t stage[2 * kDwStage];
+__global__ __launch_bounds__(64 * kDwWaves) void mlp_dw_kernel(DwArgs a) {
+  __shared__ __attribute__((aligned(16))) uint8_t ring[kDwSlots * kDwSlot];
   const int lane = threadIdx.x & 63, g = lane >> 4, j = lane & 15;
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int item = blockIdx.x;
@@ -844,91 +857,102 @@ __global__ __launch_bounds__(256) void mlp_dw_kernel(DwArgs a) {
   const int local = item - jb.work_begin;
   const int nblk = jb.mblocks * jb.nblocks;
   const int tile = local % nblk, chunk = local / nblk;
-  const int m0 = 128 * (tile % jb.mblocks), n0 = 128 * (tile / jb.mblocks);
-  const int pb = chunk * kChunk;
-  const int pe = min(pb + kChunk, a.ldP);
+  const int m0 = kDwTile * (tile % jb.mblocks), n0 = kDwTile * (tile / jb.mblocks);
+  const int pb = chunk * a.chunk;
+  const int pe = min(pb + a.chunk, a.ldP);
   const size_t ldP = a.ldP;
   constexpr int ELEM = P == kF32 ? 4 : 2;
   constexpr int KPTS = P == kF32 ? 16 : 32;  // points per k-unit
-  // this wave's quadrant: row tiles 4*(w>>1)+i, column tiles 4*(w&1)+k
-  const int qm = 4 * (w >> 1), qn = 4 * (w & 1);
-  bool mv[4], nv[4];
+  const int qm = 4 * (w >> 1), qn = 8 * (w & 1);
+  bool mv[4], nv[8];
 #pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    mv[i] = m0 + 16 * (qm + i) < jb.dz_rows;
-    nv[i] = n0 + 16 * (qn + i) < jb.x_rows;
-  }
-  // DMA of the 16 fragments of k-unit at point p into buf: fragment f < 8 is dZ row tile f,
-  // f >= 8 is X row tile f - 8; wave w copies f = w, w+4, w+8, w+12 (if the tile exists).
+  for (int i = 0; i < 4; ++i) mv[i] = m0 + 16 * (qm + i) < jb.dz_rows;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) nv[k] = n0 + 16 * (qn + k) < jb.x_rows;
+  bool any_m = false, any_n = false;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) any_m |= mv[i];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) any_n |= nv[k];
+  // DMA of the 32 fragments of the k-unit at point p into a ring slot: fragment f < 16 is dZ row
+  // tile f, f >= 16 is X row tile f - 16; wave w copies f = w, w + 8, w + 16, w + 24.  Every wave
+  // issues exactly four loads per k-unit (a tile beyond the job's rows re-reads its first row
+  // tile into the unused slot), so the vmcnt waits below count whole k-units.
   auto dma_unit = [&](int p, uint8_t* buf) SRF_AI {
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-      const int f = w + 4 * q;
-      const bool is_a = f < 8;
-      const int rt = is_a ? f : f - 8;
-      const int row0 = (is_a ? m0 : n0) + 16 * rt;
-      if (row0 < (is_a ? jb.dz_rows : jb.x_rows)) {
-        const uint8_t* base = is_a ? jb.dz : jb.x;
-        const int pg = P == kF32 ? 4 * g : 8 * g;
-        const uint8_t* src = base + (((size_t)(row0 + j)) * ldP + p + pg) * ELEM;
-        __builtin_amdgcn_global_load_lds((const void*)src,
-                                         (__attribute__((address_space(3))) void*)(buf + f * 1024), 16, 0, 0);
-      }
+      const int f = w + kDwWaves * q;
+      const bool is_a = f < 16;
+      const int rt = is_a ? f : f - 16;
+      int row0 = (is_a ? m0 : n0) + 16 * rt;
+      if (row0 >= (is_a ? jb.dz_rows : jb.x_rows)) row0 = is_a ? m0 : n0;
+      const uint8_t* base = is_a ? jb.dz : jb.x;
+      const int pg = P == kF32 ? 4 * g : 8 * g;
+      const uint8_t* src = base + (((size_t)(row0 + j)) * ldP + p + pg) * ELEM;
+      __builtin_amdgcn_global_load_lds((const void*)src,
+                                       (__attribute__((address_space(3))) void*)(buf + f * 1024), 16, 0, 0);
     }
   };
-  f32x4 acc[4][4];
+  f32x4 acc[4][8];
   float rs[4];
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
     rs[i] = 0.f;
 #pragma unroll
-    for (int k = 0; k < 4; ++k) acc[i][k] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int k = 0; k < 8; ++k) acc[i][k] = f32x4{0.f, 0.f, 0.f, 0.f};
   }
   const bool do_db = jb.db != nullptr && n0 == 0 && qn == 0;
-  const int n_units = (pe - pb) / KPTS;  // even: chunks and ldP are multiples of 256 points
-  dma_unit(pb, stage);
-  dma_unit(pb + KPTS, stage + 16 * 1024);
-  wait_vmem();
-  __syncthreads();
+  const bool busy = any_m && (any_n || do_db);
+  const int n_units = (pe - pb) / KPTS;
+  // prologue: k-units 0 .. kDwSlots - 2 in flight
+#pragma unroll
+  for (int u = 0; u < kDwSlots - 1; ++u)
+    if (u < n_units) dma_unit(pb + u * KPTS, ring + u * kDwSlot);
 #pragma unroll 1
-  for (int u = 0; u < n_units; u += 2) {
-    const uint8_t* cur = stage + ((u >> 1) & 1) * kDwStage;
-    if (u + 2 < n_units) {
-      uint8_t* nxt = stage + (((u >> 1) + 1) & 1) * kDwStage;
-      dma_unit(pb + (u + 2) * KPTS, nxt);
-      dma_unit(pb + (u + 3) * KPTS, nxt + 16 * 1024);
-    }
-#pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      const uint8_t* fr = cur + h * 16 * 1024;
+  for (int u = 0; u < n_units; ++u) {
+    // k-unit u has landed once at most (issued after it) k-units' loads are outstanding
+    const int after = min(kDwSlots - 2, n_units - 1 - u);
+    if (after >= 2) __builtin_amdgcn_s_waitcnt(vmcnt_only(8));
+    else if (after == 1) __builtin_amdgcn_s_waitcnt(vmcnt_only(4));
+    else __builtin_amdgcn_s_waitcnt(vmcnt_only(0));
+    // After the barrier every wave's part of k-unit u is in LDS (each wave waited for its own
+    // copies above) and slot (u - 1) % kDwSlots is free (each wave consumed its LDS reads of it
+    // in the MFMAs of iteration u - 1).  A plain s_barrier with compiler-only memory barriers:
+    // __syncthreads()'s workgroup fences would drain the prefetches still in flight
+    // (vmcnt(0)) and defeat the ring.
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_s_barrier();
+    asm volatile("" ::: "memory");
+    if (u + kDwSlots - 1 < n_units)
+      dma_unit(pb + (u + kDwSlots - 1) * KPTS, ring + ((u + kDwSlots - 1) % kDwSlots) * kDwSlot);
+    if (busy) {
+      const uint8_t* fr = ring + (u % kDwSlots) * kDwSlot;
       if constexpr (P == kF32) {
-        f32x4 av[4], bv[4];
+        f32x4 av[4], bv[8];
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-          av[i] = *reinterpret_cast<const f32x4*>(fr + (qm + i) * 1024 + lane * 16);
-          bv[i] = *reinterpret_cast<const f32x4*>(fr + (8 + qn + i) * 1024 + lane * 16);
-        }
+        for (int i = 0; i < 4; ++i) av[i] = *reinterpret_cast<const f32x4*>(fr + (qm + i) * 1024 + lane * 16);
+#pragma unroll
+        for (int k = 0; k < 8; ++k) bv[k] = *reinterpret_cast<const f32x4*>(fr + (16 + qn + k) * 1024 + lane * 16);
 #pragma unroll
         for (int q = 0; q < 4; ++q)
 #pragma unroll
           for (int i = 0; i < 4; ++i)
 #pragma unroll
-            for (int k = 0; k < 4; ++k)
+            for (int k = 0; k < 8; ++k)
               if (mv[i] && nv[k]) acc[i][k] = mfma32(av[i][q], bv[k][q], acc[i][k]);
         if (do_db)
 #pragma unroll
           for (int i = 0; i < 4; ++i) rs[i] += (av[i][0] + av[i][1]) + (av[i][2] + av[i][3]);
       } else {
-        u32x4 av[4], bv[4];
+        u32x4 av[4], bv[8];
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-          av[i] = *reinterpret_cast<const u32x4*>(fr + (qm + i) * 1024 + lane * 16);
-          bv[i] = *reinterpret_cast<const u32x4*>(fr + (8 + qn + i) * 1024 + lane * 16);
-        }
+        for (int i = 0; i < 4; ++i) av[i] = *reinterpret_cast<const u32x4*>(fr + (qm + i) * 1024 + lane * 16);
+#pragma unroll
+        for (int k = 0; k < 8; ++k) bv[k] = *reinterpret_cast<const u32x4*>(fr + (16 + qn + k) * 1024 + lane * 16);
 #pragma unroll
         for (int i = 0; i < 4; ++i)
 #pragma unroll
-          for (int k = 0; k < 4; ++k)
+          for (int k = 0; k < 8; ++k)
             if (mv[i] && nv[k]) acc[i][k] = mfma16<P>(av[i], bv[k], acc[i][k]);
         if (do_db)
 #pragma unroll
@@ -937,14 +961,12 @@ __global__ __launch_bounds__(256) void mlp_dw_kernel(DwArgs a) {
             for (int q = 0; q < 4; ++q) rs[i] += unpack_lo<P>(av[i][q]) + unpack_hi<P>(av[i][q]);
       }
     }
-    wait_vmem();
-    __syncthreads();
   }
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
     if (!mv[i]) continue;
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
+    for (int k = 0; k < 8; ++k) {
       if (!nv[k]) continue;
       const int c = n0 + 16 * (qn + k) + j;
 #pragma unroll
@@ -984,7 +1006,7 @@ int launch_bwd(const BwdArgs& a, hipStream_t s) {
 
 template <int P>
 int launch_dw(const DwArgs& d, hipStream_t s) {
-  hipLaunchKernelGGL(mlp_dw_kernel<P>, dim3(d.total_work), dim3(256), 0, s, d);
+  hipLaunchKernelGGL(mlp_dw_kernel<P>, dim3(d.total_work), dim3(64 * kDwWaves), 0, s, d);
   return launch_status();
 }
 
