@@ -251,6 +251,7 @@ static int mlp_backward_impl(const srf_mlp_desc* desc, const void* packed, int n
   DwArgs d;
   memset(&d, 0, sizeof(d));
   d.ldP = w.ldP;
+  d.cols = w.cols;
 
   int nj = 0;
   auto job = [&](size_t dz_off, int dz_row0, int dz_rows, size_t x_off, int x_rows, int M, int N, float* dw,
@@ -291,13 +292,13 @@ static int mlp_backward_impl(const srf_mlp_desc* desc, const void* packed, int n
   d.n_jobs = nj;
   // points per work item: about 1024 blocks over all jobs, whole multiples of 256 points (so every
   // item holds an even number of k-units), between 2048 and 16384 points
-  long long target = ((long long)w.ldP * nj / 1024 + 255) / 256 * 256;
+  long long target = ((long long)w.cols * nj / 1024 + 255) / 256 * 256;
   if (const char* env = getenv("SRF_DW_CHUNK")) {  // tuning override (points, rounded to 256)
     const long long v = atoll(env);
     if (v > 0) target = (v + 255) / 256 * 256;
   }
   d.chunk = (int)(target < 2048 ? 2048 : (target > 262144 ? 262144 : target));
-  d.n_chunks = (w.ldP + d.chunk - 1) / d.chunk;
+  d.n_chunks = (w.cols + d.chunk - 1) / d.chunk;
   int work = 0;
   for (int i = 0; i < nj; ++i) {
     d.jobs[i].work_begin = work;

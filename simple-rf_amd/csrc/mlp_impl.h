@@ -125,7 +125,8 @@ inline bool make_plan(const srf_mlp_desc* d, Plan* p) {
 
 // Training workspace: feature-major arrays with leading dimension ldP (points, multiple of 256).
 struct Ws {
-  int ldP;
+  int cols;  // points processed: n rounded up to whole blocks of 256
+  int ldP;   // row stride in points (cols, plus one block when that is an even number of blocks)
   size_t x0, h[8], f, ex, v, dz[kLayers], mask, bytes;  // byte offsets
   int dz_rows[kLayers];
 };
@@ -133,7 +134,11 @@ struct Ws {
 inline Ws make_ws(const Plan& p, int n) {
   Ws w;
   memset(&w, 0, sizeof(w));
-  w.ldP = round_up(n > 0 ? n : 1, 256);  // whole blocks of 4 waves (64 or 32 points each)
+  w.cols = round_up(n > 0 ? n : 1, 256);  // whole blocks of 4 waves (64 or 32 points each)
+  // Rows are spaced by an odd number of 256-point blocks, so they are not a multiple of 1 KB x a
+  // power of two apart and the 16 rows a wave touches at once do not fall on the same HBM
+  // channels.  Only the stride grows; the kernels still process `cols` points.
+  w.ldP = (w.cols / 256) % 2 == 0 ? w.cols + 256 : w.cols;
   const size_t row = (size_t)w.ldP * p.elem;
   size_t off = 0;
   auto take = [&](int rows) {
@@ -473,7 +478,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
   const int lane = threadIdx.x & 63, g = lane >> 4, j = lane & 15;
   const int wv = threadIdx.x >> 6;
   const int wave = blockIdx.x * kWaves + wv;
-  const int p0 = wave * 16 * CB;  // the grid covers ldP exactly: every wave has points
+  const int p0 = wave * 16 * CB;  // the grid covers cols exactly: every wave has points
   uint8_t* lds = lds_all + wv * 16 * kLdsRowBytes;
   const Plan& pl = a.p;
   const int ldP = a.w.ldP;
@@ -683,7 +688,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
   const int lane = threadIdx.x & 63, g = lane >> 4, j = lane & 15;
   const int wv = threadIdx.x >> 6;
   const int wave = blockIdx.x * kWaves + wv;
-  const int p0 = wave * 16 * CB;  // the grid covers ldP exactly: every wave has points
+  const int p0 = wave * 16 * CB;  // the grid covers cols exactly: every wave has points
   uint8_t* lds = lds_all + wv * 16 * kLdsRowBytes;
   const Plan& pl = a.p;
   const int ldP = a.w.ldP;
@@ -821,7 +826,7 @@ struct DwJob {
 
 struct DwArgs {
   DwJob jobs[kMaxJobs];
-  int n_jobs, ldP, chunk, n_chunks, total_work;
+  int n_jobs, ldP, cols, chunk, n_chunks, total_work;
 };
 
 // One block of 8 waves reduces a 256 x 256 tile of dW = dZ X^T (a whole layer of the MLP) over a
@@ -859,7 +864,7 @@ __global__ __launch_bounds__(64 * kDwWaves) void mlp_dw_kernel(DwArgs a) {
   const int tile = local % nblk, chunk = local / nblk;
   const int m0 = kDwTile * (tile % jb.mblocks), n0 = kDwTile * (tile / jb.mblocks);
   const int pb = chunk * a.chunk;
-  const int pe = min(pb + a.chunk, a.ldP);
+  const int pe = min(pb + a.chunk, a.cols);
   const size_t ldP = a.ldP;
   constexpr int ELEM = P == kF32 ? 4 : 2;
   constexpr int KPTS = P == kF32 ? 16 : 32;  // points per k-unit
@@ -989,7 +994,7 @@ __global__ __launch_bounds__(64 * kDwWaves) void mlp_dw_kernel(DwArgs a) {
 template <int P, bool VD, bool TRAIN>
 int launch_fwd(const FwdArgs& a, hipStream_t s) {
   const int ppw = 16 * Tr<P>::CB;
-  const int waves = a.w.ldP / ppw;
+  const int waves = a.w.cols / ppw;
   const int blocks = (waves + kWaves - 1) / kWaves;
   hipLaunchKernelGGL((mlp_fwd_kernel<P, VD, TRAIN>), dim3(blocks), dim3(64 * kWaves), 0, s, a);
   return launch_status();
@@ -998,7 +1003,7 @@ int launch_fwd(const FwdArgs& a, hipStream_t s) {
 template <int P, bool VD>
 int launch_bwd(const BwdArgs& a, hipStream_t s) {
   const int ppw = 16 * Tr<P>::CB;
-  const int waves = a.w.ldP / ppw;
+  const int waves = a.w.cols / ppw;
   const int blocks = (waves + kWaves - 1) / kWaves;
   hipLaunchKernelGGL((mlp_bwd_kernel<P, VD>), dim3(blocks), dim3(64 * kWaves), 0, s, a);
   return launch_status();
