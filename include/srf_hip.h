@@ -293,6 +293,51 @@ int srf_aabb_samples(int n_rays, int n_samples, const float* rays_o, const float
                      const float* bbox, float near, float far, float step, const float* u,
                      float* z, void* stream);
 
+/* ----------------------------------------------------------- TensoRF colour network */
+
+/* Structure of the colour path after the appearance gather: basis_matrix_color
+ * (Linear(n_features -> basis_dim), no bias; src/models/SimpleTensoRF09.py:1151) and
+ * MlpFeaturesColorPredictor (:1370-1421): [PE_Lf(features), PE_Lv(view dir)] -> hidden ReLU ->
+ * hidden ReLU -> 3 sigmoid.  n_features % 4 == 0 and <= 256, basis_dim <= 32, hidden == 128,
+ * first-layer width basis_dim (2 features_pe + 1) + 3 (2 views_pe + 1) <= 256. */
+typedef struct srf_color_desc {
+  int n_features;    /* sum of the appearance components */
+  int basis_dim;     /* features_dimension_color (27) */
+  int features_pe;   /* features_positional_encoding_degree */
+  int use_view_dirs; /* view_dependent_color */
+  int views_pe;      /* views_positional_encoding_degree */
+  int hidden;        /* num_units_color_predictor */
+} srf_color_desc;
+
+/* Size of the MFMA-fragment copy of the weights written by srf_color_pack (bytes; 0 if the
+ * descriptor is unsupported). */
+size_t srf_color_packed_bytes(const srf_color_desc* desc);
+
+/* Training workspace for n_points rows (bytes). */
+size_t srf_color_workspace_bytes(const srf_color_desc* desc, int n_points);
+
+/* Repacks the reference-layout parameters (basis [basis_dim, n_features]; mlp.0 / mlp.2 / mlp.4
+ * weights [out, in] and biases) into the kernels' fragments, once per parameter update. */
+int srf_color_pack(const srf_color_desc* desc, const float* basis, const float* w1, const float* b1,
+                   const float* w2, const float* b2, const float* w3, const float* b3, void* packed,
+                   void* stream);
+
+/* rgb [n,3] of the n_points appearance rows app [n, n_features] (16-byte aligned).  View
+ * direction of row j: view_dirs[r] with r = (idx ? idx[j] : j) / n_samples (the sample's ray in
+ * the dense [rays, n_samples] layout).  Replaces basis_matrix_color + color_predictor in
+ * VmDecomposedTensor.get_color (src/models/SimpleTensoRF09.py:1262-1270).  workspace NULL:
+ * inference; otherwise the layer inputs are kept for srf_color_backward. */
+int srf_color_forward(const srf_color_desc* desc, const void* packed, int n_points, const float* app,
+                      const int32_t* idx, int n_samples, const float* view_dirs, float* rgb,
+                      void* workspace, void* stream);
+
+/* Backward of srf_color_forward (same workspace): d_app [n, n_features] (overwritten, 16-byte
+ * aligned) for srf_vm_appearance_bwd, and the parameter gradients in the reference layouts,
+ * accumulated with FP32 atomics (zero them first). */
+int srf_color_backward(const srf_color_desc* desc, const void* packed, int n_points, const float* rgb,
+                       const float* d_rgb, void* workspace, float* d_app, float* d_basis, float* d_w1,
+                       float* d_b1, float* d_w2, float* d_b2, float* d_w3, float* d_b3, void* stream);
+
 /* Lane-layout self-test of the MFMA building blocks (one 16x16 tile): d = a b with
  * a [16,K], b [K,16] row-major FP32 (rounded to the operand type), K = 4 for SRF_PREC_F32 and
  * 32 for SRF_PREC_BF16 / SRF_PREC_F16.  No counterpart in the reference: a unit check of the

@@ -1,0 +1,651 @@
+// TensoRF colour network on gfx950 MFMA (FP32 operands): basis matrix, positional encoding of the
+// features and view directions, 128 -> 128 -> 3 MLP with sigmoid, its hand-written backward chain
+// and the weight gradients.
+//
+// Reference semantics: VmDecomposedTensor.get_color after the gather, basis_matrix_color
+// (src/models/SimpleTensoRF09.py:1151, :1262) and MlpFeaturesColorPredictor
+// (:1370-1421): x = [PE_Lf(F), PE_Lv(d)], h1 = relu(W1 x + b1), h2 = relu(W2 h1 + b2),
+// rgb = sigmoid(W3 h2 + b3), F = B app (no bias), PE blocks [v, sin(2^0 v), cos(2^0 v), ...].
+//
+// Layout and mapping follow the NeRF MLP kernels (mfma.h): the network runs transposed,
+// H_out^T = W H_in^T with the weights as the MFMA A operand (pre-packed into per-lane fragments
+// by srf_color_pack) and 16-point column blocks as N; a wave owns 32 points (2 column blocks), so
+// one layer's D tiles are the next layer's B operand without data movement.  The basis output F
+// and the view directions go through a per-wave LDS tile from which every lane builds its
+// positional-encoding inputs.  Training stores the layer inputs feature-major (row = feature,
+// column = point) for the weight-gradient GEMM, which is the grouped dW kernel of the MLP.
+#include <string.h>
+
+#include "mlp_impl.h"
+
+namespace srf {
+namespace color {
+namespace {
+
+using mlp::DwArgs;
+using mlp::DwJob;
+
+constexpr int kHid = 128;     // num_units_color_predictor of the shipped configurations
+constexpr int kHT = kHid / 16;
+constexpr int kCB = 2;        // 16-point column blocks per wave
+constexpr int kPW = 16 * kCB; // points per wave
+constexpr int kWaves = 4;
+constexpr int kMaxC = 256;    // appearance components
+constexpr int kMaxB = 32;     // basis outputs (features_dimension_color)
+constexpr int kMaxD1 = 256;   // first-layer inputs
+
+struct Geo {
+  int C, Cp;     // components, padded to 16
+  int NB;        // basis outputs (27)
+  int Lf, Lv, view;
+  int D1, D1p;   // first-layer inputs, padded to 16
+  // fragment offsets (floats) in the packed buffer
+  size_t fB, fW1, fW2, fW3, fW3T, fW2T, fW1T, fBT, b1, b2, b3, total;
+};
+
+bool make_geo(const srf_color_desc* d, Geo* g) {
+  if (!d || d->n_features < 4 || d->n_features % 4 != 0 || d->n_features > kMaxC || d->basis_dim < 1 ||
+      d->basis_dim > kMaxB || d->features_pe < 0 || d->views_pe < 0 || d->hidden != kHid)
+    return false;
+  memset(g, 0, sizeof(*g));
+  g->C = d->n_features;
+  g->Cp = (g->C + 15) / 16 * 16;
+  g->NB = d->basis_dim;
+  g->Lf = d->features_pe;
+  g->Lv = d->views_pe;
+  g->view = d->use_view_dirs != 0;
+  g->D1 = g->NB * (2 * g->Lf + 1) + (g->view ? 3 * (2 * g->Lv + 1) : 0);
+  g->D1p = (g->D1 + 15) / 16 * 16;
+  if (g->D1p > kMaxD1) return false;
+  size_t o = 0;
+  auto take = [&](size_t n) { size_t r = o; o += n; return r; };
+  g->fB = take((size_t)2 * (g->Cp / 4) * 64);          // [2 tiles][Cp/4 k-steps][64]
+  g->fW1 = take((size_t)kHT * (g->D1p / 4) * 64);
+  g->fW2 = take((size_t)kHT * (kHid / 4) * 64);
+  g->fW3 = take((size_t)1 * (kHid / 4) * 64);
+  g->fW3T = take((size_t)kHT * 4 * 64);                // rows 128, k 16
+  g->fW2T = take((size_t)kHT * (kHid / 4) * 64);
+  g->fW1T = take((size_t)(g->D1p / 16) * (kHid / 4) * 64);
+  g->fBT = take((size_t)(g->Cp / 16) * 8 * 64);        // rows Cp, k 32
+  g->b1 = take(kHid);
+  g->b2 = take(kHid);
+  g->b3 = take(16);
+  g->total = o;
+  return true;
+}
+
+// Training workspace (FP32 rows of `ld` points): appT, F, x1, h1, h2, dz3, dz2, dz1, dF.
+struct Ws {
+  int cols, ld;
+  size_t appT, F, x1, h1, h2, dz3, dz2, dz1, dF, bytes;
+};
+
+Ws make_ws(const Geo& g, int n) {
+  Ws w;
+  w.cols = ((n > 0 ? n : 1) + 255) / 256 * 256;
+  w.ld = (w.cols / 256) % 2 == 0 ? w.cols + 256 : w.cols;  // rows an odd number of blocks apart
+  const size_t row = (size_t)w.ld * 4;
+  size_t o = 0;
+  auto take = [&](int rows) { size_t r = o; o += (size_t)rows * row; return r; };
+  w.appT = take(g.Cp);
+  w.F = take(kMaxB);
+  w.x1 = take(g.D1p);
+  w.h1 = take(kHid);
+  w.h2 = take(kHid);
+  w.dz3 = take(16);
+  w.dz2 = take(kHid);
+  w.dz1 = take(kHid);
+  w.dF = take(kMaxB);
+  w.bytes = o;
+  return w;
+}
+
+// ------------------------------------------------------------------------ packing
+// Fragment f of the A operand of a transposed layer: tile T (16 output rows), k-step s (lane group
+// g supplies input feature 16 (s / 4) + 4 g + s % 4), lane l = 16 g + j holds W'[16 T + j][k]
+// where W' = W (rows x cols, leading dimension ld) or its transpose.
+struct PackJob {
+  const float* src;
+  int rows, cols, ld, trans, tiles, ksteps;
+  size_t off;
+};
+
+struct PackArgs {
+  PackJob jobs[8];
+  int n_jobs;
+  const float* b1;
+  const float* b2;
+  const float* b3;
+  size_t ob1, ob2, ob3;
+  float* out;
+};
+
+__global__ __launch_bounds__(256) void pack_kernel(PackArgs a) {
+  const int jid = blockIdx.y;
+  if (jid < a.n_jobs) {
+    const PackJob& jb = a.jobs[jid];
+    const int total = jb.tiles * jb.ksteps * 64;
+    for (int e = blockIdx.x * 256 + threadIdx.x; e < total; e += gridDim.x * 256) {
+      const int l = e & 63, s = (e >> 6) % jb.ksteps, T = (e >> 6) / jb.ksteps;
+      const int g = l >> 4, j = l & 15;
+      const int r = 16 * T + j, k = 16 * (s >> 2) + 4 * g + (s & 3);
+      float v = 0.f;
+      if (!jb.trans) {
+        if (r < jb.rows && k < jb.cols) v = jb.src[(size_t)r * jb.ld + k];
+      } else {
+        if (r < jb.cols && k < jb.rows) v = jb.src[(size_t)k * jb.ld + r];
+      }
+      a.out[jb.off + e] = v;
+    }
+  } else if (blockIdx.x == 0) {
+    for (int i = threadIdx.x; i < kHid; i += 256) {
+      a.out[a.ob1 + i] = a.b1[i];
+      a.out[a.ob2 + i] = a.b2[i];
+    }
+    if (threadIdx.x < 16) a.out[a.ob3 + threadIdx.x] = threadIdx.x < 3 ? a.b3[threadIdx.x] : 0.f;
+  }
+}
+
+// ------------------------------------------------------------------------ kernels
+__device__ __forceinline__ float ldf(const float* p) { return *p; }
+
+// acc[T][cb] += A(T, k-step) x in(k-step) over the k-steps 4 t + r of input tiles t < TI.
+template <int TO, int TI>
+__device__ __forceinline__ void layer(f32x4 (&acc)[TO][kCB], const float* __restrict__ frag, int ksteps,
+                                      const f32x4 (&in)[TI][kCB], int lane) {
+#pragma unroll
+  for (int t = 0; t < TI; ++t)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int s = 4 * t + r;
+#pragma unroll
+      for (int T = 0; T < TO; ++T) {
+        const float a = ldf(frag + ((size_t)T * ksteps + s) * 64 + lane);
+#pragma unroll
+        for (int cb = 0; cb < kCB; ++cb) acc[T][cb] = mfma32(a, in[t][cb][r], acc[T][cb]);
+      }
+    }
+}
+
+// Positional-encoding input i of the first layer for one point, from its basis outputs F (LDS
+// column) and view direction (reference order: [F, sin(F), cos(F), sin(2F), ...], then the same
+// blocks of the direction).
+__device__ __forceinline__ float pe_input(const Geo& g, int i, const float* Fcol, int fstride, const float* dir) {
+  const int nf = g.NB * (2 * g.Lf + 1);
+  if (i < nf) {
+    const int b = i / g.NB, f = i - b * g.NB;
+    const float v = Fcol[f * fstride];
+    if (b == 0) return v;
+    const float x = __fmul_rn(v, (float)(1 << ((b - 1) >> 1)));
+    return (b & 1) ? sinf(x) : cosf(x);
+  }
+  if (g.view && i < g.D1) {
+    const int i2 = i - nf;
+    const int b = i2 / 3, d = i2 - 3 * b;
+    const float v = dir[d];
+    if (b == 0) return v;
+    const float x = __fmul_rn(v, (float)(1 << ((b - 1) >> 1)));
+    return (b & 1) ? sinf(x) : cosf(x);
+  }
+  return 0.f;
+}
+
+struct FwdArgs {
+  Geo g;
+  Ws w;
+  const float* pk;
+  int n;
+  const float* app;  // [n, C]
+  const int32_t* idx;
+  int n_samples;
+  const float* view_dirs;
+  float* rgb;
+  uint8_t* ws;  // null: inference
+};
+
+__device__ __forceinline__ void store_fm(uint8_t* base, size_t row_off, int ld, int row, int p, float v) {
+  reinterpret_cast<float*>(base + row_off)[(size_t)row * ld + p] = v;
+}
+
+template <int CT>
+__global__ __launch_bounds__(64 * kWaves) void color_fwd_kernel(FwdArgs a) {
+  constexpr int D1T_MAX = kMaxD1 / 16;
+  __shared__ float sF[kWaves][kMaxB][kPW + 1];
+  __shared__ float sD[kWaves][3][kPW];
+  const Geo& g = a.g;
+  const int lane = threadIdx.x & 63, gq = lane >> 4, j = lane & 15;
+  const int wv = threadIdx.x >> 6;
+  const int p0 = (blockIdx.x * kWaves + wv) * kPW;
+  const bool train = a.ws != nullptr;
+  // 1) basis: F^T = B app^T (2 tiles of 16 outputs)
+  f32x4 accF[2][kCB];
+#pragma unroll
+  for (int T = 0; T < 2; ++T)
+#pragma unroll
+    for (int cb = 0; cb < kCB; ++cb) accF[T][cb] = f32x4{0.f, 0.f, 0.f, 0.f};
+  for (int t = 0; t < g.Cp / 16; ++t) {
+    f32x4 in[1][kCB];
+#pragma unroll
+    for (int cb = 0; cb < kCB; ++cb) {
+      const int p = p0 + 16 * cb + j;
+      const int c = 16 * t + 4 * gq;
+      float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+      if (p < a.n && c < g.C) v = *reinterpret_cast<const float4*>(a.app + (size_t)p * g.C + c);
+      in[0][cb] = f32x4{v.x, v.y, v.z, v.w};
+      if (train && c < g.C) {
+        store_fm(a.ws, a.w.appT, a.w.ld, c + 0, p, v.x);
+        store_fm(a.ws, a.w.appT, a.w.ld, c + 1, p, v.y);
+        store_fm(a.ws, a.w.appT, a.w.ld, c + 2, p, v.z);
+        store_fm(a.ws, a.w.appT, a.w.ld, c + 3, p, v.w);
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int s = 4 * t + r;
+#pragma unroll
+      for (int T = 0; T < 2; ++T) {
+        const float av = ldf(a.pk + g.fB + ((size_t)T * (g.Cp / 4) + s) * 64 + lane);
+#pragma unroll
+        for (int cb = 0; cb < kCB; ++cb) accF[T][cb] = mfma32(av, in[0][cb][r], accF[T][cb]);
+      }
+    }
+  }
+  // F and the view directions to LDS (and F feature-major for the backward)
+#pragma unroll
+  for (int T = 0; T < 2; ++T)
+#pragma unroll
+    for (int cb = 0; cb < kCB; ++cb)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int f = 16 * T + 4 * gq + r;
+        sF[wv][f][16 * cb + j] = accF[T][cb][r];
+        if (train) store_fm(a.ws, a.w.F, a.w.ld, f, p0 + 16 * cb + j, accF[T][cb][r]);
+      }
+  if (g.view && lane < kPW) {
+    const int p = p0 + lane;
+    float d0 = 0.f, d1 = 0.f, d2 = 0.f;
+    if (p < a.n) {
+      const int64_t smp = a.idx ? (int64_t)a.idx[p] : (int64_t)p;
+      const int64_t ray = smp / a.n_samples;
+      d0 = a.view_dirs[3 * ray + 0];
+      d1 = a.view_dirs[3 * ray + 1];
+      d2 = a.view_dirs[3 * ray + 2];
+    }
+    sD[wv][0][lane] = d0;
+    sD[wv][1][lane] = d1;
+    sD[wv][2][lane] = d2;
+  }
+  __syncthreads();
+  // 2) first layer over the positional-encoding inputs built per k-step from LDS
+  f32x4 h1[kHT][kCB];
+#pragma unroll
+  for (int T = 0; T < kHT; ++T)
+#pragma unroll
+    for (int cb = 0; cb < kCB; ++cb)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) h1[T][cb][r] = a.pk[g.b1 + 16 * T + 4 * gq + r];
+  const int d1t = g.D1p / 16;
+  for (int t = 0; t < d1t && t < D1T_MAX; ++t) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int i = 16 * t + 4 * gq + r;
+      float b[kCB];
+#pragma unroll
+      for (int cb = 0; cb < kCB; ++cb) {
+        const int col = 16 * cb + j;
+        const float dir[3] = {sD[wv][0][col], sD[wv][1][col], sD[wv][2][col]};
+        b[cb] = pe_input(g, i, &sF[wv][0][col], kPW + 1, dir);
+        if (train && i < g.D1) store_fm(a.ws, a.w.x1, a.w.ld, i, p0 + col, b[cb]);
+      }
+      const int s = 4 * t + r;
+#pragma unroll
+      for (int T = 0; T < kHT; ++T) {
+        const float av = ldf(a.pk + g.fW1 + ((size_t)T * (g.D1p / 4) + s) * 64 + lane);
+#pragma unroll
+        for (int cb = 0; cb < kCB; ++cb) h1[T][cb] = mfma32(av, b[cb], h1[T][cb]);
+      }
+    }
+  }
+#pragma unroll
+  for (int T = 0; T < kHT; ++T)
+#pragma unroll
+    for (int cb = 0; cb < kCB; ++cb)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const float v = h1[T][cb][r] > 0.f ? h1[T][cb][r] : 0.f;
+        h1[T][cb][r] = v;
+        if (train) store_fm(a.ws, a.w.h1, a.w.ld, 16 * T + 4 * gq + r, p0 + 16 * cb + j, v);
+      }
+  // 3) second layer
+  f32x4 h2[kHT][kCB];
+#pragma unroll
+  for (int T = 0; T < kHT; ++T)
+#pragma unroll
+    for (int cb = 0; cb < kCB; ++cb)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) h2[T][cb][r] = a.pk[g.b2 + 16 * T + 4 * gq + r];
+  layer<kHT, kHT>(h2, a.pk + g.fW2, kHid / 4, h1, lane);
+#pragma unroll
+  for (int T = 0; T < kHT; ++T)
+#pragma unroll
+    for (int cb = 0; cb < kCB; ++cb)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const float v = h2[T][cb][r] > 0.f ? h2[T][cb][r] : 0.f;
+        h2[T][cb][r] = v;
+        if (train) store_fm(a.ws, a.w.h2, a.w.ld, 16 * T + 4 * gq + r, p0 + 16 * cb + j, v);
+      }
+  // 4) output layer + sigmoid
+  f32x4 o[1][kCB];
+#pragma unroll
+  for (int cb = 0; cb < kCB; ++cb)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) o[0][cb][r] = a.pk[g.b3 + 4 * gq + r];
+  layer<1, kHT>(o, a.pk + g.fW3, kHid / 4, h2, lane);
+  if (gq == 0) {
+#pragma unroll
+    for (int cb = 0; cb < kCB; ++cb) {
+      const int p = p0 + 16 * cb + j;
+      if (p < a.n)
+#pragma unroll
+        for (int r = 0; r < 3; ++r) a.rgb[3 * (size_t)p + r] = 1.f / (1.f + expf(-o[0][cb][r]));
+    }
+  }
+}
+
+struct BwdArgs {
+  Geo g;
+  Ws w;
+  const float* pk;
+  int n;
+  const float* rgb;
+  const float* d_rgb;
+  float* d_app;
+  uint8_t* ws;
+};
+
+__device__ __forceinline__ float load_fm(const uint8_t* base, size_t row_off, int ld, int row, int p) {
+  return reinterpret_cast<const float*>(base + row_off)[(size_t)row * ld + p];
+}
+
+// Rows of the per-wave LDS tile of the backward: d x1 (D1p rows), later d F (kMaxB rows).
+__host__ __device__ inline int bwd_rows(const Geo& g) { return g.D1p > kMaxB ? g.D1p : kMaxB; }
+
+__global__ __launch_bounds__(64 * kWaves) void color_bwd_kernel(BwdArgs a) {
+  extern __shared__ float lds_dyn[];
+  const Geo& g = a.g;
+  const int lane = threadIdx.x & 63, gq = lane >> 4, j = lane & 15;
+  const int wv = threadIdx.x >> 6;
+  const int xrows = bwd_rows(g);
+  float* sXw = lds_dyn + (size_t)wv * xrows * (kPW + 1);
+  auto sX = [&](int row, int col) -> float& { return sXw[row * (kPW + 1) + col]; };
+  const int p0 = (blockIdx.x * kWaves + wv) * kPW;
+  // 1) dz3 = d_rgb * s (1 - s)
+  f32x4 dz3[1][kCB];
+#pragma unroll
+  for (int cb = 0; cb < kCB; ++cb) {
+    const int p = p0 + 16 * cb + j;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      float v = 0.f;
+      if (gq == 0 && r < 3 && p < a.n) {
+        const float s = a.rgb[3 * (size_t)p + r];
+        v = __fmul_rn(__fmul_rn(a.d_rgb[3 * (size_t)p + r], __fsub_rn(1.f, s)), s);
+      }
+      dz3[0][cb][r] = v;
+      store_fm(a.ws, a.w.dz3, a.w.ld, 4 * gq + r, p, v);
+    }
+  }
+  // 2) dz2 = (W3^T dz3) * [h2 > 0]
+  f32x4 dz2[kHT][kCB];
+#pragma unroll
+  for (int T = 0; T < kHT; ++T)
+#pragma unroll
+    for (int cb = 0; cb < kCB; ++cb) dz2[T][cb] = f32x4{0.f, 0.f, 0.f, 0.f};
+  layer<kHT, 1>(dz2, a.pk + g.fW3T, 4, dz3, lane);
+#pragma unroll
+  for (int T = 0; T < kHT; ++T)
+#pragma unroll
+    for (int cb = 0; cb < kCB; ++cb)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = 16 * T + 4 * gq + r, p = p0 + 16 * cb + j;
+        const float v = load_fm(a.ws, a.w.h2, a.w.ld, row, p) > 0.f ? dz2[T][cb][r] : 0.f;
+        dz2[T][cb][r] = v;
+        store_fm(a.ws, a.w.dz2, a.w.ld, row, p, v);
+      }
+  // 3) dz1 = (W2^T dz2) * [h1 > 0]
+  f32x4 dz1[kHT][kCB];
+#pragma unroll
+  for (int T = 0; T < kHT; ++T)
+#pragma unroll
+    for (int cb = 0; cb < kCB; ++cb) dz1[T][cb] = f32x4{0.f, 0.f, 0.f, 0.f};
+  layer<kHT, kHT>(dz1, a.pk + g.fW2T, kHid / 4, dz2, lane);
+#pragma unroll
+  for (int T = 0; T < kHT; ++T)
+#pragma unroll
+    for (int cb = 0; cb < kCB; ++cb)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = 16 * T + 4 * gq + r, p = p0 + 16 * cb + j;
+        const float v = load_fm(a.ws, a.w.h1, a.w.ld, row, p) > 0.f ? dz1[T][cb][r] : 0.f;
+        dz1[T][cb][r] = v;
+        store_fm(a.ws, a.w.dz1, a.w.ld, row, p, v);
+      }
+  // 4) dx1 = W1^T dz1, one output tile at a time into LDS
+  for (int T = 0; T < g.D1p / 16; ++T) {
+    f32x4 dx[1][kCB];
+#pragma unroll
+    for (int cb = 0; cb < kCB; ++cb) dx[0][cb] = f32x4{0.f, 0.f, 0.f, 0.f};
+    layer<1, kHT>(dx, a.pk + g.fW1T + (size_t)T * (kHid / 4) * 64, kHid / 4, dz1, lane);
+#pragma unroll
+    for (int cb = 0; cb < kCB; ++cb)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) sX(16 * T + 4 * gq + r, 16 * cb + j) = dx[0][cb][r];
+  }
+  __syncthreads();
+  // 5) positional-encoding backward: dF[f] = dx[f] + sum_k 2^k (dx[sin_k f] cos(2^k F) - dx[cos_k f] sin(2^k F))
+  constexpr int kIt = kMaxB * kPW / 64;  // (feature, point) pairs per lane
+  float dFv[kIt];
+#pragma unroll
+  for (int it = 0; it < kIt; ++it) {
+    const int e = lane + 64 * it;
+    const int f = e / kPW, col = e - f * kPW;
+    float d = 0.f;
+    if (f < g.NB) {
+      const float Fv = load_fm(a.ws, a.w.F, a.w.ld, f, p0 + col);
+      d = sX(f, col);
+      for (int k = 0; k < g.Lf; ++k) {
+        const float fr = (float)(1 << k);
+        const float x = __fmul_rn(Fv, fr);
+        const float gs = sX((2 * k + 1) * g.NB + f, col);
+        const float gc = sX((2 * k + 2) * g.NB + f, col);
+        d = __fadd_rn(d, __fmul_rn(__fmul_rn(gs, cosf(x)), fr));
+        d = __fsub_rn(d, __fmul_rn(__fmul_rn(gc, sinf(x)), fr));
+      }
+    }
+    dFv[it] = d;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int it = 0; it < kIt; ++it) {
+    const int e = lane + 64 * it;
+    const int f = e / kPW, col = e - f * kPW;
+    sX(f, col) = dFv[it];
+    store_fm(a.ws, a.w.dF, a.w.ld, f, p0 + col, dFv[it]);
+  }
+  __syncthreads();
+  // 6) d_app = B^T dF (point-major, float4 per lane)
+  f32x4 dFt[2][kCB];
+#pragma unroll
+  for (int t = 0; t < 2; ++t)
+#pragma unroll
+    for (int cb = 0; cb < kCB; ++cb)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) dFt[t][cb][r] = sX(16 * t + 4 * gq + r, 16 * cb + j);
+  for (int T = 0; T < g.Cp / 16; ++T) {
+    f32x4 da[1][kCB];
+#pragma unroll
+    for (int cb = 0; cb < kCB; ++cb) da[0][cb] = f32x4{0.f, 0.f, 0.f, 0.f};
+    layer<1, 2>(da, a.pk + g.fBT + (size_t)T * 8 * 64, 8, dFt, lane);
+    const int c = 16 * T + 4 * gq;
+    if (c < g.C)
+#pragma unroll
+      for (int cb = 0; cb < kCB; ++cb) {
+        const int p = p0 + 16 * cb + j;
+        if (p < a.n)
+          *reinterpret_cast<float4*>(a.d_app + (size_t)p * g.C + c) =
+              make_float4(da[0][cb][0], da[0][cb][1], da[0][cb][2], da[0][cb][3]);
+      }
+  }
+}
+
+}  // namespace
+}  // namespace color
+}  // namespace srf
+
+using namespace srf;
+using namespace srf::color;
+
+extern "C" size_t srf_color_packed_bytes(const srf_color_desc* desc) {
+  Geo g;
+  return make_geo(desc, &g) ? g.total * sizeof(float) : 0;
+}
+
+extern "C" size_t srf_color_workspace_bytes(const srf_color_desc* desc, int n_points) {
+  Geo g;
+  if (!make_geo(desc, &g) || n_points < 0) return 0;
+  return make_ws(g, n_points).bytes;
+}
+
+extern "C" int srf_color_pack(const srf_color_desc* desc, const float* basis, const float* w1, const float* b1,
+                              const float* w2, const float* b2, const float* w3, const float* b3, void* packed,
+                              void* stream) {
+  Geo g;
+  if (!make_geo(desc, &g)) return SRF_EINVAL;
+  if (!basis || !w1 || !b1 || !w2 || !b2 || !w3 || !b3 || !packed) return SRF_EINVAL;
+  PackArgs a;
+  memset(&a, 0, sizeof(a));
+  auto job = [&](const float* src, int rows, int cols, int trans, int tiles, int ksteps, size_t off) {
+    PackJob& jb = a.jobs[a.n_jobs++];
+    jb.src = src;
+    jb.rows = rows;
+    jb.cols = cols;
+    jb.ld = cols;
+    jb.trans = trans;
+    jb.tiles = tiles;
+    jb.ksteps = ksteps;
+    jb.off = off;
+  };
+  // forward fragments: W [out, in] as A (rows = out)
+  job(basis, g.NB, g.C, 0, 2, g.Cp / 4, g.fB);
+  job(w1, kHid, g.D1, 0, kHT, g.D1p / 4, g.fW1);
+  job(w2, kHid, kHid, 0, kHT, kHid / 4, g.fW2);
+  job(w3, 3, kHid, 0, 1, kHid / 4, g.fW3);
+  // backward fragments: W^T as A (rows = in)
+  job(w3, 3, kHid, 1, kHT, 4, g.fW3T);
+  job(w2, kHid, kHid, 1, kHT, kHid / 4, g.fW2T);
+  job(w1, kHid, g.D1, 1, g.D1p / 16, kHid / 4, g.fW1T);
+  job(basis, g.NB, g.C, 1, g.Cp / 16, 8, g.fBT);
+  a.b1 = b1;
+  a.b2 = b2;
+  a.b3 = b3;
+  a.ob1 = g.b1;
+  a.ob2 = g.b2;
+  a.ob3 = g.b3;
+  a.out = reinterpret_cast<float*>(packed);
+  hipLaunchKernelGGL(pack_kernel, dim3(32, a.n_jobs + 1), dim3(256), 0, as_stream(stream), a);
+  return launch_status();
+}
+
+extern "C" int srf_color_forward(const srf_color_desc* desc, const void* packed, int n_points, const float* app,
+                                 const int32_t* idx, int n_samples, const float* view_dirs, float* rgb,
+                                 void* workspace, void* stream) {
+  Geo g;
+  if (!make_geo(desc, &g) || n_points < 0 || (g.view && n_samples < 1)) return SRF_EINVAL;
+  if (n_points == 0) return SRF_OK;
+  if (!packed || !app || !rgb || (g.view && !view_dirs)) return SRF_EINVAL;
+  if (reinterpret_cast<uintptr_t>(app) & 15) return SRF_EINVAL;
+  FwdArgs a;
+  a.g = g;
+  a.w = make_ws(g, n_points);
+  a.pk = reinterpret_cast<const float*>(packed);
+  a.n = n_points;
+  a.app = app;
+  a.idx = idx;
+  a.n_samples = n_samples > 0 ? n_samples : 1;
+  a.view_dirs = view_dirs;
+  a.rgb = rgb;
+  a.ws = reinterpret_cast<uint8_t*>(workspace);
+  const int blocks = a.w.cols / (kPW * kWaves);
+  hipLaunchKernelGGL(color_fwd_kernel<0>, dim3(blocks), dim3(64 * kWaves), 0, as_stream(stream), a);
+  return launch_status();
+}
+
+extern "C" int srf_color_backward(const srf_color_desc* desc, const void* packed, int n_points, const float* rgb,
+                                  const float* d_rgb, void* workspace, float* d_app, float* d_basis, float* d_w1,
+                                  float* d_b1, float* d_w2, float* d_b2, float* d_w3, float* d_b3, void* stream) {
+  Geo g;
+  if (!make_geo(desc, &g) || n_points < 0) return SRF_EINVAL;
+  if (n_points == 0) return SRF_OK;
+  if (!packed || !rgb || !d_rgb || !workspace || !d_app || !d_basis || !d_w1 || !d_b1 || !d_w2 || !d_b2 || !d_w3 ||
+      !d_b3)
+    return SRF_EINVAL;
+  if (reinterpret_cast<uintptr_t>(d_app) & 15) return SRF_EINVAL;
+  hipStream_t s = as_stream(stream);
+  BwdArgs b;
+  b.g = g;
+  b.w = make_ws(g, n_points);
+  b.pk = reinterpret_cast<const float*>(packed);
+  b.n = n_points;
+  b.rgb = rgb;
+  b.d_rgb = d_rgb;
+  b.d_app = d_app;
+  b.ws = reinterpret_cast<uint8_t*>(workspace);
+  const size_t lds = (size_t)kWaves * bwd_rows(g) * (kPW + 1) * sizeof(float);
+  if (lds > 65536 &&
+      hipFuncSetAttribute(reinterpret_cast<const void*>(&color_bwd_kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
+                          (int)lds) != hipSuccess)
+    return SRF_EHIP;
+  hipLaunchKernelGGL(color_bwd_kernel, dim3(b.w.cols / (kPW * kWaves)), dim3(64 * kWaves), lds, s, b);
+  int st = launch_status();
+  if (st != SRF_OK) return st;
+  // weight gradients: the grouped dW GEMM over the stored dZ / input rows
+  const Ws& w = b.w;
+  uint8_t* ws = b.ws;
+  DwArgs d;
+  memset(&d, 0, sizeof(d));
+  d.ldP = w.ld;
+  d.cols = w.cols;
+  int nj = 0;
+  auto job = [&](size_t dz, int dz_rows, size_t x, int x_rows, int M, int N, float* dw, int ld, float* db) {
+    DwJob& jb = d.jobs[nj++];
+    jb.dz = ws + dz;
+    jb.dz_rows = dz_rows;
+    jb.x = ws + x;
+    jb.x_rows = x_rows;
+    jb.M = M;
+    jb.N = N;
+    jb.dw = dw;
+    jb.ld_dw = ld;
+    jb.db = db;
+    jb.mblocks = (M + mlp::kDwTile - 1) / mlp::kDwTile;
+    jb.nblocks = (N + mlp::kDwTile - 1) / mlp::kDwTile;
+  };
+  job(w.dz1, kHid, w.x1, g.D1p, kHid, g.D1, d_w1, g.D1, d_b1);
+  job(w.dz2, kHid, w.h1, kHid, kHid, kHid, d_w2, kHid, d_b2);
+  job(w.dz3, 16, w.h2, kHid, 3, kHid, d_w3, kHid, d_b3);
+  job(w.dF, kMaxB, w.appT, g.Cp, g.NB, g.C, d_basis, g.C, nullptr);
+  d.n_jobs = nj;
+  long long target = ((long long)w.cols * nj / 1024 + 255) / 256 * 256;
+  d.chunk = (int)(target < 2048 ? 2048 : (target > 65536 ? 65536 : target));
+  d.n_chunks = (w.cols + d.chunk - 1) / d.chunk;
+  int work = 0;
+  for (int i = 0; i < nj; ++i) {
+    d.jobs[i].work_begin = work;
+    d.jobs[i].work_count = d.jobs[i].mblocks * d.jobs[i].nblocks * d.n_chunks;
+    work += d.jobs[i].work_count;
+  }
+  d.total_work = work;
+  return mlp::dw_f32(d, s);
+}

@@ -15,7 +15,9 @@ on the HIP kernels:
                                     srf_vm_appearance_fwd / _bwd       (:726, :1241-1263)
   volume_render                  -> srf_volume_render_fwd / _bwd       (:814-817)
 
-The basis matrix and the colour MLP (:1151, :1370-1421) run as torch GEMMs.  The factor
+  basis_matrix_color + color MLP -> srf_color_forward / _backward        (:1262-1270, :1370-1421)
+
+The factor
 parameters keep the reference's shapes but are stored texel-major (component axis innermost),
 which is the layout the gather kernels read.  Random draws of the path (t_rand :376, the
 per-ray offset :396, sample_pdf's u) come from the GPU generator and are passed to the kernels.
@@ -29,7 +31,7 @@ import torch
 import torch.nn.functional as F
 from torch.nn import ModuleDict, ModuleList
 
-from srf_hip import compositing, geometry, sampling, tensorf
+from srf_hip import color as hip_color, compositing, geometry, sampling, tensorf
 
 from .SimpleNeRFHip17 import ExtrinsicsLearner, IntrinsicsLearner
 
@@ -425,10 +427,19 @@ class VmDecomposedTensor(torch.nn.Module):
         self.density_kind = tensorf.predictor_code(self.tensor_configs['density_predictor'])
         if self.tensor_configs['color_predictor'] != 'MLP_Features':
             raise NotImplementedError(self.tensor_configs['color_predictor'])
+        if bool(self.tensor_configs['use_view_dirs']) != bool(self.tensor_configs['view_dependent_color']):
+            raise NotImplementedError('colour MLP with use_view_dirs != view_dependent_color')
         self.color_predictor = MlpFeaturesColorPredictor(
             self.tensor_configs, self.tensor_configs['features_dimension_color'],
             self.tensor_configs['features_positional_encoding_degree'],
             self.tensor_configs['views_positional_encoding_degree'], self.tensor_configs['num_units_color_predictor'])
+        tc = self.tensor_configs
+        self.color_spec = hip_color.ColorSpec(
+            n_features=sum(tc['num_components_color']), basis_dim=tc['features_dimension_color'],
+            features_pe=tc['features_positional_encoding_degree'], use_view_dirs=bool(tc['view_dependent_color']),
+            views_pe=tc['views_positional_encoding_degree'], hidden=tc['num_units_color_predictor'])
+        if not self.color_spec.supported():
+            raise NotImplementedError(f'colour network {self.color_spec} is outside the HIP kernels')
 
     # ------------------------------------------------------------- geometry
     @staticmethod
@@ -561,16 +572,16 @@ class VmDecomposedTensor(torch.nn.Module):
             sidx = idx[:m]
             app = tensorf.vm_appearance(pts, list(self.matrices_color), list(self.vectors_color), hp['bbox'],
                                         idx=sidx, n_rows=m)
-            feats = self.basis_matrix_color(app)
-            cin = {'features': feats}
+            vd, per_ray = None, n_samples
             if self.tensor_configs['view_dependent_color']:
                 vd = input_dict['view_dirs']
-                if vd.ndim == 2:
-                    vd = vd[torch.div(sidx, n_samples, rounding_mode='floor').long()]
-                else:
-                    vd = vd.reshape(-1, 3)[sidx.long()]
-                cin['view_dirs'] = vd
-            color = self.color_predictor(cin)['color']
+                if vd.ndim != 2:          # per-sample directions [N, S, 3]
+                    vd, per_ray = vd.reshape(-1, 3), 1
+                vd = vd.contiguous()
+            mlp = self.color_predictor.mlp
+            color = hip_color.color_forward(self.color_spec, app, self.basis_matrix_color.weight,
+                                            (mlp[0].weight, mlp[0].bias, mlp[2].weight, mlp[2].bias, mlp[4].weight,
+                                             mlp[4].bias), view_dirs=vd, idx=sidx, n_samples=per_ray)
             rgb_flat = rgb_flat.index_put((sidx.long(),), color)
         rgb = rgb_flat.view(n_rays, n_samples, 3)
         rgb_map = compositing.volume_render(rgb, weights)

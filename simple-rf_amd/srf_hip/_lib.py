@@ -62,9 +62,15 @@ _PROTOS = {
     'srf_volume_render_fwd': [_I, _I, _P, _P, _P, _I, _P, _P],
     'srf_volume_render_bwd': [_I, _I, _P, _P, _P, _I, _P, _P, _P, _P],
     'srf_aabb_samples': [_I, _I, _P, _P, _P, _F, _F, _F, _P, _P, _P],
+    'srf_color_packed_bytes': [_P],
+    'srf_color_workspace_bytes': [_P, _I],
+    'srf_color_pack': [_P, _P, _P, _P, _P, _P, _P, _P, _P, _P],
+    'srf_color_forward': [_P, _P, _I, _P, _P, _I, _P, _P, _P, _P],
+    'srf_color_backward': [_P, _P, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P],
 }
 _RESTYPES = {'srf_status_string': C.c_char_p, 'srf_mlp_packed_bytes': C.c_size_t,
-             'srf_mlp_workspace_bytes': C.c_size_t, 'srf_compact_workspace_bytes': C.c_size_t}
+             'srf_mlp_workspace_bytes': C.c_size_t, 'srf_compact_workspace_bytes': C.c_size_t,
+             'srf_color_packed_bytes': C.c_size_t, 'srf_color_workspace_bytes': C.c_size_t}
 
 _lib = None
 
