@@ -252,6 +252,7 @@ static int mlp_backward_impl(const srf_mlp_desc* desc, const void* packed, int n
   memset(&d, 0, sizeof(d));
   d.ldP = w.ldP;
   d.cols = w.cols;
+  d.paired = 1;
 
   int nj = 0;
   auto job = [&](size_t dz_off, int dz_row0, int dz_rows, size_t x_off, int x_rows, int M, int N, float* dw,

@@ -19,10 +19,10 @@
 //     (gfx950 global_load_lds_dwordx4) and shared by its four waves.
 //   * Weights are repacked once per parameter update (srf_mlp_pack) into per-lane 16-byte
 //     fragments: one 1 KB fragment per (output tile, k-unit).
-//   * Training stores, feature-major (row = feature, column = point), the inputs of every
-//     layer (16-bit or FP32) for the weight gradients, plus one 64-bit ReLU mask per lane per
-//     layer for the backward chain.  The transposition to feature-major goes through a 2.3 KB
-//     per-wave LDS tile so global stores are 16-byte vectors.
+//   * Training stores the inputs of every layer (16-bit or FP32) for the weight gradients, in
+//     16-feature tiles laid out as the GEMM's operand slices (store_tile: each lane writes one
+//     coalesced 32 B record straight from its accumulator registers after one pair exchange),
+//     plus one 64-bit ReLU mask per lane per layer for the backward chain.
 //   * Backward chain: the same transposed product with W^T tiles walks from the heads to
 //     layer 0, applying the stored ReLU masks, and writes every layer's pre-activation
 //     gradient dZ feature-major.
@@ -48,7 +48,6 @@ constexpr int kLayers = 11;        // kernel layers: trunk 0-7, head 8, view 9, 
 constexpr int kParams = SRF_MLP_NUM_PARAMS;
 constexpr int kMaskSlots = 9;      // ReLU masks of layers 0-7 and the view layer
 constexpr int kWaves = 4;          // waves per block
-constexpr int kLdsRowBytes = 144;  // 16 rows x 144 B transpose tile per wave
 constexpr int kMaxJobs = 16;
 
 // ----------------------------------------------------------------------------- plan
@@ -123,7 +122,8 @@ inline bool make_plan(const srf_mlp_desc* d, Plan* p) {
   return true;
 }
 
-// Training workspace: feature-major arrays with leading dimension ldP (points, multiple of 256).
+// Training workspace: arrays of 16-feature tiles (each the bytes of 16 rows of ldP points; the
+// layout inside a tile is store_tile's), ldP a multiple of 256 points.
 struct Ws {
   int cols;  // points processed: n rounded up to whole blocks of 256
   int ldP;   // row stride in points (cols, plus one block when that is an even number of blocks)
@@ -424,39 +424,54 @@ __device__ __forceinline__ void run_layer(const uint8_t* __restrict__ A, FIn&& f
   for (int to = 0; to < OUT; ++to) epi(TB + to, acc[to]);
 }
 
-// Writes one 16-feature tile of the wave's points feature-major: dst row r, column p0 + point.
+// Writes one 16-feature tile of the wave's points in the paired tile layout read by the weight-
+// gradient GEMM.  A tile of the whole workspace array occupies the bytes of 16 feature-major rows;
+// inside it, the wave owning points p0 .. p0 + 16 CB - 1 writes one 2 KB block of 64 lane records
+// of 32 B.  The lanes of a pair (j = 2m, 2m + 1) first swap half of their D-tile values
+// (__shfl_xor 1), so that record (g, 2m + e) holds features 4 g + 2 e + c (c = 0, 1), each as one
+// 16 B chunk of that feature at the points 16 cb + 2 m and 16 cb + 2 m + 1 (cb = 0 .. CB - 1), in
+// point order.  A 16 B chunk is then exactly one lane's MFMA operand slice of one feature, so the
+// GEMM reads fragments with plain 16 B copies and every store here is a coalesced 32 B per lane.
 template <int P>
-__device__ __forceinline__ void store_tile(uint8_t* __restrict__ dst, int ldP, int p0,
-                                           const typename Tr<P>::Act (&v)[Tr<P>::CB],
-                                           uint8_t* lds) {
+__device__ __forceinline__ void store_tile(uint8_t* __restrict__ dst, int p0, const typename Tr<P>::Act (&v)[Tr<P>::CB]) {
   constexpr int CB = Tr<P>::CB;
-  const int lane = threadIdx.x & 63, g = lane >> 4, j = lane & 15;
+  const int lane = threadIdx.x & 63;
+  const bool odd = lane & 1;
+  u32x4* rec = reinterpret_cast<u32x4*>(dst + (size_t)(p0 / (16 * CB)) * 2048 + lane * 32);
   if constexpr (P == kF32) {
-    float* L = reinterpret_cast<float*>(lds);
+    // v[cb] = features 4g + 0..3 at point 16 cb + j; even lanes keep r0, r1 and take the
+    // partner's; odd lanes keep r2, r3
+    float rc[CB][2];
 #pragma unroll
-    for (int cb = 0; cb < CB; ++cb)
+    for (int cb = 0; cb < CB; ++cb) {
+      rc[cb][0] = __shfl_xor(odd ? v[cb][0] : v[cb][2], 1, 64);
+      rc[cb][1] = __shfl_xor(odd ? v[cb][1] : v[cb][3], 1, 64);
+    }
+    f32x4 c0, c1;
 #pragma unroll
-      for (int r = 0; r < 4; ++r) L[(4 * g + r) * 36 + 16 * cb + j] = v[cb][r];
+    for (int cb = 0; cb < CB; ++cb) {
+      const float own0 = odd ? v[cb][2] : v[cb][0], own1 = odd ? v[cb][3] : v[cb][1];
+      c0[2 * cb + 0] = odd ? rc[cb][0] : own0;
+      c0[2 * cb + 1] = odd ? own0 : rc[cb][0];
+      c1[2 * cb + 0] = odd ? rc[cb][1] : own1;
+      c1[2 * cb + 1] = odd ? own1 : rc[cb][1];
+    }
+    rec[0] = __builtin_bit_cast(u32x4, c0);
+    rec[1] = __builtin_bit_cast(u32x4, c1);
   } else {
-    uint16_t* L = reinterpret_cast<uint16_t*>(lds);
+    // v[cb] = {bf16 (r0, r1), bf16 (r2, r3)} at point 16 cb + j
+    u32x4 c0, c1;
 #pragma unroll
-    for (int cb = 0; cb < CB; ++cb)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) L[(4 * g + r) * 72 + 16 * cb + j] = (uint16_t)act_elem_bits<P>(v[cb], r);
+    for (int cb = 0; cb < CB; ++cb) {
+      const uint32_t recv = __shfl_xor(odd ? v[cb][0] : v[cb][1], 1, 64);
+      const uint32_t own = odd ? v[cb][1] : v[cb][0];
+      const uint32_t first = odd ? recv : own, second = odd ? own : recv;  // points 2m, 2m + 1
+      c0[cb] = (first & 0xffffu) | (second << 16);
+      c1[cb] = (first >> 16) | (second & 0xffff0000u);
+    }
+    rec[0] = c0;
+    rec[1] = c1;
   }
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-  const int row = lane >> 2, seg = lane & 3;
-  const u32x4* src = reinterpret_cast<const u32x4*>(lds + row * kLdsRowBytes + seg * 32);
-  const u32x4 a = src[0], b = src[1];
-  const int col = P == kF32 ? p0 + 8 * seg : p0 + 16 * seg;
-  u32x4* out = reinterpret_cast<u32x4*>(dst + ((size_t)row * ldP + col) * (P == kF32 ? 4 : 2));
-  out[0] = a;
-  out[1] = b;
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 __device__ __forceinline__ float sigmoidf_(float x) { return 1.0f / (1.0f + expf(-x)); }
@@ -469,7 +484,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
     FwdArgs a) {
   constexpr int CB = Tr<P>::CB;
   using Act = typename Tr<P>::Act;
-  __shared__ __attribute__((aligned(16))) uint8_t lds_all[kWaves * 16 * kLdsRowBytes];
   __shared__ __attribute__((aligned(16))) uint8_t stage[2 * kStageBytes];
   // computed input features of each wave's points (trunk PE for layers 0 and 5, then the view
   // layer's [PE_xyz[trunk_pe:], PE_dir]) as B-operand tiles: 6 tiles x CB column blocks x 64
@@ -479,7 +493,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
   const int wv = threadIdx.x >> 6;
   const int wave = blockIdx.x * kWaves + wv;
   const int p0 = wave * 16 * CB;  // the grid covers cols exactly: every wave has points
-  uint8_t* lds = lds_all + wv * 16 * kLdsRowBytes;
   const Plan& pl = a.p;
   const int ldP = a.w.ldP;
 
@@ -516,7 +529,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
           tl[cb] = to_act<P>(v);
           pe_lds[wv][t][cb][lane] = tl[cb];
         }
-        if (TRAIN) store_tile<P>(a.ws + a.w.x0 + (size_t)16 * t * ldP * pl.elem, ldP, p0, tl, lds);
+        if (TRAIN) store_tile<P>(a.ws + a.w.x0 + (size_t)16 * t * ldP * pl.elem, p0, tl);
       }
     });
   };
@@ -530,7 +543,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
     }
   };
   auto bias_of = [&](int l) SRF_AI { return a.packed + pl.bias_off[l]; };
-  // ReLU epilogue: tile T of layer l into h[T] (in place), its feature-major copy at ws offset
+  // ReLU epilogue: tile T of layer l into h[T] (in place), its stored copy at ws offset
   // store_off and the mask bits.
   auto relu_epi = [&](int l, size_t store_off) SRF_AI {
     return [&, l, store_off](int T, f32x4(&acc)[CB]) SRF_AI {
@@ -547,7 +560,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
         t[cb] = to_act<P>(v);
         h[T][cb] = t[cb];
       }
-      if (TRAIN) store_tile<P>(a.ws + store_off + (size_t)16 * T * ldP * pl.elem, ldP, p0, t, lds);
+      if (TRAIN) store_tile<P>(a.ws + store_off + (size_t)16 * T * ldP * pl.elem, p0, t);
     };
   };
   auto fin_h = [&](int t, int cb) SRF_AI { return h[t][cb]; };
@@ -620,7 +633,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
         t[cb] = to_act<P>(acc[cb]);
         h[T][cb] = t[cb];
       }
-      if (TRAIN) store_tile<P>(a.ws + a.w.f + (size_t)16 * T * ldP * pl.elem, ldP, p0, t, lds);
+      if (TRAIN) store_tile<P>(a.ws + a.w.f + (size_t)16 * T * ldP * pl.elem, p0, t);
     }, stage, bias_of(8));
     // view layer input extras: [PE_xyz[trunk_pe:], PE_dir]
     const int n_hi = 3 * (2 * pl.pts_deg + 1) - pl.trunk_pe;
@@ -648,7 +661,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
           tl[cb] = to_act<P>(v);
           pe_lds[wv][t][cb][lane] = tl[cb];
         }
-        if (TRAIN) store_tile<P>(a.ws + a.w.ex + (size_t)16 * t * ldP * pl.elem, ldP, p0, tl, lds);
+        if (TRAIN) store_tile<P>(a.ws + a.w.ex + (size_t)16 * t * ldP * pl.elem, p0, tl);
       }
     });
 #pragma unroll
@@ -683,13 +696,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
     BwdArgs a) {
   constexpr int CB = Tr<P>::CB;
   using Act = typename Tr<P>::Act;
-  __shared__ __attribute__((aligned(16))) uint8_t lds_all[kWaves * 16 * kLdsRowBytes];
   __shared__ __attribute__((aligned(16))) uint8_t stage[2 * kStageBytes];
   const int lane = threadIdx.x & 63, g = lane >> 4, j = lane & 15;
   const int wv = threadIdx.x >> 6;
   const int wave = blockIdx.x * kWaves + wv;
   const int p0 = wave * 16 * CB;  // the grid covers cols exactly: every wave has points
-  uint8_t* lds = lds_all + wv * 16 * kLdsRowBytes;
   const Plan& pl = a.p;
   const int ldP = a.w.ldP;
 
@@ -719,7 +730,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
   Act z[18][CB];
   uint32_t mb[CB][2];
   auto store_dz = [&](int l, int T, const Act(&t)[CB]) SRF_AI {
-    store_tile<P>(a.ws + a.w.dz[l] + (size_t)16 * T * ldP * pl.elem, ldP, p0, t, lds);
+    store_tile<P>(a.ws + a.w.dz[l] + (size_t)16 * T * ldP * pl.elem, p0, t);
   };
   auto load_mask = [&](int slot) SRF_AI {
     const uint64_t* m = reinterpret_cast<const uint64_t*>(a.ws + a.w.mask + (size_t)slot * ldP * 32);
@@ -827,6 +838,7 @@ struct DwJob {
 struct DwArgs {
   DwJob jobs[kMaxJobs];
   int n_jobs, ldP, cols, chunk, n_chunks, total_work;
+  int paired;  // 1: dZ / X in the paired tile layout of store_tile; 0: plain feature-major rows
 };
 
 // One block of 8 waves reduces a 256 x 256 tile of dW = dZ X^T (a whole layer of the MLP) over a
@@ -892,8 +904,18 @@ __global__ __launch_bounds__(64 * kDwWaves) void mlp_dw_kernel(DwArgs a) {
       int row0 = (is_a ? m0 : n0) + 16 * rt;
       if (row0 >= (is_a ? jb.dz_rows : jb.x_rows)) row0 = is_a ? m0 : n0;
       const uint8_t* base = is_a ? jb.dz : jb.x;
-      const int pg = P == kF32 ? 4 * g : 8 * g;
-      const uint8_t* src = base + (((size_t)(row0 + j)) * ldP + p + pg) * ELEM;
+      const uint8_t* src;
+      if (a.paired) {
+        // lane (g, j) takes feature j's chunk of the record pair m = 4 u + g of the 2 KB block of
+        // the points' wave (see store_tile): record (j >> 2, 2 m + ((j >> 1) & 1)), chunk j & 1
+        constexpr int PPW = P == kF32 ? 32 : 64;  // points per stored wave block
+        const int wb = p / PPW, u = (p % PPW) / KPTS;
+        const int rec = 16 * (j >> 2) + 2 * (4 * u + g) + ((j >> 1) & 1);
+        src = base + (size_t)row0 * ldP * ELEM + (size_t)wb * 2048 + rec * 32 + (j & 1) * 16;
+      } else {
+        const int pg = P == kF32 ? 4 * g : 8 * g;
+        src = base + (((size_t)(row0 + j)) * ldP + p + pg) * ELEM;
+      }
       __builtin_amdgcn_global_load_lds((const void*)src,
                                        (__attribute__((address_space(3))) void*)(buf + f * 1024), 16, 0, 0);
     }
