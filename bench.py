@@ -132,6 +132,47 @@ def hip_train_leg(args, dev):
     return dict(seconds=t1 - t0, loss=float(loss.item()), kernels=kern, model=model)
 
 
+def hip_graph_train_leg(args, dev):
+    """The same train step captured once in a HIP graph (torch.cuda.graph: forward, hand-written
+    backward kernels, capturable Adam) and replayed: one launch per step instead of hundreds."""
+    import torch
+    from srf_hip import synthetic
+    _, model = build_model('bf16', dev)
+    model.train()
+    params = [p for p in model.parameters() if p.requires_grad]
+    opt = torch.optim.Adam(params, lr=5e-4, betas=(0.9, 0.999), capturable=True)
+    batch = synthetic.train_batch(args.rays, seed=0, device=dev)
+    inp = {'pixel_id': batch['pixel_id'], 'num_frames': 3, 'iter_num': 0, 'sub_batch_index': 0}
+
+    def step():
+        out = model(inp)
+        loss = synthetic.train_loss(out, batch)
+        loss.backward()
+        opt.step()
+        return loss
+
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        for _ in range(max(args.warmup, 2)):
+            opt.zero_grad(set_to_none=True)
+            step()
+    torch.cuda.current_stream().wait_stream(side)
+    graph = torch.cuda.CUDAGraph()
+    opt.zero_grad(set_to_none=True)
+    with torch.cuda.graph(graph):
+        loss = step()
+    for _ in range(2):
+        graph.replay()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        graph.replay()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    return dict(seconds=t1 - t0, loss=float(loss.item()))
+
+
 def hip_render_leg(args, dev, model):
     """Forward-only eval render of one chunk (deterministic sampling, no augmentations)."""
     import torch
@@ -396,6 +437,7 @@ def main():
     ap.add_argument('--tensorf-rays', type=int, default=8192)
     ap.add_argument('--tensorf-steps', type=int, default=5)
     ap.add_argument('--no-tensorf', action='store_true')
+    ap.add_argument('--no-graph', action='store_true', help='report the eager step only')
     ap.add_argument('--copy-calibration-mb', type=int, default=0,
                     help='after the timed legs, run 3 device copies of this many MiB (a plain copy kernel '
                          'of known size for scaling rocprof FETCH_SIZE / WRITE_SIZE)')
@@ -410,8 +452,16 @@ def main():
     srf_hip.lib()  # fail loudly without the HIP library
 
     tr = hip_train_leg(args, dev)
-    ms = tr['seconds'] * 1000.0 / args.steps
-    value = args.rays * args.steps / tr['seconds']
+    eager_value = args.rays * args.steps / tr['seconds']
+    graph_info = None
+    value, ms = eager_value, tr['seconds'] * 1000.0 / args.steps
+    if not args.no_graph:
+        try:
+            gr = hip_graph_train_leg(args, dev)
+            value, ms = args.rays * args.steps / gr['seconds'], gr['seconds'] * 1000.0 / args.steps
+            graph_info = {'captured': True}
+        except Exception as exc:  # reported, not hidden: value is then the eager step
+            graph_info = {'captured': False, 'error': f'{type(exc).__name__}: {exc}'[:300]}
     kern = tr['kernels']
     # dominant kernel: the single-kernel entry point with the largest total time
     single = {k: v for k, v in kern.items() if k not in ('srf_mlp_pack',)}
@@ -442,6 +492,9 @@ def main():
                                '192 merged samples, fwd+bwd+Adam', 'rays_per_step': args.rays,
                    'mlp': '8x256 skip@4 + 128 view head, BF16 MFMA / FP32 accumulate', 'parallelism': 'none'},
         'roofline': roof,
+        'step_mode': 'hip_graph_replay' if graph_info and graph_info['captured'] else 'eager',
+        'eager_rays_per_s': round(eager_value, 1),
+        'graph': graph_info,
         'render_rays_per_s': round(render, 1),
         'kernels': per_kernel,
     }

@@ -107,7 +107,7 @@ class SimpleNeRFHip(torch.nn.Module):
         image_id = input_batch['pixel_id'][:, 0]
         intrinsics = self.intrinsics_learner(image_id)
         extrinsics = self.extrinsics_learner(image_id)
-        all_extrinsics = self.extrinsics_learner(torch.arange(input_batch['num_frames']).to(image_id))
+        all_extrinsics = self.extrinsics_learner(torch.arange(input_batch['num_frames'], device=image_id.device, dtype=image_id.dtype))
         input_batch['intrinsics'] = intrinsics
         input_batch['extrinsics'] = extrinsics
         if mode == 'camera_params_only':
@@ -434,7 +434,7 @@ class ExtrinsicsLearner(torch.nn.Module):
         c2w = torch.cat([R, t.unsqueeze(2)], dim=2)
         c2w = torch.cat([c2w, torch.zeros_like(c2w[:, 0:1])], dim=1)
         c2w[:, 3, 3] = 1.0
-        return torch.linalg.inv(c2w)
+        return torch.linalg.inv_ex(c2w)[0]  # torch.linalg.inv without the host-synchronising check
 
     @staticmethod
     def Exp(r):

@@ -131,7 +131,7 @@ class SimpleTensoRFHip(torch.nn.Module):
         image_id = input_batch['pixel_id'][:, 0]
         intrinsics = self.intrinsics_learner(image_id)
         extrinsics = self.extrinsics_learner(image_id)
-        all_extrinsics = self.extrinsics_learner(torch.arange(input_batch['num_frames']).to(image_id))
+        all_extrinsics = self.extrinsics_learner(torch.arange(input_batch['num_frames'], device=image_id.device, dtype=image_id.dtype))
         input_batch['intrinsics'] = intrinsics
         input_batch['extrinsics'] = extrinsics
         if mode == 'camera_params_only':

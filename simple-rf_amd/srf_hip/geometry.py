@@ -23,7 +23,9 @@ def get_rays_tr(pixel_id: torch.Tensor, intrinsic: torch.Tensor, extrinsics: tor
         x = x + 0.5
         y = y + 0.5
     points_homo = torch.stack([x, y, torch.ones_like(x)], dim=1)
-    dirs = (torch.linalg.inv(intrinsic) @ points_homo[:, :, None])[:, :, 0]
+    # the same LU inverse as torch.linalg.inv, without its host-synchronising singularity check
+    # (keeps the step capturable in a HIP graph)
+    dirs = (torch.linalg.inv_ex(intrinsic)[0] @ points_homo[:, :, None])[:, :, 0]
     dirs[:, 1:] *= -1
     if nerf_synthetic:
         dirs[:, 1:] *= -1
