@@ -258,6 +258,44 @@ int srf_vm_appearance_bwd(const srf_vm_factors* factors, float* const* d_planes,
                           const int32_t* idx, const float* pts, const float* d_features,
                           void* stream);
 
+/* The CANDECOMP/PARAFAC factors of one CpDecomposedTensor field (src/models/SimpleTensoRF09.py:
+ * 964-1127): three lines of the same rank n_comp, line i being the reference's vectors_*[i] of
+ * shape [1, n_comp, line_len[i], 1] (built at :989-996), stored texel-major as
+ * [line_len[i]][n_comp] (16-byte aligned bases).  Line i is sampled at pts[2 - i] (vector_axes
+ * at :968).  n_comp % 4 == 0 and n_comp <= 256. */
+typedef struct srf_cp_factors {
+  int n_comp;
+  int line_len[3];
+  const float* lines[3];
+} srf_cp_factors;
+
+/* srf_vm_density_fwd for the CP decomposition: the density feature is
+ * sum_r line_0[r] * line_1[r] * line_2[r] (CpDecomposedTensor.get_volume_density,
+ * src/models/SimpleTensoRF09.py:1043-1062), with the same validity, alpha mask, predictor and
+ * outputs. */
+int srf_cp_density_fwd(const srf_cp_factors* factors, const float* bbox,
+                       const srf_alpha_grid* alpha, int64_t n_points, const float* pts,
+                       int predictor, float offset, float* sigma, float* feat, uint8_t* valid,
+                       void* stream);
+
+/* Backward of srf_cp_density_fwd into d_lines[3] (texel-major, FP32 atomics, accumulating). */
+int srf_cp_density_bwd(const srf_cp_factors* factors, float* const* d_lines, const float* bbox,
+                       int64_t n_points, const float* pts, const uint8_t* valid, const float* feat,
+                       int predictor, float offset, const float* d_sigma, void* stream);
+
+/* CP appearance features: features[j, r] = line_0[r] * line_1[r] * line_2[r] at sample idx[j]
+ * (idx NULL: sample j), [n_rows, n_comp] row-major.  Replaces the gather part of
+ * CpDecomposedTensor.get_color (src/models/SimpleTensoRF09.py:1064-1078, before
+ * basis_matrix_color). */
+int srf_cp_appearance_fwd(const srf_cp_factors* factors, const float* bbox, int64_t n_rows,
+                          const int32_t* idx, const float* pts, float* features, void* stream);
+
+/* Backward of srf_cp_appearance_fwd: d_features [n_rows, n_comp] scattered into d_lines[3]
+ * with FP32 atomics (accumulating). */
+int srf_cp_appearance_bwd(const srf_cp_factors* factors, float* const* d_lines, const float* bbox,
+                          int64_t n_rows, const int32_t* idx, const float* pts,
+                          const float* d_features, void* stream);
+
 /* Trilinear occupancy value of each point (AlphaGridMask.sample_alpha,
  * src/models/SimpleTensoRF09.py:1342-1345): out [n_points]. */
 int srf_alpha_mask_sample(const srf_alpha_grid* alpha, int64_t n_points, const float* pts,

@@ -89,6 +89,35 @@ def vm_appearance_features(planes, lines, pts_n: torch.Tensor) -> torch.Tensor:
     return (torch.cat(pcs) * torch.cat(lcs)).T
 
 
+def _line_coords(pts_n: torch.Tensor):
+    """Line coordinates of the CP decomposition, SimpleTensoRF09.py:1051-1052 (same as :1226-1227)."""
+    cl = torch.stack((pts_n[..., VECTOR_AXES[0]], pts_n[..., VECTOR_AXES[1]], pts_n[..., VECTOR_AXES[2]]))
+    return torch.stack((torch.zeros_like(cl), cl), dim=-1).detach().view(3, -1, 1, 2)
+
+
+def cp_component_products(lines, pts_n: torch.Tensor) -> torch.Tensor:
+    """[R, M] = (line_0 * line_1) * line_2 per component, the three lines of rank R sampled at
+    pts[2], pts[1], pts[0].  CpDecomposedTensor.get_volume_density / get_color,
+    SimpleTensoRF09.py:1051-1056 and :1072-1077; lines[i]: [1,R,L_i,1] (built at :989-996)."""
+    cl = _line_coords(pts_n)
+    m = pts_n.shape[0]
+    prod = F.grid_sample(lines[0], cl[[0]], align_corners=True).view(-1, m)
+    prod = prod * F.grid_sample(lines[1], cl[[1]], align_corners=True).view(-1, m)
+    prod = prod * F.grid_sample(lines[2], cl[[2]], align_corners=True).view(-1, m)
+    return prod
+
+
+def cp_density_features(lines, pts_n: torch.Tensor) -> torch.Tensor:
+    """sigma features = sum_r of the component products, SimpleTensoRF09.py:1057 (before the
+    density predictor)."""
+    return torch.sum(cp_component_products(lines, pts_n), dim=0)
+
+
+def cp_appearance_features(lines, pts_n: torch.Tensor) -> torch.Tensor:
+    """[M, R] component products, the input of basis_matrix_color at SimpleTensoRF09.py:1078."""
+    return cp_component_products(lines, pts_n).T
+
+
 def bilinear_align_corners(img: torch.Tensor, x: torch.Tensor, y: torch.Tensor) -> torch.Tensor:
     """Closed-form bilinear sampling with align_corners=True and zero padding (the
     textbook definition grid_sample implements); img [C,H,W], x,y [M] in [-1,1].
@@ -159,12 +188,20 @@ def tensorf_render(model: dict, pts: torch.Tensor, z_vals_ndc: torch.Tensor, ray
                    view_dirs: torch.Tensor, *, alpha_volume=None, distance_scale=25.0, weight_threshold=1e-4,
                    density_kind='ReLU', density_offset=-10.0, white_bkgd=False, features_pe_degree=0,
                    views_pe_degree=0):
-    """LowRankTensor.forward for the VM decomposition in NDC, SimpleTensoRF09.py:701-761.
+    """LowRankTensor.forward in NDC, SimpleTensoRF09.py:701-761, for the VM decomposition
+    (:1214-1263) or, when model['decomposition'] == 'CandecompParafac', the CP one (:1043-1089).
 
-    model: dict(bbox [2,3], planes_density, lines_density, planes_color, lines_color,
-    basis [27, sum c], color_mlp params).  view_dirs [N,3] (expanded per sample as at :735).
-    Returns the compositing dict plus raw sigma/rgb.
+    model: dict(bbox [2,3], planes_density, lines_density, planes_color, lines_color (no planes
+    for CP), basis [27, sum c], color_mlp params).  view_dirs [N,3] (expanded per sample as at
+    :735).  Returns the compositing dict plus raw sigma/rgb.
     """
+    cp = model.get('decomposition', 'VectorMatrix') == 'CandecompParafac'
+    if cp:
+        density_features = lambda q: cp_density_features(model['lines_density'], q)
+        appearance_features = lambda q: cp_appearance_features(model['lines_color'], q)
+    else:
+        density_features = lambda q: vm_density_features(model['planes_density'], model['lines_density'], q)
+        appearance_features = lambda q: vm_appearance_features(model['planes_color'], model['lines_color'], q)
     from .nerf import volume_rendering
     bbox = model['bbox']
     valid = bbox_validity(pts, bbox)
@@ -174,7 +211,7 @@ def tensorf_render(model: dict, pts: torch.Tensor, z_vals_ndc: torch.Tensor, ray
     pts_n = normalize_points(pts, bbox)
     sigma = torch.zeros(pts.shape[:-1], dtype=pts.dtype)
     if valid.any():
-        feat = vm_density_features(model['planes_density'], model['lines_density'], pts_n[valid])
+        feat = density_features(pts_n[valid])
         sigma[valid] = density_predictor(feat, density_kind, density_offset)
     out = volume_rendering(sigma, torch.zeros(*pts.shape[:-1], 3, dtype=pts.dtype), z_vals_ndc=z_vals_ndc,
                            rays_o=rays_o, rays_d=rays_d, rays_d_ndc=rays_d_ndc, ndc=True,
@@ -182,7 +219,7 @@ def tensorf_render(model: dict, pts: torch.Tensor, z_vals_ndc: torch.Tensor, ray
     surface = out['weights'] > weight_threshold
     rgb = torch.zeros(*pts.shape[:-1], 3, dtype=pts.dtype)
     if surface.any():
-        app = vm_appearance_features(model['planes_color'], model['lines_color'], pts_n[surface])
+        app = appearance_features(pts_n[surface])
         feats = F.linear(app, model['basis'])
         vd = view_dirs[:, None].expand(pts.shape)[surface]
         rgb[surface] = color_mlp_forward(model['color_mlp'], feats, vd, features_pe_degree, views_pe_degree)

@@ -1,8 +1,10 @@
-// TensoRF vector-matrix (VM) factor sampling on gfx950.
+// TensoRF factor sampling on gfx950: the vector-matrix (VM) and CANDECOMP/PARAFAC (CP)
+// decompositions.
 //
 // Reference semantics: LowRankTensor.forward validity + normalize_points
 // (src/models/SimpleTensoRF09.py:701-765), AlphaGridMask.sample_alpha (:1342-1349),
 // VmDecomposedTensor.get_volume_density / get_color before the basis matrix (:1214-1263),
+// CpDecomposedTensor.get_volume_density / get_color before the basis matrix (:1043-1078),
 // the density predictors (:671-679) and the backward of torch.nn.functional.grid_sample
 // (bilinear, align_corners=True, zero padding) that autograd runs for them.
 //
@@ -449,6 +451,194 @@ __global__ __launch_bounds__(256) void appearance_bwd_kernel(VmArgs a, int64_t n
   }
 }
 
+// ------------------------------------------------------------ CP decomposition (3 lines)
+// CpDecomposedTensor (SimpleTensoRF09.py:1043-1089): every component r is the product of the
+// three lines at pts[2], pts[1], pts[0], multiplied in that order.  The lines share one rank, so
+// a lane's quad reads the same 4 components of all three lines.
+
+struct CpArgs {
+  const float* line[kPlanes];
+  float* dline[kPlanes];
+  int ll[kPlanes];
+  int comp, n_quads;
+  float lo[3], hi[3], size[3];
+};
+
+struct CpTaps {
+  LineTaps t[kPlanes];
+};
+
+__device__ __forceinline__ CpTaps cp_taps(const CpArgs& a, const float pn[3]) {
+  CpTaps s;
+#pragma unroll
+  for (int k = 0; k < kPlanes; ++k) s.t[k] = line_taps(pn[axis_v(k)], a.ll[k]);
+  return s;
+}
+
+// Linear line values of the 4 components starting at c, for the three lines.
+__device__ __forceinline__ void cp_values(const CpArgs& a, const CpTaps& s, int c, float4 v[kPlanes]) {
+#pragma unroll
+  for (int k = 0; k < kPlanes; ++k) {
+    v[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+    for (int t = 0; t < 2; ++t)
+      if (s.t[k].idx[t] >= 0) v[k] = madd4(v[k], ld4(a.line[k] + (int64_t)s.t[k].idx[t] * a.comp + c), s.t[k].w[t]);
+  }
+}
+
+__device__ __forceinline__ float4 cp_product(const float4 v[kPlanes]) { return mul4(mul4(v[0], v[1]), v[2]); }
+
+// Run-length gradient accumulation for the three lines of one quad (as RunAcc for VM).
+struct CpRun {
+  int idx[kPlanes][2];
+  float4 g[kPlanes][2];
+};
+
+__device__ __forceinline__ void cp_run_init(CpRun& r) {
+#pragma unroll
+  for (int k = 0; k < kPlanes; ++k)
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+      r.idx[k][t] = -1;
+      r.g[k][t] = make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+}
+
+__device__ __forceinline__ void cp_run_flush(const CpArgs& a, int k, int c, CpRun& r) {
+#pragma unroll
+  for (int t = 0; t < 2; ++t) {
+    if (r.idx[k][t] >= 0) atomic_add4v(a.dline[k] + (int64_t)r.idx[k][t] * a.comp + c, r.g[k][t]);
+    r.g[k][t] = make_float4(0.f, 0.f, 0.f, 0.f);
+  }
+}
+
+// Adds the gradient g of the quad's component products: line k receives g times the other two
+// lines' values, spread over its two taps.
+__device__ __forceinline__ void cp_run_add(const CpArgs& a, const CpTaps& s, int c, CpRun& r, float4 g,
+                                           const float4 v[kPlanes]) {
+  const float4 d[kPlanes] = {mul4(mul4(g, v[1]), v[2]), mul4(mul4(g, v[0]), v[2]), mul4(mul4(g, v[0]), v[1])};
+#pragma unroll
+  for (int k = 0; k < kPlanes; ++k) {
+    if (s.t[k].idx[0] != r.idx[k][0] || s.t[k].idx[1] != r.idx[k][1]) {
+      cp_run_flush(a, k, c, r);
+      r.idx[k][0] = s.t[k].idx[0];
+      r.idx[k][1] = s.t[k].idx[1];
+    }
+#pragma unroll
+    for (int t = 0; t < 2; ++t) r.g[k][t] = madd4(r.g[k][t], d[k], s.t[k].w[t]);
+  }
+}
+
+template <int G>
+__global__ __launch_bounds__(256) void cp_density_fwd_kernel(CpArgs a, AlphaArgs m, int64_t n,
+                                                             const float* __restrict__ pts, int kind,
+                                                             float offset, float* __restrict__ sigma,
+                                                             float* __restrict__ feat_out,
+                                                             uint8_t* __restrict__ valid_out) {
+  const GroupPos gp = group_pos<G>();
+  for (int64_t i = gp.first; i < n; i += gp.stride) {
+    const float p[3] = {pts[3 * i + 0], pts[3 * i + 1], pts[3 * i + 2]};
+    bool valid = in_box(p, a.lo, a.hi);
+    if (valid && m.vol) valid = alpha_value(m, p) > 0.f;
+    float pn[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) pn[k] = normalize_coord(p[k], a.lo[k], a.size[k]);
+    float s = 0.f;
+    if (valid) {
+      const CpTaps taps = cp_taps(a, pn);
+      for (int q = gp.ql; q < a.n_quads; q += G) {
+        float4 v[kPlanes];
+        cp_values(a, taps, 4 * q, v);
+        s = __fadd_rn(s, sum4(cp_product(v)));
+      }
+    }
+    s = group_sum<G>(s);
+    if (gp.ql == 0) {
+      sigma[i] = valid ? predict_density(s, kind, offset) : 0.f;
+      if (feat_out) feat_out[i] = valid ? s : 0.f;
+      if (valid_out) valid_out[i] = valid ? 1 : 0;
+    }
+  }
+}
+
+template <int G>
+__global__ __launch_bounds__(256) void cp_density_bwd_kernel(CpArgs a, int64_t n, const float* __restrict__ pts,
+                                                             const uint8_t* __restrict__ valid,
+                                                             const float* __restrict__ feat, int kind,
+                                                             float offset, const float* __restrict__ d_sigma) {
+  const GroupPos gp = group_pos<G>();
+  const int64_t n_runs = (n + kRun - 1) / kRun;
+  for (int64_t run = gp.first; run < n_runs; run += gp.stride) {
+    const int64_t i0 = run * kRun;
+    const int64_t i1 = i0 + kRun < n ? i0 + kRun : n;
+    for (int q = gp.ql; q < a.n_quads; q += G) {
+      CpRun r;
+      cp_run_init(r);
+      for (int64_t i = i0; i < i1; ++i) {
+        if (!valid[i]) continue;
+        const float g = __fmul_rn(d_sigma[i], predict_density_grad(feat[i], kind, offset));
+        if (g == 0.f) continue;
+        float pn[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) pn[k] = normalize_coord(pts[3 * i + k], a.lo[k], a.size[k]);
+        const CpTaps taps = cp_taps(a, pn);
+        float4 v[kPlanes];
+        cp_values(a, taps, 4 * q, v);
+        cp_run_add(a, taps, 4 * q, r, make_float4(g, g, g, g), v);
+      }
+#pragma unroll
+      for (int k = 0; k < kPlanes; ++k) cp_run_flush(a, k, 4 * q, r);
+    }
+  }
+}
+
+template <int G>
+__global__ __launch_bounds__(256) void cp_appearance_fwd_kernel(CpArgs a, int64_t n, const int32_t* __restrict__ idx,
+                                                                const float* __restrict__ pts,
+                                                                float* __restrict__ out) {
+  const GroupPos gp = group_pos<G>();
+  for (int64_t j = gp.first; j < n; j += gp.stride) {
+    const int64_t i = idx ? (int64_t)idx[j] : j;
+    float pn[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) pn[k] = normalize_coord(pts[3 * i + k], a.lo[k], a.size[k]);
+    const CpTaps taps = cp_taps(a, pn);
+    for (int q = gp.ql; q < a.n_quads; q += G) {
+      float4 v[kPlanes];
+      cp_values(a, taps, 4 * q, v);
+      *reinterpret_cast<float4*>(out + j * a.comp + 4 * q) = cp_product(v);
+    }
+  }
+}
+
+template <int G>
+__global__ __launch_bounds__(256) void cp_appearance_bwd_kernel(CpArgs a, int64_t n, const int32_t* __restrict__ idx,
+                                                                const float* __restrict__ pts,
+                                                                const float* __restrict__ d_out) {
+  const GroupPos gp = group_pos<G>();
+  const int64_t n_runs = (n + kRun - 1) / kRun;
+  for (int64_t run = gp.first; run < n_runs; run += gp.stride) {
+    const int64_t j0 = run * kRun;
+    const int64_t j1 = j0 + kRun < n ? j0 + kRun : n;
+    for (int q = gp.ql; q < a.n_quads; q += G) {
+      CpRun r;
+      cp_run_init(r);
+      for (int64_t j = j0; j < j1; ++j) {
+        const int64_t i = idx ? (int64_t)idx[j] : j;
+        float pn[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) pn[k] = normalize_coord(pts[3 * i + k], a.lo[k], a.size[k]);
+        const CpTaps taps = cp_taps(a, pn);
+        float4 v[kPlanes];
+        cp_values(a, taps, 4 * q, v);
+        cp_run_add(a, taps, 4 * q, r, ld4(d_out + j * a.comp + 4 * q), v);
+      }
+#pragma unroll
+      for (int k = 0; k < kPlanes; ++k) cp_run_flush(a, k, 4 * q, r);
+    }
+  }
+}
+
 __global__ __launch_bounds__(256) void alpha_sample_kernel(AlphaArgs m, int64_t n, const float* __restrict__ pts,
                                                            float* __restrict__ out) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
@@ -576,6 +766,32 @@ int make_vm_args(const srf_vm_factors* f, const float* bbox, VmArgs* a, bool nee
   return SRF_OK;
 }
 
+int make_cp_args(const srf_cp_factors* f, const float* bbox, CpArgs* a) {
+  if (!f || !bbox) return SRF_EINVAL;
+  memset(a, 0, sizeof(*a));
+  if (f->n_comp < 4 || f->n_comp % 4 != 0) return SRF_EINVAL;
+  if (f->n_comp / 4 > kMaxQuads) return SRF_EUNSUPPORTED;
+  for (int k = 0; k < kPlanes; ++k) {
+    if (f->line_len[k] < 1 || !f->lines[k]) return SRF_EINVAL;
+    if (reinterpret_cast<uintptr_t>(f->lines[k]) & 15) return SRF_EINVAL;  // float4 taps
+    a->line[k] = f->lines[k];
+    a->ll[k] = f->line_len[k];
+  }
+  a->comp = f->n_comp;
+  a->n_quads = f->n_comp / 4;
+  set_box(a->lo, a->hi, a->size, bbox);
+  return SRF_OK;
+}
+
+int set_cp_grads(CpArgs* a, float* const* d_lines) {
+  if (!d_lines) return SRF_EINVAL;
+  for (int k = 0; k < kPlanes; ++k) {
+    if (!d_lines[k] || (reinterpret_cast<uintptr_t>(d_lines[k]) & 3)) return SRF_EINVAL;
+    a->dline[k] = d_lines[k];
+  }
+  return SRF_OK;
+}
+
 int make_alpha_args(const srf_alpha_grid* g, AlphaArgs* m) {
   memset(m, 0, sizeof(*m));
   if (!g || !g->volume) return SRF_OK;
@@ -687,6 +903,70 @@ extern "C" int srf_vm_appearance_bwd(const srf_vm_factors* factors, float* const
   SRF_VM_DISPATCH(G, appearance_bwd_kernel, dim3(grid_for((n_rows + kRun - 1) / kRun, G)), dim3(256), 0,
                   as_stream(stream), a,
                   n_rows, idx, pts, d_features);
+  return launch_status();
+}
+
+extern "C" int srf_cp_density_fwd(const srf_cp_factors* factors, const float* bbox,
+                                  const srf_alpha_grid* alpha, int64_t n_points, const float* pts,
+                                  int predictor, float offset, float* sigma, float* feat,
+                                  uint8_t* valid, void* stream) {
+  if (n_points < 0 || (predictor != SRF_DENSITY_RELU && predictor != SRF_DENSITY_SOFTPLUS)) return SRF_EINVAL;
+  if (n_points == 0) return SRF_OK;
+  CpArgs a;
+  int st = make_cp_args(factors, bbox, &a);
+  if (st != SRF_OK) return st;
+  AlphaArgs m;
+  if ((st = make_alpha_args(alpha, &m)) != SRF_OK) return st;
+  if (!pts || !sigma) return SRF_EINVAL;
+  const int G = next_pow2(a.n_quads);
+  SRF_VM_DISPATCH(G, cp_density_fwd_kernel, dim3(grid_for(n_points, G)), dim3(256), 0, as_stream(stream), a, m,
+                  n_points, pts, predictor, offset, sigma, feat, valid);
+  return launch_status();
+}
+
+extern "C" int srf_cp_density_bwd(const srf_cp_factors* factors, float* const* d_lines, const float* bbox,
+                                  int64_t n_points, const float* pts, const uint8_t* valid, const float* feat,
+                                  int predictor, float offset, const float* d_sigma, void* stream) {
+  if (n_points < 0 || (predictor != SRF_DENSITY_RELU && predictor != SRF_DENSITY_SOFTPLUS)) return SRF_EINVAL;
+  if (n_points == 0) return SRF_OK;
+  CpArgs a;
+  int st = make_cp_args(factors, bbox, &a);
+  if (st != SRF_OK) return st;
+  if ((st = set_cp_grads(&a, d_lines)) != SRF_OK) return st;
+  if (!pts || !valid || !feat || !d_sigma) return SRF_EINVAL;
+  const int G = next_pow2(a.n_quads);
+  SRF_VM_DISPATCH(G, cp_density_bwd_kernel, dim3(grid_for((n_points + kRun - 1) / kRun, G)), dim3(256), 0,
+                  as_stream(stream), a, n_points, pts, valid, feat, predictor, offset, d_sigma);
+  return launch_status();
+}
+
+extern "C" int srf_cp_appearance_fwd(const srf_cp_factors* factors, const float* bbox, int64_t n_rows,
+                                     const int32_t* idx, const float* pts, float* features, void* stream) {
+  if (n_rows < 0) return SRF_EINVAL;
+  if (n_rows == 0) return SRF_OK;
+  CpArgs a;
+  int st = make_cp_args(factors, bbox, &a);
+  if (st != SRF_OK) return st;
+  if (!pts || !features || (reinterpret_cast<uintptr_t>(features) & 15)) return SRF_EINVAL;
+  const int G = next_pow2(a.n_quads);
+  SRF_VM_DISPATCH(G, cp_appearance_fwd_kernel, dim3(grid_for(n_rows, G)), dim3(256), 0, as_stream(stream), a,
+                  n_rows, idx, pts, features);
+  return launch_status();
+}
+
+extern "C" int srf_cp_appearance_bwd(const srf_cp_factors* factors, float* const* d_lines, const float* bbox,
+                                     int64_t n_rows, const int32_t* idx, const float* pts,
+                                     const float* d_features, void* stream) {
+  if (n_rows < 0) return SRF_EINVAL;
+  if (n_rows == 0) return SRF_OK;
+  CpArgs a;
+  int st = make_cp_args(factors, bbox, &a);
+  if (st != SRF_OK) return st;
+  if ((st = set_cp_grads(&a, d_lines)) != SRF_OK) return st;
+  if (!pts || !d_features || (reinterpret_cast<uintptr_t>(d_features) & 15)) return SRF_EINVAL;
+  const int G = next_pow2(a.n_quads);
+  SRF_VM_DISPATCH(G, cp_appearance_bwd_kernel, dim3(grid_for((n_rows + kRun - 1) / kRun, G)), dim3(256), 0,
+                  as_stream(stream), a, n_rows, idx, pts, d_features);
   return launch_status();
 }
 

@@ -3,16 +3,18 @@
 Selected by the reference's plugin loader (src/models/ModelFactory02.py:10-22) with
 ``configs['model']['name'] = 'SimpleTensoRFHip09'`` (class ``SimpleTensoRFHip``).  The class
 surface, config keys, output-dict keys and state_dict names/shapes are those of SimpleTensoRF
-(src/models/SimpleTensoRF09.py:17-1421) for the vector-matrix decomposition.  The hot path runs
-on the HIP kernels:
+(src/models/SimpleTensoRF09.py:17-1421) for the vector-matrix and CP decompositions.  The hot
+path runs on the HIP kernels:
 
   get_z_vals_coarse (NDC)        -> srf_stratified_samples             (SimpleTensoRF09.py:364-381)
   get_z_vals_coarse (AABB)       -> srf_aabb_samples                   (:382-400)
   get_z_vals_fine                -> srf_fine_z_vals                    (:403-414, sample_pdf :427-460)
   validity + alpha mask + density-> srf_vm_density_fwd / _bwd          (:701-721, :1214-1239, :1342-1345)
+                                    (CP: srf_cp_density_fwd / _bwd      :1043-1062)
   get_volume_rendering_weights   -> srf_composite_fwd / _bwd (no rgb)  (:767-811)
   surface mask + colour gather   -> srf_threshold_compact,
                                     srf_vm_appearance_fwd / _bwd       (:726, :1241-1263)
+                                    (CP: srf_cp_appearance_fwd / _bwd   :1064-1078)
   volume_render                  -> srf_volume_render_fwd / _bwd       (:814-817)
 
   basis_matrix_color + color MLP -> srf_color_forward / _backward        (:1262-1270, :1370-1421)
@@ -21,8 +23,8 @@ The factor
 parameters keep the reference's shapes but are stored texel-major (component axis innermost),
 which is the layout the gather kernels read.  Random draws of the path (t_rand :376, the
 per-ray offset :396, sample_pdf's u) come from the GPU generator and are passed to the kernels.
-Options outside the path raise NotImplementedError: visibility prediction, the CP
-decomposition and the periodic model modifications (alpha-mask update, shrink, upsampling).
+Options outside the path raise NotImplementedError: visibility prediction and the periodic
+model modifications (alpha-mask update, shrink, upsampling).
 """
 from __future__ import annotations
 
@@ -324,7 +326,7 @@ class SimpleTensoRFHip(torch.nn.Module):
         if kind == 'VectorMatrix':
             return VmDecomposedTensor(name, configs, tensor_configs, model_configs)
         if kind == 'CandecompParafac':
-            raise NotImplementedError('the CP decomposition is not on the HIP path yet')
+            return CpDecomposedTensor(name, configs, tensor_configs, model_configs)
         raise RuntimeError(f'Unknown tensor decomposition: {kind}')
 
 
@@ -396,8 +398,12 @@ class AlphaGridMask(torch.nn.Module):
         return tensorf.alpha_mask_sample(self.kernel_mask(), pts)
 
 
-class VmDecomposedTensor(torch.nn.Module):
-    """LowRankTensor + VmDecomposedTensor (src/models/SimpleTensoRF09.py:582-1272), VM factors only."""
+class LowRankTensor(torch.nn.Module):
+    """LowRankTensor (src/models/SimpleTensoRF09.py:582-961): geometry buffers, alpha mask, colour
+    network and the forward on the HIP kernels.  The decompositions below supply the factor
+    parameters and their density / appearance gathers."""
+
+    FACTOR_LISTS = ()  # names of the ParameterLists holding texel-major factors
 
     def __init__(self, name, configs, tensor_configs, model_configs):
         super().__init__()
@@ -484,33 +490,22 @@ class VmDecomposedTensor(torch.nn.Module):
 
     # ------------------------------------------------------------- factors
     def build_tensor(self):
-        """src/models/SimpleTensoRF09.py:1144-1152."""
-        self.matrices_density, self.vectors_density = self.create_decomposed_tensor(
-            self.tensor_configs['num_components_density'], self.resolution, init_params_scale=0.1)
-        self.matrices_color, self.vectors_color = self.create_decomposed_tensor(
-            self.tensor_configs['num_components_color'], self.resolution, init_params_scale=0.1)
-        self.basis_matrix_color = torch.nn.Linear(sum(self.tensor_configs['num_components_color']),
-                                                  self.tensor_configs['features_dimension_color'], bias=False)
+        raise NotImplementedError
 
-    @staticmethod
-    def create_decomposed_tensor(num_components, resolution, init_params_scale):
-        """src/models/SimpleTensoRF09.py:1154-1165, with texel-major storage."""
-        matrices, vectors = [], []
-        for i in range(3):
-            a0, a1 = MATRIX_AXES[i]
-            v = VECTOR_AXES[i]
-            h, w, l = int(resolution[a1]), int(resolution[a0]), int(resolution[v])
-            c = num_components[i]
-            matrices.append(tensorf.texel_major_param(c, h, w, init_params_scale * torch.randn((1, c, h, w))))
-            vectors.append(tensorf.texel_major_param(c, l, 1, init_params_scale * torch.randn((1, c, l, 1))))
-        return torch.nn.ParameterList(matrices), torch.nn.ParameterList(vectors)
+    def _density(self, pts, bbox, alpha, offset):
+        """sigma [N, S] of every sample (0 where invalid)."""
+        raise NotImplementedError
+
+    def _appearance(self, pts, bbox, idx, n_rows):
+        """Appearance features [n_rows, n_features] of the samples idx[:n_rows]."""
+        raise NotImplementedError
 
     def _apply(self, fn, recurse=True):
         """Module.to() / .cuda() rebuild parameters with fn, which makes them contiguous in the
         reference's [1, c, H, W] layout; restore the texel-major storage afterwards."""
         out = super()._apply(fn, recurse)
-        for plist in (getattr(self, 'matrices_density', None), getattr(self, 'vectors_density', None),
-                      getattr(self, 'matrices_color', None), getattr(self, 'vectors_color', None)):
+        for name in self.FACTOR_LISTS:
+            plist = getattr(self, name, None)
             if plist is None:
                 continue
             for i, p in enumerate(plist):
@@ -521,22 +516,6 @@ class VmDecomposedTensor(torch.nn.Module):
                     plist[i] = np_
         self._host = None
         return out
-
-    def get_trainable_parameters(self, optimizer_configs):
-        """src/models/SimpleTensoRF09.py:1167-1194."""
-        tensor_params = torch.nn.ParameterList()
-        network_params = torch.nn.ParameterList()
-        tensor_params.extend(self.vectors_density)
-        tensor_params.extend(self.matrices_density)
-        tensor_params.extend(self.vectors_color)
-        tensor_params.extend(self.matrices_color)
-        network_params.extend(self.basis_matrix_color.parameters())
-        network_params.extend(self.color_predictor.parameters())
-        return [
-            {'name': f'{self.name}_tensor_params', 'params': tensor_params, 'lr': optimizer_configs['lr_initial_tensor']},
-            {'name': f'{self.name}_network_params', 'params': network_params,
-             'lr': optimizer_configs['lr_initial_network']},
-        ]
 
     def run_model_modifications(self, iter_num):
         """src/models/SimpleTensoRF09.py:820-829.  The alpha-mask update, shrink and upsampling
@@ -560,9 +539,7 @@ class VmDecomposedTensor(torch.nn.Module):
         shape = pts.shape[:-1]
         n_rays, n_samples = shape
         alpha = self.alpha_mask.kernel_mask() if self.alpha_mask is not None else None
-        sigma, _ = tensorf.vm_density(pts, list(self.matrices_density), list(self.vectors_density), hp['bbox'],
-                                      alpha=alpha, predictor=self.density_kind,
-                                      offset=float(self.tensor_configs.get('density_offset', 0.0)))
+        sigma = self._density(pts, hp['bbox'], alpha, float(self.tensor_configs.get('density_offset', 0.0)))
         out = self.get_volume_rendering_weights(input_dict, sigma)
         weights = out['weights']
         idx, count = tensorf.threshold_compact(weights, self.tensor_configs['ray_marching_weight_threshold'])
@@ -570,8 +547,7 @@ class VmDecomposedTensor(torch.nn.Module):
         rgb_flat = torch.zeros((n_rays * n_samples, 3), dtype=torch.float32, device=pts.device)
         if m > 0:
             sidx = idx[:m]
-            app = tensorf.vm_appearance(pts, list(self.matrices_color), list(self.vectors_color), hp['bbox'],
-                                        idx=sidx, n_rows=m)
+            app = self._appearance(pts, hp['bbox'], sidx, m)
             vd, per_ray = None, n_samples
             if self.tensor_configs['view_dependent_color']:
                 vd = input_dict['view_dirs']
@@ -609,3 +585,106 @@ class VmDecomposedTensor(torch.nn.Module):
     def volume_render(raw_value, weights):
         """src/models/SimpleTensoRF09.py:814-817."""
         return compositing.volume_render(raw_value, weights)
+
+
+class VmDecomposedTensor(LowRankTensor):
+    """VmDecomposedTensor (src/models/SimpleTensoRF09.py:1126-1272): three planes and three lines
+    per field, gathered by srf_vm_density_* / srf_vm_appearance_*."""
+
+    FACTOR_LISTS = ('matrices_density', 'vectors_density', 'matrices_color', 'vectors_color')
+
+    def build_tensor(self):
+        """src/models/SimpleTensoRF09.py:1144-1152."""
+        self.matrices_density, self.vectors_density = self.create_decomposed_tensor(
+            self.tensor_configs['num_components_density'], self.resolution, init_params_scale=0.1)
+        self.matrices_color, self.vectors_color = self.create_decomposed_tensor(
+            self.tensor_configs['num_components_color'], self.resolution, init_params_scale=0.1)
+        self.basis_matrix_color = torch.nn.Linear(sum(self.tensor_configs['num_components_color']),
+                                                  self.tensor_configs['features_dimension_color'], bias=False)
+
+    @staticmethod
+    def create_decomposed_tensor(num_components, resolution, init_params_scale):
+        """src/models/SimpleTensoRF09.py:1154-1165, with texel-major storage."""
+        matrices, vectors = [], []
+        for i in range(3):
+            a0, a1 = MATRIX_AXES[i]
+            v = VECTOR_AXES[i]
+            h, w, l = int(resolution[a1]), int(resolution[a0]), int(resolution[v])
+            c = num_components[i]
+            matrices.append(tensorf.texel_major_param(c, h, w, init_params_scale * torch.randn((1, c, h, w))))
+            vectors.append(tensorf.texel_major_param(c, l, 1, init_params_scale * torch.randn((1, c, l, 1))))
+        return torch.nn.ParameterList(matrices), torch.nn.ParameterList(vectors)
+
+    def get_trainable_parameters(self, optimizer_configs):
+        """src/models/SimpleTensoRF09.py:1167-1194."""
+        tensor_params = torch.nn.ParameterList()
+        network_params = torch.nn.ParameterList()
+        tensor_params.extend(self.vectors_density)
+        tensor_params.extend(self.matrices_density)
+        tensor_params.extend(self.vectors_color)
+        tensor_params.extend(self.matrices_color)
+        network_params.extend(self.basis_matrix_color.parameters())
+        network_params.extend(self.color_predictor.parameters())
+        return [
+            {'name': f'{self.name}_tensor_params', 'params': tensor_params, 'lr': optimizer_configs['lr_initial_tensor']},
+            {'name': f'{self.name}_network_params', 'params': network_params,
+             'lr': optimizer_configs['lr_initial_network']},
+        ]
+
+    def _density(self, pts, bbox, alpha, offset):
+        sigma, _ = tensorf.vm_density(pts, list(self.matrices_density), list(self.vectors_density), bbox,
+                                      alpha=alpha, predictor=self.density_kind, offset=offset)
+        return sigma
+
+    def _appearance(self, pts, bbox, idx, n_rows):
+        return tensorf.vm_appearance(pts, list(self.matrices_color), list(self.vectors_color), bbox, idx=idx,
+                                     n_rows=n_rows)
+
+
+class CpDecomposedTensor(LowRankTensor):
+    """CpDecomposedTensor (src/models/SimpleTensoRF09.py:964-1127): three lines of one rank per
+    field (num_components_*[0]), gathered by srf_cp_density_* / srf_cp_appearance_*."""
+
+    FACTOR_LISTS = ('vectors_density', 'vectors_color')
+
+    def build_tensor(self):
+        """src/models/SimpleTensoRF09.py:979-987."""
+        self.vectors_density = self.create_decomposed_tensor(self.tensor_configs['num_components_density'],
+                                                             self.resolution, init_params_scale=0.1)
+        self.vectors_color = self.create_decomposed_tensor(self.tensor_configs['num_components_color'],
+                                                           self.resolution, init_params_scale=0.1)
+        self.basis_matrix_color = torch.nn.Linear(sum(self.tensor_configs['num_components_color']),
+                                                  self.tensor_configs['features_dimension_color'], bias=False)
+
+    @staticmethod
+    def create_decomposed_tensor(num_components, resolution, init_params_scale):
+        """src/models/SimpleTensoRF09.py:989-996 (every line has num_components[0] components),
+        with texel-major storage."""
+        vectors = []
+        c = num_components[0]
+        for i in range(3):
+            l = int(resolution[VECTOR_AXES[i]])
+            vectors.append(tensorf.texel_major_param(c, l, 1, init_params_scale * torch.randn((1, c, l, 1))))
+        return torch.nn.ParameterList(vectors)
+
+    def get_trainable_parameters(self, optimizer_configs):
+        """src/models/SimpleTensoRF09.py:998-1023."""
+        tensor_params = torch.nn.ParameterList()
+        network_params = torch.nn.ParameterList()
+        tensor_params.extend(self.vectors_density)
+        tensor_params.extend(self.vectors_color)
+        network_params.extend(self.basis_matrix_color.parameters())
+        network_params.extend(self.color_predictor.parameters())
+        return [
+            {'name': f'{self.name}_tensor_params', 'params': tensor_params, 'lr': optimizer_configs['lr_initial_tensor']},
+            {'name': f'{self.name}_network_params', 'params': network_params,
+             'lr': optimizer_configs['lr_initial_network']},
+        ]
+
+    def _density(self, pts, bbox, alpha, offset):
+        sigma, _ = tensorf.cp_density(pts, list(self.vectors_density), bbox, alpha=alpha, predictor=self.density_kind,
+                                      offset=offset)
+        return sigma
+
+    def _appearance(self, pts, bbox, idx, n_rows):
+        return tensorf.cp_appearance(pts, list(self.vectors_color), bbox, idx=idx, n_rows=n_rows)

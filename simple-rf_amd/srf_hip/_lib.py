@@ -25,6 +25,11 @@ class VmFactors(C.Structure):
                 ('line_len', C.c_int * 3), ('planes', C.c_void_p * 3), ('lines', C.c_void_p * 3)]
 
 
+class CpFactors(C.Structure):
+    """struct srf_cp_factors."""
+    _fields_ = [('n_comp', C.c_int), ('line_len', C.c_int * 3), ('lines', C.c_void_p * 3)]
+
+
 class AlphaGrid(C.Structure):
     """struct srf_alpha_grid."""
     _fields_ = [('volume', C.c_void_p), ('depth', C.c_int), ('height', C.c_int), ('width', C.c_int),
@@ -56,6 +61,10 @@ _PROTOS = {
     'srf_vm_density_bwd': [_P, _P, _P, _P, _L, _P, _P, _P, _I, _F, _P, _P],
     'srf_vm_appearance_fwd': [_P, _P, _L, _P, _P, _P, _P],
     'srf_vm_appearance_bwd': [_P, _P, _P, _P, _L, _P, _P, _P, _P],
+    'srf_cp_density_fwd': [_P, _P, _P, _L, _P, _I, _F, _P, _P, _P, _P],
+    'srf_cp_density_bwd': [_P, _P, _P, _L, _P, _P, _P, _I, _F, _P, _P],
+    'srf_cp_appearance_fwd': [_P, _P, _L, _P, _P, _P, _P],
+    'srf_cp_appearance_bwd': [_P, _P, _P, _L, _P, _P, _P, _P],
     'srf_alpha_mask_sample': [_P, _L, _P, _P, _P],
     'srf_compact_workspace_bytes': [_L],
     'srf_threshold_compact': [_L, _P, _F, _P, _P, _P, _P],

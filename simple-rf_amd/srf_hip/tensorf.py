@@ -1,8 +1,10 @@
-"""TensoRF vector-matrix factor sampling on the HIP kernels, with hand-written backwards.
+"""TensoRF factor sampling (vector-matrix and CP decompositions) on the HIP kernels, with
+hand-written backwards.
 
-Mirrors LowRankTensor.forward's validity and normalisation, AlphaGridMask.sample_alpha and
-VmDecomposedTensor.get_volume_density / get_color (src/models/SimpleTensoRF09.py:701-765,
-1214-1263, 1342-1345), the surface test of :726 and the non-NDC sampler of :382-400.
+Mirrors LowRankTensor.forward's validity and normalisation, AlphaGridMask.sample_alpha,
+VmDecomposedTensor.get_volume_density / get_color, CpDecomposedTensor.get_volume_density /
+get_color (src/models/SimpleTensoRF09.py:701-765, 1214-1263, 1043-1089, 1342-1345), the surface
+test of :726 and the non-NDC sampler of :382-400.
 
 Factor layout: the kernels read plane p as [H, W, c] and line p as [L, c] (component axis
 innermost).  Parameters keep the reference's shapes ([1, c, H, W] and [1, c, L, 1], so
@@ -203,8 +205,11 @@ class _VmAppearance(torch.autograd.Function):
         planes, lines = factors[:3], factors[3:]
         f, keep = _factors_struct(planes, lines)
         dp, dl, grads = _grad_buffers(planes, lines, needs)
+        g = g_out.contiguous()
+        if g.data_ptr() % 16:   # rows are read as float4
+            g = g.clone()
         _lib.call('srf_vm_appearance_bwd', C.byref(f), dp, dl, _bbox_arg(bbox), n_rows, _lib.ptr(idx),
-                  _lib.ptr(pts), _lib.ptr(g_out.contiguous()), _lib.stream_ptr(pts.device))
+                  _lib.ptr(pts), _lib.ptr(g), _lib.stream_ptr(pts.device))
         return (None, None, None, None, *grads)
 
 
@@ -219,6 +224,118 @@ def vm_appearance(pts: torch.Tensor, planes, lines, bbox, idx: torch.Tensor | No
         idx = dev_tensor(idx, 'idx', dtype=torch.int32)
         n_rows = idx.shape[0] if n_rows is None else int(n_rows)
     return _VmAppearance.apply(p, idx, n_rows, tuple(float(x) for x in bbox), *planes, *lines)
+
+
+def _cp_struct(lines):
+    """struct srf_cp_factors of the three CP lines [1, R, L_i, 1] (plus the views it points into)."""
+    f = _lib.CpFactors()
+    keep = []
+    rank = None
+    for i in range(3):
+        L = _texel_view(lines[i], f'line {i}')
+        if L.shape[1] != 1 or (rank is not None and L.shape[2] != rank):
+            raise _lib.SrfError(f'line {i}: expected [1, {rank or "R"}, L, 1], got {tuple(lines[i].shape)}')
+        rank = L.shape[2]
+        f.line_len[i] = L.shape[0]
+        f.lines[i] = L.data_ptr()
+        keep.append(L)
+    f.n_comp = rank
+    return f, keep
+
+
+def _cp_grad_buffers(lines, needs):
+    """Zeroed texel-major gradient buffers of the three lines."""
+    bufs = []
+    for i in range(3):
+        _, c, h, w = lines[i].shape
+        bufs.append(torch.zeros((1, h, w, c), dtype=torch.float32, device=lines[i].device))
+    arr = (C.c_void_p * 3)(*[g.data_ptr() for g in bufs])
+    return arr, [g.permute(0, 3, 1, 2) if needs[i] else None for i, g in enumerate(bufs)]
+
+
+class _CpDensity(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, pts, bbox, alpha, predictor, offset, *lines):
+        f, keep = _cp_struct(lines)
+        n = pts.shape[0]
+        dev = pts.device
+        sigma = torch.empty((n,), dtype=torch.float32, device=dev)
+        feat = torch.empty((n,), dtype=torch.float32, device=dev)
+        valid = torch.empty((n,), dtype=torch.uint8, device=dev)
+        g = alpha.struct() if alpha is not None else None
+        _lib.call('srf_cp_density_fwd', C.byref(f), _bbox_arg(bbox), C.byref(g) if g is not None else None, n,
+                  _lib.ptr(pts), predictor, float(offset), _lib.ptr(sigma), _lib.ptr(feat), _lib.ptr(valid),
+                  _lib.stream_ptr(dev))
+        ctx.save_for_backward(pts, feat, valid, *lines)
+        ctx.cfg = (bbox, predictor, float(offset))
+        ctx.mark_non_differentiable(valid)
+        return sigma, valid
+
+    @staticmethod
+    def backward(ctx, g_sigma, _g_valid):
+        pts, feat, valid, *lines = ctx.saved_tensors
+        bbox, predictor, offset = ctx.cfg
+        needs = ctx.needs_input_grad[5:]
+        if not any(needs) or g_sigma is None:
+            return (None,) * 8
+        f, keep = _cp_struct(lines)
+        dl, grads = _cp_grad_buffers(lines, needs)
+        _lib.call('srf_cp_density_bwd', C.byref(f), dl, _bbox_arg(bbox), pts.shape[0], _lib.ptr(pts),
+                  _lib.ptr(valid), _lib.ptr(feat), predictor, offset, _lib.ptr(g_sigma.contiguous()),
+                  _lib.stream_ptr(pts.device))
+        return (None, None, None, None, None, *grads)
+
+
+def cp_density(pts: torch.Tensor, lines, bbox, *, alpha: AlphaMask | None = None, predictor: int = RELU,
+               offset: float = 0.0):
+    """vm_density for the CP decomposition: sigma and validity of pts [..., 3] with the density
+    feature sum_r line_0[r] line_1[r] line_2[r] (CpDecomposedTensor.get_volume_density,
+    SimpleTensoRF09.py:1043-1062).  Gradients flow to the three lines."""
+    shape = pts.shape[:-1]
+    p = dev_tensor(pts.detach(), 'pts').reshape(-1, 3)
+    sigma, valid = _CpDensity.apply(p, tuple(float(x) for x in bbox), alpha, int(predictor), float(offset), *lines)
+    return sigma.view(shape), valid.view(shape)
+
+
+class _CpAppearance(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, pts, idx, n_rows, bbox, *lines):
+        f, keep = _cp_struct(lines)
+        out = torch.empty((n_rows, f.n_comp), dtype=torch.float32, device=pts.device)
+        _lib.call('srf_cp_appearance_fwd', C.byref(f), _bbox_arg(bbox), n_rows, _lib.ptr(idx), _lib.ptr(pts),
+                  _lib.ptr(out), _lib.stream_ptr(pts.device))
+        ctx.save_for_backward(pts, idx, *lines)
+        ctx.cfg = (n_rows, bbox)
+        return out
+
+    @staticmethod
+    def backward(ctx, g_out):
+        pts, idx, *lines = ctx.saved_tensors
+        n_rows, bbox = ctx.cfg
+        needs = ctx.needs_input_grad[4:]
+        if not any(needs):
+            return (None,) * 7
+        f, keep = _cp_struct(lines)
+        dl, grads = _cp_grad_buffers(lines, needs)
+        g = g_out.contiguous()
+        if g.data_ptr() % 16:
+            g = g.clone()
+        _lib.call('srf_cp_appearance_bwd', C.byref(f), dl, _bbox_arg(bbox), n_rows, _lib.ptr(idx), _lib.ptr(pts),
+                  _lib.ptr(g), _lib.stream_ptr(pts.device))
+        return (None, None, None, None, *grads)
+
+
+def cp_appearance(pts: torch.Tensor, lines, bbox, idx: torch.Tensor | None = None, n_rows: int | None = None):
+    """CP appearance features [n_rows, R] (per-component products of the three lines) of the
+    samples idx[:n_rows] of pts [..., 3] (all samples when idx is None): the gather of
+    CpDecomposedTensor.get_color before basis_matrix_color (SimpleTensoRF09.py:1064-1078)."""
+    p = dev_tensor(pts.detach(), 'pts').reshape(-1, 3)
+    if idx is None:
+        n_rows = p.shape[0]
+    else:
+        idx = dev_tensor(idx, 'idx', dtype=torch.int32)
+        n_rows = idx.shape[0] if n_rows is None else int(n_rows)
+    return _CpAppearance.apply(p, idx, n_rows, tuple(float(x) for x in bbox), *lines)
 
 
 def threshold_compact(values: torch.Tensor, threshold: float):

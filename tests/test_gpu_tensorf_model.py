@@ -11,12 +11,24 @@ pytestmark = pytest.mark.gpu
 THR = 1e-4  # ray_marching_weight_threshold
 
 
-def _model(cuda, alpha=False, white=True):
+CP = dict(decomposition_type='CandecompParafac', num_components_density=(16,), num_components_color=(48,))
+
+
+def _model(cuda, alpha=False, white=True, cp=False):
     torch.manual_seed(0)
-    c = cfgs.simple_tensorf_configs(white_bkgd=white, perturb=False, coarse=dict(num_voxels_initial=32 ** 3),
+    coarse = dict(num_voxels_initial=32 ** 3)
+    if cp:
+        coarse.update(CP)
+    c = cfgs.simple_tensorf_configs(white_bkgd=white, perturb=False, coarse=coarse,
                                     augmented=dict(num_voxels_initial=16 ** 3))
     mc = cfgs.model_configs()
     model = get_model(c, model_configs=mc).to(cuda)
+    if cp:
+        # at the reference's 0.1 init a CP feature is a sum of triple products (~1e-3): scale the
+        # lines so that the densities put weight on enough samples to exercise the colour path
+        with torch.no_grad():
+            for p in list(model.coarse_model.vectors_density) + list(model.coarse_model.vectors_color):
+                p.mul_(6.0)
     if alpha:
         from models.SimpleTensoRFHip09 import AlphaGridMask
         g = torch.Generator().manual_seed(1)
@@ -25,13 +37,22 @@ def _model(cuda, alpha=False, white=True):
     return model, mc
 
 
+def _is_cp(tm):
+    return tm.tensor_configs['decomposition_type'] == 'CandecompParafac'
+
+
 def _oracle_params(tm, dtype=torch.float64, grad=False):
     cast = lambda t: t.detach().cpu().to(dtype).contiguous().requires_grad_(grad)
-    return {'bbox': tm.bounding_box.detach().cpu().to(dtype),
-            'planes_density': [cast(p) for p in tm.matrices_density], 'lines_density': [cast(p) for p in tm.vectors_density],
-            'planes_color': [cast(p) for p in tm.matrices_color], 'lines_color': [cast(p) for p in tm.vectors_color],
-            'basis': cast(tm.basis_matrix_color.weight),
-            'color_mlp': {k: cast(v) for k, v in tm.color_predictor.state_dict().items()}}
+    prm = {'bbox': tm.bounding_box.detach().cpu().to(dtype),
+           'lines_density': [cast(p) for p in tm.vectors_density], 'lines_color': [cast(p) for p in tm.vectors_color],
+           'basis': cast(tm.basis_matrix_color.weight),
+           'color_mlp': {k: cast(v) for k, v in tm.color_predictor.state_dict().items()}}
+    if _is_cp(tm):
+        prm['decomposition'] = 'CandecompParafac'
+    else:
+        prm['planes_density'] = [cast(p) for p in tm.matrices_density]
+        prm['planes_color'] = [cast(p) for p in tm.matrices_color]
+    return prm
 
 
 def _oracle_render(out, prm, tm, dtype=torch.float64, alpha=None):
@@ -44,9 +65,9 @@ def _oracle_render(out, prm, tm, dtype=torch.float64, alpha=None):
                              weight_threshold=THR, white_bkgd=True)
 
 
-@pytest.mark.parametrize('alpha', [False, True])
-def test_render_matches_oracle(cuda, alpha):
-    model, mc = _model(cuda, alpha=alpha)
+@pytest.mark.parametrize('alpha,cp', [(False, False), (True, False), (False, True), (True, True)])
+def test_render_matches_oracle(cuda, alpha, cp):
+    model, mc = _model(cuda, alpha=alpha, cp=cp)
     model.eval()
     batch = synthetic.train_batch(256, seed=3, device=cuda)
     with torch.no_grad():
@@ -61,6 +82,7 @@ def test_render_matches_oracle(cuda, alpha):
     # the colour of a sample is evaluated only where its weight exceeds the threshold; a ray with a
     # weight within rounding distance of it may legitimately switch, so such rays are excluded
     w64 = ref['weights']
+    assert (w64 > THR).sum() > 1000   # the colour path is exercised
     near = ((w64 - THR).abs() < 1e-9).any(-1)
     assert near.sum() <= 2
     keep = ~near
@@ -80,8 +102,9 @@ def test_render_matches_oracle(cuda, alpha):
     assert (a - ref['raw_sigma'].detach()[keep]).abs().max().item() < 1e-5
 
 
-def test_train_grads_match_oracle(cuda):
-    model, mc = _model(cuda)
+@pytest.mark.parametrize('cp', [False, True])
+def test_train_grads_match_oracle(cuda, cp):
+    model, mc = _model(cuda, cp=cp)
     model.eval()   # deterministic sampling; gradients still flow
     batch = synthetic.train_batch(128, seed=4, device=cuda)
     out = model({'pixel_id': batch['pixel_id'], 'num_frames': 3}, retraw=True)
@@ -95,8 +118,9 @@ def test_train_grads_match_oracle(cuda):
     ref = _oracle_render(out, prm, tm)
     assert not ((ref['weights'] - THR).abs() < 1e-9).any()
     ((ref['rgb'] * g_rgb.double()).sum() + (ref['depth_ndc'] * g_d.double()).sum()).backward()
-    pairs = [(tm.matrices_density, prm['planes_density']), (tm.vectors_density, prm['lines_density']),
-             (tm.matrices_color, prm['planes_color']), (tm.vectors_color, prm['lines_color'])]
+    pairs = [(tm.vectors_density, prm['lines_density']), (tm.vectors_color, prm['lines_color'])]
+    if not cp:
+        pairs += [(tm.matrices_density, prm['planes_density']), (tm.matrices_color, prm['planes_color'])]
     for ours, theirs in pairs:
         for a, b in zip(ours, theirs):
             err = (a.grad.cpu().double() - b.grad).abs().max() / b.grad.abs().max().clamp_min(1e-30)

@@ -88,11 +88,44 @@ def test_tensorf_checkpoint_round_trip_keeps_values_and_layout():
 
 
 def test_tensorf_unsupported_options_raise():
-    c = cfgs.simple_tensorf_configs(augmentations=False, coarse=dict(decomposition_type='CandecompParafac'))
-    with pytest.raises(NotImplementedError):
+    c = cfgs.simple_tensorf_configs(augmentations=False, coarse=dict(decomposition_type='Tucker'))
+    with pytest.raises(RuntimeError):   # SimpleTensoRF09.py:542-543
         get_model(c, model_configs=cfgs.model_configs())
     c = cfgs.simple_tensorf_configs(augmentations=False, coarse=dict(num_voxels_initial=16 ** 3))
     m = get_model(c, model_configs=cfgs.model_configs())
     m.train()
     with pytest.raises(NotImplementedError):
         m.coarse_model.run_model_modifications(2500)
+
+
+def _cp_configs(**kw):
+    coarse = dict(decomposition_type='CandecompParafac', num_components_density=(16,), num_components_color=(48,),
+                  num_voxels_initial=32 ** 3)
+    coarse.update(kw)
+    return cfgs.simple_tensorf_configs(augmentations=False, coarse=coarse)
+
+
+def test_tensorf_cp_dropin_state_dict_matches_reference_layout():
+    """CpDecomposedTensor (src/models/SimpleTensoRF09.py:979-996): three lines of rank
+    num_components[0] per field along z, y, x, no planes, basis Linear(sum(comps) -> 27)."""
+    import oracle.tensorf as ot
+    c = _cp_configs()
+    model = get_model(c, model_configs=cfgs.model_configs())
+    sd = model.state_dict()
+    tc = c['model']['coarse_model']
+    res = ot.compute_resolution_in_voxels(tc['num_voxels_initial'], torch.tensor(tc['bounding_box'])).tolist()
+    for kind, r in (('density', 16), ('color', 48)):
+        for i in range(3):
+            assert tuple(sd[f'coarse_model.vectors_{kind}.{i}'].shape) == (1, r, res[ot.VECTOR_AXES[i]], 1)
+    assert not any('matrices_' in k for k in sd)
+    assert tuple(sd['coarse_model.basis_matrix_color.weight'].shape) == (27, 48)
+    assert model.coarse_model.vectors_color[2].permute(0, 2, 3, 1).is_contiguous()
+    groups = model.coarse_model.get_trainable_parameters({'lr_initial_tensor': 0.02, 'lr_initial_network': 1e-3})
+    assert [g['name'] for g in groups] == ['coarse_model_tensor_params', 'coarse_model_network_params']
+    assert [tuple(p.shape) for p in groups[0]['params']] == \
+        [(1, 16, res[v], 1) for v in ot.VECTOR_AXES] + [(1, 48, res[v], 1) for v in ot.VECTOR_AXES]
+    b = get_model(_cp_configs(), model_configs=cfgs.model_configs())
+    b.load_state_dict(sd)
+    for k, v in sd.items():
+        assert torch.equal(b.state_dict()[k], v), k
+    assert b.coarse_model.vectors_density[1].permute(0, 2, 3, 1).is_contiguous()

@@ -122,3 +122,56 @@ def test_num_samples_rule():
     assert ot.compute_num_samples(res, 0.5, 1000) == 443
     assert ot.compute_num_samples(res, 0.5, 300) == 300
     assert round(math.sqrt(3) * 128 / 0.5) == 443
+
+
+def rand_cp_lines(rank, res, seed, dtype=torch.float64):
+    """Lines [1,R,res[v_i],1] for v = (2, 1, 0), as created at SimpleTensoRF09.py:989-996."""
+    g = torch.Generator().manual_seed(seed)
+    return [0.1 * torch.randn(1, rank, res[ot.VECTOR_AXES[i]], 1, generator=g, dtype=dtype) for i in range(3)]
+
+
+def test_cp_texel_centres_read_stored_values():
+    """At a texel centre the CP feature is sum_r L0[r, z] L1[r, y] L2[r, x] exactly: line 0 runs
+    along pts[2], line 1 along pts[1], line 2 along pts[0]."""
+    res = [5, 7, 9]
+    lines = rand_cp_lines(6, res, 11)
+    idx = torch.tensor([[1, 2, 3], [4, 6, 8], [0, 0, 0], [2, 5, 7]])
+    pts_n = torch.stack([2 * idx[:, k].double() / (res[k] - 1) - 1 for k in range(3)], -1)
+    got = ot.cp_appearance_features(lines, pts_n)
+    want = lines[0][0, :, idx[:, 2], 0] * lines[1][0, :, idx[:, 1], 0] * lines[2][0, :, idx[:, 0], 0]
+    assert torch.allclose(got, want.T, rtol=0, atol=1e-15)
+    assert torch.allclose(ot.cp_density_features(lines, pts_n), want.sum(0), rtol=0, atol=1e-15)
+
+
+def test_cp_separable_linear_lines_interpolate_exactly():
+    """Lines linear in their coordinate are reproduced exactly, so a rank-1 CP field is the
+    product of the three linear functions everywhere in the box."""
+    res = [6, 8, 10]
+    lin = [lambda t, a=a: 0.5 + 0.25 * a + 0.3 * t for a in range(3)]
+    lines = []
+    for i in range(3):
+        v = ot.VECTOR_AXES[i]
+        lines.append(lin[v](torch.linspace(-1, 1, res[v], dtype=torch.float64))[None, None, :, None])
+    g = torch.Generator().manual_seed(12)
+    pts_n = torch.rand(300, 3, generator=g, dtype=torch.float64) * 2 - 1
+    want = lin[0](pts_n[:, 0]) * lin[1](pts_n[:, 1]) * lin[2](pts_n[:, 2])
+    assert torch.allclose(ot.cp_density_features(lines, pts_n), want, rtol=0, atol=1e-13)
+
+
+def test_cp_equals_vm_with_outer_product_plane():
+    """A CP component L0(z) L1(y) L2(x) is the VM component with plane 0 = L2 (x) L1 (an outer
+    product on the (x, y) plane, whose bilinear sample is the product of the two linear samples)
+    and line 0 = L0, the other planes zero: the CP path agrees with the pinned VM path."""
+    res = [6, 7, 8]
+    rank = 5
+    lines = rand_cp_lines(rank, res, 13)
+    plane0 = (lines[1][0, :, :, 0][:, :, None] * lines[2][0, :, :, 0][:, None, :])[None]   # [1,R,H=res1,W=res0]
+    planes = [plane0, torch.zeros(1, rank, res[2], res[0], dtype=torch.float64),
+              torch.zeros(1, rank, res[2], res[1], dtype=torch.float64)]
+    vm_lines = [lines[0], torch.zeros(1, rank, res[1], 1, dtype=torch.float64),
+                torch.zeros(1, rank, res[0], 1, dtype=torch.float64)]
+    g = torch.Generator().manual_seed(14)
+    pts_n = torch.rand(400, 3, generator=g, dtype=torch.float64) * 2 - 1
+    cp = ot.cp_appearance_features(lines, pts_n)
+    vm = ot.vm_appearance_features(planes, vm_lines, pts_n)[:, :rank]
+    assert torch.allclose(cp, vm, rtol=0, atol=1e-15)
