@@ -40,6 +40,8 @@ def simple_nerf_configs(name='SimpleNeRFHip17', precision='bf16', augmentations=
             {'name': 'points_augmentation', 'coarse_model': _mlp(points_sigma_positional_encoding_degree=3)},
             {'name': 'views_augmentation', 'coarse_model': _mlp(use_view_dirs=False, view_dependent_rgb=False)},
         ]
+        # the view-independent model has no view encoding (src/NerfLlffTrainerTester10.py:347-358)
+        del model['augmentations'][1]['coarse_model']['views_positional_encoding_degree']
     return {'data_loader': {'ndc': True}, 'model': model, 'database': 'NeRF_LLFF'}
 
 
