@@ -194,6 +194,89 @@ def hip_render_leg(args, dev, model):
     return args.render_rays * n / (t1 - t0)
 
 
+IMAGE_W, IMAGE_H = 1008, 756
+RENDER_FLOP_PER_RAY = 2.0 * 256 * 593408   # coarse 64 + fine 192 samples through the main MLP
+
+
+def render_image_leg(args, dev, precisions=('bf16', 'fp16', 'fp32'), chunk=None):
+    """Full-image render: the 1008 x 756 = 762,048 rays of one frame through the coarse + fine model
+    in eval (deterministic sampling, no augmentations), in fixed chunks of `chunk` rays (the tail
+    chunk padded with repeats of its last pixel).  Timed two ways per MLP precision: the eager chunk
+    loop, and one chunk forward captured in a HIP graph and replayed per chunk.  The weights are the
+    same seeded Xavier init for every precision, so the images are compared with the FP32 one."""
+    import torch
+    from models.ModelFactory02 import get_model
+    from srf_hip import configs as cfgs, synthetic
+    chunk = chunk or args.render_chunk
+    pid = synthetic.full_image_pixel_ids(IMAGE_W, IMAGE_H, 0).to(dev)
+    n = pid.shape[0]
+    n_chunks = (n + chunk - 1) // chunk
+    pad = n_chunks * chunk - n
+    pid_pad = torch.cat([pid, pid[-1:].expand(pad, 3)]) if pad else pid
+    mc = cfgs.model_configs(width=IMAGE_W, height=IMAGE_H)
+    out, images = {}, {}
+    for prec in precisions:
+        torch.manual_seed(0)
+        model = get_model(cfgs.simple_nerf_configs(precision=prec), model_configs=mc).to(dev).eval()
+        img = torch.empty((n_chunks * chunk, 3), device=dev)
+
+        def eager_image():
+            for c in range(n_chunks):
+                o = model({'pixel_id': pid_pad[c * chunk:(c + 1) * chunk], 'num_frames': 3})
+                img[c * chunk:(c + 1) * chunk].copy_(o['rgb_fine'])
+
+        res = {}
+        with torch.no_grad():
+            eager_image()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            eager_image()
+            torch.cuda.synchronize()
+            res['eager_rays_per_s'] = round(n / (time.perf_counter() - t0), 1)
+            try:
+                static_pid = pid_pad[:chunk].clone()
+                side = torch.cuda.Stream()
+                side.wait_stream(torch.cuda.current_stream())
+                with torch.cuda.stream(side):
+                    for _ in range(2):
+                        model({'pixel_id': static_pid, 'num_frames': 3})
+                torch.cuda.current_stream().wait_stream(side)
+                graph = torch.cuda.CUDAGraph()
+                with torch.cuda.graph(graph):
+                    o = model({'pixel_id': static_pid, 'num_frames': 3})
+                rgb_static = o['rgb_fine']
+
+                def graph_image():
+                    for c in range(n_chunks):
+                        static_pid.copy_(pid_pad[c * chunk:(c + 1) * chunk])
+                        graph.replay()
+                        img[c * chunk:(c + 1) * chunk].copy_(rgb_static)
+
+                graph_image()
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                graph_image()
+                torch.cuda.synchronize()
+                dt = time.perf_counter() - t0
+                res['graph_rays_per_s'] = round(n / dt, 1)
+                del graph
+            except Exception as exc:  # reported, not hidden
+                res['graph_error'] = f'{type(exc).__name__}: {exc}'[:300]
+        best = max(res.get('graph_rays_per_s', 0.0), res['eager_rays_per_s'])
+        res['ms_per_image'] = round(n / best * 1e3, 2)
+        res['tflops'] = round(RENDER_FLOP_PER_RAY * best / 1e12, 1)
+        images[prec] = img[:n].clone()
+        out[prec] = res
+        del model
+    if 'fp32' in images:
+        for prec in images:
+            if prec != 'fp32':
+                out[f'{prec}_vs_fp32_max_rgb'] = float((images[prec] - images['fp32']).abs().max())
+    out['workload'] = (f'{IMAGE_W}x{IMAGE_H} = {n:,} rays, coarse 64 + fine 192 samples, eval, chunks of {chunk:,} '
+                       f'rays ({n_chunks} chunks)')
+    return out
+
+
 PEAK_HBM_BYTES = PEAK_HBM_GBS * 1e9
 
 
@@ -430,6 +513,8 @@ def main():
     ap.add_argument('--warmup', type=int, default=5)
     ap.add_argument('--rays', type=int, default=4096)
     ap.add_argument('--render-rays', type=int, default=65536)
+    ap.add_argument('--render-chunk', type=int, default=65536)
+    ap.add_argument('--no-image', action='store_true', help='skip the full-image render leg')
     ap.add_argument('--plain-steps', type=int, default=5)
     ap.add_argument('--cpu-rays', type=int, default=128)
     ap.add_argument('--cpu-seconds', type=float, default=20.0)
@@ -477,6 +562,7 @@ def main():
                       **({'tflops': round(v['flop'] / (v['ms'] * 1e-3) / 1e12, 1)} if v['flop'] else {})}
                   for k, v in kern.items()}
     render = hip_render_leg(args, dev, tr['model'])
+    image = None if args.no_image else render_image_leg(args, dev)
     if args.copy_calibration_mb:
         n = args.copy_calibration_mb * (1 << 20) // 4
         src = torch.rand(n, device=dev)
@@ -496,6 +582,7 @@ def main():
         'eager_rays_per_s': round(eager_value, 1),
         'graph': graph_info,
         'render_rays_per_s': round(render, 1),
+        **({'render_image': image} if image else {}),
         'kernels': per_kernel,
     }
     if not args.no_tensorf:

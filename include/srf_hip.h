@@ -131,7 +131,9 @@ int srf_composite_bwd(int n_rays, int n_samples, const int32_t* ray_offsets, con
  *                   the view head); 0 -> 4-channel pts_output_linear (views-augmented model);
  *   sigma_softplus: 0 -> relu(raw + noise) (reference, :738-743); 1 -> softplus(raw + noise);
  *   prec:           SRF_PREC_F32 / SRF_PREC_BF16 / SRF_PREC_F16 operands of the MFMA tiles
- *                   (accumulation is always FP32). */
+ *                   (accumulation is always FP32).  SRF_PREC_F16 covers the inference forward
+ *                   only (srf_mlp_forward with workspace NULL); its training entry points
+ *                   return SRF_EUNSUPPORTED. */
 typedef struct srf_mlp_desc {
   int trunk_pe;
   int pts_pe_degree;

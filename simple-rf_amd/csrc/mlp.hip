@@ -123,7 +123,8 @@ int fwd_launch(int prec, const FwdArgs& a, bool vd, bool tr, hipStream_t s) {
   switch (prec) {
     case SRF_PREC_F32: return fwd_f32(a, vd, tr, s);
     case SRF_PREC_BF16: return fwd_bf16(a, vd, tr, s);
-    default: return SRF_EUNSUPPORTED;  // FP16 operands: not built into this library
+    case SRF_PREC_F16: return fwd_f16(a, vd, tr, s);  // inference only (tr -> SRF_EUNSUPPORTED)
+    default: return SRF_EUNSUPPORTED;
   }
 }
 

@@ -1037,12 +1037,14 @@ int launch_dw(const DwArgs& d, hipStream_t s) {
   return launch_status();
 }
 
-// Per-precision launchers, one translation unit each (mlp_f32.hip, mlp_bf16.hip).
+// Per-precision launchers, one translation unit each (mlp_f32.hip, mlp_bf16.hip; mlp_f16.hip holds
+// the inference forward only).
 int fwd_launch(int prec, const FwdArgs& a, bool view_dep, bool train, hipStream_t s);
 int bwd_launch(int prec, const BwdArgs& a, bool view_dep, hipStream_t s);
 int dw_launch(int prec, const DwArgs& d, hipStream_t s);
 int fwd_f32(const FwdArgs& a, bool vd, bool tr, hipStream_t s);
 int fwd_bf16(const FwdArgs& a, bool vd, bool tr, hipStream_t s);
+int fwd_f16(const FwdArgs& a, bool vd, bool tr, hipStream_t s);
 int bwd_f32(const BwdArgs& a, bool vd, hipStream_t s);
 int bwd_bf16(const BwdArgs& a, bool vd, hipStream_t s);
 int dw_f32(const DwArgs& d, hipStream_t s);
