@@ -117,6 +117,8 @@ def hip_train_leg(args, dev):
 
     for _ in range(args.warmup):
         step()
+    if TUNER is not None:   # the weight-gradient schedule, measured on this step (untimed)
+        TUNER.tune(f'nerf_train_{args.rays}rays', 'dw_chunk', step)
     torch.cuda.synchronize()
     timer = KernelTimer(_lib)
     with timer:
@@ -194,6 +196,8 @@ def hip_render_leg(args, dev, model):
     return args.render_rays * n / (t1 - t0)
 
 
+TUNER = None  # srf_hip.autotune.Autotuner of this run (None: the library's built-in schedule)
+
 IMAGE_W, IMAGE_H = 1008, 756
 RENDER_FLOP_PER_RAY = 2.0 * 256 * 593408   # coarse 64 + fine 192 samples through the main MLP
 
@@ -207,13 +211,27 @@ def render_image_leg(args, dev, precisions=('bf16', 'fp16', 'fp32'), chunk=None)
     import torch
     from models.ModelFactory02 import get_model
     from srf_hip import configs as cfgs, synthetic
-    chunk = chunk or args.render_chunk
     pid = synthetic.full_image_pixel_ids(IMAGE_W, IMAGE_H, 0).to(dev)
     n = pid.shape[0]
+    mc = cfgs.model_configs(width=IMAGE_W, height=IMAGE_H)
+    if chunk is None and TUNER is not None:
+        # the ray chunk of the render schedule, measured on 262,144 rays of the frame (BF16)
+        torch.manual_seed(0)
+        m = get_model(cfgs.simple_nerf_configs(precision='bf16'), model_configs=mc).to(dev).eval()
+        probe = pid[:1 << 18]
+
+        def run(c):
+            with torch.no_grad():
+                for i in range(0, probe.shape[0], c):
+                    m({'pixel_id': probe[i:i + c], 'num_frames': 3})
+
+        chunk = TUNER.tune_param(f'render_{IMAGE_W}x{IMAGE_H}', 'render_chunk', [32768, 65536, 131072, 262144],
+                                 run)['chosen']
+        del m
+    chunk = chunk or args.render_chunk
     n_chunks = (n + chunk - 1) // chunk
     pad = n_chunks * chunk - n
     pid_pad = torch.cat([pid, pid[-1:].expand(pad, 3)]) if pad else pid
-    mc = cfgs.model_configs(width=IMAGE_W, height=IMAGE_H)
     out, images = {}, {}
     for prec in precisions:
         torch.manual_seed(0)
@@ -325,6 +343,9 @@ def tensorf_train_leg(args, dev):
 
     for _ in range(args.warmup):
         step()
+    if TUNER is not None:   # gather and colour weight-gradient schedule, measured on this step (untimed)
+        for knob in ('gather_run', 'gather_blocks', 'color_dw_chunk'):
+            TUNER.tune(f'tensorf_train_{args.tensorf_rays}rays', knob, step, reps=2)
     torch.cuda.synchronize()
     timer = KernelTimer(_lib)
     with timer:
@@ -523,6 +544,7 @@ def main():
     ap.add_argument('--tensorf-steps', type=int, default=5)
     ap.add_argument('--no-tensorf', action='store_true')
     ap.add_argument('--no-graph', action='store_true', help='report the eager step only')
+    ap.add_argument('--no-autotune', action='store_true', help="keep the library's built-in launch schedule")
     ap.add_argument('--copy-calibration-mb', type=int, default=0,
                     help='after the timed legs, run 3 device copies of this many MiB (a plain copy kernel '
                          'of known size for scaling rocprof FETCH_SIZE / WRITE_SIZE)')
@@ -535,6 +557,10 @@ def main():
     dev = torch.device('cuda:0')
     import srf_hip
     srf_hip.lib()  # fail loudly without the HIP library
+    global TUNER
+    if not args.no_autotune:
+        from srf_hip import autotune
+        TUNER = autotune.Autotuner()
 
     tr = hip_train_leg(args, dev)
     eager_value = args.rays * args.steps / tr['seconds']
@@ -590,6 +616,8 @@ def main():
         result['tensorf_train'] = {k: v for k, v in tf.items() if k not in ('model', 'out')}
         result['hbm_kernels'] = hbm_leg(dev, tf['model'], tf['out'])
         del tf
+    if TUNER is not None:
+        result['autotune'] = TUNER.results
     if not args.no_baselines:
         plain = plain_gpu_leg(args, dev, tr['model'])
         result['plain_fp32_gpu'] = {'value': round(plain, 1), 'unit': 'rays/s',

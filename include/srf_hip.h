@@ -12,7 +12,8 @@
  *     allocates.  Tensors are contiguous row-major float32 unless stated otherwise.
  *   - `stream` is a hipStream_t (passed as void*); all work is stream-ordered, there
  *     are no host synchronisations, so every entry point is hipGraph-capturable.
- *   - Entry points are re-entrant (no global mutable state).
+ *   - Entry points are re-entrant.  The only process-global state is the table of
+ *     launch-schedule knobs (srf_tune_set), read atomically when a kernel is launched.
  */
 #ifndef SRF_HIP_H_
 #define SRF_HIP_H_
@@ -377,6 +378,29 @@ int srf_color_forward(const srf_color_desc* desc, const void* packed, int n_poin
 int srf_color_backward(const srf_color_desc* desc, const void* packed, int n_points, const float* rgb,
                        const float* d_rgb, void* workspace, float* d_app, float* d_basis, float* d_w1,
                        float* d_b1, float* d_w2, float* d_b2, float* d_w3, float* d_b3, void* stream);
+
+/* ------------------------------------------------------- launch-schedule tuning */
+
+/* Knobs of the launch schedule, picked by an autotuner (srf_hip.autotune) for a workload.  No
+ * counterpart in the reference, whose PyTorch ops have no schedule.  Value 0 restores the
+ * built-in choice; results are the same for every value up to the order of FP32 atomic
+ * additions.  Launches read the table when they are issued, so a captured graph keeps the
+ * values it was captured with. */
+#define SRF_TUNE_DW_CHUNK 0       /* points per work item of the MLP weight-gradient GEMM (srf_mlp_weight_grads):
+                                     0 = about 1024 work items; else 2048..262144, a multiple of 256 */
+#define SRF_TUNE_COLOR_DW_CHUNK 1 /* the same for the colour network's weight gradients (srf_color_backward):
+                                     0 = adaptive; else 2048..65536, a multiple of 256 */
+#define SRF_TUNE_GATHER_BLOCKS 2  /* grid cap of the factor-gather kernels (srf_vm_*, srf_cp_*): 0 = 8192;
+                                     else 256..65536 blocks of 256 threads */
+#define SRF_TUNE_GATHER_RUN 3     /* consecutive samples one lane group walks in the gather backwards, summing
+                                     equal-corner gradients before the atomics: 0 = 32; else 1..256 */
+#define SRF_TUNE_KNOBS 4
+
+/* Sets knob to value (SRF_EINVAL for an unknown knob or a value outside its range). */
+int srf_tune_set(int knob, int64_t value);
+
+/* Current value of knob (0 = built-in choice), -1 for an unknown knob. */
+int64_t srf_tune_get(int knob);
 
 /* Lane-layout self-test of the MFMA building blocks (one 16x16 tile): d = a b with
  * a [16,K], b [K,16] row-major FP32 (rounded to the operand type), K = 4 for SRF_PREC_F32 and

@@ -1,8 +1,31 @@
-// Library-level entry points: version, status strings and the MFMA lane-layout self-test.
+// Library-level entry points: version, status strings, the launch-schedule knobs and the MFMA
+// lane-layout self-test.
+#include <atomic>
+
 #include "common.h"
 #include "mfma.h"
 
 namespace srf {
+
+namespace {
+std::atomic<int64_t> g_tune[SRF_TUNE_KNOBS];
+
+bool tune_in_range(int knob, int64_t v) {
+  if (v == 0) return true;
+  switch (knob) {
+    case SRF_TUNE_DW_CHUNK: return v >= 2048 && v <= 262144 && v % 256 == 0;
+    case SRF_TUNE_COLOR_DW_CHUNK: return v >= 2048 && v <= 65536 && v % 256 == 0;
+    case SRF_TUNE_GATHER_BLOCKS: return v >= 256 && v <= 65536;
+    case SRF_TUNE_GATHER_RUN: return v >= 1 && v <= 256;
+    default: return false;
+  }
+}
+}  // namespace
+
+int64_t tune_value(int knob) {
+  return knob >= 0 && knob < SRF_TUNE_KNOBS ? g_tune[knob].load(std::memory_order_relaxed) : 0;
+}
+
 namespace {
 
 // One wave computes D = A B for a single 16x16 tile with the operand layouts of mfma.h
@@ -50,6 +73,17 @@ __global__ __launch_bounds__(256) void selftest_lds_dma_kernel(const uint8_t* __
 }  // namespace srf
 
 using namespace srf;
+
+extern "C" int srf_tune_set(int knob, int64_t value) {
+  if (knob < 0 || knob >= SRF_TUNE_KNOBS || !tune_in_range(knob, value)) return SRF_EINVAL;
+  g_tune[knob].store(value, std::memory_order_relaxed);
+  return SRF_OK;
+}
+
+extern "C" int64_t srf_tune_get(int knob) {
+  if (knob < 0 || knob >= SRF_TUNE_KNOBS) return -1;
+  return g_tune[knob].load(std::memory_order_relaxed);
+}
 
 extern "C" int srf_selftest_lds_dma(const void* src, void* dst, void* stream) {
   if (!src || !dst) return SRF_EINVAL;

@@ -638,6 +638,7 @@ extern "C" int srf_color_backward(const srf_color_desc* desc, const void* packed
   job(w.dF, kMaxB, w.appT, g.Cp, g.NB, g.C, d_basis, g.C, nullptr);
   d.n_jobs = nj;
   long long target = ((long long)w.cols * nj / 1024 + 255) / 256 * 256;
+  if (const int64_t v = tune_value(SRF_TUNE_COLOR_DW_CHUNK)) target = v;  // autotuned schedule
   d.chunk = (int)(target < 2048 ? 2048 : (target > 65536 ? 65536 : target));
   d.n_chunks = (w.cols + d.chunk - 1) / d.chunk;
   int work = 0;

@@ -21,6 +21,9 @@ namespace srf {
 
 inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
 
+// Current value of a launch-schedule knob (SRF_TUNE_*; 0 = the built-in choice).
+int64_t tune_value(int knob);
+
 inline int launch_status() {
   hipError_t e = hipGetLastError();
   return e == hipSuccess ? SRF_OK : SRF_EHIP;

@@ -1,7 +1,5 @@
 // C-ABI of the fused PE + MLP kernels: planning, weight packing and dispatch.
 // Kernel design notes are at the top of mlp_impl.h.
-#include <stdlib.h>
-
 #include "mlp_impl.h"
 
 namespace srf {
@@ -295,10 +293,7 @@ static int mlp_backward_impl(const srf_mlp_desc* desc, const void* packed, int n
   // points per work item: about 1024 blocks over all jobs, whole multiples of 256 points (so every
   // item holds an even number of k-units), between 2048 and 16384 points
   long long target = ((long long)w.cols * nj / 1024 + 255) / 256 * 256;
-  if (const char* env = getenv("SRF_DW_CHUNK")) {  // tuning override (points, rounded to 256)
-    const long long v = atoll(env);
-    if (v > 0) target = (v + 255) / 256 * 256;
-  }
+  if (const int64_t v = tune_value(SRF_TUNE_DW_CHUNK)) target = v;  // autotuned schedule
   d.chunk = (int)(target < 2048 ? 2048 : (target > 262144 ? 262144 : target));
   d.n_chunks = (w.cols + d.chunk - 1) / d.chunk;
   int work = 0;

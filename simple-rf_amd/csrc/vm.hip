@@ -287,7 +287,7 @@ __global__ __launch_bounds__(256) void density_fwd_kernel(VmArgs a, AlphaArgs m,
 // to HBM with atomics only when the corners change.  Consecutive samples of a ray mostly share
 // them (the step is half a voxel, and in NDC a ray's x, y barely move), so the atomics to the
 // most contended texels drop by up to the run length.
-constexpr int kRun = 32;  // samples walked by one lane group
+constexpr int kRun = 32;  // samples walked by one lane group (SRF_TUNE_GATHER_RUN overrides it)
 
 struct RunAcc {
   int pidx[4], lidx[2];
@@ -362,17 +362,17 @@ __device__ __forceinline__ void run_add(const VmArgs& a, const QuadSite& s, RunA
 }
 
 // d_feat = d_sigma * predictor'(feat) on valid samples; planes receive d_feat * line and
-// lines d_feat * plane, per component and corner.  A lane group walks kRun consecutive samples.
+// lines d_feat * plane, per component and corner.  A lane group walks run_len consecutive samples.
 template <int G>
-__global__ __launch_bounds__(256) void density_bwd_kernel(VmArgs a, int64_t n, const float* __restrict__ pts,
+__global__ __launch_bounds__(256) void density_bwd_kernel(VmArgs a, int64_t n, int run_len, const float* __restrict__ pts,
                                                           const uint8_t* __restrict__ valid,
                                                           const float* __restrict__ feat, int kind,
                                                           float offset, const float* __restrict__ d_sigma) {
   const GroupPos gp = group_pos<G>();
-  const int64_t n_runs = (n + kRun - 1) / kRun;
+  const int64_t n_runs = (n + run_len - 1) / run_len;
   for (int64_t run = gp.first; run < n_runs; run += gp.stride) {
-    const int64_t i0 = run * kRun;
-    const int64_t i1 = i0 + kRun < n ? i0 + kRun : n;
+    const int64_t i0 = run * run_len;
+    const int64_t i1 = i0 + run_len < n ? i0 + run_len : n;
     for (int q = gp.ql; q < a.n_quads; q += G) {
       RunAcc r;
       run_init(r);
@@ -420,14 +420,14 @@ __global__ __launch_bounds__(256) void appearance_fwd_kernel(VmArgs a, int64_t n
 }
 
 template <int G>
-__global__ __launch_bounds__(256) void appearance_bwd_kernel(VmArgs a, int64_t n, const int32_t* __restrict__ idx,
+__global__ __launch_bounds__(256) void appearance_bwd_kernel(VmArgs a, int64_t n, int run_len, const int32_t* __restrict__ idx,
                                                              const float* __restrict__ pts,
                                                              const float* __restrict__ d_out) {
   const GroupPos gp = group_pos<G>();
-  const int64_t n_runs = (n + kRun - 1) / kRun;
+  const int64_t n_runs = (n + run_len - 1) / run_len;
   for (int64_t run = gp.first; run < n_runs; run += gp.stride) {
-    const int64_t j0 = run * kRun;
-    const int64_t j1 = j0 + kRun < n ? j0 + kRun : n;
+    const int64_t j0 = run * run_len;
+    const int64_t j1 = j0 + run_len < n ? j0 + run_len : n;
     for (int q = gp.ql; q < a.n_quads; q += G) {
       RunAcc r;
       run_init(r);
@@ -562,15 +562,15 @@ __global__ __launch_bounds__(256) void cp_density_fwd_kernel(CpArgs a, AlphaArgs
 }
 
 template <int G>
-__global__ __launch_bounds__(256) void cp_density_bwd_kernel(CpArgs a, int64_t n, const float* __restrict__ pts,
+__global__ __launch_bounds__(256) void cp_density_bwd_kernel(CpArgs a, int64_t n, int run_len, const float* __restrict__ pts,
                                                              const uint8_t* __restrict__ valid,
                                                              const float* __restrict__ feat, int kind,
                                                              float offset, const float* __restrict__ d_sigma) {
   const GroupPos gp = group_pos<G>();
-  const int64_t n_runs = (n + kRun - 1) / kRun;
+  const int64_t n_runs = (n + run_len - 1) / run_len;
   for (int64_t run = gp.first; run < n_runs; run += gp.stride) {
-    const int64_t i0 = run * kRun;
-    const int64_t i1 = i0 + kRun < n ? i0 + kRun : n;
+    const int64_t i0 = run * run_len;
+    const int64_t i1 = i0 + run_len < n ? i0 + run_len : n;
     for (int q = gp.ql; q < a.n_quads; q += G) {
       CpRun r;
       cp_run_init(r);
@@ -612,14 +612,14 @@ __global__ __launch_bounds__(256) void cp_appearance_fwd_kernel(CpArgs a, int64_
 }
 
 template <int G>
-__global__ __launch_bounds__(256) void cp_appearance_bwd_kernel(CpArgs a, int64_t n, const int32_t* __restrict__ idx,
+__global__ __launch_bounds__(256) void cp_appearance_bwd_kernel(CpArgs a, int64_t n, int run_len, const int32_t* __restrict__ idx,
                                                                 const float* __restrict__ pts,
                                                                 const float* __restrict__ d_out) {
   const GroupPos gp = group_pos<G>();
-  const int64_t n_runs = (n + kRun - 1) / kRun;
+  const int64_t n_runs = (n + run_len - 1) / run_len;
   for (int64_t run = gp.first; run < n_runs; run += gp.stride) {
-    const int64_t j0 = run * kRun;
-    const int64_t j1 = j0 + kRun < n ? j0 + kRun : n;
+    const int64_t j0 = run * run_len;
+    const int64_t j1 = j0 + run_len < n ? j0 + run_len : n;
     for (int q = gp.ql; q < a.n_quads; q += G) {
       CpRun r;
       cp_run_init(r);
@@ -807,10 +807,17 @@ int make_alpha_args(const srf_alpha_grid* g, AlphaArgs* m) {
 unsigned grid_for(int64_t n, int G) {
   const int64_t per_block = (int64_t)(256 / SRF_WAVE) * (SRF_WAVE / G);
   int64_t blocks = (n + per_block - 1) / per_block;
-  // 256 CUs x 8 resident blocks of 256 threads: beyond that the grid-stride loop takes over
-  const int64_t cap = 256 * 8 * 4;
+  // 256 CUs x 8 resident blocks of 256 threads, 4 rounds: beyond that the grid-stride loop takes
+  // over (SRF_TUNE_GATHER_BLOCKS overrides the cap)
+  const int64_t tuned = tune_value(SRF_TUNE_GATHER_BLOCKS);
+  const int64_t cap = tuned > 0 ? tuned : 256 * 8 * 4;
   if (blocks > cap) blocks = cap;
   return (unsigned)(blocks < 1 ? 1 : blocks);
+}
+
+int run_length() {
+  const int64_t v = tune_value(SRF_TUNE_GATHER_RUN);
+  return v > 0 ? (int)v : kRun;
 }
 
 #define SRF_VM_DISPATCH(G_, KERNEL, ...)                                                   \
@@ -863,9 +870,9 @@ extern "C" int srf_vm_density_bwd(const srf_vm_factors* factors, float* const* d
     a.dline[p] = d_lines[p];
   }
   const int G = next_pow2(a.n_quads);
-  SRF_VM_DISPATCH(G, density_bwd_kernel, dim3(grid_for((n_points + kRun - 1) / kRun, G)), dim3(256), 0,
-                  as_stream(stream), a,
-                  n_points, pts, valid, feat, predictor, offset, d_sigma);
+  const int run = run_length();
+  SRF_VM_DISPATCH(G, density_bwd_kernel, dim3(grid_for((n_points + run - 1) / run, G)), dim3(256), 0,
+                  as_stream(stream), a, n_points, run, pts, valid, feat, predictor, offset, d_sigma);
   return launch_status();
 }
 
@@ -900,9 +907,9 @@ extern "C" int srf_vm_appearance_bwd(const srf_vm_factors* factors, float* const
     a.dline[p] = d_lines[p];
   }
   const int G = next_pow2(a.n_quads);
-  SRF_VM_DISPATCH(G, appearance_bwd_kernel, dim3(grid_for((n_rows + kRun - 1) / kRun, G)), dim3(256), 0,
-                  as_stream(stream), a,
-                  n_rows, idx, pts, d_features);
+  const int run = run_length();
+  SRF_VM_DISPATCH(G, appearance_bwd_kernel, dim3(grid_for((n_rows + run - 1) / run, G)), dim3(256), 0,
+                  as_stream(stream), a, n_rows, run, idx, pts, d_features);
   return launch_status();
 }
 
@@ -935,8 +942,9 @@ extern "C" int srf_cp_density_bwd(const srf_cp_factors* factors, float* const* d
   if ((st = set_cp_grads(&a, d_lines)) != SRF_OK) return st;
   if (!pts || !valid || !feat || !d_sigma) return SRF_EINVAL;
   const int G = next_pow2(a.n_quads);
-  SRF_VM_DISPATCH(G, cp_density_bwd_kernel, dim3(grid_for((n_points + kRun - 1) / kRun, G)), dim3(256), 0,
-                  as_stream(stream), a, n_points, pts, valid, feat, predictor, offset, d_sigma);
+  const int run = run_length();
+  SRF_VM_DISPATCH(G, cp_density_bwd_kernel, dim3(grid_for((n_points + run - 1) / run, G)), dim3(256), 0,
+                  as_stream(stream), a, n_points, run, pts, valid, feat, predictor, offset, d_sigma);
   return launch_status();
 }
 
@@ -965,8 +973,9 @@ extern "C" int srf_cp_appearance_bwd(const srf_cp_factors* factors, float* const
   if ((st = set_cp_grads(&a, d_lines)) != SRF_OK) return st;
   if (!pts || !d_features || (reinterpret_cast<uintptr_t>(d_features) & 15)) return SRF_EINVAL;
   const int G = next_pow2(a.n_quads);
-  SRF_VM_DISPATCH(G, cp_appearance_bwd_kernel, dim3(grid_for((n_rows + kRun - 1) / kRun, G)), dim3(256), 0,
-                  as_stream(stream), a, n_rows, idx, pts, d_features);
+  const int run = run_length();
+  SRF_VM_DISPATCH(G, cp_appearance_bwd_kernel, dim3(grid_for((n_rows + run - 1) / run, G)), dim3(256), 0,
+                  as_stream(stream), a, n_rows, run, idx, pts, d_features);
   return launch_status();
 }
 

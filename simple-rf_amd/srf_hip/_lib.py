@@ -41,6 +41,8 @@ _PROTOS = {
     'srf_status_string': [_I],
     'srf_selftest_mfma': [_I, _P, _P, _P, _P],
     'srf_selftest_lds_dma': [_P, _P, _P],
+    'srf_tune_set': [_I, _L],
+    'srf_tune_get': [_I],
     'srf_stratified_samples': [_I, _I, _F, _F, _I, _P, _P, _P],
     'srf_sample_pdf': [_I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P],
     'srf_fine_z_vals': [_I, _I, _I, _P, _P, _P, _P, _P, _P],
@@ -79,7 +81,8 @@ _PROTOS = {
 }
 _RESTYPES = {'srf_status_string': C.c_char_p, 'srf_mlp_packed_bytes': C.c_size_t,
              'srf_mlp_workspace_bytes': C.c_size_t, 'srf_compact_workspace_bytes': C.c_size_t,
-             'srf_color_packed_bytes': C.c_size_t, 'srf_color_workspace_bytes': C.c_size_t}
+             'srf_color_packed_bytes': C.c_size_t, 'srf_color_workspace_bytes': C.c_size_t,
+             'srf_tune_get': C.c_int64}
 
 _lib = None
 

@@ -57,3 +57,36 @@ def test_bad_arguments_return_status(call):
 def test_call_raises_on_error_status():
     with pytest.raises(_lib.SrfError):
         _lib.call('srf_stratified_samples', -1, 64, 0., 1., 0, None, None, None)
+
+
+def test_tune_knobs_validate_and_round_trip():
+    """The schedule table is host state: set/get work without a GPU and reject bad values."""
+    from srf_hip import autotune
+    lib = _lib.lib()
+    assert lib.srf_tune_get(99) == -1
+    assert lib.srf_tune_set(99, 0) == 2
+    for name, bad in (('dw_chunk', 1000), ('dw_chunk', 2048 + 128), ('color_dw_chunk', 131072),
+                      ('gather_blocks', 10), ('gather_run', 0 - 1), ('gather_run', 1000)):
+        assert lib.srf_tune_set(autotune.KNOBS[name], bad) == 2, (name, bad)
+    try:
+        for name, cands in autotune.CANDIDATES.items():
+            for v in cands:
+                autotune.set_knob(name, v)
+                assert autotune.get_knob(name) == v
+    finally:
+        autotune.reset()
+    assert all(autotune.get_knob(n) == 0 for n in autotune.KNOBS)
+
+
+def test_autotuner_cache_round_trip(tmp_path):
+    from srf_hip import autotune
+    path = tmp_path / 'tune.json'
+    t = autotune.Autotuner(path)
+    t.results['w'] = {'gather_run': {'chosen': 16, 'ms': {'0': 2.0, '16': 1.0}}}
+    t._save()
+    try:
+        assert autotune.Autotuner(path).apply('w')
+        assert autotune.get_knob('gather_run') == 16
+        assert not autotune.Autotuner(path).apply('other')
+    finally:
+        autotune.reset()
