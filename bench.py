@@ -366,6 +366,74 @@ def tensorf_train_leg(args, dev):
             'model': model, 'out': out}
 
 
+def tensorf_ragged_leg(args, dev):
+    """Uneven-batch TensoRF train step at scale: args.ragged_rays rays whose sample counts follow a
+    truncated log-normal (median 96, sigma_log 0.5, at most 512), the shape occupancy pruning of a
+    128^3 mask leaves; VM grids of 512^2 planes with 24 density / 64 appearance components per
+    plane (277 MB of factors), features PE L = 2; metric samples at half-voxel steps, composited
+    ragged (CSR ray offsets); fwd + hand-written bwd + Adam, FP32."""
+    import torch
+    import torch.nn.functional as F
+    from models.ModelFactory02 import get_model
+    from srf_hip import _lib, configs as cfgs
+    torch.manual_seed(0)
+    c = cfgs.simple_tensorf_configs(
+        augmentations=False, ndc=False, perturb=False,
+        coarse=dict(num_components_density=(24, 24, 24), num_components_color=(64, 64, 64),
+                    bounding_box=((-1.0, -1.0, -1.0), (1.0, 1.0, 1.0)), num_voxels_initial=512 ** 3,
+                    features_positional_encoding_degree=2))
+    tm = get_model(c, model_configs=cfgs.model_configs()).coarse_model.to(dev).train()
+    n = args.ragged_rays
+    g = torch.Generator().manual_seed(11)
+    counts = (96 * torch.exp(0.5 * torch.randn(n, generator=g))).round().clamp(1, 512).long()
+    off = torch.zeros(n + 1, dtype=torch.int64)
+    off[1:] = torch.cumsum(counts, 0)
+    step = float(tm.step_size)
+    o = -0.9 + 1.8 * torch.rand(n, 3, generator=g)
+    d = F.normalize(torch.randn(n, 3, generator=g), dim=-1)
+    ray = torch.repeat_interleave(torch.arange(n), counts)
+    k = torch.arange(int(off[-1])) - off[:-1][ray]
+    z = step * (k.float() + torch.rand(n, generator=g)[ray])
+    inp = {'pts': (o[ray] + d[ray] * z[:, None]).to(dev), 'z_vals': z.to(dev), 'ray_offsets': off.int().to(dev),
+           'rays_o': o.to(dev), 'rays_d': d.to(dev), 'view_dirs': d.to(dev), 'max_samples': int(counts.max())}
+    target = torch.rand(n, 3, generator=g).to(dev)
+    groups = tm.get_trainable_parameters({'lr_initial_tensor': 2e-2, 'lr_initial_network': 1e-3})
+    opt = torch.optim.Adam(groups, betas=(0.9, 0.99))
+
+    def train_step():
+        out = tm.forward_ragged(inp)
+        loss = ((out['rgb'] - target) ** 2).mean()
+        opt.zero_grad(set_to_none=True)
+        loss.backward()
+        opt.step()
+        return out
+
+    for _ in range(2):
+        out = train_step()
+    if TUNER is not None:
+        for knob in ('gather_run', 'gather_blocks'):
+            TUNER.tune(f'tensorf_ragged_{n}rays', knob, train_step, reps=1)
+    torch.cuda.synchronize()
+    timer = KernelTimer(_lib)
+    with timer:
+        timer.enabled = True
+        t0 = time.perf_counter()
+        for _ in range(args.ragged_steps):
+            out = train_step()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        timer.enabled = False
+    kern = timer.summary()
+    s = args.ragged_steps
+    n_samples = int(off[-1])
+    return {'value': round(n * s / (t1 - t0), 1), 'unit': 'rays/s', 'ms_per_step': round((t1 - t0) * 1e3 / s, 3),
+            'samples_per_step': n_samples, 'surface_samples': int((out['weights'] > 1e-4).sum()),
+            'workload': f'{n} rays, ragged counts (truncated log-normal, median 96, max 512; {n_samples:,} '
+                        'samples), VM 512^2 density 3x24 / appearance 3x64 (277 MB), features PE L=2, '
+                        'fwd+bwd+Adam, FP32',
+            'kernels': {k: {'calls': v['calls'], 'ms_per_step': round(v['ms'] / s, 3)} for k, v in kern.items()}}
+
+
 def _events_time(fn, reps):
     """Average GPU time of fn() over reps back-to-back calls, bracketed by HIP events on the
     current stream (the stream every entry point is launched on)."""
@@ -543,6 +611,9 @@ def main():
     ap.add_argument('--tensorf-rays', type=int, default=8192)
     ap.add_argument('--tensorf-steps', type=int, default=5)
     ap.add_argument('--no-tensorf', action='store_true')
+    ap.add_argument('--ragged-rays', type=int, default=32768)
+    ap.add_argument('--ragged-steps', type=int, default=3)
+    ap.add_argument('--no-ragged', action='store_true', help='skip the uneven-batch TensoRF leg')
     ap.add_argument('--no-graph', action='store_true', help='report the eager step only')
     ap.add_argument('--no-autotune', action='store_true', help="keep the library's built-in launch schedule")
     ap.add_argument('--copy-calibration-mb', type=int, default=0,
@@ -616,6 +687,8 @@ def main():
         result['tensorf_train'] = {k: v for k, v in tf.items() if k not in ('model', 'out')}
         result['hbm_kernels'] = hbm_leg(dev, tf['model'], tf['out'])
         del tf
+    if not args.no_ragged:
+        result['tensorf_ragged_train'] = tensorf_ragged_leg(args, dev)
     if TUNER is not None:
         result['autotune'] = TUNER.results
     if not args.no_baselines:

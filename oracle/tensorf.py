@@ -187,13 +187,15 @@ def tensorf_z_vals_ndc(n_rays: int, n_samples: int, near: float, far: float, u=N
 def tensorf_render(model: dict, pts: torch.Tensor, z_vals_ndc: torch.Tensor, rays_o, rays_d, rays_d_ndc,
                    view_dirs: torch.Tensor, *, alpha_volume=None, distance_scale=25.0, weight_threshold=1e-4,
                    density_kind='ReLU', density_offset=-10.0, white_bkgd=False, features_pe_degree=0,
-                   views_pe_degree=0):
+                   views_pe_degree=0, ndc=True):
     """LowRankTensor.forward in NDC, SimpleTensoRF09.py:701-761, for the VM decomposition
     (:1214-1263) or, when model['decomposition'] == 'CandecompParafac', the CP one (:1043-1089).
 
     model: dict(bbox [2,3], planes_density, lines_density, planes_color, lines_color (no planes
     for CP), basis [27, sum c], color_mlp params).  view_dirs [N,3] (expanded per sample as at
-    :735).  Returns the compositing dict plus raw sigma/rgb.
+    :735).  With ndc=False, z_vals_ndc holds the metric depths and the non-NDC branch of :775-778
+    applies (delta from the metric depths, infinite last interval).  Returns the compositing dict
+    plus raw sigma/rgb.
     """
     cp = model.get('decomposition', 'VectorMatrix') == 'CandecompParafac'
     if cp:
@@ -213,9 +215,13 @@ def tensorf_render(model: dict, pts: torch.Tensor, z_vals_ndc: torch.Tensor, ray
     if valid.any():
         feat = density_features(pts_n[valid])
         sigma[valid] = density_predictor(feat, density_kind, density_offset)
-    out = volume_rendering(sigma, torch.zeros(*pts.shape[:-1], 3, dtype=pts.dtype), z_vals_ndc=z_vals_ndc,
-                           rays_o=rays_o, rays_d=rays_d, rays_d_ndc=rays_d_ndc, ndc=True,
-                           distance_scale=distance_scale)
+    zero_rgb = torch.zeros(*pts.shape[:-1], 3, dtype=pts.dtype)
+    if ndc:
+        out = volume_rendering(sigma, zero_rgb, z_vals_ndc=z_vals_ndc, rays_o=rays_o, rays_d=rays_d,
+                               rays_d_ndc=rays_d_ndc, ndc=True, distance_scale=distance_scale)
+    else:
+        out = volume_rendering(sigma, zero_rgb, z_vals=z_vals_ndc, rays_o=rays_o, rays_d=rays_d, ndc=False,
+                               distance_scale=distance_scale)
     surface = out['weights'] > weight_threshold
     rgb = torch.zeros(*pts.shape[:-1], 3, dtype=pts.dtype)
     if surface.any():
@@ -230,3 +236,22 @@ def tensorf_render(model: dict, pts: torch.Tensor, z_vals_ndc: torch.Tensor, ray
     out['raw_sigma'] = sigma[..., None]
     out['raw_rgb'] = rgb
     return out
+
+
+def tensorf_render_ragged(model: dict, pts: torch.Tensor, z_vals: torch.Tensor, ray_offsets, rays_o, rays_d,
+                          view_dirs, **kw):
+    """LowRankTensor.forward (non-NDC, SimpleTensoRF09.py:701-761, :767-819) over a ragged batch:
+    ray r owns the flat samples [ray_offsets[r], ray_offsets[r+1]) of pts [P,3] / z_vals [P]
+    (metric depths).  Each ray is rendered on its own by tensorf_render, which is what the
+    reference computes for a batch of that one ray.  Returns per-ray rgb/acc/depth/depth_var and
+    per-sample weights/raw_sigma/raw_rgb, flat."""
+    outs = []
+    off = [int(x) for x in ray_offsets]
+    for r in range(len(off) - 1):
+        a, b = off[r], off[r + 1]
+        outs.append(tensorf_render(model, pts[a:b][None], z_vals[a:b][None], rays_o[r:r + 1], rays_d[r:r + 1], None,
+                                   view_dirs[r:r + 1], ndc=False, **kw))
+    res = {k: torch.cat([o[k] for o in outs]) for k in ('rgb', 'acc', 'depth', 'depth_var')}
+    for k in ('weights', 'raw_sigma', 'raw_rgb'):
+        res[k] = torch.cat([o[k][0] for o in outs])
+    return res

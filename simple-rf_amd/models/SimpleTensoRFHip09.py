@@ -569,6 +569,49 @@ class LowRankTensor(torch.nn.Module):
             out['raw_rgb'] = rgb
         return out
 
+    def forward_ragged(self, input_dict: dict, retraw: bool = False):
+        """LowRankTensor.forward (src/models/SimpleTensoRF09.py:701-761, non-NDC branch of
+        :767-819) over a ragged batch of occupancy-pruned samples: ray r owns the flat samples
+        [ray_offsets[r], ray_offsets[r+1]).  input_dict: pts [P,3], z_vals [P] (metric depths),
+        ray_offsets [N+1] int32, rays_o / rays_d / view_dirs [N,3], max_samples (largest count, a
+        host int).  Same arithmetic per ray as forward() on a dense batch of that ray alone; the
+        samples of a ray are composited with the infinite last interval of :776."""
+        hp = self.host_params()
+        pts, z, off = input_dict['pts'], input_dict['z_vals'], input_dict['ray_offsets']
+        n_pts = pts.shape[0]
+        n_rays = off.shape[0] - 1
+        alpha = self.alpha_mask.kernel_mask() if self.alpha_mask is not None else None
+        sigma = self._density(pts, hp['bbox'], alpha, float(self.tensor_configs.get('density_offset', 0.0)))
+        out = compositing.composite(sigma, None, z, input_dict['rays_o'], input_dict['rays_d'], None, ndc=False,
+                                    distance_scale=float(self.tensor_configs['distance_scale']), ray_offsets=off,
+                                    max_samples=int(input_dict['max_samples']))
+        out.pop('rgb')
+        weights = out['weights']
+        idx, count = tensorf.threshold_compact(weights, self.tensor_configs['ray_marching_weight_threshold'])
+        m = int(count.item())  # the reference's boolean indexing synchronises here too (:1249)
+        rgb_flat = torch.zeros((n_pts, 3), dtype=torch.float32, device=pts.device)
+        if m > 0:
+            sidx = idx[:m]
+            app = self._appearance(pts, hp['bbox'], sidx, m)
+            vd = None
+            if self.tensor_configs['view_dependent_color']:
+                counts = (off[1:] - off[:-1]).long()
+                ray_of = torch.repeat_interleave(torch.arange(n_rays, device=pts.device), counts, output_size=n_pts)
+                vd = input_dict['view_dirs'][ray_of].contiguous()
+            mlp = self.color_predictor.mlp
+            color = hip_color.color_forward(self.color_spec, app, self.basis_matrix_color.weight,
+                                            (mlp[0].weight, mlp[0].bias, mlp[2].weight, mlp[2].bias, mlp[4].weight,
+                                             mlp[4].bias), view_dirs=vd, idx=sidx, n_samples=1)
+            rgb_flat = rgb_flat.index_put((sidx.long(),), color)
+        rgb_map = compositing.volume_render(rgb_flat, weights, ray_offsets=off)
+        if self.configs['model']['white_bkgd'] or (self.training and torch.rand((1,)) < 0.5):
+            rgb_map = rgb_map + (1 - out['acc'][..., None])
+        out['rgb'] = rgb_map
+        if retraw:
+            out['raw_sigma'] = sigma[..., None]
+            out['raw_rgb'] = rgb_flat
+        return out
+
     def get_volume_rendering_weights(self, input_dict: dict, sigma: torch.Tensor):
         """src/models/SimpleTensoRF09.py:767-811 on srf_composite_fwd (colours composited later)."""
         ds = float(self.tensor_configs['distance_scale'])

@@ -175,3 +175,53 @@ def test_cp_equals_vm_with_outer_product_plane():
     cp = ot.cp_appearance_features(lines, pts_n)
     vm = ot.vm_appearance_features(planes, vm_lines, pts_n)[:, :rank]
     assert torch.allclose(cp, vm, rtol=0, atol=1e-15)
+
+
+def test_ragged_render_gradient_matches_finite_differences():
+    """The oracle's ragged non-NDC TensoRF render (every ray rendered on its own) differentiated by
+    autograd (grid_sample's backward, cumprod, the colour MLP) agrees with central differences in
+    FP64 on a tiny scene: this pins the reference gradient the GPU tests compare against."""
+    g = torch.Generator().manual_seed(21)
+    res = [5, 6, 7]
+    planes_d, lines_d = rand_factors([4, 4, 4], res, 22)
+    planes_c, lines_c = rand_factors([4, 4, 4], res, 23)
+    planes_d = [p * 8 for p in planes_d]
+    lin = lambda o, i: (torch.randn(o, i, generator=g, dtype=torch.float64) / i ** 0.5,
+                        0.1 * torch.randn(o, generator=g, dtype=torch.float64))
+    (w1, b1), (w2, b2), (w3, b3) = lin(16, 30), lin(16, 16), lin(3, 16)
+    model = {'bbox': torch.tensor([[-1.0, -1.0, -1.0], [1.0, 1.0, 1.0]], dtype=torch.float64),
+             'planes_density': planes_d, 'lines_density': lines_d, 'planes_color': planes_c, 'lines_color': lines_c,
+             'basis': torch.randn(27, 12, generator=g, dtype=torch.float64) / 12 ** 0.5,
+             'color_mlp': {'mlp.0.weight': w1, 'mlp.0.bias': b1, 'mlp.2.weight': w2, 'mlp.2.bias': b2,
+                           'mlp.4.weight': w3, 'mlp.4.bias': b3}}
+    counts = [3, 7, 1, 5]
+    off = torch.tensor([0] + list(torch.cumsum(torch.tensor(counts), 0)), dtype=torch.int32)
+    o = torch.rand(4, 3, generator=g, dtype=torch.float64) * 0.8 - 0.4
+    d = torch.nn.functional.normalize(torch.randn(4, 3, generator=g, dtype=torch.float64), dim=-1)
+    ray = torch.repeat_interleave(torch.arange(4), torch.tensor(counts))
+    k = torch.arange(sum(counts)) - off[:-1].long()[ray]
+    z = 0.05 + 0.15 * (k.double() + 0.3)
+    pts = o[ray] + d[ray] * z[:, None]
+    gr = torch.randn(4, 3, generator=g, dtype=torch.float64)
+
+    def loss():
+        out = ot.tensorf_render_ragged(model, pts, z, off, o, d, d, distance_scale=25.0, weight_threshold=1e-4)
+        return (out['rgb'] * gr).sum() + out['acc'].sum()
+
+    targets = [(planes_d[0], (0, 1, 2, 3)), (lines_d[1], (0, 2, 3, 0)), (planes_c[2], (0, 3, 4, 2)),
+               (model['basis'], (5, 7)), (w1, (3, 11))]
+    for t, _ in targets:
+        t.requires_grad_(True)
+    base = loss()
+    assert (ot.tensorf_render_ragged(model, pts, z, off, o, d, d)['weights'] > 1e-4).sum() >= 6
+    base.backward()
+    h = 1e-6
+    for t, i in targets:
+        with torch.no_grad():
+            t[i] += h
+            up = loss().item()
+            t[i] -= 2 * h
+            dn = loss().item()
+            t[i] += h
+        fd = (up - dn) / (2 * h)
+        assert abs(t.grad[i].item() - fd) <= 1e-6 * max(1.0, abs(fd)), (i, t.grad[i].item(), fd)
