@@ -432,6 +432,8 @@ __device__ __forceinline__ void run_layer(const uint8_t* __restrict__ A, FIn&& f
 // 16 B chunk of that feature at the points 16 cb + 2 m and 16 cb + 2 m + 1 (cb = 0 .. CB - 1), in
 // point order.  A 16 B chunk is then exactly one lane's MFMA operand slice of one feature, so the
 // GEMM reads fragments with plain 16 B copies and every store here is a coalesced 32 B per lane.
+// The stores are non-temporal: a layer's tiles of all the CUs of an XCD (~4 MB) would otherwise
+// displace the streamed weights from that XCD's L2, and they are read back only by the backward.
 template <int P>
 __device__ __forceinline__ void store_tile(uint8_t* __restrict__ dst, int p0, const typename Tr<P>::Act (&v)[Tr<P>::CB]) {
   constexpr int CB = Tr<P>::CB;
@@ -456,8 +458,8 @@ __device__ __forceinline__ void store_tile(uint8_t* __restrict__ dst, int p0, co
       c1[2 * cb + 0] = odd ? rc[cb][1] : own1;
       c1[2 * cb + 1] = odd ? own1 : rc[cb][1];
     }
-    rec[0] = __builtin_bit_cast(u32x4, c0);
-    rec[1] = __builtin_bit_cast(u32x4, c1);
+    __builtin_nontemporal_store(__builtin_bit_cast(u32x4, c0), rec);
+    __builtin_nontemporal_store(__builtin_bit_cast(u32x4, c1), rec + 1);
   } else {
     // v[cb] = {bf16 (r0, r1), bf16 (r2, r3)} at point 16 cb + j
     u32x4 c0, c1;
@@ -469,8 +471,8 @@ __device__ __forceinline__ void store_tile(uint8_t* __restrict__ dst, int p0, co
       c0[cb] = (first & 0xffffu) | (second << 16);
       c1[cb] = (first >> 16) | (second & 0xffff0000u);
     }
-    rec[0] = c0;
-    rec[1] = c1;
+    __builtin_nontemporal_store(c0, rec);
+    __builtin_nontemporal_store(c1, rec + 1);
   }
 }
 
