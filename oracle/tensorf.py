@@ -239,19 +239,57 @@ def tensorf_render(model: dict, pts: torch.Tensor, z_vals_ndc: torch.Tensor, ray
 
 
 def tensorf_render_ragged(model: dict, pts: torch.Tensor, z_vals: torch.Tensor, ray_offsets, rays_o, rays_d,
-                          view_dirs, **kw):
+                          view_dirs, *, alpha_volume=None, distance_scale=25.0, weight_threshold=1e-4,
+                          density_kind='ReLU', density_offset=-10.0, white_bkgd=False, features_pe_degree=0,
+                          views_pe_degree=0):
     """LowRankTensor.forward (non-NDC, SimpleTensoRF09.py:701-761, :767-819) over a ragged batch:
     ray r owns the flat samples [ray_offsets[r], ray_offsets[r+1]) of pts [P,3] / z_vals [P]
-    (metric depths).  Each ray is rendered on its own by tensorf_render, which is what the
-    reference computes for a batch of that one ray.  Returns per-ray rgb/acc/depth/depth_var and
-    per-sample weights/raw_sigma/raw_rgb, flat."""
-    outs = []
+    (metric depths).  Per ray this is the reference's forward on a batch of that one ray: the
+    validity, density, surface test, appearance gather, basis and colour MLP are per sample (so
+    they run over all samples at once), and the compositing (:775-803, infinite last interval) runs
+    ray by ray.  Returns per-ray rgb/acc/depth/depth_var and per-sample weights/raw_sigma/raw_rgb."""
+    from .nerf import volume_rendering
+    cp = model.get('decomposition', 'VectorMatrix') == 'CandecompParafac'
+    bbox = model['bbox']
+    valid = bbox_validity(pts, bbox)
+    if alpha_volume is not None:
+        alphas = alpha_mask_sample(alpha_volume, bbox, pts[valid])
+        valid[valid.clone()] &= alphas > 0
+    pts_n = normalize_points(pts, bbox)
+    sigma = torch.zeros(pts.shape[:-1], dtype=pts.dtype)
+    if valid.any():
+        if cp:
+            feat = cp_density_features(model['lines_density'], pts_n[valid])
+        else:
+            feat = vm_density_features(model['planes_density'], model['lines_density'], pts_n[valid])
+        sigma[valid] = density_predictor(feat, density_kind, density_offset)
     off = [int(x) for x in ray_offsets]
-    for r in range(len(off) - 1):
+    n = len(off) - 1
+    counts = torch.tensor([off[r + 1] - off[r] for r in range(n)])
+    ray_of = torch.repeat_interleave(torch.arange(n), counts)
+    per = {'acc': [], 'depth': [], 'depth_var': [], 'weights': []}
+    for r in range(n):
         a, b = off[r], off[r + 1]
-        outs.append(tensorf_render(model, pts[a:b][None], z_vals[a:b][None], rays_o[r:r + 1], rays_d[r:r + 1], None,
-                                   view_dirs[r:r + 1], ndc=False, **kw))
-    res = {k: torch.cat([o[k] for o in outs]) for k in ('rgb', 'acc', 'depth', 'depth_var')}
-    for k in ('weights', 'raw_sigma', 'raw_rgb'):
-        res[k] = torch.cat([o[k][0] for o in outs])
+        o = volume_rendering(sigma[a:b][None], torch.zeros(1, b - a, 3, dtype=pts.dtype), z_vals=z_vals[a:b][None],
+                             rays_o=rays_o[r:r + 1], rays_d=rays_d[r:r + 1], ndc=False, distance_scale=distance_scale)
+        for k in per:
+            per[k].append(o[k][0] if k == 'weights' else o[k])
+    res = {k: torch.cat(v) for k, v in per.items()}
+    weights = res['weights']
+    surface = weights > weight_threshold
+    rgb = torch.zeros(*pts.shape[:-1], 3, dtype=pts.dtype)
+    if surface.any():
+        if cp:
+            app = cp_appearance_features(model['lines_color'], pts_n[surface])
+        else:
+            app = vm_appearance_features(model['planes_color'], model['lines_color'], pts_n[surface])
+        feats = F.linear(app, model['basis'])
+        rgb[surface] = color_mlp_forward(model['color_mlp'], feats, view_dirs[ray_of][surface], features_pe_degree,
+                                         views_pe_degree)
+    rgb_map = torch.zeros(n, 3, dtype=pts.dtype).index_add(0, ray_of, weights[:, None] * rgb)
+    if white_bkgd:
+        rgb_map = rgb_map + (1 - res['acc'][..., None])
+    res['rgb'] = rgb_map
+    res['raw_sigma'] = sigma[..., None]
+    res['raw_rgb'] = rgb
     return res

@@ -1,0 +1,252 @@
+"""Parity at the workload shapes themselves (GPU):
+
+* the plain coarse-only Simple-NeRF FP32 step on 1024 rays (504 x 378 camera, 64 samples, white background);
+* the 300^2 VM TensoRF model (density 3 x 16, appearance 3 x 48, features PE L = 2, 64^3 alpha mask at 40 %,
+  443 samples) on 8192 rays: a 64-ray subset of the full batch against the FP64 oracle, and the factor and
+  network gradients of a 256-ray batch on the full-size grids;
+* the uneven-batch TensoRF model (512^2 planes, 24 / 64 components, ragged log-normal sample counts) on
+  32,768 rays: a 64-ray subset against the oracle rendering each ray on its own, and gradients of a 256-ray
+  ragged batch.
+"""
+import pytest
+import torch
+import torch.nn.functional as F
+
+import oracle.tensorf as ot
+import oracle.train_step as ots
+from models.ModelFactory02 import get_model
+from srf_hip import configs as cfgs, synthetic, tensorf
+
+pytestmark = pytest.mark.gpu
+
+THR = 1e-4
+
+
+def _rel(a, b):
+    return ((a - b).abs().max() / b.abs().max().clamp_min(1e-30)).item()
+
+
+def _assert_grads(pairs, what, tol=1e-3):
+    for name, a, b in pairs:
+        e = _rel(a.cpu().double(), b)
+        print(f'{what} {name}: grad rel err {e:.2e}')
+        assert e < tol, (name, e)
+
+
+# FP32 and FP64 may take a discrete decision differently where a sample sits within rounding of its
+# threshold: the surface test (weight > 1e-4; FP32 weights carry ~2e-5 relative error), the
+# occupancy test (trilinear alpha > 0, exactly 0 on one side of a voxel plane) and the ReLU kink of
+# the density.  Such a sample sends its whole gradient in one precision and none in the other to
+# the few texels it touches, so the gradient checks leave out the rays that hold one.
+def _unstable_rays(weights, ray_of, n_rays, feat=None, valid_mismatch=None):
+    bad = torch.zeros(n_rays, dtype=torch.bool)
+    near_thr = (weights - THR).abs() < 1e-3 * THR
+    bad[ray_of[near_thr]] = True
+    if feat is not None:
+        bad[ray_of[feat]] = True
+    if valid_mismatch is not None:
+        bad[ray_of[valid_mismatch]] = True
+    return bad
+
+
+def test_coarse_only_fp32_step_1024_rays(cuda):
+    torch.manual_seed(0)
+    c = cfgs.simple_nerf_configs(precision='fp32', augmentations=False, fine=False, perturb=False,
+                                 raw_noise_std=0.0, white_bkgd=True)
+    mc = cfgs.model_configs()
+    model = get_model(c, model_configs=mc).to(cuda).train()
+    batch = synthetic.train_batch(1024, seed=11, device=cuda)
+    out = model({'pixel_id': batch['pixel_id'], 'num_frames': 3, 'iter_num': 0, 'sub_batch_index': 0})
+    loss = ((out['rgb_coarse'] - batch['target_rgb']) ** 2).mean()
+    loss.backward()
+    pid = batch['pixel_id'].cpu()
+    K = torch.from_numpy(mc['intrinsics'])[pid[:, 0].long()].double()
+    E = torch.from_numpy(mc['extrinsics'])[pid[:, 0].long()].double()
+    def oracle(dtype):
+        prm = {'coarse': {k: v.detach().cpu().to(dtype).requires_grad_(True)
+                          for k, v in model.coarse_model.state_dict().items()}}
+        ref = ots.render_train(prm, pid, K.to(dtype), E.to(dtype), mc['resolution'], augment=False, white_bkgd=True)
+        ((ref['rgb_coarse'] - batch['target_rgb'].cpu().to(dtype)) ** 2).mean().backward()
+        return ref, prm
+
+    ref, p64 = oracle(torch.float64)
+    _, p32 = oracle(torch.float32)
+    assert (out['rgb_coarse'].detach().cpu().double() - ref['rgb_coarse']).abs().max().item() < 1e-5
+    a = out['depth_ndc_coarse'].detach().cpu().double() * (out['acc_coarse'].detach().cpu().double() + 1e-6)
+    b = (ref['depth_ndc_coarse'] * (ref['acc_coarse'] + 1e-6)).detach()
+    assert (a - b).abs().max().item() < 1e-5
+    # gradients of a random-init network are ~1e-8 here: relative error in norm, within 1e-3 or within 3x the
+    # FP32 oracle's own error against FP64 (the same bar as the full-model test)
+    for n, p in model.coarse_model.named_parameters():
+        r = p64['coarse'][n].grad
+        ek = ((p.grad.cpu().double() - r).norm() / r.norm().clamp_min(1e-30)).item()
+        eo = ((p32['coarse'][n].grad.double() - r).norm() / r.norm().clamp_min(1e-30)).item()
+        assert ek < max(1e-3, 3 * eo), (n, ek, eo)
+
+
+def _vm300(cuda):
+    torch.manual_seed(0)
+    box = ((-1.0, -1.0, -1.0), (1.0, 1.0, 1.0))
+    c = cfgs.simple_tensorf_configs(
+        perturb=False, white_bkgd=True, augmentations=False,
+        coarse=dict(num_components_density=(16, 16, 16), num_components_color=(48, 48, 48), bounding_box=box,
+                    num_voxels_initial=300 ** 3 + 4, num_samples_max=443, features_positional_encoding_degree=2))
+    model = get_model(c, model_configs=cfgs.model_configs()).to(cuda).eval()
+    from models.SimpleTensoRFHip09 import AlphaGridMask
+    g = torch.Generator().manual_seed(7)
+    vol = (torch.rand(64, 64, 64, generator=g) < 0.4).float()
+    model.coarse_model.alpha_mask = AlphaGridMask(vol, model.coarse_model.bounding_box.cpu()).to(cuda)
+    return model, vol
+
+
+def _vm_params(tm, grad):
+    f64 = lambda t: t.detach().cpu().double().contiguous().requires_grad_(grad)
+    return {'bbox': tm.bounding_box.detach().cpu().double(),
+            'planes_density': [f64(p) for p in tm.matrices_density], 'lines_density': [f64(p) for p in tm.vectors_density],
+            'planes_color': [f64(p) for p in tm.matrices_color], 'lines_color': [f64(p) for p in tm.vectors_color],
+            'basis': f64(tm.basis_matrix_color.weight),
+            'color_mlp': {k: f64(v) for k, v in tm.color_predictor.state_dict().items()}}
+
+
+def _vm_oracle(out, prm, vol, rows):
+    d = lambda k: out[k].detach().cpu().double()[rows]
+    o, dn, z = d('rays_o_ndc'), d('rays_d_ndc'), d('z_vals_coarse')
+    return ot.tensorf_render(prm, o[:, None] + dn[:, None] * z[..., None], z, d('rays_o'), d('rays_d'), dn,
+                             d('view_dirs'), alpha_volume=vol.double(), distance_scale=25.0, weight_threshold=THR,
+                             white_bkgd=True, features_pe_degree=2)
+
+
+def test_vm300_full_batch_subset_and_gradients(cuda):
+    model, vol = _vm300(cuda)
+    tm = model.coarse_model
+    assert tm.resolution.tolist() == [300, 300, 300] and int(tm.num_samples) == 443
+    batch = synthetic.train_batch(8192, seed=12, device=cuda)
+    with torch.no_grad():
+        out = model({'pixel_id': batch['pixel_id'], 'num_frames': 3}, retraw=True)
+    rows = torch.randperm(8192, generator=torch.Generator().manual_seed(13))[:64]
+    ref = _vm_oracle(out, _vm_params(tm, False), vol, rows)
+    keep = ~((ref['weights'] - THR).abs() < 1e-9).any(-1)
+    assert keep.sum() >= 62 and (ref['weights'] > THR).sum() > 2000
+    for k in ('rgb', 'acc', 'depth_ndc', 'weights'):
+        e = (out[f'{k}_coarse'].cpu().double()[rows][keep] - ref[k][keep]).abs().max().item()
+        assert e < 1e-5, (k, e)
+    # gradients of a 256-ray batch on the full-size grids
+    b2 = synthetic.train_batch(256, seed=14, device=cuda)
+    out2 = model({'pixel_id': b2['pixel_id'], 'num_frames': 3}, retraw=True)
+    prm = _vm_params(tm, True)
+    ref2 = _vm_oracle(out2, prm, vol, torch.arange(256))
+    d = lambda k: out2[k].detach().cpu().double()
+    pts64 = (d('rays_o_ndc')[:, None] + d('rays_d_ndc')[:, None] * d('z_vals_coarse')[..., None]).reshape(-1, 3)
+    pts32 = (out2['rays_o_ndc'][:, None] + out2['rays_d_ndc'][:, None] * out2['z_vals_coarse'][..., None]).reshape(-1, 3)
+    in64 = ot.bbox_validity(pts64, prm['bbox'])
+    occ64 = torch.zeros_like(in64)
+    occ64[in64] = ot.alpha_mask_sample(vol.double(), prm['bbox'], pts64[in64]) > 0
+    occ32 = (tensorf.alpha_mask_sample(tm.alpha_mask.kernel_mask(), pts32.detach()) > 0).cpu() & in64
+    feat = torch.zeros(pts64.shape[0], dtype=torch.float64)
+    feat[occ64] = ot.vm_density_features([p.detach() for p in prm['planes_density']],
+                                         [p.detach() for p in prm['lines_density']],
+                                         ot.normalize_points(pts64[occ64], prm['bbox']))
+    ray_of = torch.arange(256).repeat_interleave(443)
+    bad = _unstable_rays(ref2['weights'].detach().reshape(-1), ray_of, 256, feat=occ64 & (feat.abs() < 1e-6),
+                         valid_mismatch=occ64 != occ32)
+    print('vm300: rays left out of the gradient check:', int(bad.sum()))
+    assert bad.sum() <= 8
+    gr = torch.randn(256, 3, generator=torch.Generator().manual_seed(15))
+    gr[bad] = 0
+    (out2['rgb_coarse'] * gr.to(cuda)).sum().backward()
+    (ref2['rgb'] * gr.double()).sum().backward()
+    pairs = []
+    for nm, key in (('matrices_density', 'planes_density'), ('vectors_density', 'lines_density'),
+                    ('matrices_color', 'planes_color'), ('vectors_color', 'lines_color')):
+        pairs += [(f'{nm}.{i}', x.grad, y.grad) for i, (x, y) in enumerate(zip(getattr(tm, nm), prm[key]))]
+    pairs.append(('basis', tm.basis_matrix_color.weight.grad, prm['basis'].grad))
+    pairs += [(k, dict(tm.color_predictor.named_parameters())[k].grad, v.grad) for k, v in prm['color_mlp'].items()]
+    _assert_grads(pairs, 'vm300')
+
+
+def _vm512(cuda):
+    torch.manual_seed(0)
+    c = cfgs.simple_tensorf_configs(
+        augmentations=False, ndc=False, perturb=False, white_bkgd=True,
+        coarse=dict(num_components_density=(24, 24, 24), num_components_color=(64, 64, 64),
+                    bounding_box=((-1.0, -1.0, -1.0), (1.0, 1.0, 1.0)), num_voxels_initial=512 ** 3,
+                    features_positional_encoding_degree=2))
+    return get_model(c, model_configs=cfgs.model_configs()).coarse_model.to(cuda).eval()
+
+
+def _ragged(n, step, seed):
+    g = torch.Generator().manual_seed(seed)
+    counts = (96 * torch.exp(0.5 * torch.randn(n, generator=g))).round().clamp(1, 512).long()
+    off = torch.zeros(n + 1, dtype=torch.int64)
+    off[1:] = torch.cumsum(counts, 0)
+    o = -0.9 + 1.8 * torch.rand(n, 3, generator=g)
+    d = F.normalize(torch.randn(n, 3, generator=g), dim=-1)
+    ray = torch.repeat_interleave(torch.arange(n), counts)
+    k = torch.arange(int(off[-1])) - off[:-1][ray]
+    z = step * (k.float() + torch.rand(n, generator=g)[ray])
+    return {'pts': o[ray] + d[ray] * z[:, None], 'z_vals': z, 'ray_offsets': off.int(), 'rays_o': o, 'rays_d': d,
+            'view_dirs': d, 'max_samples': int(counts.max())}
+
+
+def _subset(b, rows):
+    off = b['ray_offsets'].long()
+    segs = [torch.arange(int(off[r]), int(off[r + 1])) for r in rows]
+    idx = torch.cat(segs)
+    counts = torch.tensor([len(s) for s in segs])
+    o2 = torch.zeros(len(rows) + 1, dtype=torch.int64)
+    o2[1:] = torch.cumsum(counts, 0)
+    return {'pts': b['pts'][idx].double(), 'z_vals': b['z_vals'][idx].double(), 'ray_offsets': o2,
+            'rays_o': b['rays_o'][rows].double(), 'rays_d': b['rays_d'][rows].double(),
+            'view_dirs': b['view_dirs'][rows].double()}, idx
+
+
+def test_vm512_ragged_full_batch_subset_and_gradients(cuda):
+    tm = _vm512(cuda)
+    assert tm.resolution.tolist() == [512, 512, 512]
+    step = float(tm.step_size)
+    b = _ragged(32768, step, 21)
+    inp = {k: (v.to(cuda) if torch.is_tensor(v) else v) for k, v in b.items()}
+    with torch.no_grad():
+        out = tm.forward_ragged(inp, retraw=True)
+    rows = torch.randperm(32768, generator=torch.Generator().manual_seed(22))[:64]
+    sub, idx = _subset(b, rows)
+    prm = _vm_params(tm, False)
+    ref = ot.tensorf_render_ragged(prm, sub['pts'], sub['z_vals'], sub['ray_offsets'], sub['rays_o'], sub['rays_d'],
+                                   sub['view_dirs'], distance_scale=25.0, weight_threshold=THR, white_bkgd=True,
+                                   features_pe_degree=2)
+    assert (ref['weights'] > THR).sum() > 500
+    counts = sub['ray_offsets'][1:] - sub['ray_offsets'][:-1]
+    keep = ~_unstable_rays(ref['weights'], torch.arange(64).repeat_interleave(counts), 64)
+    assert keep.sum() >= 62
+    for k in ('rgb', 'acc'):
+        assert (out[k].cpu().double()[rows][keep] - ref[k][keep]).abs().max().item() < 1e-5, k
+    assert (out['weights'].cpu().double()[idx] - ref['weights']).abs().max().item() < 1e-5
+    # gradients of a 256-ray ragged batch on the full-size grids
+    b2 = _ragged(256, step, 23)
+    inp2 = {k: (v.to(cuda) if torch.is_tensor(v) else v) for k, v in b2.items()}
+    out2 = tm.forward_ragged(inp2)
+    prm2 = _vm_params(tm, True)
+    sub2, _ = _subset(b2, torch.arange(256))
+    ref2 = ot.tensorf_render_ragged(prm2, sub2['pts'], sub2['z_vals'], sub2['ray_offsets'], sub2['rays_o'],
+                                    sub2['rays_d'], sub2['view_dirs'], distance_scale=25.0, weight_threshold=THR,
+                                    white_bkgd=True, features_pe_degree=2)
+    counts = (sub2['ray_offsets'][1:] - sub2['ray_offsets'][:-1])
+    ray_of = torch.arange(256).repeat_interleave(counts)
+    in64 = ot.bbox_validity(sub2['pts'], prm2['bbox'])
+    feat = torch.zeros(sub2['pts'].shape[0], dtype=torch.float64)
+    feat[in64] = ot.vm_density_features([p.detach() for p in prm2['planes_density']],
+                                        [p.detach() for p in prm2['lines_density']],
+                                        ot.normalize_points(sub2['pts'][in64], prm2['bbox']))
+    bad = _unstable_rays(ref2['weights'].detach(), ray_of, 256, feat=in64 & (feat.abs() < 1e-6))
+    print('vm512 ragged: rays left out of the gradient check:', int(bad.sum()))
+    assert bad.sum() <= 8
+    gr = torch.randn(256, 3, generator=torch.Generator().manual_seed(24))
+    gr[bad] = 0
+    (out2['rgb'] * gr.to(cuda)).sum().backward()
+    (ref2['rgb'] * gr.double()).sum().backward()
+    pairs = []
+    for nm, key in (('matrices_density', 'planes_density'), ('vectors_density', 'lines_density'),
+                    ('matrices_color', 'planes_color'), ('vectors_color', 'lines_color')):
+        pairs += [(f'{nm}.{i}', x.grad, y.grad) for i, (x, y) in enumerate(zip(getattr(tm, nm), prm2[key]))]
+    pairs += [(k, dict(tm.color_predictor.named_parameters())[k].grad, v.grad) for k, v in prm2['color_mlp'].items()]
+    _assert_grads(pairs, 'vm512 ragged')
