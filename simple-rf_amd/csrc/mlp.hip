@@ -1,5 +1,7 @@
 // C-ABI of the fused PE + MLP kernels: planning, weight packing and dispatch.
 // Kernel design notes are at the top of mlp_impl.h.
+#include <string.h>
+
 #include "mlp_impl.h"
 
 namespace srf {
@@ -65,7 +67,8 @@ __device__ inline int slot_k(int s, int g, int i) {
   return 32 * s + (i < 4 ? 4 * g + i : 16 + 4 * g + i - 4);
 }
 
-__global__ void pack_kernel(PackArgs a, int l, int bwd) {
+// Element idx of the forward (bwd = 0) or transposed (bwd = 1) operand image of layer l.
+__device__ void pack_element(const PackArgs& a, int l, int bwd, long idx) {
   const Plan& p = a.p;
   const bool f32 = p.prec == SRF_PREC_F32;
   const int kpl = f32 ? 1 : 8;
@@ -79,7 +82,6 @@ __global__ void pack_kernel(PackArgs a, int l, int bwd) {
   }
   const int ks = f32 ? 4 * k_tiles : k_tiles / 2;
   const long total = (long)rows_t * ks * 64 * kpl;
-  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= total) return;
   const int slot = (int)(idx % kpl);
   const int lane = (int)((idx / kpl) % 64);
@@ -104,8 +106,7 @@ __global__ void pack_kernel(PackArgs a, int l, int bwd) {
   }
 }
 
-__global__ void pack_bias_kernel(PackArgs a, int l) {
-  const int o = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ void pack_bias_element(const PackArgs& a, int l, int o) {
   if (o >= 16 * a.p.out_tiles[l]) return;
   float v = 0.f;
   if (o < a.p.out_rows[l]) {
@@ -114,6 +115,23 @@ __global__ void pack_bias_kernel(PackArgs a, int l) {
     v = a.b[t][r];
   }
   reinterpret_cast<float*>(a.out + a.p.bias_off[l])[o] = v;
+}
+
+// Every operand image and bias row of one model in a single launch: job j covers the blocks
+// [first_block[j], first_block[j + 1]) (kind 0: forward image, 1: transposed image, 2: bias).
+constexpr int kPackJobs = 3 * kLayers;
+struct PackJobs {
+  int n_jobs;
+  int layer[kPackJobs], kind[kPackJobs];
+  unsigned first_block[kPackJobs + 1];
+};
+
+__global__ __launch_bounds__(256) void pack_all_kernel(PackArgs a, PackJobs jobs) {
+  int j = 0;
+  while (j + 1 < jobs.n_jobs && jobs.first_block[j + 1] <= blockIdx.x) ++j;  // block-uniform
+  const long idx = (long)(blockIdx.x - jobs.first_block[j]) * blockDim.x + threadIdx.x;
+  if (jobs.kind[j] == 2) pack_bias_element(a, jobs.layer[j], (int)idx);
+  else pack_element(a, jobs.layer[j], jobs.kind[j], idx);
 }
 
 
@@ -174,8 +192,17 @@ extern "C" int srf_mlp_pack(const srf_mlp_desc* desc, const float* const* weight
                               (bool)p.view_dep, true, (bool)p.view_dep, (bool)p.view_dep};
   for (int i = 0; i < kParams; ++i)
     if (need[i] && (!a.w[i] || !a.b[i])) return SRF_EINVAL;
-  hipStream_t s = as_stream(stream);
   const int kpl = p.prec == SRF_PREC_F32 ? 1 : 8;
+  PackJobs jobs;
+  memset(&jobs, 0, sizeof(jobs));
+  unsigned blocks = 0;
+  auto add = [&](int l, int kind, long total) {
+    jobs.layer[jobs.n_jobs] = l;
+    jobs.kind[jobs.n_jobs] = kind;
+    jobs.first_block[jobs.n_jobs] = blocks;
+    blocks += (unsigned)((total + 255) / 256);
+    ++jobs.n_jobs;
+  };
   for (int l = 0; l < kLayers; ++l) {
     if (!p.present[l]) continue;
     for (int bwd = 0; bwd < 2; ++bwd) {
@@ -183,11 +210,12 @@ extern "C" int srf_mlp_pack(const srf_mlp_desc* desc, const float* const* weight
       const int rows_t = bwd ? p.reg_tiles[l] : p.out_tiles[l];
       const int k_tiles = bwd ? p.bwd_tiles[l] : p.reg_tiles[l] + p.comp_tiles[l];
       const int ks = p.prec == SRF_PREC_F32 ? 4 * k_tiles : k_tiles / 2;
-      const long total = (long)rows_t * ks * 64 * kpl;
-      hipLaunchKernelGGL(pack_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, a, l, bwd);
+      add(l, bwd, (long)rows_t * ks * 64 * kpl);
     }
-    hipLaunchKernelGGL(pack_bias_kernel, dim3((16 * p.out_tiles[l] + 255) / 256), dim3(256), 0, s, a, l);
+    add(l, 2, 16L * p.out_tiles[l]);
   }
+  jobs.first_block[jobs.n_jobs] = blocks;
+  hipLaunchKernelGGL(pack_all_kernel, dim3(blocks), dim3(256), 0, as_stream(stream), a, jobs);
   return launch_status();
 }
 
