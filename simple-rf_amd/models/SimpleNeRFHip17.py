@@ -106,9 +106,18 @@ class SimpleNeRFHip(torch.nn.Module):
                     input_batch['common_data'][key] = input_batch['common_data'][key][0]
         image_id = input_batch['pixel_id'][:, 0]
         intrinsics = self.intrinsics_learner(image_id)
-        extrinsics = self.extrinsics_learner(image_id)
-        all_extrinsics = self.extrinsics_learner(torch.arange(input_batch['num_frames'], device=image_id.device, dtype=image_id.dtype))
+        # the camera of a ray is a function of its frame: evaluate the learners and the intrinsics
+        # inverse once per frame and gather them per ray (the reference evaluates them per ray)
+        cams = torch.arange(self.extrinsics_learner.num_frames, device=image_id.device, dtype=image_id.dtype)
+        cam_extrinsics = self.extrinsics_learner(cams)
+        extrinsics = cam_extrinsics[image_id]
+        n_frames = input_batch['num_frames']
+        all_extrinsics = cam_extrinsics if n_frames == cams.shape[0] else \
+            self.extrinsics_learner(torch.arange(n_frames, device=image_id.device, dtype=image_id.dtype))
         input_batch['intrinsics'] = intrinsics
+        k_cams = torch.arange(self.intrinsics_learner.initial_intrinsics.shape[0], device=image_id.device,
+                              dtype=image_id.dtype)
+        input_batch['intrinsics_inv'] = torch.linalg.inv_ex(self.intrinsics_learner(k_cams))[0][image_id]
         input_batch['extrinsics'] = extrinsics
         if mode == 'camera_params_only':
             out = {}
@@ -146,7 +155,8 @@ class SimpleNeRFHip(torch.nn.Module):
         ret = {}
         pixel_id = input_dict['pixel_id']
         intrinsics, extrinsics = input_dict['intrinsics'], input_dict['extrinsics']
-        rays_o, rays_d = geometry.get_rays_tr(pixel_id, intrinsics, extrinsics, mip_nerf_used=False)
+        rays_o, rays_d = geometry.get_rays_tr(pixel_id, intrinsics, extrinsics, mip_nerf_used=False,
+                                              intrinsic_inv=input_dict.get('intrinsics_inv'))
         ret['rays_o'], ret['rays_d'] = rays_o, rays_d
         rays_o_ndc = rays_d_ndc = None
         if self.ndc:

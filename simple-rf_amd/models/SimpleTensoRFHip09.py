@@ -132,9 +132,18 @@ class SimpleTensoRFHip(torch.nn.Module):
             self.run_model_modifications(input_batch.get('iter_num'))
         image_id = input_batch['pixel_id'][:, 0]
         intrinsics = self.intrinsics_learner(image_id)
-        extrinsics = self.extrinsics_learner(image_id)
-        all_extrinsics = self.extrinsics_learner(torch.arange(input_batch['num_frames'], device=image_id.device, dtype=image_id.dtype))
+        # the camera of a ray is a function of its frame: evaluate the learners and the intrinsics
+        # inverse once per frame and gather them per ray (the reference evaluates them per ray)
+        cams = torch.arange(self.extrinsics_learner.num_frames, device=image_id.device, dtype=image_id.dtype)
+        cam_extrinsics = self.extrinsics_learner(cams)
+        extrinsics = cam_extrinsics[image_id]
+        n_frames = input_batch['num_frames']
+        all_extrinsics = cam_extrinsics if n_frames == cams.shape[0] else \
+            self.extrinsics_learner(torch.arange(n_frames, device=image_id.device, dtype=image_id.dtype))
         input_batch['intrinsics'] = intrinsics
+        k_cams = torch.arange(self.intrinsics_learner.initial_intrinsics.shape[0], device=image_id.device,
+                              dtype=image_id.dtype)
+        input_batch['intrinsics_inv'] = torch.linalg.inv_ex(self.intrinsics_learner(k_cams))[0][image_id]
         input_batch['extrinsics'] = extrinsics
         if mode == 'camera_params_only':
             out = {}
@@ -174,7 +183,8 @@ class SimpleTensoRFHip(torch.nn.Module):
         intrinsics, extrinsics = input_dict['intrinsics'], input_dict['extrinsics']
         num_rays = pixel_id.shape[0]
         rays_o, rays_d = geometry.get_rays_tr(pixel_id, intrinsics, extrinsics, mip_nerf_used=True,
-                                              nerf_synthetic=self.nerf_synthetic)
+                                              nerf_synthetic=self.nerf_synthetic,
+                                              intrinsic_inv=input_dict.get('intrinsics_inv'))
         rays_o = rays_o.clone()
         rays_d = rays_d.clone()
         rays_o[:, :1] *= -1

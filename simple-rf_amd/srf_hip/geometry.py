@@ -1,8 +1,8 @@
 """Ray generation and NDC warping (host side of the drop-in, PyTorch ops on the GPU).
 
-These produce the inputs of the hot path exactly as the reference does
-(src/utils/CommonUtils04.py:73-149).  They are small per-ray computations kept in
-PyTorch; fusing them into the sampler is the next step after the hot path.
+These produce the inputs of the hot path as the reference does (src/utils/CommonUtils04.py:73-149).
+They are small per-ray computations kept in PyTorch; fusing them into the sampler is the next step
+after the hot path.
 """
 from __future__ import annotations
 
@@ -10,8 +10,9 @@ import torch
 
 
 def get_rays_tr(pixel_id: torch.Tensor, intrinsic: torch.Tensor, extrinsics: torch.Tensor, mip_nerf_used: bool,
-                nerf_synthetic: bool = False):
-    """src/utils/CommonUtils04.py:73-95."""
+                nerf_synthetic: bool = False, intrinsic_inv: torch.Tensor | None = None):
+    """src/utils/CommonUtils04.py:73-95.  intrinsic_inv: the inverse intrinsics per ray when the caller
+    has them (inverted once per frame); otherwise each ray's matrix is inverted here."""
     if nerf_synthetic:
         p = torch.eye(4).to(extrinsics)
         p[1, 1] = -1
@@ -23,9 +24,12 @@ def get_rays_tr(pixel_id: torch.Tensor, intrinsic: torch.Tensor, extrinsics: tor
         x = x + 0.5
         y = y + 0.5
     points_homo = torch.stack([x, y, torch.ones_like(x)], dim=1)
-    # the same LU inverse as torch.linalg.inv, without its host-synchronising singularity check
-    # (keeps the step capturable in a HIP graph)
-    dirs = (torch.linalg.inv_ex(intrinsic)[0] @ points_homo[:, :, None])[:, :, 0]
+    if intrinsic_inv is None:
+        # the same LU inverse as torch.linalg.inv, without its host-synchronising singularity check
+        # (keeps the step capturable in a HIP graph)
+        intrinsic_inv = torch.linalg.inv_ex(intrinsic)[0]
+    # K^-1 [x, y, 1] per ray as one elementwise product and sum (no batched 3 x 3 GEMM)
+    dirs = torch.sum(intrinsic_inv * points_homo[:, None, :], dim=-1)
     dirs[:, 1:] *= -1
     if nerf_synthetic:
         dirs[:, 1:] *= -1
