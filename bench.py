@@ -5,7 +5,7 @@ Workload (one "step"): the Simple-NeRF training step of the drop-in model Simple
 4096 synthetic rays -- coarse 64 stratified samples through the main, points-augmented and
 views-augmented MLPs, 128 inverse-CDF fine samples merged to 192 through the fine MLP, alpha
 compositing of all four, the synthetic loss (photometric + sparse depth + depth consistency),
-the hand-written backward of every stage, and an Adam update.  MLP operands in BF16 on MFMA
+the hand-written backward of every stage, and an Adam update (torch's fused Adam).  MLP operands in BF16 on MFMA
 with FP32 accumulation; sampling, transmittance and compositing in FP32.  Inputs (pixel ids,
 targets, weights) are resident in HBM before the timed region.
 
@@ -103,7 +103,7 @@ def hip_train_leg(args, dev):
     _, model = build_model('bf16', dev)
     model.train()
     params = [p for p in model.parameters() if p.requires_grad]
-    opt = torch.optim.Adam(params, lr=5e-4, betas=(0.9, 0.999))
+    opt = torch.optim.Adam(params, lr=5e-4, betas=(0.9, 0.999), fused=True)
     batch = synthetic.train_batch(args.rays, seed=0, device=dev)
     inp = {'pixel_id': batch['pixel_id'], 'num_frames': 3, 'iter_num': 0, 'sub_batch_index': 0}
 
@@ -142,7 +142,7 @@ def hip_graph_train_leg(args, dev):
     _, model = build_model('bf16', dev)
     model.train()
     params = [p for p in model.parameters() if p.requires_grad]
-    opt = torch.optim.Adam(params, lr=5e-4, betas=(0.9, 0.999), capturable=True)
+    opt = torch.optim.Adam(params, lr=5e-4, betas=(0.9, 0.999), capturable=True, fused=True)
     batch = synthetic.train_batch(args.rays, seed=0, device=dev)
     inp = {'pixel_id': batch['pixel_id'], 'num_frames': 3, 'iter_num': 0, 'sub_batch_index': 0}
 
@@ -329,7 +329,7 @@ def tensorf_train_leg(args, dev):
     _, model = tensorf_model(dev)
     model.train()
     groups = model.get_trainable_parameters({'lr_initial_tensor': 2e-2, 'lr_initial_network': 1e-3})
-    opt = torch.optim.Adam(groups, betas=(0.9, 0.99))
+    opt = torch.optim.Adam(groups, betas=(0.9, 0.99), fused=True)
     batch = synthetic.train_batch(args.tensorf_rays, seed=5, device=dev)
     inp = {'pixel_id': batch['pixel_id'], 'num_frames': 3, 'iter_num': 1, 'sub_batch_index': 0}
 
@@ -398,7 +398,7 @@ def tensorf_ragged_leg(args, dev):
            'rays_o': o.to(dev), 'rays_d': d.to(dev), 'view_dirs': d.to(dev), 'max_samples': int(counts.max())}
     target = torch.rand(n, 3, generator=g).to(dev)
     groups = tm.get_trainable_parameters({'lr_initial_tensor': 2e-2, 'lr_initial_network': 1e-3})
-    opt = torch.optim.Adam(groups, betas=(0.9, 0.99))
+    opt = torch.optim.Adam(groups, betas=(0.9, 0.99), fused=True)
 
     def train_step():
         out = tm.forward_ragged(inp)
