@@ -10,7 +10,9 @@
 // Layout and mapping follow the NeRF MLP kernels (mfma.h): the network runs transposed,
 // H_out^T = W H_in^T with the weights as the MFMA A operand (pre-packed into per-lane fragments
 // by srf_color_pack) and 16-point column blocks as N; a wave owns 32 points (2 column blocks), so
-// one layer's D tiles are the next layer's B operand without data movement.  The basis output F
+// one layer's D tiles are the next layer's B operand without data movement.  The A fragments are
+// streamed global -> LDS by the four waves of a block (double-buffered, as in mlp::mma_tiles) and
+// shared by them.  The basis output F
 // and the view directions go through a per-wave LDS tile from which every lane builds its
 // positional-encoding inputs.  Training stores the layer inputs feature-major (row = feature,
 // column = point) for the weight-gradient GEMM, which is the grouped dW kernel of the MLP.
@@ -32,7 +34,9 @@ constexpr int kPW = 16 * kCB; // points per wave
 constexpr int kWaves = 4;
 constexpr int kMaxC = 256;    // appearance components
 constexpr int kMaxB = 32;     // basis outputs (features_dimension_color)
-constexpr int kMaxD1 = 256;   // first-layer inputs
+constexpr int kMaxD1 = 240;   // first-layer inputs (the backward's per-wave LDS tile fits beside the stage)
+constexpr int kST = 8;        // output tiles per weight-stage buffer (the widest layer has kHT = 8)
+constexpr int kStage = 2 * kST * mlp::kKC * 1024;  // double-buffered weight stage (32 KB)
 
 struct Geo {
   int C, Cp;     // components, padded to 16
@@ -147,24 +151,61 @@ __global__ __launch_bounds__(256) void pack_kernel(PackArgs a) {
 }
 
 // ------------------------------------------------------------------------ kernels
-__device__ __forceinline__ float ldf(const float* p) { return *p; }
-
-// acc[T][cb] += A(T, k-step) x in(k-step) over the k-steps 4 t + r of input tiles t < TI.
-template <int TO, int TI>
-__device__ __forceinline__ void layer(f32x4 (&acc)[TO][kCB], const float* __restrict__ frag, int ksteps,
-                                      const f32x4 (&in)[TI][kCB], int lane) {
-#pragma unroll
-  for (int t = 0; t < TI; ++t)
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int s = 4 * t + r;
-#pragma unroll
-      for (int T = 0; T < TO; ++T) {
-        const float a = ldf(frag + ((size_t)T * ksteps + s) * 64 + lane);
-#pragma unroll
-        for (int cb = 0; cb < kCB; ++cb) acc[T][cb] = mfma32(a, in[t][cb][r], acc[T][cb]);
-      }
+// acc[T][cb] += sum over k-units u < nu of A(tile T, u) x fin(u, cb) for T < OUT: mlp::mma_tiles
+// for a reduction depth nu known only at run time.  The A fragments (1 KB per tile and k-unit, upr
+// k-units per packed tile row) are staged through LDS in chunks of kKC k-units, double-buffered;
+// fin may load or compute its operand (it is called once per k-unit and column block, so it must
+// not index register arrays).  Block-collective: every wave calls it with the same arguments.
+template <int OUT, typename FIn>
+__device__ __forceinline__ void staged_layer_rt(f32x4 (&acc)[OUT][kCB], const float* __restrict__ A, int upr,
+                                                int nu, FIn&& fin, uint8_t* stage) {
+  constexpr int kKC = mlp::kKC;
+  constexpr int kHalf = kST * kKC * 1024;
+  static_assert(OUT <= kST, "the stage buffer holds kST output tiles");
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  const uint8_t* Ab = reinterpret_cast<const uint8_t*>(A);
+  const int nch = (nu + kKC - 1) / kKC;
+  auto dma = [&](int c, uint8_t* buf) {
+    const int kn = nu - c * kKC < kKC ? nu - c * kKC : kKC;
+    for (int f = w; f < OUT * kn; f += kWaves) {
+      const int to = f / kn, kc = f - (f / kn) * kn;
+      const int frag = __builtin_amdgcn_readfirstlane(to * upr + c * kKC + kc);
+      const int dst = __builtin_amdgcn_readfirstlane((to * kKC + kc) * 1024);
+      lds_dma_1k(Ab + (size_t)frag * 1024, buf + dst);
     }
+  };
+  dma(0, stage);
+  wait_vmem();
+  __syncthreads();
+  for (int c = 0; c < nch; ++c) {
+    const uint8_t* cur = stage + (c & 1) * kHalf;
+    if (c + 1 < nch) dma(c + 1, stage + ((c + 1) & 1) * kHalf);
+    const int kn = nu - c * kKC < kKC ? nu - c * kKC : kKC;
+    for (int kc = 0; kc < kn; ++kc) {
+      f32x4 b[kCB];
+#pragma unroll
+      for (int cb = 0; cb < kCB; ++cb) b[cb] = fin(c * kKC + kc, cb);
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int T = 0; T < OUT; ++T) {
+          const float a = *reinterpret_cast<const float*>(cur + (T * kKC + kc) * 1024 + (r * 64 + lane) * 4);
+#pragma unroll
+          for (int cb = 0; cb < kCB; ++cb) acc[T][cb] = mfma32(a, b[cb][r], acc[T][cb]);
+        }
+    }
+    wait_vmem();
+    __syncthreads();
+  }
+}
+
+// acc += A(tiles of the packed block at A) x in with a compile-time depth of NU k-units, through
+// mlp::mma_tiles on the kST-tile stage.
+template <int OUT, int NU, typename FIn>
+__device__ __forceinline__ void staged_layer(f32x4 (&acc)[OUT][kCB], const float* __restrict__ A, int upr,
+                                             FIn&& fin, uint8_t* stage) {
+  mlp::mma_tiles<kF32, NU, OUT, 0, kST>(acc, reinterpret_cast<const uint8_t*>(A), upr, 0, fin, stage);
 }
 
 // Positional-encoding input i of the first layer for one point, from its basis outputs F (LDS
@@ -204,12 +245,12 @@ struct FwdArgs {
 };
 
 __device__ __forceinline__ void store_fm(uint8_t* base, size_t row_off, int ld, int row, int p, float v) {
-  reinterpret_cast<float*>(base + row_off)[(size_t)row * ld + p] = v;
+  // streamed past L2 (non-temporal): the rows are read back only by the backward
+  __builtin_nontemporal_store(v, reinterpret_cast<float*>(base + row_off) + (size_t)row * ld + p);
 }
 
-template <int CT>
 __global__ __launch_bounds__(64 * kWaves) void color_fwd_kernel(FwdArgs a) {
-  constexpr int D1T_MAX = kMaxD1 / 16;
+  __shared__ __attribute__((aligned(16))) uint8_t stage[kStage];
   __shared__ float sF[kWaves][kMaxB][kPW + 1];
   __shared__ float sD[kWaves][3][kPW];
   const Geo& g = a.g;
@@ -223,33 +264,20 @@ __global__ __launch_bounds__(64 * kWaves) void color_fwd_kernel(FwdArgs a) {
   for (int T = 0; T < 2; ++T)
 #pragma unroll
     for (int cb = 0; cb < kCB; ++cb) accF[T][cb] = f32x4{0.f, 0.f, 0.f, 0.f};
-  for (int t = 0; t < g.Cp / 16; ++t) {
-    f32x4 in[1][kCB];
-#pragma unroll
-    for (int cb = 0; cb < kCB; ++cb) {
-      const int p = p0 + 16 * cb + j;
-      const int c = 16 * t + 4 * gq;
-      float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-      if (p < a.n && c < g.C) v = *reinterpret_cast<const float4*>(a.app + (size_t)p * g.C + c);
-      in[0][cb] = f32x4{v.x, v.y, v.z, v.w};
-      if (train && c < g.C) {
-        store_fm(a.ws, a.w.appT, a.w.ld, c + 0, p, v.x);
-        store_fm(a.ws, a.w.appT, a.w.ld, c + 1, p, v.y);
-        store_fm(a.ws, a.w.appT, a.w.ld, c + 2, p, v.z);
-        store_fm(a.ws, a.w.appT, a.w.ld, c + 3, p, v.w);
-      }
+  auto fin_app = [&](int u, int cb) -> f32x4 {
+    const int p = p0 + 16 * cb + j;
+    const int c = 16 * u + 4 * gq;
+    float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (p < a.n && c < g.C) v = *reinterpret_cast<const float4*>(a.app + (size_t)p * g.C + c);
+    if (train && c < g.C) {
+      store_fm(a.ws, a.w.appT, a.w.ld, c + 0, p, v.x);
+      store_fm(a.ws, a.w.appT, a.w.ld, c + 1, p, v.y);
+      store_fm(a.ws, a.w.appT, a.w.ld, c + 2, p, v.z);
+      store_fm(a.ws, a.w.appT, a.w.ld, c + 3, p, v.w);
     }
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int s = 4 * t + r;
-#pragma unroll
-      for (int T = 0; T < 2; ++T) {
-        const float av = ldf(a.pk + g.fB + ((size_t)T * (g.Cp / 4) + s) * 64 + lane);
-#pragma unroll
-        for (int cb = 0; cb < kCB; ++cb) accF[T][cb] = mfma32(av, in[0][cb][r], accF[T][cb]);
-      }
-    }
-  }
+    return f32x4{v.x, v.y, v.z, v.w};
+  };
+  staged_layer_rt<2>(accF, a.pk + g.fB, g.Cp / 16, g.Cp / 16, fin_app, stage);
   // F and the view directions to LDS (and F feature-major for the backward)
 #pragma unroll
   for (int T = 0; T < 2; ++T)
@@ -284,28 +312,19 @@ __global__ __launch_bounds__(64 * kWaves) void color_fwd_kernel(FwdArgs a) {
     for (int cb = 0; cb < kCB; ++cb)
 #pragma unroll
       for (int r = 0; r < 4; ++r) h1[T][cb][r] = a.pk[g.b1 + 16 * T + 4 * gq + r];
-  const int d1t = g.D1p / 16;
-  for (int t = 0; t < d1t && t < D1T_MAX; ++t) {
+  auto fin_pe = [&](int u, int cb) -> f32x4 {
+    const int col = 16 * cb + j;
+    const float dir[3] = {sD[wv][0][col], sD[wv][1][col], sD[wv][2][col]};
+    f32x4 b;
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      const int i = 16 * t + 4 * gq + r;
-      float b[kCB];
-#pragma unroll
-      for (int cb = 0; cb < kCB; ++cb) {
-        const int col = 16 * cb + j;
-        const float dir[3] = {sD[wv][0][col], sD[wv][1][col], sD[wv][2][col]};
-        b[cb] = pe_input(g, i, &sF[wv][0][col], kPW + 1, dir);
-        if (train && i < g.D1) store_fm(a.ws, a.w.x1, a.w.ld, i, p0 + col, b[cb]);
-      }
-      const int s = 4 * t + r;
-#pragma unroll
-      for (int T = 0; T < kHT; ++T) {
-        const float av = ldf(a.pk + g.fW1 + ((size_t)T * (g.D1p / 4) + s) * 64 + lane);
-#pragma unroll
-        for (int cb = 0; cb < kCB; ++cb) h1[T][cb] = mfma32(av, b[cb], h1[T][cb]);
-      }
+      const int i = 16 * u + 4 * gq + r;
+      b[r] = pe_input(g, i, &sF[wv][0][col], kPW + 1, dir);
+      if (train && i < g.D1) store_fm(a.ws, a.w.x1, a.w.ld, i, p0 + col, b[r]);
     }
-  }
+    return b;
+  };
+  staged_layer_rt<kHT>(h1, a.pk + g.fW1, g.D1p / 16, g.D1p / 16, fin_pe, stage);
 #pragma unroll
   for (int T = 0; T < kHT; ++T)
 #pragma unroll
@@ -324,7 +343,7 @@ __global__ __launch_bounds__(64 * kWaves) void color_fwd_kernel(FwdArgs a) {
     for (int cb = 0; cb < kCB; ++cb)
 #pragma unroll
       for (int r = 0; r < 4; ++r) h2[T][cb][r] = a.pk[g.b2 + 16 * T + 4 * gq + r];
-  layer<kHT, kHT>(h2, a.pk + g.fW2, kHid / 4, h1, lane);
+  staged_layer<kHT, kHT>(h2, a.pk + g.fW2, kHid / 16, [&](int u, int cb) { return h1[u][cb]; }, stage);
 #pragma unroll
   for (int T = 0; T < kHT; ++T)
 #pragma unroll
@@ -341,7 +360,7 @@ __global__ __launch_bounds__(64 * kWaves) void color_fwd_kernel(FwdArgs a) {
   for (int cb = 0; cb < kCB; ++cb)
 #pragma unroll
     for (int r = 0; r < 4; ++r) o[0][cb][r] = a.pk[g.b3 + 4 * gq + r];
-  layer<1, kHT>(o, a.pk + g.fW3, kHid / 4, h2, lane);
+  staged_layer<1, kHT>(o, a.pk + g.fW3, kHid / 16, [&](int u, int cb) { return h2[u][cb]; }, stage);
   if (gq == 0) {
 #pragma unroll
     for (int cb = 0; cb < kCB; ++cb) {
@@ -372,6 +391,7 @@ __device__ __forceinline__ float load_fm(const uint8_t* base, size_t row_off, in
 __host__ __device__ inline int bwd_rows(const Geo& g) { return g.D1p > kMaxB ? g.D1p : kMaxB; }
 
 __global__ __launch_bounds__(64 * kWaves) void color_bwd_kernel(BwdArgs a) {
+  __shared__ __attribute__((aligned(16))) uint8_t stage[kStage];
   extern __shared__ float lds_dyn[];
   const Geo& g = a.g;
   const int lane = threadIdx.x & 63, gq = lane >> 4, j = lane & 15;
@@ -402,7 +422,7 @@ __global__ __launch_bounds__(64 * kWaves) void color_bwd_kernel(BwdArgs a) {
   for (int T = 0; T < kHT; ++T)
 #pragma unroll
     for (int cb = 0; cb < kCB; ++cb) dz2[T][cb] = f32x4{0.f, 0.f, 0.f, 0.f};
-  layer<kHT, 1>(dz2, a.pk + g.fW3T, 4, dz3, lane);
+  staged_layer<kHT, 1>(dz2, a.pk + g.fW3T, 1, [&](int u, int cb) { return dz3[u][cb]; }, stage);
 #pragma unroll
   for (int T = 0; T < kHT; ++T)
 #pragma unroll
@@ -420,7 +440,7 @@ __global__ __launch_bounds__(64 * kWaves) void color_bwd_kernel(BwdArgs a) {
   for (int T = 0; T < kHT; ++T)
 #pragma unroll
     for (int cb = 0; cb < kCB; ++cb) dz1[T][cb] = f32x4{0.f, 0.f, 0.f, 0.f};
-  layer<kHT, kHT>(dz1, a.pk + g.fW2T, kHid / 4, dz2, lane);
+  staged_layer<kHT, kHT>(dz1, a.pk + g.fW2T, kHid / 16, [&](int u, int cb) { return dz2[u][cb]; }, stage);
 #pragma unroll
   for (int T = 0; T < kHT; ++T)
 #pragma unroll
@@ -432,17 +452,26 @@ __global__ __launch_bounds__(64 * kWaves) void color_bwd_kernel(BwdArgs a) {
         dz1[T][cb][r] = v;
         store_fm(a.ws, a.w.dz1, a.w.ld, row, p, v);
       }
-  // 4) dx1 = W1^T dz1, one output tile at a time into LDS
-  for (int T = 0; T < g.D1p / 16; ++T) {
-    f32x4 dx[1][kCB];
+  // 4) dx1 = W1^T dz1 into LDS, kST output tiles per staged pass and single tiles for the rest
+  auto fin_dz1 = [&](int u, int cb) { return dz1[u][cb]; };
+  auto dx_tiles = [&](auto OUTC, int T0) {
+    constexpr int OUT = decltype(OUTC)::value;
+    f32x4 dx[OUT][kCB];
 #pragma unroll
-    for (int cb = 0; cb < kCB; ++cb) dx[0][cb] = f32x4{0.f, 0.f, 0.f, 0.f};
-    layer<1, kHT>(dx, a.pk + g.fW1T + (size_t)T * (kHid / 4) * 64, kHid / 4, dz1, lane);
+    for (int T = 0; T < OUT; ++T)
 #pragma unroll
-    for (int cb = 0; cb < kCB; ++cb)
+      for (int cb = 0; cb < kCB; ++cb) dx[T][cb] = f32x4{0.f, 0.f, 0.f, 0.f};
+    staged_layer<OUT, kHT>(dx, a.pk + g.fW1T + (size_t)T0 * (kHid / 4) * 64, kHid / 16, fin_dz1, stage);
 #pragma unroll
-      for (int r = 0; r < 4; ++r) sX(16 * T + 4 * gq + r, 16 * cb + j) = dx[0][cb][r];
-  }
+    for (int T = 0; T < OUT; ++T)
+#pragma unroll
+      for (int cb = 0; cb < kCB; ++cb)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) sX(16 * (T0 + T) + 4 * gq + r, 16 * cb + j) = dx[T][cb][r];
+  };
+  int T0 = 0;
+  for (; T0 + kST <= g.D1p / 16; T0 += kST) dx_tiles(std::integral_constant<int, kST>{}, T0);
+  for (; T0 < g.D1p / 16; ++T0) dx_tiles(std::integral_constant<int, 1>{}, T0);
   __syncthreads();
   // 5) positional-encoding backward: dF[f] = dx[f] + sum_k 2^k (dx[sin_k f] cos(2^k F) - dx[cos_k f] sin(2^k F))
   constexpr int kIt = kMaxB * kPW / 64;  // (feature, point) pairs per lane
@@ -483,21 +512,31 @@ __global__ __launch_bounds__(64 * kWaves) void color_bwd_kernel(BwdArgs a) {
     for (int cb = 0; cb < kCB; ++cb)
 #pragma unroll
       for (int r = 0; r < 4; ++r) dFt[t][cb][r] = sX(16 * t + 4 * gq + r, 16 * cb + j);
-  for (int T = 0; T < g.Cp / 16; ++T) {
-    f32x4 da[1][kCB];
+  auto fin_dF = [&](int u, int cb) { return dFt[u][cb]; };
+  auto da_tiles = [&](auto OUTC, int T0) {
+    constexpr int OUT = decltype(OUTC)::value;
+    f32x4 da[OUT][kCB];
 #pragma unroll
-    for (int cb = 0; cb < kCB; ++cb) da[0][cb] = f32x4{0.f, 0.f, 0.f, 0.f};
-    layer<1, 2>(da, a.pk + g.fBT + (size_t)T * 8 * 64, 8, dFt, lane);
-    const int c = 16 * T + 4 * gq;
-    if (c < g.C)
+    for (int T = 0; T < OUT; ++T)
 #pragma unroll
-      for (int cb = 0; cb < kCB; ++cb) {
-        const int p = p0 + 16 * cb + j;
-        if (p < a.n)
-          *reinterpret_cast<float4*>(a.d_app + (size_t)p * g.C + c) =
-              make_float4(da[0][cb][0], da[0][cb][1], da[0][cb][2], da[0][cb][3]);
-      }
-  }
+      for (int cb = 0; cb < kCB; ++cb) da[T][cb] = f32x4{0.f, 0.f, 0.f, 0.f};
+    staged_layer<OUT, 2>(da, a.pk + g.fBT + (size_t)T0 * 8 * 64, 2, fin_dF, stage);
+#pragma unroll
+    for (int T = 0; T < OUT; ++T) {
+      const int c = 16 * (T0 + T) + 4 * gq;
+      if (c < g.C)
+#pragma unroll
+        for (int cb = 0; cb < kCB; ++cb) {
+          const int p = p0 + 16 * cb + j;
+          if (p < a.n)
+            *reinterpret_cast<float4*>(a.d_app + (size_t)p * g.C + c) =
+                make_float4(da[T][cb][0], da[T][cb][1], da[T][cb][2], da[T][cb][3]);
+        }
+    }
+  };
+  int Ta = 0;
+  for (; Ta + kST <= g.Cp / 16; Ta += kST) da_tiles(std::integral_constant<int, kST>{}, Ta);
+  for (; Ta < g.Cp / 16; ++Ta) da_tiles(std::integral_constant<int, 1>{}, Ta);
 }
 
 }  // namespace
@@ -578,7 +617,7 @@ extern "C" int srf_color_forward(const srf_color_desc* desc, const void* packed,
   a.rgb = rgb;
   a.ws = reinterpret_cast<uint8_t*>(workspace);
   const int blocks = a.w.cols / (kPW * kWaves);
-  hipLaunchKernelGGL(color_fwd_kernel<0>, dim3(blocks), dim3(64 * kWaves), 0, as_stream(stream), a);
+  hipLaunchKernelGGL(color_fwd_kernel, dim3(blocks), dim3(64 * kWaves), 0, as_stream(stream), a);
   return launch_status();
 }
 
@@ -603,7 +642,7 @@ extern "C" int srf_color_backward(const srf_color_desc* desc, const void* packed
   b.d_app = d_app;
   b.ws = reinterpret_cast<uint8_t*>(workspace);
   const size_t lds = (size_t)kWaves * bwd_rows(g) * (kPW + 1) * sizeof(float);
-  if (lds > 65536 &&
+  if (lds + kStage > 65536 &&
       hipFuncSetAttribute(reinterpret_cast<const void*>(&color_bwd_kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
                           (int)lds) != hipSuccess)
     return SRF_EHIP;

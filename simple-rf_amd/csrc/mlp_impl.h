@@ -297,16 +297,17 @@ __host__ __device__ constexpr int units(int tiles) { return P == kF32 ? tiles : 
 // acc[OUT][CB] += W[TB .. TB+OUT-1, k-units u0 .. u0+NT/TPU-1] x In for all OUT output tiles at
 // once (the accumulators fill the AGPR file while the activations stay in arch VGPRs).  upr is
 // the number of k-units per packed row of the layer; fin(t, cb) returns input tile t (counted
-// from the first tile of k-unit u0) for column block cb.  Must be reached by all waves of the
-// block (it synchronises them).
-template <int P, int NT, int OUT, int TB, typename FIn>
+// from the first tile of k-unit u0) for column block cb.  stage holds 2 x ST x kKC KB.  Must be
+// reached by all waves of the block (it synchronises them).
+template <int P, int NT, int OUT, int TB, int ST = 16, typename FIn>
 __device__ __forceinline__ void mma_tiles(f32x4 (&acc)[OUT][Tr<P>::CB], const uint8_t* __restrict__ A,
                                           int upr, int u0, FIn&& fin, uint8_t* stage) {
   constexpr int CB = Tr<P>::CB;
   constexpr int TPU = P == kF32 ? 1 : 2;
   constexpr int NU = NT / TPU;
   constexpr int NCH = (NU + kKC - 1) / kKC;
-  static_assert(OUT <= 16, "stage buffer holds 16 output tiles");
+  constexpr int kHalf = ST * kKC * 1024;  // one buffer of the double-buffered stage (2 kHalf bytes)
+  static_assert(OUT <= ST, "the stage buffer holds ST output tiles");
   // wave index and every fragment address are wave-uniform: keep them in SGPRs
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
@@ -329,8 +330,8 @@ __device__ __forceinline__ void mma_tiles(f32x4 (&acc)[OUT][Tr<P>::CB], const ui
   __syncthreads();
   static_for<0, NCH>([&](auto CI) SRF_AI {
     constexpr int c = decltype(CI)::value;
-    const uint8_t* cur = stage + (c & 1) * kStageBytes;
-    if constexpr (c + 1 < NCH) dma_chunk(std::integral_constant<int, c + 1>{}, stage + ((c + 1) & 1) * kStageBytes);
+    const uint8_t* cur = stage + (c & 1) * kHalf;
+    if constexpr (c + 1 < NCH) dma_chunk(std::integral_constant<int, c + 1>{}, stage + ((c + 1) & 1) * kHalf);
     constexpr int kn = (NU - c * kKC) < kKC ? (NU - c * kKC) : kKC;
     static_for<0, kn>([&](auto KI) SRF_AI {
       constexpr int kc = decltype(KI)::value;
