@@ -668,7 +668,7 @@ extern "C" int srf_color_backward(const srf_color_desc* desc, const void* packed
     jb.dw = dw;
     jb.ld_dw = ld;
     jb.db = db;
-    jb.mblocks = (M + mlp::kDwTile - 1) / mlp::kDwTile;
+    jb.mblocks = (M + 127) / 128;  // 128-row tiles (dw_f32_narrow): every job has at most 128 rows
     jb.nblocks = (N + mlp::kDwTile - 1) / mlp::kDwTile;
   };
   job(w.dz1, kHid, w.x1, g.D1p, kHid, g.D1, d_w1, g.D1, d_b1);
@@ -687,5 +687,5 @@ extern "C" int srf_color_backward(const srf_color_desc* desc, const void* packed
     work += d.jobs[i].work_count;
   }
   d.total_work = work;
-  return mlp::dw_f32(d, s);
+  return mlp::dw_f32_narrow(d, s);
 }

@@ -864,8 +864,13 @@ constexpr int kDwSlots = 4;               // ring depth (prefetch distance kDwSl
 // operations and leaves expcnt / lgkmcnt unconstrained.
 constexpr int vmcnt_only(int n) { return 0x0F70 | (n & 0xF) | ((n >> 4) << 14); }
 
-template <int P>
+// RM row tiles per wave: RM = 4 covers 256-row tiles (the NeRF layers), RM = 2 covers 128-row
+// tiles, so that a job of at most 128 rows (the TensoRF colour network) keeps all eight waves
+// busy instead of two.  The DMA is the same for both (it loads the 16 dZ row tiles from m0).
+template <int P, int RM = 4>
 __global__ __launch_bounds__(64 * kDwWaves) void mlp_dw_kernel(DwArgs a) {
+  static_assert(RM == 4 || RM == 2, "256- or 128-row tiles");
+  constexpr int kTM = 64 * RM;  // tile rows
   __shared__ __attribute__((aligned(16))) uint8_t ring[kDwSlots * kDwSlot];
   const int lane = threadIdx.x & 63, g = lane >> 4, j = lane & 15;
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -877,21 +882,21 @@ __global__ __launch_bounds__(64 * kDwWaves) void mlp_dw_kernel(DwArgs a) {
   const int local = item - jb.work_begin;
   const int nblk = jb.mblocks * jb.nblocks;
   const int tile = local % nblk, chunk = local / nblk;
-  const int m0 = kDwTile * (tile % jb.mblocks), n0 = kDwTile * (tile / jb.mblocks);
+  const int m0 = kTM * (tile % jb.mblocks), n0 = kDwTile * (tile / jb.mblocks);
   const int pb = chunk * a.chunk;
   const int pe = min(pb + a.chunk, a.cols);
   const size_t ldP = a.ldP;
   constexpr int ELEM = P == kF32 ? 4 : 2;
   constexpr int KPTS = P == kF32 ? 16 : 32;  // points per k-unit
-  const int qm = 4 * (w >> 1), qn = 8 * (w & 1);
-  bool mv[4], nv[8];
+  const int qm = RM * (w >> 1), qn = 8 * (w & 1);
+  bool mv[RM], nv[8];
 #pragma unroll
-  for (int i = 0; i < 4; ++i) mv[i] = m0 + 16 * (qm + i) < jb.dz_rows;
+  for (int i = 0; i < RM; ++i) mv[i] = m0 + 16 * (qm + i) < jb.dz_rows;
 #pragma unroll
   for (int k = 0; k < 8; ++k) nv[k] = n0 + 16 * (qn + k) < jb.x_rows;
   bool any_m = false, any_n = false;
 #pragma unroll
-  for (int i = 0; i < 4; ++i) any_m |= mv[i];
+  for (int i = 0; i < RM; ++i) any_m |= mv[i];
 #pragma unroll
   for (int k = 0; k < 8; ++k) any_n |= nv[k];
   // DMA of the 32 fragments of the k-unit at point p into a ring slot: fragment f < 16 is dZ row
@@ -923,10 +928,10 @@ __global__ __launch_bounds__(64 * kDwWaves) void mlp_dw_kernel(DwArgs a) {
                                        (__attribute__((address_space(3))) void*)(buf + f * 1024), 16, 0, 0);
     }
   };
-  f32x4 acc[4][8];
-  float rs[4];
+  f32x4 acc[RM][8];
+  float rs[RM];
 #pragma unroll
-  for (int i = 0; i < 4; ++i) {
+  for (int i = 0; i < RM; ++i) {
     rs[i] = 0.f;
 #pragma unroll
     for (int k = 0; k < 8; ++k) acc[i][k] = f32x4{0.f, 0.f, 0.f, 0.f};
@@ -958,42 +963,42 @@ __global__ __launch_bounds__(64 * kDwWaves) void mlp_dw_kernel(DwArgs a) {
     if (busy) {
       const uint8_t* fr = ring + (u % kDwSlots) * kDwSlot;
       if constexpr (P == kF32) {
-        f32x4 av[4], bv[8];
+        f32x4 av[RM], bv[8];
 #pragma unroll
-        for (int i = 0; i < 4; ++i) av[i] = *reinterpret_cast<const f32x4*>(fr + (qm + i) * 1024 + lane * 16);
+        for (int i = 0; i < RM; ++i) av[i] = *reinterpret_cast<const f32x4*>(fr + (qm + i) * 1024 + lane * 16);
 #pragma unroll
         for (int k = 0; k < 8; ++k) bv[k] = *reinterpret_cast<const f32x4*>(fr + (16 + qn + k) * 1024 + lane * 16);
 #pragma unroll
         for (int q = 0; q < 4; ++q)
 #pragma unroll
-          for (int i = 0; i < 4; ++i)
+          for (int i = 0; i < RM; ++i)
 #pragma unroll
             for (int k = 0; k < 8; ++k)
               if (mv[i] && nv[k]) acc[i][k] = mfma32(av[i][q], bv[k][q], acc[i][k]);
         if (do_db)
 #pragma unroll
-          for (int i = 0; i < 4; ++i) rs[i] += (av[i][0] + av[i][1]) + (av[i][2] + av[i][3]);
+          for (int i = 0; i < RM; ++i) rs[i] += (av[i][0] + av[i][1]) + (av[i][2] + av[i][3]);
       } else {
-        u32x4 av[4], bv[8];
+        u32x4 av[RM], bv[8];
 #pragma unroll
-        for (int i = 0; i < 4; ++i) av[i] = *reinterpret_cast<const u32x4*>(fr + (qm + i) * 1024 + lane * 16);
+        for (int i = 0; i < RM; ++i) av[i] = *reinterpret_cast<const u32x4*>(fr + (qm + i) * 1024 + lane * 16);
 #pragma unroll
         for (int k = 0; k < 8; ++k) bv[k] = *reinterpret_cast<const u32x4*>(fr + (16 + qn + k) * 1024 + lane * 16);
 #pragma unroll
-        for (int i = 0; i < 4; ++i)
+        for (int i = 0; i < RM; ++i)
 #pragma unroll
           for (int k = 0; k < 8; ++k)
             if (mv[i] && nv[k]) acc[i][k] = mfma16<P>(av[i], bv[k], acc[i][k]);
         if (do_db)
 #pragma unroll
-          for (int i = 0; i < 4; ++i)
+          for (int i = 0; i < RM; ++i)
 #pragma unroll
             for (int q = 0; q < 4; ++q) rs[i] += unpack_lo<P>(av[i][q]) + unpack_hi<P>(av[i][q]);
       }
     }
   }
 #pragma unroll
-  for (int i = 0; i < 4; ++i) {
+  for (int i = 0; i < RM; ++i) {
     if (!mv[i]) continue;
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
@@ -1034,9 +1039,9 @@ int launch_bwd(const BwdArgs& a, hipStream_t s) {
   return launch_status();
 }
 
-template <int P>
+template <int P, int RM = 4>
 int launch_dw(const DwArgs& d, hipStream_t s) {
-  hipLaunchKernelGGL(mlp_dw_kernel<P>, dim3(d.total_work), dim3(64 * kDwWaves), 0, s, d);
+  hipLaunchKernelGGL((mlp_dw_kernel<P, RM>), dim3(d.total_work), dim3(64 * kDwWaves), 0, s, d);
   return launch_status();
 }
 
@@ -1052,6 +1057,7 @@ int bwd_f32(const BwdArgs& a, bool vd, hipStream_t s);
 int bwd_bf16(const BwdArgs& a, bool vd, hipStream_t s);
 int dw_f32(const DwArgs& d, hipStream_t s);
 int dw_bf16(const DwArgs& d, hipStream_t s);
+int dw_f32_narrow(const DwArgs& d, hipStream_t s);  // 128-row tiles (jobs of at most 128 rows)
 
 }  // namespace mlp
 }  // namespace srf
