@@ -334,31 +334,69 @@ __device__ __forceinline__ void run_flush_line(const VmArgs& a, int p, int c, Ru
   }
 }
 
+// Per-corner run accumulation.  Slot k keeps corner role k (plane: nw, ne, sw, se; line: lower,
+// upper), so a sample whose cell is the previous one moved by one texel along an axis carries
+// the shared half of the slots over (the texel row or column both cells touch) and flushes only
+// the half that left: along a ray, a texel is flushed once instead of once per cell that uses it.
+// A slot is carried only when its texel index equals the new corner's, so any move is handled
+// correctly (unmatched slots are flushed); index -1 marks a corner outside the factor, whose
+// slot is never flushed.  Static selects only: the slots stay in registers.
+__device__ __forceinline__ void flush_slot(float* grad, int idx, float4 acc, int C, int c) {
+  if (idx >= 0) atomic_add4v(grad + (int64_t)idx * C + c, acc);
+}
+
+__device__ __forceinline__ float4 pick4(bool b, float4 x, float4 y) {
+  return make_float4(b ? x.x : y.x, b ? x.y : y.y, b ? x.z : y.z, b ? x.w : y.w);
+}
+
+__device__ __forceinline__ void plane_accum(int (&si)[4], float4 (&sa)[4], const int (&ci)[4], const float (&cw)[4],
+                                            float4 g, float* grad, int C, int c) {
+  const bool same = ci[0] == si[0] && ci[1] == si[1] && ci[2] == si[2] && ci[3] == si[3];
+  const bool px = !same && ci[0] == si[1] && ci[2] == si[3];         // cell moved +x: nw<-ne, sw<-se
+  const bool nx = !same && !px && ci[1] == si[0] && ci[3] == si[2];  // -x: ne<-nw, se<-sw
+  const bool py = !same && !px && !nx && ci[0] == si[2] && ci[1] == si[3];          // +y: nw<-sw, ne<-se
+  const bool ny = !same && !px && !nx && !py && ci[2] == si[0] && ci[3] == si[1];  // -y: sw<-nw, se<-ne
+  if (!(same || nx || ny)) flush_slot(grad, si[0], sa[0], C, c);
+  if (!(same || px || ny)) flush_slot(grad, si[1], sa[1], C, c);
+  if (!(same || nx || py)) flush_slot(grad, si[2], sa[2], C, c);
+  if (!(same || px || py)) flush_slot(grad, si[3], sa[3], C, c);
+  const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
+  const float4 n0 = pick4(same, sa[0], pick4(px, sa[1], pick4(py, sa[2], z)));
+  const float4 n1 = pick4(same, sa[1], pick4(nx, sa[0], pick4(py, sa[3], z)));
+  const float4 n2 = pick4(same, sa[2], pick4(px, sa[3], pick4(ny, sa[0], z)));
+  const float4 n3 = pick4(same, sa[3], pick4(nx, sa[2], pick4(ny, sa[1], z)));
+  sa[0] = madd4(n0, g, cw[0]);
+  sa[1] = madd4(n1, g, cw[1]);
+  sa[2] = madd4(n2, g, cw[2]);
+  sa[3] = madd4(n3, g, cw[3]);
+#pragma unroll
+  for (int k = 0; k < 4; ++k) si[k] = ci[k];
+}
+
+__device__ __forceinline__ void line_accum(int (&si)[2], float4 (&sa)[2], const int (&ci)[2], const float (&cw)[2],
+                                           float4 g, float* grad, int C, int c) {
+  const bool same = ci[0] == si[0] && ci[1] == si[1];
+  const bool up = !same && ci[0] == si[1];          // lower <- upper
+  const bool down = !same && !up && ci[1] == si[0];  // upper <- lower
+  if (!(same || down)) flush_slot(grad, si[0], sa[0], C, c);
+  if (!(same || up)) flush_slot(grad, si[1], sa[1], C, c);
+  const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
+  const float4 n0 = pick4(same, sa[0], pick4(up, sa[1], z));
+  const float4 n1 = pick4(same, sa[1], pick4(down, sa[0], z));
+  sa[0] = madd4(n0, g, cw[0]);
+  sa[1] = madd4(n1, g, cw[1]);
+  si[0] = ci[0];
+  si[1] = ci[1];
+}
+
 // Adds one sample's plane gradient gp and line gradient gl (before the corner weights).
 __device__ __forceinline__ void run_add(const VmArgs& a, const QuadSite& s, RunAcc& r, float4 gp, float4 gl) {
-  if (s.pt.idx[0] != r.pidx[0] || s.pt.idx[1] != r.pidx[1] || s.pt.idx[2] != r.pidx[2] ||
-      s.pt.idx[3] != r.pidx[3]) {
-    run_flush_plane(a, s.p, s.c, r);
-#pragma unroll
-    for (int t = 0; t < 4; ++t) r.pidx[t] = s.pt.idx[t];
-  }
-  if (s.lt.idx[0] != r.lidx[0] || s.lt.idx[1] != r.lidx[1]) {
-    run_flush_line(a, s.p, s.c, r);
-    r.lidx[0] = s.lt.idx[0];
-    r.lidx[1] = s.lt.idx[1];
-  }
-#pragma unroll
-  for (int t = 0; t < 4; ++t) {
-    const float w = s.pt.w[t];
-    r.pg[t] = make_float4(__fadd_rn(r.pg[t].x, __fmul_rn(w, gp.x)), __fadd_rn(r.pg[t].y, __fmul_rn(w, gp.y)),
-                          __fadd_rn(r.pg[t].z, __fmul_rn(w, gp.z)), __fadd_rn(r.pg[t].w, __fmul_rn(w, gp.w)));
-  }
-#pragma unroll
-  for (int t = 0; t < 2; ++t) {
-    const float w = s.lt.w[t];
-    r.lg[t] = make_float4(__fadd_rn(r.lg[t].x, __fmul_rn(w, gl.x)), __fadd_rn(r.lg[t].y, __fmul_rn(w, gl.y)),
-                          __fadd_rn(r.lg[t].z, __fmul_rn(w, gl.z)), __fadd_rn(r.lg[t].w, __fmul_rn(w, gl.w)));
-  }
+  const int C = sel3(s.p, a.comp[0], a.comp[1], a.comp[2]);
+  float* dP = sel3(s.p, a.dplane[0], a.dplane[1], a.dplane[2]);
+  float* dL = sel3(s.p, a.dline[0], a.dline[1], a.dline[2]);
+  const int c = s.c;
+  plane_accum(r.pidx, r.pg, s.pt.idx, s.pt.w, gp, dP, C, c);
+  line_accum(r.lidx, r.lg, s.lt.idx, s.lt.w, gl, dL, C, c);
 }
 
 // d_feat = d_sigma * predictor'(feat) on valid samples; planes receive d_feat * line and
@@ -518,15 +556,8 @@ __device__ __forceinline__ void cp_run_add(const CpArgs& a, const CpTaps& s, int
                                            const float4 v[kPlanes]) {
   const float4 d[kPlanes] = {mul4(mul4(g, v[1]), v[2]), mul4(mul4(g, v[0]), v[2]), mul4(mul4(g, v[0]), v[1])};
 #pragma unroll
-  for (int k = 0; k < kPlanes; ++k) {
-    if (s.t[k].idx[0] != r.idx[k][0] || s.t[k].idx[1] != r.idx[k][1]) {
-      cp_run_flush(a, k, c, r);
-      r.idx[k][0] = s.t[k].idx[0];
-      r.idx[k][1] = s.t[k].idx[1];
-    }
-#pragma unroll
-    for (int t = 0; t < 2; ++t) r.g[k][t] = madd4(r.g[k][t], d[k], s.t[k].w[t]);
-  }
+  for (int k = 0; k < kPlanes; ++k)
+    line_accum(r.idx[k], r.g[k], s.t[k].idx, s.t[k].w, d[k], a.dline[k], a.comp, c);
 }
 
 template <int G>
