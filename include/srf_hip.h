@@ -340,7 +340,7 @@ int srf_aabb_samples(int n_rays, int n_samples, const float* rays_o, const float
  * (Linear(n_features -> basis_dim), no bias; src/models/SimpleTensoRF09.py:1151) and
  * MlpFeaturesColorPredictor (:1370-1421): [PE_Lf(features), PE_Lv(view dir)] -> hidden ReLU ->
  * hidden ReLU -> 3 sigmoid.  n_features % 4 == 0 and <= 256, basis_dim <= 32, hidden == 128,
- * first-layer width basis_dim (2 features_pe + 1) + 3 (2 views_pe + 1) <= 256. */
+ * first-layer width basis_dim (2 features_pe + 1) + 3 (2 views_pe + 1) <= 240. */
 typedef struct srf_color_desc {
   int n_features;    /* sum of the appearance components */
   int basis_dim;     /* features_dimension_color (27) */
