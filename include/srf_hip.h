@@ -164,7 +164,9 @@ int srf_mlp_pack(const srf_mlp_desc* desc, const float* const* weights, const fl
                  void* packed, void* stream);
 
 /* Bytes of the training workspace for n_points (saved activations, ReLU bit masks and
- * activation gradients, all feature-major). */
+ * activation gradients, all feature-major).  Training launches (forward with a workspace,
+ * backward) take up to about 3.9 M points (FP32) or 7.8 M points (16-bit operands) per call, so
+ * that each workspace array stays below 4 GB; larger calls return SRF_EINVAL. */
 size_t srf_mlp_workspace_bytes(const srf_mlp_desc* desc, int n_points);
 
 /* Fused PE + MLP forward of n_points points.

@@ -237,6 +237,7 @@ extern "C" int srf_mlp_forward(const srf_mlp_desc* desc, const void* packed, int
   a.sigma = sigma;
   a.rgb = rgb;
   a.ws = reinterpret_cast<uint8_t*>(workspace);
+  if (workspace && !ws_addressable(p, a.w)) return SRF_EINVAL;
   return fwd_launch(p.prec, a, p.view_dep, workspace != nullptr, as_stream(stream));
 }
 
@@ -258,6 +259,7 @@ static int mlp_backward_impl(const srf_mlp_desc* desc, const void* packed, int n
   }
   hipStream_t s = as_stream(stream);
   const Ws w = make_ws(p, n_points);
+  if (!ws_addressable(p, w)) return SRF_EINVAL;
   uint8_t* ws = reinterpret_cast<uint8_t*>(workspace);
   if (chain) {
     BwdArgs b;

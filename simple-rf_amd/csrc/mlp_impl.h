@@ -165,6 +165,15 @@ inline Ws make_ws(const Plan& p, int n) {
   return w;
 }
 
+// The kernels address every workspace array (one per layer input, dZ and mask slot) with 32-bit
+// byte offsets from the array's base, so each must stay below 4 GB: a training launch of up to
+// about 3.9 M points (FP32) or 7.8 M points (16-bit operands).
+inline bool ws_addressable(const Plan& p, const Ws& w) {
+  int rows = 256;
+  for (int l = 0; l < kLayers; ++l) rows = w.dz_rows[l] > rows ? w.dz_rows[l] : rows;
+  return (uint64_t)rows * (uint64_t)w.ldP * (uint64_t)p.elem < (1ull << 32);
+}
+
 // --------------------------------------------------------------------- type traits
 template <int P>
 struct Tr;
@@ -209,6 +218,40 @@ __device__ __forceinline__ uint32_t act_elem_bits(const typename Tr<P>::Act& a, 
   if constexpr (P == kF32) return __float_as_uint(a[r]);
   else return (r < 2 ? a.x : a.y) >> (16 * (r & 1)) & 0xffffu;
 }
+
+// A wave's 16 activation tiles x CB column blocks in registers, one Act per tile (ActTiles) or,
+// for 16-bit operands, tiles 2u and 2u + 1 of a column block in one 128-bit register quad
+// (ActQuads), which is exactly the MFMA B operand of k-unit u, so that no register copies
+// assemble it.  Measured on gfx950: the quads make the training forward 8 % faster and the
+// inference forward 20 % slower (register allocation differs), so each uses its own.  Indices
+// are compile-time constants after unrolling (the arrays stay in VGPRs).
+template <int P>
+struct ActTiles {
+  using Act = typename Tr<P>::Act;
+  Act q[16][Tr<P>::CB];
+  __device__ SRF_AI Act get(int t, int cb) const { return q[t][cb]; }
+  __device__ SRF_AI void set(int t, int cb, Act a) { q[t][cb] = a; }
+};
+template <int P>
+struct ActQuads {
+  using Act = typename Tr<P>::Act;
+  u32x4 q[8][Tr<P>::CB];
+  __device__ SRF_AI Act get(int t, int cb) const {
+    const u32x4 v = q[t >> 1][cb];
+    return (t & 1) ? u32x2{v.z, v.w} : u32x2{v.x, v.y};
+  }
+  __device__ SRF_AI void set(int t, int cb, Act a) {
+    if (t & 1) {
+      q[t >> 1][cb].z = a.x;
+      q[t >> 1][cb].w = a.y;
+    } else {
+      q[t >> 1][cb].x = a.x;
+      q[t >> 1][cb].y = a.y;
+    }
+  }
+};
+template <>
+struct ActQuads<kF32> : ActTiles<kF32> {};
 
 // sin and cos of x for |x| up to ~1e4 (positional-encoding arguments are |2^m x| < 1e3):
 // Cody-Waite reduction x = k pi/2 + r with pi/2 split into three floats applied by fma, then
@@ -294,14 +337,32 @@ constexpr int kStageBytes = 16 * kKC * 1024;
 template <int P>
 __host__ __device__ constexpr int units(int tiles) { return P == kF32 ? tiles : tiles / 2; }
 
+// s_waitcnt immediate (gfx9 encoding) that waits for at most n outstanding vector-memory
+// operations and leaves expcnt / lgkmcnt unconstrained.
+constexpr int vmcnt_only(int n) { return 0x0F70 | (n & 0xF) | ((n >> 4) << 14); }
+
+// No deferred stores (see mma_tiles).
+struct NoSide {
+  template <typename I>
+  SRF_AI void operator()(I) const {}
+};
+
 // acc[OUT][CB] += W[TB .. TB+OUT-1, k-units u0 .. u0+NT/TPU-1] x In for all OUT output tiles at
 // once (the accumulators fill the AGPR file while the activations stay in arch VGPRs).  upr is
 // the number of k-units per packed row of the layer; fin(t, cb) returns input tile t (counted
 // from the first tile of k-unit u0) for column block cb.  stage holds 2 x ST x kKC KB.  Must be
 // reached by all waves of the block (it synchronises them).
-template <int P, int NT, int OUT, int TB, int ST = 16, typename FIn>
+//
+// Deferred stores: side(i) for i in [0, NSIDE) (compile-time indices) issues at least two
+// vector-memory stores each (the previous layer's activation tiles and ReLU masks).  The items
+// are spread over the weight chunks, issued right after the DMA of the next chunk, and each
+// chunk then waits with vmcnt(stores issued after that DMA): vector-memory operations complete
+// in issue order, so the DMA has landed while the stores keep draining behind the MFMAs.  Without
+// this, a layer's stores would all be waited for at the next chunk boundary, when every wave of
+// the chip issues them at once.
+template <int P, int NT, int OUT, int TB, int ST = 16, int NSIDE = 0, typename FIn, typename Side = NoSide>
 __device__ __forceinline__ void mma_tiles(f32x4 (&acc)[OUT][Tr<P>::CB], const uint8_t* __restrict__ A,
-                                          int upr, int u0, FIn&& fin, uint8_t* stage) {
+                                          int upr, int u0, FIn&& fin, uint8_t* stage, Side&& side = Side{}) {
   constexpr int CB = Tr<P>::CB;
   constexpr int TPU = P == kF32 ? 1 : 2;
   constexpr int NU = NT / TPU;
@@ -328,10 +389,24 @@ __device__ __forceinline__ void mma_tiles(f32x4 (&acc)[OUT][Tr<P>::CB], const ui
   dma_chunk(std::integral_constant<int, 0>{}, stage);
   wait_vmem();
   __syncthreads();
+  constexpr int kPer = (NSIDE + NCH - 1) / NCH;  // deferred-store items per chunk
   static_for<0, NCH>([&](auto CI) SRF_AI {
     constexpr int c = decltype(CI)::value;
     const uint8_t* cur = stage + (c & 1) * kHalf;
     if constexpr (c + 1 < NCH) dma_chunk(std::integral_constant<int, c + 1>{}, stage + ((c + 1) & 1) * kHalf);
+    constexpr int i0 = c * kPer < NSIDE ? c * kPer : NSIDE;
+    constexpr int i1 = (c + 1) * kPer < NSIDE ? (c + 1) * kPer : NSIDE;
+    if constexpr (i1 > i0) {
+      // the stores must stay after the DMA above and before the wait below; one item at a time
+      // (scheduling barriers), so that only one item's store data is live at once
+      asm volatile("" ::: "memory");
+      static_for<i0, i1>([&](auto II) SRF_AI {
+        __builtin_amdgcn_sched_barrier(0);
+        side(II);
+        __builtin_amdgcn_sched_barrier(0);
+      });
+      asm volatile("" ::: "memory");
+    }
     constexpr int kn = (NU - c * kKC) < kKC ? (NU - c * kKC) : kKC;
     static_for<0, kn>([&](auto KI) SRF_AI {
       constexpr int kc = decltype(KI)::value;
@@ -385,8 +460,18 @@ __device__ __forceinline__ void mma_tiles(f32x4 (&acc)[OUT][Tr<P>::CB], const ui
         });
       }
     });
-    wait_vmem();
-    __syncthreads();
+    if constexpr (NSIDE == 0) {
+      wait_vmem();
+      __syncthreads();
+    } else {
+      // the next chunk's DMA (issued before this chunk's 2 (i1 - i0) or more stores) has landed
+      // once at most that many operations are outstanding; the barrier then also frees the
+      // buffer this chunk read (its LDS reads were consumed by the MFMAs above)
+      if constexpr (c + 1 < NCH) __builtin_amdgcn_s_waitcnt(vmcnt_only(2 * (i1 - i0)));
+      asm volatile("" ::: "memory");
+      __builtin_amdgcn_s_barrier();
+      asm volatile("" ::: "memory");
+    }
   });
 }
 
@@ -413,42 +498,81 @@ __device__ __forceinline__ void bias_acc(f32x4 (&acc)[OUT][Tr<P>::CB], const uin
 
 // One whole layer with IN input tiles: accumulate (from the bias if `bias` is non-null), then
 // epi(T, acc) for every output tile (after the last k-unit, so epi may overwrite the input
-// tiles in place).
-template <int P, int IN, int OUT, int TB, typename FIn, typename Epi>
+// tiles in place).  side / NSIDE: deferred stores issued during the product (mma_tiles).
+template <int P, int IN, int OUT, int TB, int NSIDE = 0, typename FIn, typename Epi, typename Side = NoSide>
 __device__ __forceinline__ void run_layer(const uint8_t* __restrict__ A, FIn&& fin, Epi&& epi, uint8_t* stage,
-                                          const uint8_t* __restrict__ bias = nullptr) {
+                                          const uint8_t* __restrict__ bias = nullptr, Side&& side = Side{}) {
   f32x4 acc[OUT][Tr<P>::CB];
   if (bias) bias_acc<P, OUT, TB>(acc, bias);
   else zero_acc<P, OUT>(acc);
-  mma_tiles<P, IN, OUT, TB>(acc, A, units<P>(IN), 0, fin, stage);
+  mma_tiles<P, IN, OUT, TB, 16, NSIDE>(acc, A, units<P>(IN), 0, fin, stage, side);
 #pragma unroll
   for (int to = 0; to < OUT; ++to) epi(TB + to, acc[to]);
 }
 
+// Value of the same register in the neighbouring lane (lane ^ 1), by a DPP quad permutation
+// [1, 0, 3, 2]: no index register and no LDS round trip, unlike a general lane shuffle.
+__device__ __forceinline__ uint32_t swap_adjacent(uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0xB1, 0xF, 0xF, false);
+}
+__device__ __forceinline__ float swap_adjacent(float v) {
+  return __builtin_bit_cast(float, swap_adjacent(__builtin_bit_cast(uint32_t, v)));
+}
+
+// Training stores go through raw buffer descriptors: the array base (the same in every lane) in
+// scalar registers, the wave's block offset as the 32-bit scalar offset, and only the lane's byte
+// offset in a vector register.  Per-lane 64-bit addresses for every tile would otherwise be kept
+// live (or spilled) across the layer loop.  Descriptor word 3 0x00020000: 32-bit data format, as
+// gfx9 raw buffer accesses require; the record count is the maximum (no clipping).  Offsets are
+// 32-bit: the host keeps every workspace array below 4 GB.
+using Rsrc = __amdgpu_buffer_rsrc_t;
+constexpr int kBufNT = 2;  // cache-policy bit of the non-temporal store
+__device__ __forceinline__ Rsrc uniform_rsrc(const void* base) {
+  const uint64_t v = reinterpret_cast<uint64_t>(base);
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+  return __builtin_amdgcn_make_buffer_rsrc(reinterpret_cast<void*>(((uint64_t)hi << 32) | lo), (short)0, -1,
+                                           0x00020000);
+}
+__device__ __forceinline__ void store_nt16(Rsrc r, uint32_t lane_off, uint32_t blk, u32x4 v) {
+  __builtin_amdgcn_raw_buffer_store_b128(v, r, lane_off, blk, kBufNT);
+}
+__device__ __forceinline__ void store_nt8(Rsrc r, uint32_t lane_off, uint32_t blk, u32x2 v) {
+  __builtin_amdgcn_raw_buffer_store_b64(v, r, lane_off, blk, kBufNT);
+}
+
+// Byte offset, inside an array of 16-feature tiles, of tile t's 2 KB record block written by the
+// wave owning points p0 .. p0 + 16 CB - 1 (p0 wave-uniform).
+template <int P>
+__device__ __forceinline__ uint32_t tile_block(int t, int ldP, int elem, int p0) {
+  return (uint32_t)t * 16u * (uint32_t)ldP * (uint32_t)elem + (uint32_t)(p0 / (16 * Tr<P>::CB)) * 2048u;
+}
+
 // Writes one 16-feature tile of the wave's points in the paired tile layout read by the weight-
-// gradient GEMM.  A tile of the whole workspace array occupies the bytes of 16 feature-major rows;
+// gradient GEMM: the 2 KB block at byte offset blk (tile_block) of the array described by dst.
+// A tile of the whole workspace array occupies the bytes of 16 feature-major rows;
 // inside it, the wave owning points p0 .. p0 + 16 CB - 1 writes one 2 KB block of 64 lane records
 // of 32 B.  The lanes of a pair (j = 2m, 2m + 1) first swap half of their D-tile values
-// (__shfl_xor 1), so that record (g, 2m + e) holds features 4 g + 2 e + c (c = 0, 1), each as one
+// (swap_adjacent), so that record (g, 2m + e) holds features 4 g + 2 e + c (c = 0, 1), each as one
 // 16 B chunk of that feature at the points 16 cb + 2 m and 16 cb + 2 m + 1 (cb = 0 .. CB - 1), in
 // point order.  A 16 B chunk is then exactly one lane's MFMA operand slice of one feature, so the
 // GEMM reads fragments with plain 16 B copies and every store here is a coalesced 32 B per lane.
 // The stores are non-temporal: a layer's tiles of all the CUs of an XCD (~4 MB) would otherwise
 // displace the streamed weights from that XCD's L2, and they are read back only by the backward.
 template <int P>
-__device__ __forceinline__ void store_tile(uint8_t* __restrict__ dst, int p0, const typename Tr<P>::Act (&v)[Tr<P>::CB]) {
+__device__ __forceinline__ void store_tile(Rsrc dst, uint32_t blk, const typename Tr<P>::Act (&v)[Tr<P>::CB]) {
   constexpr int CB = Tr<P>::CB;
   const int lane = threadIdx.x & 63;
   const bool odd = lane & 1;
-  u32x4* rec = reinterpret_cast<u32x4*>(dst + (size_t)(p0 / (16 * CB)) * 2048 + lane * 32);
+  const uint32_t rec = (uint32_t)lane * 32u;
   if constexpr (P == kF32) {
     // v[cb] = features 4g + 0..3 at point 16 cb + j; even lanes keep r0, r1 and take the
     // partner's; odd lanes keep r2, r3
     float rc[CB][2];
 #pragma unroll
     for (int cb = 0; cb < CB; ++cb) {
-      rc[cb][0] = __shfl_xor(odd ? v[cb][0] : v[cb][2], 1, 64);
-      rc[cb][1] = __shfl_xor(odd ? v[cb][1] : v[cb][3], 1, 64);
+      rc[cb][0] = swap_adjacent(odd ? v[cb][0] : v[cb][2]);
+      rc[cb][1] = swap_adjacent(odd ? v[cb][1] : v[cb][3]);
     }
     f32x4 c0, c1;
 #pragma unroll
@@ -459,21 +583,21 @@ __device__ __forceinline__ void store_tile(uint8_t* __restrict__ dst, int p0, co
       c1[2 * cb + 0] = odd ? rc[cb][1] : own1;
       c1[2 * cb + 1] = odd ? own1 : rc[cb][1];
     }
-    __builtin_nontemporal_store(__builtin_bit_cast(u32x4, c0), rec);
-    __builtin_nontemporal_store(__builtin_bit_cast(u32x4, c1), rec + 1);
+    store_nt16(dst, rec, blk, __builtin_bit_cast(u32x4, c0));
+    store_nt16(dst, rec, blk + 16, __builtin_bit_cast(u32x4, c1));
   } else {
     // v[cb] = {bf16 (r0, r1), bf16 (r2, r3)} at point 16 cb + j
     u32x4 c0, c1;
 #pragma unroll
     for (int cb = 0; cb < CB; ++cb) {
-      const uint32_t recv = __shfl_xor(odd ? v[cb][0] : v[cb][1], 1, 64);
+      const uint32_t recv = swap_adjacent(odd ? v[cb][0] : v[cb][1]);
       const uint32_t own = odd ? v[cb][1] : v[cb][0];
       const uint32_t first = odd ? recv : own, second = odd ? own : recv;  // points 2m, 2m + 1
       c0[cb] = (first & 0xffffu) | (second << 16);
       c1[cb] = (first >> 16) | (second & 0xffff0000u);
     }
-    __builtin_nontemporal_store(c0, rec);
-    __builtin_nontemporal_store(c1, rec + 1);
+    store_nt16(dst, rec, blk, c0);
+    store_nt16(dst, rec, blk + 16, c1);
   }
 }
 
@@ -493,7 +617,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
   // lanes.  Each lane reads back only its own entries, so no barrier is needed.
   __shared__ __attribute__((aligned(16))) Act pe_lds[kWaves][6][CB][64];
   const int lane = threadIdx.x & 63, g = lane >> 4, j = lane & 15;
-  const int wv = threadIdx.x >> 6;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform: SGPR addresses
   const int wave = blockIdx.x * kWaves + wv;
   const int p0 = wave * 16 * CB;  // the grid covers cols exactly: every wave has points
   const Plan& pl = a.p;
@@ -509,10 +633,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
   };
   auto valid = [&](int cb) SRF_AI { return p0 + 16 * cb + j < a.n; };
 
-  Act h[16][CB];
+  std::conditional_t<TRAIN, ActQuads<P>, ActTiles<P>> h;
   uint32_t mbits[CB][2];  // ReLU mask bits 4T+r of tiles 0-7 (word 0) and 8-15 (word 1)
 
-  // trunk PE tiles -> pe_lds (and, training, the feature-major X0 copy for the weight grads)
+  // trunk PE tiles -> pe_lds (training stores them for the weight gradients during layer 0)
   auto pe_trunk = [&]() SRF_AI {
     float x[CB][3];
 #pragma unroll
@@ -520,7 +644,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
     static_for<0, 4>([&](auto TI) SRF_AI {
       constexpr int t = decltype(TI)::value;
       if (t < pl.comp_tiles[0]) {
-        Act tl[CB];
 #pragma unroll
         for (int cb = 0; cb < CB; ++cb) {
           f32x4 v;
@@ -529,76 +652,95 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
             const int f = 16 * t + 4 * g + r;
             v[r] = f < pl.trunk_pe ? pe_feature(x[cb][0], x[cb][1], x[cb][2], f) : 0.f;
           }
-          tl[cb] = to_act<P>(v);
-          pe_lds[wv][t][cb][lane] = tl[cb];
+          pe_lds[wv][t][cb][lane] = to_act<P>(v);
         }
-        if (TRAIN) store_tile<P>(a.ws + a.w.x0 + (size_t)16 * t * ldP * pl.elem, p0, tl);
       }
     });
   };
   auto fin_pe = [&](int t, int cb) SRF_AI { return pe_lds[wv][t][cb][lane]; };
-  auto mask_store = [&](int slot) SRF_AI {
-    if (TRAIN) {
-      uint64_t* m = reinterpret_cast<uint64_t*>(a.ws + a.w.mask + (size_t)slot * ldP * 32);
+  auto mask_store = [&](Rsrc m) SRF_AI {
 #pragma unroll
-      for (int cb = 0; cb < CB; ++cb)
-        m[((size_t)wave * CB + cb) * 64 + lane] = (uint64_t)mbits[cb][0] | ((uint64_t)mbits[cb][1] << 32);
-    }
+    for (int cb = 0; cb < CB; ++cb)
+      store_nt8(m, (uint32_t)lane * 8u, (uint32_t)(wave * CB + cb) * 512u, u32x2{mbits[cb][0], mbits[cb][1]});
   };
   auto bias_of = [&](int l) SRF_AI { return a.packed + pl.bias_off[l]; };
-  // ReLU epilogue: tile T of layer l into h[T] (in place), its stored copy at ws offset
-  // store_off and the mask bits.
-  auto relu_epi = [&](int l, size_t store_off) SRF_AI {
-    return [&, l, store_off](int T, f32x4(&acc)[CB]) SRF_AI {
-      Act t[CB];
+  // ReLU epilogue: tile T into h[T] (in place) and its mask bits.  Nothing is stored here: the
+  // training copies of h and mbits are written during the next layer's product (side_of).
+  auto relu_epi = [&](int T, f32x4(&acc)[CB]) SRF_AI {
+    if (T == 0) {
 #pragma unroll
-      for (int cb = 0; cb < CB; ++cb) {
-        f32x4 v = acc[cb];  // bias already in the accumulator
+      for (int cb = 0; cb < CB; ++cb) mbits[cb][0] = mbits[cb][1] = 0;
+    }
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const bool pos = v[r] > 0.f;
-          v[r] = pos ? v[r] : 0.f;
-          mbits[cb][T >> 3] |= pos ? (1u << (4 * (T & 7) + r)) : 0u;
-        }
-        t[cb] = to_act<P>(v);
-        h[T][cb] = t[cb];
+    for (int cb = 0; cb < CB; ++cb) {
+      f32x4 v = acc[cb];  // bias already in the accumulator
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const bool pos = v[r] > 0.f;
+        v[r] = pos ? v[r] : 0.f;
+        mbits[cb][T >> 3] |= pos ? (1u << (4 * (T & 7) + r)) : 0u;
       }
-      if (TRAIN) store_tile<P>(a.ws + store_off + (size_t)16 * T * ldP * pl.elem, p0, t);
+      h.set(T, cb, to_act<P>(v));
+    }
+  };
+  // Deferred training stores (mma_tiles' side items) of the tiles the layer being computed
+  // reads: item i < NH stores activation tile i of h at ws offset h_off, the next NPE items store
+  // the pe_lds tiles at pe_off, and a last item (when mask_slot >= 0) stores the ReLU mask bits of
+  // the layer that produced h.  Every item issues at least two vector-memory stores.
+  constexpr int kT = TRAIN ? 1 : 0;
+  auto side_of = [&](auto NHI, size_t h_off, auto NPEI, size_t pe_off, int mask_slot) SRF_AI {
+    constexpr int nh = decltype(NHI)::value, npe = decltype(NPEI)::value;
+    const Rsrc rh = uniform_rsrc(a.ws + h_off), rpe = uniform_rsrc(a.ws + pe_off);
+    const Rsrc rm = uniform_rsrc(a.ws + a.w.mask + (size_t)(mask_slot < 0 ? 0 : mask_slot) * ldP * 32);
+    return [&, rh, rpe, rm](auto II) SRF_AI {
+      constexpr int i = decltype(II)::value;
+      if constexpr (i < nh) {
+        Act tl[CB];
+#pragma unroll
+        for (int cb = 0; cb < CB; ++cb) tl[cb] = h.get(i, cb);
+        store_tile<P>(rh, tile_block<P>(i, ldP, pl.elem, p0), tl);
+      } else if constexpr (i < nh + npe) {
+        constexpr int t = i - nh;
+        Act tl[CB];
+#pragma unroll
+        for (int cb = 0; cb < CB; ++cb) tl[cb] = pe_lds[wv][t][cb][lane];
+        store_tile<P>(rpe, tile_block<P>(t, ldP, pl.elem, p0), tl);
+      } else {
+        mask_store(rm);
+      }
     };
   };
-  auto fin_h = [&](int t, int cb) SRF_AI { return h[t][cb]; };
-  auto fin_hc = [&](int t, int cb) SRF_AI { return t < 16 ? h[t][cb] : pe_lds[wv][t - 16][cb][lane]; };
+  using I0 = std::integral_constant<int, 0>;
+  using I16 = std::integral_constant<int, 16>;
+  auto fin_h = [&](int t, int cb) SRF_AI { return h.get(t, cb); };
+  auto fin_hc = [&](int t, int cb) SRF_AI { return t < 16 ? h.get(t, cb) : pe_lds[wv][t - 16][cb][lane]; };
 
-  // ---- layer 0: PE(trunk) -> 256
+  // ---- layer 0: PE(trunk) -> 256 (stores the PE tiles X0)
   pe_trunk();
-#pragma unroll
-  for (int cb = 0; cb < CB; ++cb) mbits[cb][0] = mbits[cb][1] = 0;
   if (pl.comp_tiles[0] == 4)
-    run_layer<P, 4, 16, 0>(a.packed + pl.fwd_off[0], fin_pe, relu_epi(0, a.w.h[0]), stage, bias_of(0));
+    run_layer<P, 4, 16, 0, 4 * kT>(a.packed + pl.fwd_off[0], fin_pe, relu_epi, stage, bias_of(0),
+                                   side_of(I0{}, 0, std::integral_constant<int, 4>{}, a.w.x0, -1));
   else
-    run_layer<P, 2, 16, 0>(a.packed + pl.fwd_off[0], fin_pe, relu_epi(0, a.w.h[0]), stage, bias_of(0));
-  mask_store(0);
+    run_layer<P, 2, 16, 0, 2 * kT>(a.packed + pl.fwd_off[0], fin_pe, relu_epi, stage, bias_of(0),
+                                   side_of(I0{}, 0, std::integral_constant<int, 2>{}, a.w.x0, -1));
 
   // ---- layers 1..7, each in place in h; layer 5 adds the trunk PE k-steps after the 16
   // register tiles (one loop body for all seven layers keeps register pressure flat)
+  // (stores layer l - 1's output and mask)
   for (int l = 1; l < 8; ++l) {
-#pragma unroll
-    for (int cb = 0; cb < CB; ++cb) mbits[cb][0] = mbits[cb][1] = 0;
     f32x4 acc[16][CB];
     bias_acc<P, 16, 0>(acc, bias_of(l));
     const uint8_t* A = a.packed + pl.fwd_off[l];
     const int upr = units<P>(16 + pl.comp_tiles[l]);
-    mma_tiles<P, 16, 16, 0>(acc, A, upr, 0, fin_h, stage);
+    mma_tiles<P, 16, 16, 0, 16, 17 * kT>(acc, A, upr, 0, fin_h, stage, side_of(I16{}, a.w.h[l - 1], I0{}, 0, l - 1));
     if (l == 5) {
       if (pl.comp_tiles[5] == 4)
         mma_tiles<P, 4, 16, 0>(acc, A, upr, units<P>(16), fin_pe, stage);
       else
         mma_tiles<P, 2, 16, 0>(acc, A, upr, units<P>(16), fin_pe, stage);
     }
-    auto epi = relu_epi(l, a.w.h[l]);
 #pragma unroll
-    for (int T = 0; T < 16; ++T) epi(T, acc[T]);
-    mask_store(l);
+    for (int T = 0; T < 16; ++T) relu_epi(T, acc[T]);
   }
 
   // ---- heads
@@ -608,8 +750,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
     a.sigma[p] = pl.softplus ? softplusf_(raw) : (raw > 0.f ? raw : 0.f);
   };
   if constexpr (!VD) {
-    // 4-channel output: sigma, rgb (src/models/SimpleNeRF17.py:735-750)
-    run_layer<P, 16, 1, 0>(a.packed + pl.fwd_off[8], fin_h, [&](int T, f32x4(&acc)[CB]) SRF_AI {
+    // 4-channel output: sigma, rgb (src/models/SimpleNeRF17.py:735-750); stores layer 7
+    run_layer<P, 16, 1, 0, 17 * kT>(a.packed + pl.fwd_off[8], fin_h, [&](int T, f32x4(&acc)[CB]) SRF_AI {
 #pragma unroll
       for (int cb = 0; cb < CB; ++cb) {
         const int p = p0 + 16 * cb + j;
@@ -621,22 +763,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
           a.rgb[3 * p + 2] = sigmoidf_(v[3]);
         }
       }
-    }, stage, bias_of(8));
+    }, stage, bias_of(8), side_of(I16{}, a.w.h[7], I0{}, 0, 7));
   } else {
-    // sigma (row 256, tile 16) first, then feature rows 0..255 (identity) in place in h
-    run_layer<P, 16, 1, 16>(a.packed + pl.fwd_off[8], fin_h, [&](int T, f32x4(&acc)[CB]) SRF_AI {
+    // sigma (row 256, tile 16) first (storing layer 7), then feature rows 0..255 (identity) in
+    // place in h
+    run_layer<P, 16, 1, 16, 17 * kT>(a.packed + pl.fwd_off[8], fin_h, [&](int T, f32x4(&acc)[CB]) SRF_AI {
 #pragma unroll
       for (int cb = 0; cb < CB; ++cb)
         if (g == 0 && valid(cb)) head_sigma(cb, acc[cb][0]);
-    }, stage, bias_of(8));
+    }, stage, bias_of(8), side_of(I16{}, a.w.h[7], I0{}, 0, 7));
     run_layer<P, 16, 16, 0>(a.packed + pl.fwd_off[8], fin_h, [&](int T, f32x4(&acc)[CB]) SRF_AI {
-      Act t[CB];
 #pragma unroll
-      for (int cb = 0; cb < CB; ++cb) {
-        t[cb] = to_act<P>(acc[cb]);
-        h[T][cb] = t[cb];
-      }
-      if (TRAIN) store_tile<P>(a.ws + a.w.f + (size_t)16 * T * ldP * pl.elem, p0, t);
+      for (int cb = 0; cb < CB; ++cb) h.set(T, cb, to_act<P>(acc[cb]));
     }, stage, bias_of(8));
     // view layer input extras: [PE_xyz[trunk_pe:], PE_dir]
     const int n_hi = 3 * (2 * pl.pts_deg + 1) - pl.trunk_pe;
@@ -649,7 +787,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
     static_for<0, 6>([&](auto TI) SRF_AI {
       constexpr int t = decltype(TI)::value;
       if (t < pl.comp_tiles[9]) {
-        Act tl[CB];
 #pragma unroll
         for (int cb = 0; cb < CB; ++cb) {
           f32x4 v;
@@ -661,24 +798,23 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
             else if (e < pl.extra) val = pe_feature(d[cb][0], d[cb][1], d[cb][2], e - n_hi);
             v[r] = val;
           }
-          tl[cb] = to_act<P>(v);
-          pe_lds[wv][t][cb][lane] = tl[cb];
+          pe_lds[wv][t][cb][lane] = to_act<P>(v);
         }
-        if (TRAIN) store_tile<P>(a.ws + a.w.ex + (size_t)16 * t * ldP * pl.elem, p0, tl);
       }
     });
-#pragma unroll
-    for (int cb = 0; cb < CB; ++cb) mbits[cb][0] = mbits[cb][1] = 0;
-    // view layer [feature, extras] -> 128 (ReLU), in place into h[0..7]
+    // view layer [feature, extras] -> 128 (ReLU), in place into h[0..7]; stores the feature
+    // tiles and the extras
     if (pl.comp_tiles[9] == 2)
-      run_layer<P, 18, 8, 0>(a.packed + pl.fwd_off[9], fin_hc, relu_epi(9, a.w.v), stage, bias_of(9));
+      run_layer<P, 18, 8, 0, 18 * kT>(a.packed + pl.fwd_off[9], fin_hc, relu_epi, stage, bias_of(9),
+                                      side_of(I16{}, a.w.f, std::integral_constant<int, 2>{}, a.w.ex, -1));
     else if (pl.comp_tiles[9] == 4)
-      run_layer<P, 20, 8, 0>(a.packed + pl.fwd_off[9], fin_hc, relu_epi(9, a.w.v), stage, bias_of(9));
+      run_layer<P, 20, 8, 0, 20 * kT>(a.packed + pl.fwd_off[9], fin_hc, relu_epi, stage, bias_of(9),
+                                      side_of(I16{}, a.w.f, std::integral_constant<int, 4>{}, a.w.ex, -1));
     else
-      run_layer<P, 22, 8, 0>(a.packed + pl.fwd_off[9], fin_hc, relu_epi(9, a.w.v), stage, bias_of(9));
-    mask_store(8);
-    // rgb head 128 -> 3, sigmoid
-    run_layer<P, 8, 1, 0>(a.packed + pl.fwd_off[10], fin_h, [&](int T, f32x4(&acc)[CB]) SRF_AI {
+      run_layer<P, 22, 8, 0, 22 * kT>(a.packed + pl.fwd_off[9], fin_hc, relu_epi, stage, bias_of(9),
+                                      side_of(I16{}, a.w.f, std::integral_constant<int, 6>{}, a.w.ex, -1));
+    // rgb head 128 -> 3, sigmoid (stores the view layer's output and mask)
+    run_layer<P, 8, 1, 0, 9 * kT>(a.packed + pl.fwd_off[10], fin_h, [&](int T, f32x4(&acc)[CB]) SRF_AI {
 #pragma unroll
       for (int cb = 0; cb < CB; ++cb) {
         const int p = p0 + 16 * cb + j;
@@ -689,7 +825,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
           a.rgb[3 * p + 2] = sigmoidf_(v[2]);
         }
       }
-    }, stage, bias_of(10));
+    }, stage, bias_of(10), side_of(std::integral_constant<int, 8>{}, a.w.v, I0{}, 0, 8));
   }
 }
 
@@ -701,7 +837,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
   using Act = typename Tr<P>::Act;
   __shared__ __attribute__((aligned(16))) uint8_t stage[2 * kStageBytes];
   const int lane = threadIdx.x & 63, g = lane >> 4, j = lane & 15;
-  const int wv = threadIdx.x >> 6;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform: scalar store offsets
   const int wave = blockIdx.x * kWaves + wv;
   const int p0 = wave * 16 * CB;  // the grid covers cols exactly: every wave has points
   const Plan& pl = a.p;
@@ -729,12 +865,24 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
     }
   }
 
-  // z holds the current layer's pre-activation gradients; every product writes in place.
+  // z holds the current layer's pre-activation gradients; every product writes in place.  Each
+  // layer's dZ tiles are stored for the weight gradients during the next product (mma_tiles'
+  // deferred stores: dz_side), while z still holds them as that product's input.
   Act z[18][CB];
   uint32_t mb[CB][2];
   auto store_dz = [&](int l, int T, const Act(&t)[CB]) SRF_AI {
-    store_tile<P>(a.ws + a.w.dz[l] + (size_t)16 * T * ldP * pl.elem, p0, t);
+    store_tile<P>(uniform_rsrc(a.ws + a.w.dz[l]), tile_block<P>(T, ldP, pl.elem, p0), t);
   };
+  // deferred-store items: tile T0 + i of z as tile T0 + i of dZ_l
+  auto dz_side = [&](int l, auto T0I) SRF_AI {
+    constexpr int t0 = decltype(T0I)::value;
+    const Rsrc r = uniform_rsrc(a.ws + a.w.dz[l]);
+    return [&, r](auto II) SRF_AI {
+      constexpr int t = t0 + decltype(II)::value;
+      store_tile<P>(r, tile_block<P>(t, ldP, pl.elem, p0), z[t]);
+    };
+  };
+  using I0 = std::integral_constant<int, 0>;
   auto load_mask = [&](int slot) SRF_AI {
     const uint64_t* m = reinterpret_cast<const uint64_t*>(a.ws + a.w.mask + (size_t)slot * ldP * 32);
 #pragma unroll
@@ -744,85 +892,73 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
       mb[cb][1] = (uint32_t)(v >> 32);
     }
   };
-  // epilogue: ReLU'-masked gradient of layer l into z[T], stored as dZ_l
-  auto masked_epi = [&](int l) SRF_AI {
-    return [&, l](int T, f32x4(&acc)[CB]) SRF_AI {
-      Act t[CB];
+  // epilogue: ReLU'-masked gradient of layer l into z[T] (dZ_l, stored during the next product)
+  auto masked_epi = [&](int T, f32x4(&acc)[CB]) SRF_AI {
 #pragma unroll
-      for (int cb = 0; cb < CB; ++cb) {
-        f32x4 v = acc[cb];
+    for (int cb = 0; cb < CB; ++cb) {
+      f32x4 v = acc[cb];
 #pragma unroll
-        for (int r = 0; r < 4; ++r) v[r] = (mb[cb][T >> 3] & (1u << (4 * (T & 7) + r))) ? v[r] : 0.f;
-        t[cb] = to_act<P>(v);
-        z[T][cb] = t[cb];
-      }
-      store_dz(l, T, t);
-    };
+      for (int r = 0; r < 4; ++r) v[r] = (mb[cb][T >> 3] & (1u << (4 * (T & 7) + r))) ? v[r] : 0.f;
+      z[T][cb] = to_act<P>(v);
+    }
   };
   auto fin_z = [&](int t, int cb) SRF_AI { return z[t][cb]; };
 
   if constexpr (VD) {
     // rgb head: dZ10 tile 0 (rows 0..2), zero pad tile
     {
-      Act t[CB];
 #pragma unroll
       for (int cb = 0; cb < CB; ++cb) {
         f32x4 v = {0.f, 0.f, 0.f, 0.f};
         if (g == 0) v = f32x4{drr[cb][0], drr[cb][1], drr[cb][2], 0.f};
-        t[cb] = to_act<P>(v);
-        z[0][cb] = t[cb];
+        z[0][cb] = to_act<P>(v);
         z[1][cb] = zero_act<P>();
       }
-      store_dz(10, 0, t);
     }
-    // dV = W10^T dZ10, masked by the view-layer ReLU -> dZ9 (z[0..7])
+    // dV = W10^T dZ10, masked by the view-layer ReLU -> dZ9 (z[0..7]); stores dZ10
     load_mask(8);
-    run_layer<P, 2, 8, 0>(a.packed + pl.bwd_off[10], fin_z, masked_epi(9), stage);
-    // dF = W9^T dZ9 (feature part, identity activation) -> dZ8 rows 0..255 (z[0..15])
-    run_layer<P, 8, 16, 0>(a.packed + pl.bwd_off[9], fin_z, [&](int T, f32x4(&acc)[CB]) SRF_AI {
-      Act t[CB];
+    run_layer<P, 2, 8, 0, 1>(a.packed + pl.bwd_off[10], fin_z, masked_epi, stage, nullptr, dz_side(10, I0{}));
+    // dF = W9^T dZ9 (feature part, identity activation) -> dZ8 rows 0..255 (z[0..15]); stores dZ9
+    run_layer<P, 8, 16, 0, 8>(a.packed + pl.bwd_off[9], fin_z, [&](int T, f32x4(&acc)[CB]) SRF_AI {
 #pragma unroll
-      for (int cb = 0; cb < CB; ++cb) {
-        t[cb] = to_act<P>(acc[cb]);
-        z[T][cb] = t[cb];
-      }
-      store_dz(8, T, t);
-    }, stage);
+      for (int cb = 0; cb < CB; ++cb) z[T][cb] = to_act<P>(acc[cb]);
+    }, stage, nullptr, dz_side(9, I0{}));
     {
-      Act t[CB];
 #pragma unroll
       for (int cb = 0; cb < CB; ++cb) {
         f32x4 v = {0.f, 0.f, 0.f, 0.f};
         if (g == 0) v[0] = dsr[cb];
-        t[cb] = to_act<P>(v);
-        z[16][cb] = t[cb];
+        z[16][cb] = to_act<P>(v);
         z[17][cb] = zero_act<P>();
       }
-      store_dz(8, 16, t);
     }
+    // dZ8 (tiles 0..16) is stored here rather than during the next product: with 18 input tiles
+    // live, deferring it spills registers
+#pragma unroll
+    for (int T = 0; T < 17; ++T) store_dz(8, T, z[T]);
     load_mask(7);
-    run_layer<P, 18, 16, 0>(a.packed + pl.bwd_off[8], fin_z, masked_epi(7), stage);
+    run_layer<P, 18, 16, 0>(a.packed + pl.bwd_off[8], fin_z, masked_epi, stage);
   } else {
     {
-      Act t[CB];
 #pragma unroll
       for (int cb = 0; cb < CB; ++cb) {
         f32x4 v = {0.f, 0.f, 0.f, 0.f};
         if (g == 0) v = f32x4{dsr[cb], drr[cb][0], drr[cb][1], drr[cb][2]};
-        t[cb] = to_act<P>(v);
-        z[0][cb] = t[cb];
+        z[0][cb] = to_act<P>(v);
         z[1][cb] = zero_act<P>();
       }
-      store_dz(8, 0, t);
     }
     load_mask(7);
-    run_layer<P, 2, 16, 0>(a.packed + pl.bwd_off[8], fin_z, masked_epi(7), stage);
+    run_layer<P, 2, 16, 0, 1>(a.packed + pl.bwd_off[8], fin_z, masked_epi, stage, nullptr, dz_side(8, I0{}));
   }
-  // trunk: z holds dZ_l; dH_{l-1} = W_l^T dZ_l, masked -> dZ_{l-1}, in place
+  // trunk: z holds dZ_l; dH_{l-1} = W_l^T dZ_l, masked -> dZ_{l-1}, in place (storing dZ_l)
   for (int l = 7; l >= 1; --l) {
     load_mask(l - 1);
-    run_layer<P, 16, 16, 0>(a.packed + pl.bwd_off[l], fin_z, masked_epi(l - 1), stage);
+    run_layer<P, 16, 16, 0, 16>(a.packed + pl.bwd_off[l], fin_z, masked_epi, stage, nullptr, dz_side(l, I0{}));
   }
+  // dZ0: no product follows
+#pragma unroll
+  for (int T = 0; T < 16; ++T) store_dz(0, T, z[T]);
 }
 
 // ----------------------------------------------------------------- weight gradients
@@ -860,9 +996,6 @@ constexpr int kDwFrags = 32;              // 16 dZ + 16 X fragments per k-unit
 constexpr int kDwSlot = kDwFrags * 1024;  // one k-unit
 constexpr int kDwSlots = 4;               // ring depth (prefetch distance kDwSlots - 1)
 
-// s_waitcnt immediate (gfx9 encoding) that waits for at most n outstanding vector-memory
-// operations and leaves expcnt / lgkmcnt unconstrained.
-constexpr int vmcnt_only(int n) { return 0x0F70 | (n & 0xF) | ((n >> 4) << 14); }
 
 // RM row tiles per wave: RM = 4 covers 256-row tiles (the NeRF layers), RM = 2 covers 128-row
 // tiles, so that a job of at most 128 rows (the TensoRF colour network) keeps all eight waves
