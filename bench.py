@@ -32,6 +32,34 @@ METRIC = 'train rays/s (fwd+bwd) & render rays/s: fused BF16 vs plain FP32, 1 GP
 PEAK_BF16_TFLOPS = 2500.0
 PEAK_HBM_GBS = 8000.0
 MACS = {(63, True): 593408, (21, True): 577280, (63, False): 492032}
+
+
+def mlp_layer_dims(desc):
+    """(inputs, outputs) of every product of the NeRF MLP (width 256, view layer 128): trunk
+    layers 0-7 (skip at 5), then [feature + sigma, view, rgb] or the 4-channel head."""
+    pts_dim = 3 * (2 * desc.pts_pe_degree + 1)
+    d = desc.trunk_pe
+    dims = [(d if l == 0 else (256 + d if l == 5 else 256), 256) for l in range(8)]
+    if desc.view_dependent:
+        dims += [(256, 257), (256 + pts_dim - d + 3 * (2 * desc.dirs_pe_degree + 1), 128), (128, 3)]
+    else:
+        dims += [(256, 4)]
+    return dims
+
+
+def mlp_algorithmic_bytes(name, desc, n):
+    """Bytes an MLP entry point must move at minimum for n points (16-bit operands):
+    weight grads read every layer's input and output-gradient rows once and write the FP32
+    gradients once; the training forward writes every layer input and the ReLU masks (9 x 32 B);
+    the backward chain writes every output gradient and reads the masks."""
+    dims = mlp_layer_dims(desc)
+    elem = 4 if desc.prec == 0 else 2
+    masks = 9 * 32
+    if name == 'srf_mlp_weight_grads':
+        return n * elem * sum(i + o for i, o in dims) + 4 * sum(i * o + o for i, o in dims)
+    if name == 'srf_mlp_forward':
+        return n * (elem * sum(i for i, _ in dims[1:]) + elem * desc.trunk_pe + masks + 24 + 16)
+    return n * (elem * sum(o for _, o in dims) + masks + 32)
 # HBM bytes per launch of the dominant kernel from the rocprofv3 FETCH_SIZE / WRITE_SIZE passes
 # (scaled by a plain copy of known size) over this same train step.
 TRAFFIC_PROFILE = REPO / 'profiles' / 'mlp_train_step_4096rays.json'
@@ -80,10 +108,12 @@ class KernelTimer:
             d = out.setdefault(name, {'calls': 0, 'ms': 0.0, 'flop': 0.0})
             d['calls'] += 1
             d['ms'] += ms
+            d.setdefault('bytes', 0.0)
             if name in ('srf_mlp_forward', 'srf_mlp_backward_data', 'srf_mlp_weight_grads'):
                 desc = args[0]._obj
                 n = args[2] if name != 'srf_mlp_weight_grads' else args[1]
                 d['flop'] += 2.0 * MACS[(desc.trunk_pe, bool(desc.view_dependent))] * n
+                d['bytes'] += mlp_algorithmic_bytes(name, desc, n)
         return out
 
 
@@ -650,13 +680,28 @@ def main():
     dom_name, dom = max(single.items(), key=lambda kv: kv[1]['ms'])
     roof = None
     if dom['flop'] > 0:
-        ach = dom['flop'] / (dom['ms'] * 1e-3) / 1e12
-        roof = {'bound': 'mfma', 'kernel': dom_name, 'achieved': round(ach, 2), 'peak': PEAK_BF16_TFLOPS,
-                'unit': 'TFLOP/s', 'frac': round(ach / PEAK_BF16_TFLOPS, 4),
-                'traffic': profiled_traffic(dom_name, args.rays),
-                'avg_launch_ms': round(dom['ms'] / dom['calls'], 4)}
+        # the binding resource: the one whose peak gives the longer minimum time for the
+        # kernel's algorithmic FLOPs and bytes (the weight-gradient GEMM, ~120 FLOP per byte, sits
+        # below the 312 FLOP/B ridge of 2.5 PFLOP/s over 8 TB/s: HBM-bound)
+        sec = dom['ms'] * 1e-3
+        t_mfma = dom['flop'] / (PEAK_BF16_TFLOPS * 1e12)
+        t_hbm = dom['bytes'] / (PEAK_HBM_GBS * 1e9)
+        tflops = dom['flop'] / sec / 1e12
+        gbs = dom['bytes'] / sec / 1e9
+        if t_hbm >= t_mfma:
+            roof = {'bound': 'hbm', 'kernel': dom_name, 'achieved': round(gbs, 1), 'peak': PEAK_HBM_GBS,
+                    'unit': 'GB/s', 'frac': round(gbs / PEAK_HBM_GBS, 4)}
+        else:
+            roof = {'bound': 'mfma', 'kernel': dom_name, 'achieved': round(tflops, 2), 'peak': PEAK_BF16_TFLOPS,
+                    'unit': 'TFLOP/s', 'frac': round(tflops / PEAK_BF16_TFLOPS, 4)}
+        roof.update({'traffic': profiled_traffic(dom_name, args.rays),
+                     'algorithmic_bytes_per_launch': int(dom['bytes'] / dom['calls']),
+                     'flop_per_launch': int(dom['flop'] / dom['calls']),
+                     'tflops': round(tflops, 1), 'mfma_frac': round(tflops / PEAK_BF16_TFLOPS, 4),
+                     'avg_launch_ms': round(dom['ms'] / dom['calls'], 4)})
     per_kernel = {k: {'calls': v['calls'], 'ms_per_step': round(v['ms'] / args.steps, 3),
-                      **({'tflops': round(v['flop'] / (v['ms'] * 1e-3) / 1e12, 1)} if v['flop'] else {})}
+                      **({'tflops': round(v['flop'] / (v['ms'] * 1e-3) / 1e12, 1),
+                          'GB/s': round(v['bytes'] / (v['ms'] * 1e-3) / 1e9, 1)} if v['flop'] else {})}
                   for k, v in kern.items()}
     render = hip_render_leg(args, dev, tr['model'])
     image = None if args.no_image else render_image_leg(args, dev)

@@ -994,7 +994,7 @@ constexpr int kDwTile = 256;
 constexpr int kDwWaves = 8;
 constexpr int kDwFrags = 32;              // 16 dZ + 16 X fragments per k-unit
 constexpr int kDwSlot = kDwFrags * 1024;  // one k-unit
-constexpr int kDwSlots = 4;               // ring depth (prefetch distance kDwSlots - 1)
+constexpr int kDwSlots = 5;               // ring depth (prefetch distance kDwSlots - 1): 160 KB of LDS
 
 
 // RM row tiles per wave: RM = 4 covers 256-row tiles (the NeRF layers), RM = 2 covers 128-row
@@ -1080,7 +1080,9 @@ __global__ __launch_bounds__(64 * kDwWaves) void mlp_dw_kernel(DwArgs a) {
   for (int u = 0; u < n_units; ++u) {
     // k-unit u has landed once at most (issued after it) k-units' loads are outstanding
     const int after = min(kDwSlots - 2, n_units - 1 - u);
-    if (after >= 2) __builtin_amdgcn_s_waitcnt(vmcnt_only(8));
+    static_assert(kDwSlots <= 5, "waits below cover up to three k-units in flight after u");
+    if (after >= 3) __builtin_amdgcn_s_waitcnt(vmcnt_only(12));
+    else if (after == 2) __builtin_amdgcn_s_waitcnt(vmcnt_only(8));
     else if (after == 1) __builtin_amdgcn_s_waitcnt(vmcnt_only(4));
     else __builtin_amdgcn_s_waitcnt(vmcnt_only(0));
     // After the barrier every wave's part of k-unit u is in LDS (each wave waited for its own
