@@ -276,16 +276,69 @@ __device__ __forceinline__ void pe_sincos(float x, float* sn, float* cs) {
   *cs = q == 0 ? cr : (q == 1 ? -sr : (q == 2 ? -cr : sr));
 }
 
-// Value of positional-encoding feature f of x (degree L, include_input, log sampling):
-// [x, sin(2^0 x), cos(2^0 x), ..., sin(2^(L-1) x), cos(2^(L-1) x)], blocks of 3 coordinates.
-__device__ __forceinline__ float pe_feature(float x0, float x1, float x2, int f) {
-  const int block = f / 3, c = f - 3 * (f / 3);
-  const float v = c == 0 ? x0 : (c == 1 ? x1 : x2);
-  if (block == 0) return v;
-  const int m = (block - 1) >> 1;
-  float s, co;
-  pe_sincos(ldexpf(v, m), &s, &co);
-  return ((block - 1) & 1) ? co : s;
+// Positional encoding (degree L, include_input, log sampling) of x = (x0, x1, x2):
+// features [x, sin(2^0 x), cos(2^0 x), ..., sin(2^(L-1) x), cos(2^(L-1) x)] in blocks of 3
+// coordinates, i.e. feature c is x_c, feature 3 + 6m + c is sin(2^m x_c) and 6 + 6m + c is
+// cos(2^m x_c).  Computed point-major: each lane evaluates every feature of one of the wave's
+// points into that point's LDS row (PeRows), one pe_sincos call per (m, c) giving both the sin
+// and the cos feature; the lanes then gather their B-operand slices from the rows.
+template <int P>
+struct PeRows {
+  static constexpr int kNpt = 16 * Tr<P>::CB;  // points per wave
+  static constexpr int kParts = 64 / kNpt;     // lanes per point (FP32: two, levels split)
+  static constexpr int kStride = 100;          // elements per row: up to 96 features, and
+                                               // conflict-free 8 (16)-byte gathers
+  static constexpr int kBytes = kNpt * kStride * (P == kF32 ? 4 : 2);
+  using E = std::conditional_t<P == kF32, float, uint16_t>;
+
+  __device__ static SRF_AI E cvt(float v) {
+    if constexpr (P == kF32) return v;
+    else return (uint16_t)(pack2<P>(v, 0.f) & 0xffffu);
+  }
+  // Features f in [f_lo, f_hi) of PE_L(x), written at row[f - shift]; this lane computes the
+  // levels m = part, part + kParts, ...
+  __device__ static SRF_AI void write(E* row, const float (&x)[3], int L, int f_lo, int f_hi, int shift, int part) {
+    if (part == 0)
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+        if (c >= f_lo && c < f_hi) row[c - shift] = cvt(x[c]);
+    for (int m = part; m < L; m += kParts) {
+      if (8 + 6 * m < f_lo || 3 + 6 * m >= f_hi) continue;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        float sn, cs;
+        pe_sincos(ldexpf(x[c], m), &sn, &cs);
+        const int fs = 3 + 6 * m + c, fc = 6 + 6 * m + c;
+        if (fs >= f_lo && fs < f_hi) row[fs - shift] = cvt(sn);
+        if (fc >= f_lo && fc < f_hi) row[fc - shift] = cvt(cs);
+      }
+    }
+  }
+  // B-operand slice of 16-column tile t for column block cb: columns 16 t + 4 g + r (r = 0..3)
+  // of point 16 cb + j's row, zero at or beyond n_valid.
+  __device__ static SRF_AI typename Tr<P>::Act gather(const E* rows, int t, int cb, int n_valid) {
+    const int lane = threadIdx.x & 63, g = lane >> 4, j = lane & 15;
+    const int c0 = 16 * t + 4 * g;
+    const E* src = rows + (16 * cb + j) * kStride + c0;
+    if constexpr (P == kF32) {
+      f32x4 v = *reinterpret_cast<const f32x4*>(src);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) v[r] = c0 + r < n_valid ? v[r] : 0.f;
+      return v;
+    } else {
+      u32x2 v = *reinterpret_cast<const u32x2*>(src);
+      v.x &= (c0 + 0 < n_valid ? 0x0000ffffu : 0u) | (c0 + 1 < n_valid ? 0xffff0000u : 0u);
+      v.y &= (c0 + 2 < n_valid ? 0x0000ffffu : 0u) | (c0 + 3 < n_valid ? 0xffff0000u : 0u);
+      return v;
+    }
+  }
+};
+
+// Orders a wave's LDS writes before its later LDS reads of other lanes' entries.
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 // ------------------------------------------------------------------- kernel arguments
@@ -623,39 +676,36 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
   const Plan& pl = a.p;
   const int ldP = a.w.ldP;
 
-  // Point coordinates / view directions are re-read where needed (layer 0, layer 5, view
-  // layer) rather than kept live across the trunk: 12 bytes per point against 24 VGPRs.
-  auto load3 = [&](const float* src, int cb, float (&v)[3]) SRF_AI {
-    const int p = p0 + 16 * cb + j;
-    const bool ok = p < a.n;
-#pragma unroll
-    for (int c = 0; c < 3; ++c) v[c] = ok ? src[3 * p + c] : 0.f;
-  };
   auto valid = [&](int cb) SRF_AI { return p0 + 16 * cb + j < a.n; };
 
   std::conditional_t<TRAIN, ActQuads<P>, ActTiles<P>> h;
   uint32_t mbits[CB][2];  // ReLU mask bits 4T+r of tiles 0-7 (word 0) and 8-15 (word 1)
 
   // trunk PE tiles -> pe_lds (training stores them for the weight gradients during layer 0)
-  auto pe_trunk = [&]() SRF_AI {
-    float x[CB][3];
+  // Point-major PE rows (PeRows) in this wave's part of the weight stage, which is free before
+  // layer 0 and between the feature and view layers; a block barrier follows the gathers,
+  // before the next product's DMA refills the stage.
+  using PR = PeRows<P>;
+  static_assert(kWaves * PR::kBytes <= 2 * kStageBytes, "PE rows fit in the stage");
+  typename PR::E* pe_rows = reinterpret_cast<typename PR::E*>(stage + wv * PR::kBytes);
+  const int pe_pt = lane % PR::kNpt, pe_part = lane / PR::kNpt;
+  auto load_pt = [&](const float* src, float (&v)[3]) SRF_AI {
+    const int p = p0 + pe_pt;
 #pragma unroll
-    for (int cb = 0; cb < CB; ++cb) load3(a.pts, cb, x[cb]);
+    for (int c = 0; c < 3; ++c) v[c] = p < a.n ? src[3 * p + c] : 0.f;
+  };
+  auto pe_trunk = [&]() SRF_AI {
+    float x[3];
+    load_pt(a.pts, x);
+    PR::write(pe_rows + pe_pt * PR::kStride, x, pl.pts_deg, 0, pl.trunk_pe, 0, pe_part);
+    wave_lds_sync();
     static_for<0, 4>([&](auto TI) SRF_AI {
       constexpr int t = decltype(TI)::value;
-      if (t < pl.comp_tiles[0]) {
+      if (t < pl.comp_tiles[0])
 #pragma unroll
-        for (int cb = 0; cb < CB; ++cb) {
-          f32x4 v;
-#pragma unroll
-          for (int r = 0; r < 4; ++r) {
-            const int f = 16 * t + 4 * g + r;
-            v[r] = f < pl.trunk_pe ? pe_feature(x[cb][0], x[cb][1], x[cb][2], f) : 0.f;
-          }
-          pe_lds[wv][t][cb][lane] = to_act<P>(v);
-        }
-      }
+        for (int cb = 0; cb < CB; ++cb) pe_lds[wv][t][cb][lane] = PR::gather(pe_rows, t, cb, pl.trunk_pe);
     });
+    __syncthreads();
   };
   auto fin_pe = [&](int t, int cb) SRF_AI { return pe_lds[wv][t][cb][lane]; };
   auto mask_store = [&](Rsrc m) SRF_AI {
@@ -776,32 +826,25 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
 #pragma unroll
       for (int cb = 0; cb < CB; ++cb) h.set(T, cb, to_act<P>(acc[cb]));
     }, stage, bias_of(8));
-    // view layer input extras: [PE_xyz[trunk_pe:], PE_dir]
-    const int n_hi = 3 * (2 * pl.pts_deg + 1) - pl.trunk_pe;
-    float x[CB][3], d[CB][3];
+    // view layer input extras: [PE_xyz[trunk_pe:], PE_dir], one row per point (the feature
+    // layer's product ended with a block barrier, so the stage is free)
+    {
+      const int pts_dim = 3 * (2 * pl.pts_deg + 1), n_hi = pts_dim - pl.trunk_pe;
+      float x[3], d[3];
+      load_pt(a.pts, x);
+      load_pt(a.dirs, d);
+      typename PR::E* row = pe_rows + pe_pt * PR::kStride;
+      if (n_hi > 0) PR::write(row, x, pl.pts_deg, pl.trunk_pe, pts_dim, pl.trunk_pe, pe_part);
+      PR::write(row, d, pl.dir_deg, 0, pl.extra - n_hi, -n_hi, pe_part);
+      wave_lds_sync();
+      static_for<0, 6>([&](auto TI) SRF_AI {
+        constexpr int t = decltype(TI)::value;
+        if (t < pl.comp_tiles[9])
 #pragma unroll
-    for (int cb = 0; cb < CB; ++cb) {
-      load3(a.pts, cb, x[cb]);
-      load3(a.dirs, cb, d[cb]);
+          for (int cb = 0; cb < CB; ++cb) pe_lds[wv][t][cb][lane] = PR::gather(pe_rows, t, cb, pl.extra);
+      });
+      __syncthreads();
     }
-    static_for<0, 6>([&](auto TI) SRF_AI {
-      constexpr int t = decltype(TI)::value;
-      if (t < pl.comp_tiles[9]) {
-#pragma unroll
-        for (int cb = 0; cb < CB; ++cb) {
-          f32x4 v;
-#pragma unroll
-          for (int r = 0; r < 4; ++r) {
-            const int e = 16 * t + 4 * g + r;
-            float val = 0.f;
-            if (e < n_hi) val = pe_feature(x[cb][0], x[cb][1], x[cb][2], pl.trunk_pe + e);
-            else if (e < pl.extra) val = pe_feature(d[cb][0], d[cb][1], d[cb][2], e - n_hi);
-            v[r] = val;
-          }
-          pe_lds[wv][t][cb][lane] = to_act<P>(v);
-        }
-      }
-    });
     // view layer [feature, extras] -> 128 (ReLU), in place into h[0..7]; stores the feature
     // tiles and the extras
     if (pl.comp_tiles[9] == 2)
