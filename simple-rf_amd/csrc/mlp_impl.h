@@ -49,6 +49,8 @@ constexpr int kParams = SRF_MLP_NUM_PARAMS;
 constexpr int kMaskSlots = 9;      // ReLU masks of layers 0-7 and the view layer
 constexpr int kWaves = 4;          // waves per block
 constexpr int kMaxJobs = 16;
+// padded biases of the largest model: trunk 8 x 256, head 272, view 128, rgb 16
+constexpr int kMaxBiasFloats = 8 * 256 + 272 + 128 + 16;
 
 // ----------------------------------------------------------------------------- plan
 struct Plan {
@@ -57,6 +59,7 @@ struct Plan {
   int elem;   // bytes per stored value
   int out_rows[kLayers], out_tiles[kLayers], reg_tiles[kLayers], comp_feats[kLayers],
       comp_tiles[kLayers], bwd_tiles[kLayers], present[kLayers];
+  int bias_floats;  // padded biases of every present layer, contiguous from bias_off[0]
   size_t fwd_off[kLayers], bwd_off[kLayers], bias_off[kLayers], packed_bytes;
 };
 
@@ -117,6 +120,7 @@ inline bool make_plan(const srf_mlp_desc* d, Plan* p) {
     if (!p->present[l]) continue;
     p->bias_off[l] = off;
     off += (size_t)p->out_tiles[l] * 16 * sizeof(float);
+    p->bias_floats += p->out_tiles[l] * 16;
   }
   p->packed_bytes = round_up((int)off, 256);
   return true;
@@ -394,10 +398,29 @@ __host__ __device__ constexpr int units(int tiles) { return P == kF32 ? tiles : 
 // operations and leaves expcnt / lgkmcnt unconstrained.
 constexpr int vmcnt_only(int n) { return 0x0F70 | (n & 0xF) | ((n >> 4) << 14); }
 
+// DMA of weight chunk C of a product (kKC k-units of every one of its OUT output tiles, in the
+// stage layout mma_tiles reads) into buf, issued by the block's waves together.
+template <int P, int NT, int OUT, int TB, int C>
+__device__ __forceinline__ void stage_chunk(const uint8_t* __restrict__ A, int upr, int u0, uint8_t* buf) {
+  constexpr int NU = NT / (P == kF32 ? 1 : 2);
+  constexpr int kn = (NU - C * kKC) < kKC ? (NU - C * kKC) : kKC;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+#pragma unroll
+  for (int i = 0; i < (OUT * kn + kWaves - 1) / kWaves; ++i) {
+    const int f = w + kWaves * i;
+    if (f < OUT * kn) {
+      const int to = f / kn, kc = f - (f / kn) * kn;
+      const int frag = __builtin_amdgcn_readfirstlane((TB + to) * upr + u0 + C * kKC + kc);
+      const int dst = __builtin_amdgcn_readfirstlane((to * kKC + kc) * 1024);
+      lds_dma_1k(A + (size_t)frag * 1024, buf + dst);
+    }
+  }
+}
+
 // No deferred stores (see mma_tiles).
 struct NoSide {
   template <typename I>
-  SRF_AI void operator()(I) const {}
+  __device__ SRF_AI void operator()(I) const {}
 };
 
 // acc[OUT][CB] += W[TB .. TB+OUT-1, k-units u0 .. u0+NT/TPU-1] x In for all OUT output tiles at
@@ -422,31 +445,15 @@ __device__ __forceinline__ void mma_tiles(f32x4 (&acc)[OUT][Tr<P>::CB], const ui
   constexpr int NCH = (NU + kKC - 1) / kKC;
   constexpr int kHalf = ST * kKC * 1024;  // one buffer of the double-buffered stage (2 kHalf bytes)
   static_assert(OUT <= ST, "the stage buffer holds ST output tiles");
-  // wave index and every fragment address are wave-uniform: keep them in SGPRs
-  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
-  auto dma_chunk = [&](auto CI, uint8_t* buf) SRF_AI {
-    constexpr int c = decltype(CI)::value;
-    constexpr int kn = (NU - c * kKC) < kKC ? (NU - c * kKC) : kKC;
-#pragma unroll
-    for (int i = 0; i < (OUT * kn + kWaves - 1) / kWaves; ++i) {
-      const int f = w + kWaves * i;
-      if (f < OUT * kn) {
-        const int to = f / kn, kc = f - (f / kn) * kn;
-        const int frag = __builtin_amdgcn_readfirstlane((TB + to) * upr + u0 + c * kKC + kc);
-        const int dst = __builtin_amdgcn_readfirstlane((to * kKC + kc) * 1024);
-        lds_dma_1k(A + (size_t)frag * 1024, buf + dst);
-      }
-    }
-  };
-  dma_chunk(std::integral_constant<int, 0>{}, stage);
+  stage_chunk<P, NT, OUT, TB, 0>(A, upr, u0, stage);
   wait_vmem();
   __syncthreads();
   constexpr int kPer = (NSIDE + NCH - 1) / NCH;  // deferred-store items per chunk
   static_for<0, NCH>([&](auto CI) SRF_AI {
     constexpr int c = decltype(CI)::value;
     const uint8_t* cur = stage + (c & 1) * kHalf;
-    if constexpr (c + 1 < NCH) dma_chunk(std::integral_constant<int, c + 1>{}, stage + ((c + 1) & 1) * kHalf);
+    if constexpr (c + 1 < NCH) stage_chunk<P, NT, OUT, TB, c + 1>(A, upr, u0, stage + ((c + 1) & 1) * kHalf);
     constexpr int i0 = c * kPer < NSIDE ? c * kPer : NSIDE;
     constexpr int i1 = (c + 1) * kPer < NSIDE ? (c + 1) * kPer : NSIDE;
     if constexpr (i1 > i0) {
@@ -539,11 +546,11 @@ __device__ __forceinline__ void zero_acc(f32x4 (&acc)[OUT][Tr<P>::CB]) {
 // Accumulators start at the bias (rows TB.. of the layer's padded FP32 bias): the bias then
 // lives in the accumulator registers instead of being loaded in the epilogue.
 template <int P, int OUT, int TB>
-__device__ __forceinline__ void bias_acc(f32x4 (&acc)[OUT][Tr<P>::CB], const uint8_t* __restrict__ bias) {
+__device__ __forceinline__ void bias_acc(f32x4 (&acc)[OUT][Tr<P>::CB], const float* bias) {
   const int g = (threadIdx.x & 63) >> 4;
 #pragma unroll
   for (int to = 0; to < OUT; ++to) {
-    const f32x4 b = *reinterpret_cast<const f32x4*>(bias + (16 * (TB + to) + 4 * g) * 4);
+    const f32x4 b = *reinterpret_cast<const f32x4*>(bias + 16 * (TB + to) + 4 * g);
 #pragma unroll
     for (int cb = 0; cb < Tr<P>::CB; ++cb) acc[to][cb] = b;
   }
@@ -554,7 +561,7 @@ __device__ __forceinline__ void bias_acc(f32x4 (&acc)[OUT][Tr<P>::CB], const uin
 // tiles in place).  side / NSIDE: deferred stores issued during the product (mma_tiles).
 template <int P, int IN, int OUT, int TB, int NSIDE = 0, typename FIn, typename Epi, typename Side = NoSide>
 __device__ __forceinline__ void run_layer(const uint8_t* __restrict__ A, FIn&& fin, Epi&& epi, uint8_t* stage,
-                                          const uint8_t* __restrict__ bias = nullptr, Side&& side = Side{}) {
+                                          const float* bias = nullptr, Side&& side = Side{}) {
   f32x4 acc[OUT][Tr<P>::CB];
   if (bias) bias_acc<P, OUT, TB>(acc, bias);
   else zero_acc<P, OUT>(acc);
@@ -592,6 +599,9 @@ __device__ __forceinline__ void store_nt16(Rsrc r, uint32_t lane_off, uint32_t b
 }
 __device__ __forceinline__ void store_nt8(Rsrc r, uint32_t lane_off, uint32_t blk, u32x2 v) {
   __builtin_amdgcn_raw_buffer_store_b64(v, r, lane_off, blk, kBufNT);
+}
+__device__ __forceinline__ u32x2 load8(Rsrc r, uint32_t lane_off, uint32_t blk) {
+  return __builtin_amdgcn_raw_buffer_load_b64(r, lane_off, blk, 0);
 }
 
 // Byte offset, inside an array of 16-feature tiles, of tile t's 2 KB record block written by the
@@ -669,6 +679,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
   // layer's [PE_xyz[trunk_pe:], PE_dir]) as B-operand tiles: 6 tiles x CB column blocks x 64
   // lanes.  Each lane reads back only its own entries, so no barrier is needed.
   __shared__ __attribute__((aligned(16))) Act pe_lds[kWaves][6][CB][64];
+  // every layer's padded FP32 bias, copied once per block: the accumulators start from LDS
+  // instead of waiting on global loads at each layer boundary
+  __shared__ __attribute__((aligned(16))) float bias_lds[kMaxBiasFloats];
   const int lane = threadIdx.x & 63, g = lane >> 4, j = lane & 15;
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform: SGPR addresses
   const int wave = blockIdx.x * kWaves + wv;
@@ -713,7 +726,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
     for (int cb = 0; cb < CB; ++cb)
       store_nt8(m, (uint32_t)lane * 8u, (uint32_t)(wave * CB + cb) * 512u, u32x2{mbits[cb][0], mbits[cb][1]});
   };
-  auto bias_of = [&](int l) SRF_AI { return a.packed + pl.bias_off[l]; };
+  {
+    const float* src = reinterpret_cast<const float*>(a.packed + pl.bias_off[0]);
+    for (int i = threadIdx.x; i < pl.bias_floats; i += 64 * kWaves) bias_lds[i] = src[i];
+  }  // (visible after pe_trunk's block barrier)
+  auto bias_of = [&](int l) SRF_AI { return bias_lds + (pl.bias_off[l] - pl.bias_off[0]) / 4; };
   // ReLU epilogue: tile T into h[T] (in place) and its mask bits.  Nothing is stored here: the
   // training copies of h and mbits are written during the next layer's product (side_of).
   auto relu_epi = [&](int T, f32x4(&acc)[CB]) SRF_AI {
@@ -926,14 +943,30 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
     };
   };
   using I0 = std::integral_constant<int, 0>;
-  auto load_mask = [&](int slot) SRF_AI {
-    const uint64_t* m = reinterpret_cast<const uint64_t*>(a.ws + a.w.mask + (size_t)slot * ldP * 32);
+  // The ReLU masks of one layer (as the forward stored them), for the epilogue of the product in
+  // which they are loaded: a deferred item of that product, issued after its first weight chunk
+  // has landed and completing behind its MFMAs (a load before the product would hold up the wait
+  // for that chunk).
+  auto mask_side = [&](int slot) SRF_AI {
+    const Rsrc r = uniform_rsrc(a.ws + a.w.mask + (size_t)slot * ldP * 32);
+    return [&, r](auto) SRF_AI {
 #pragma unroll
-    for (int cb = 0; cb < CB; ++cb) {
-      const uint64_t v = m[((size_t)wave * CB + cb) * 64 + lane];
-      mb[cb][0] = (uint32_t)v;
-      mb[cb][1] = (uint32_t)(v >> 32);
-    }
+      for (int cb = 0; cb < CB; ++cb) {
+        const u32x2 v = load8(r, (uint32_t)lane * 8u, (uint32_t)(wave * CB + cb) * 512u);
+        mb[cb][0] = v.x;
+        mb[cb][1] = v.y;
+      }
+    };
+  };
+  // item 0: the masks of slot `slot`; items 1..: dz_side(l) items
+  auto mask_dz_side = [&](int slot, int l) SRF_AI {
+    const auto ms = mask_side(slot);
+    const auto ds = dz_side(l, I0{});
+    return [&, ms, ds](auto II) SRF_AI {
+      constexpr int i = decltype(II)::value;
+      if constexpr (i == 0) ms(II);
+      else ds(std::integral_constant<int, i - 1>{});
+    };
   };
   // epilogue: ReLU'-masked gradient of layer l into z[T] (dZ_l, stored during the next product)
   auto masked_epi = [&](int T, f32x4(&acc)[CB]) SRF_AI {
@@ -959,8 +992,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
       }
     }
     // dV = W10^T dZ10, masked by the view-layer ReLU -> dZ9 (z[0..7]); stores dZ10
-    load_mask(8);
-    run_layer<P, 2, 8, 0, 1>(a.packed + pl.bwd_off[10], fin_z, masked_epi, stage, nullptr, dz_side(10, I0{}));
+    run_layer<P, 2, 8, 0, 2>(a.packed + pl.bwd_off[10], fin_z, masked_epi, stage, nullptr, mask_dz_side(8, 10));
     // dF = W9^T dZ9 (feature part, identity activation) -> dZ8 rows 0..255 (z[0..15]); stores dZ9
     run_layer<P, 8, 16, 0, 8>(a.packed + pl.bwd_off[9], fin_z, [&](int T, f32x4(&acc)[CB]) SRF_AI {
 #pragma unroll
@@ -979,8 +1011,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
     // live, deferring it spills registers
 #pragma unroll
     for (int T = 0; T < 17; ++T) store_dz(8, T, z[T]);
-    load_mask(7);
-    run_layer<P, 18, 16, 0>(a.packed + pl.bwd_off[8], fin_z, masked_epi, stage);
+    run_layer<P, 18, 16, 0, 1>(a.packed + pl.bwd_off[8], fin_z, masked_epi, stage, nullptr, mask_side(7));
   } else {
     {
 #pragma unroll
@@ -991,13 +1022,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
         z[1][cb] = zero_act<P>();
       }
     }
-    load_mask(7);
-    run_layer<P, 2, 16, 0, 1>(a.packed + pl.bwd_off[8], fin_z, masked_epi, stage, nullptr, dz_side(8, I0{}));
+    run_layer<P, 2, 16, 0, 2>(a.packed + pl.bwd_off[8], fin_z, masked_epi, stage, nullptr, mask_dz_side(7, 8));
   }
   // trunk: z holds dZ_l; dH_{l-1} = W_l^T dZ_l, masked -> dZ_{l-1}, in place (storing dZ_l)
   for (int l = 7; l >= 1; --l) {
-    load_mask(l - 1);
-    run_layer<P, 16, 16, 0, 16>(a.packed + pl.bwd_off[l], fin_z, masked_epi, stage, nullptr, dz_side(l, I0{}));
+    run_layer<P, 16, 16, 0, 17>(a.packed + pl.bwd_off[l], fin_z, masked_epi, stage, nullptr, mask_dz_side(l - 1, l));
   }
   // dZ0: no product follows
 #pragma unroll
