@@ -552,6 +552,11 @@ def hbm_leg(dev, tmodel, out, reps=10):
         gbs = byts / sec / 1e9
         table[k] = {'us': round(sec * 1e6, 1), 'algorithmic_bytes': int(byts), 'GB/s': round(gbs, 1),
                     'frac_hbm': round(gbs / PEAK_HBM_GBS, 4), 'shape': what}
+        if k.startswith('vm_'):
+            # every tap counted: the factors (69 MB here) largely stay in the L2 / 256 MB MALL, so
+            # the rate can exceed the HBM peak; rocprof's FETCH / WRITE give the HBM bytes
+            table[k]['bytes_kind'] = 'factor taps (L2 / MALL hits counted)'
+
     return table
 
 
