@@ -668,8 +668,9 @@ extern "C" int srf_color_backward(const srf_color_desc* desc, const void* packed
     jb.dw = dw;
     jb.ld_dw = ld;
     jb.db = db;
-    jb.mblocks = (M + 127) / 128;  // 128-row tiles (dw_f32_narrow): every job has at most 128 rows
-    jb.nblocks = (N + mlp::kDwTile - 1) / mlp::kDwTile;
+    // 128 x 160 tiles (dw_f32_narrow): one tile covers every job at the default sizes
+    jb.mblocks = (M + mlp::kDwNarrowRows - 1) / mlp::kDwNarrowRows;
+    jb.nblocks = (N + mlp::kDwNarrowCols - 1) / mlp::kDwNarrowCols;
   };
   job(w.dz1, kHid, w.x1, g.D1p, kHid, g.D1, d_w1, g.D1, d_b1);
   job(w.dz2, kHid, w.h1, kHid, kHid, kHid, d_w2, kHid, d_b2);

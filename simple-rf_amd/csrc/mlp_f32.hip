@@ -15,7 +15,7 @@ int bwd_f32(const BwdArgs& a, bool vd, hipStream_t s) {
 
 int dw_f32(const DwArgs& d, hipStream_t s) { return launch_dw<kF32>(d, s); }
 
-int dw_f32_narrow(const DwArgs& d, hipStream_t s) { return launch_dw<kF32, 2>(d, s); }
+int dw_f32_narrow(const DwArgs& d, hipStream_t s) { return launch_dw<kF32, 1, 10>(d, s); }
 
 }  // namespace mlp
 }  // namespace srf

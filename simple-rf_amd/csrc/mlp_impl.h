@@ -417,6 +417,28 @@ __device__ __forceinline__ void stage_chunk(const uint8_t* __restrict__ A, int u
   }
 }
 
+// s_waitcnt vmcnt(n) for a wave-uniform n in 0..15 (the immediate has to be a constant).
+__device__ __forceinline__ void wait_vmem_at_most(int n) {
+  switch (n) {
+    case 0: __builtin_amdgcn_s_waitcnt(vmcnt_only(0)); break;
+    case 1: __builtin_amdgcn_s_waitcnt(vmcnt_only(1)); break;
+    case 2: __builtin_amdgcn_s_waitcnt(vmcnt_only(2)); break;
+    case 3: __builtin_amdgcn_s_waitcnt(vmcnt_only(3)); break;
+    case 4: __builtin_amdgcn_s_waitcnt(vmcnt_only(4)); break;
+    case 5: __builtin_amdgcn_s_waitcnt(vmcnt_only(5)); break;
+    case 6: __builtin_amdgcn_s_waitcnt(vmcnt_only(6)); break;
+    case 7: __builtin_amdgcn_s_waitcnt(vmcnt_only(7)); break;
+    case 8: __builtin_amdgcn_s_waitcnt(vmcnt_only(8)); break;
+    case 9: __builtin_amdgcn_s_waitcnt(vmcnt_only(9)); break;
+    case 10: __builtin_amdgcn_s_waitcnt(vmcnt_only(10)); break;
+    case 11: __builtin_amdgcn_s_waitcnt(vmcnt_only(11)); break;
+    case 12: __builtin_amdgcn_s_waitcnt(vmcnt_only(12)); break;
+    case 13: __builtin_amdgcn_s_waitcnt(vmcnt_only(13)); break;
+    case 14: __builtin_amdgcn_s_waitcnt(vmcnt_only(14)); break;
+    default: __builtin_amdgcn_s_waitcnt(vmcnt_only(15)); break;
+  }
+}
+
 // No deferred stores (see mma_tiles).
 struct NoSide {
   template <typename I>
@@ -1067,15 +1089,21 @@ constexpr int kDwWaves = 8;
 constexpr int kDwFrags = 32;              // 16 dZ + 16 X fragments per k-unit
 constexpr int kDwSlot = kDwFrags * 1024;  // one k-unit
 constexpr int kDwSlots = 5;               // ring depth (prefetch distance kDwSlots - 1): 160 KB of LDS
+constexpr int kDwNarrowRows = 128, kDwNarrowCols = 160;  // tile of the RM = 1, NC = 10 wave tiling
 
 
-// RM row tiles per wave: RM = 4 covers 256-row tiles (the NeRF layers), RM = 2 covers 128-row
-// tiles, so that a job of at most 128 rows (the TensoRF colour network) keeps all eight waves
-// busy instead of two.  The DMA is the same for both (it loads the 16 dZ row tiles from m0).
-template <int P, int RM = 4>
+// Wave tiling: RM row tiles x NC column tiles per wave.  NC = 8: the waves split the tile into
+// four row groups x two column halves, RM = 4 giving 256 x 256 tiles (the NeRF layers) and RM = 2
+// 128 x 256 tiles.  RM = 1, NC = 10: wave w takes row tile w and column tiles 0..9 of a 128 x 160
+// tile, which keeps all eight waves (and all four SIMDs) busy on jobs of at most 128 rows and 160
+// columns (the TensoRF colour network), where the half-tile split left half of them idle.  The
+// DMA is the same for all (16 dZ and 16 X row tiles from m0 / n0).
+template <int P, int RM = 4, int NC = 8>
 __global__ __launch_bounds__(64 * kDwWaves) void mlp_dw_kernel(DwArgs a) {
-  static_assert(RM == 4 || RM == 2, "256- or 128-row tiles");
-  constexpr int kTM = 64 * RM;  // tile rows
+  static_assert((NC == 8 && (RM == 4 || RM == 2)) || (RM == 1 && NC == 10), "supported wave tilings");
+  constexpr bool kRowWaves = NC != 8;
+  constexpr int kTM = kRowWaves ? 16 * kDwWaves : 64 * RM;  // tile rows
+  constexpr int kTN = kRowWaves ? 16 * NC : kDwTile;         // tile columns
   __shared__ __attribute__((aligned(16))) uint8_t ring[kDwSlots * kDwSlot];
   const int lane = threadIdx.x & 63, g = lane >> 4, j = lane & 15;
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -1087,31 +1115,41 @@ __global__ __launch_bounds__(64 * kDwWaves) void mlp_dw_kernel(DwArgs a) {
   const int local = item - jb.work_begin;
   const int nblk = jb.mblocks * jb.nblocks;
   const int tile = local % nblk, chunk = local / nblk;
-  const int m0 = kTM * (tile % jb.mblocks), n0 = kDwTile * (tile / jb.mblocks);
+  const int m0 = kTM * (tile % jb.mblocks), n0 = kTN * (tile / jb.mblocks);
   const int pb = chunk * a.chunk;
   const int pe = min(pb + a.chunk, a.cols);
   const size_t ldP = a.ldP;
   constexpr int ELEM = P == kF32 ? 4 : 2;
   constexpr int KPTS = P == kF32 ? 16 : 32;  // points per k-unit
-  const int qm = RM * (w >> 1), qn = 8 * (w & 1);
-  bool mv[RM], nv[8];
+  const int qm = kRowWaves ? w : RM * (w >> 1), qn = kRowWaves ? 0 : 8 * (w & 1);
+  bool mv[RM], nv[NC];
 #pragma unroll
   for (int i = 0; i < RM; ++i) mv[i] = m0 + 16 * (qm + i) < jb.dz_rows;
 #pragma unroll
-  for (int k = 0; k < 8; ++k) nv[k] = n0 + 16 * (qn + k) < jb.x_rows;
+  for (int k = 0; k < NC; ++k) nv[k] = n0 + 16 * (qn + k) < jb.x_rows;
   bool any_m = false, any_n = false;
 #pragma unroll
   for (int i = 0; i < RM; ++i) any_m |= mv[i];
 #pragma unroll
-  for (int k = 0; k < 8; ++k) any_n |= nv[k];
+  for (int k = 0; k < NC; ++k) any_n |= nv[k];
   // DMA of the 32 fragments of the k-unit at point p into a ring slot: fragment f < 16 is dZ row
   // tile f, f >= 16 is X row tile f - 16; wave w copies f = w, w + 8, w + 16, w + 24.  Every wave
   // issues exactly four loads per k-unit (a tile beyond the job's rows re-reads its first row
   // tile into the unused slot), so the vmcnt waits below count whole k-units.
+  // Narrow tiling: only the fragments the tile uses are copied, nA dZ row tiles and nB X row
+  // tiles (nA <= 8, nB <= 10), fragment f of the list going to wave f % 8; each wave waits for
+  // its own count (dma_n per k-unit).
+  const int nA = kRowWaves ? min(8, (jb.dz_rows - m0 + 15) / 16) : 16;
+  const int nB = kRowWaves ? min(NC, (jb.x_rows - n0 + 15) / 16) : 16;
+  const int dma_n = kRowWaves ? (nA + nB - w + kDwWaves - 1) / kDwWaves : 4;
   auto dma_unit = [&](int p, uint8_t* buf) SRF_AI {
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const int f = w + kDwWaves * q;
+    for (int q = 0; q < (kRowWaves ? 3 : 4); ++q) {
+      int f = w + kDwWaves * q;
+      if (kRowWaves) {
+        if (f >= nA + nB) break;
+        f = f < nA ? f : 16 + (f - nA);  // slot: dZ tile f or X tile 16 + k, as the MFMA loop reads
+      }
       const bool is_a = f < 16;
       const int rt = is_a ? f : f - 16;
       int row0 = (is_a ? m0 : n0) + 16 * rt;
@@ -1133,13 +1171,13 @@ __global__ __launch_bounds__(64 * kDwWaves) void mlp_dw_kernel(DwArgs a) {
                                        (__attribute__((address_space(3))) void*)(buf + f * 1024), 16, 0, 0);
     }
   };
-  f32x4 acc[RM][8];
+  f32x4 acc[RM][NC];
   float rs[RM];
 #pragma unroll
   for (int i = 0; i < RM; ++i) {
     rs[i] = 0.f;
 #pragma unroll
-    for (int k = 0; k < 8; ++k) acc[i][k] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int k = 0; k < NC; ++k) acc[i][k] = f32x4{0.f, 0.f, 0.f, 0.f};
   }
   const bool do_db = jb.db != nullptr && n0 == 0 && qn == 0;
   const bool busy = any_m && (any_n || do_db);
@@ -1153,10 +1191,14 @@ __global__ __launch_bounds__(64 * kDwWaves) void mlp_dw_kernel(DwArgs a) {
     // k-unit u has landed once at most (issued after it) k-units' loads are outstanding
     const int after = min(kDwSlots - 2, n_units - 1 - u);
     static_assert(kDwSlots <= 5, "waits below cover up to three k-units in flight after u");
-    if (after >= 3) __builtin_amdgcn_s_waitcnt(vmcnt_only(12));
-    else if (after == 2) __builtin_amdgcn_s_waitcnt(vmcnt_only(8));
-    else if (after == 1) __builtin_amdgcn_s_waitcnt(vmcnt_only(4));
-    else __builtin_amdgcn_s_waitcnt(vmcnt_only(0));
+    if constexpr (kRowWaves) {
+      wait_vmem_at_most(after * dma_n);  // at most 3 x 3
+    } else {
+      if (after >= 3) __builtin_amdgcn_s_waitcnt(vmcnt_only(12));
+      else if (after == 2) __builtin_amdgcn_s_waitcnt(vmcnt_only(8));
+      else if (after == 1) __builtin_amdgcn_s_waitcnt(vmcnt_only(4));
+      else __builtin_amdgcn_s_waitcnt(vmcnt_only(0));
+    }
     // After the barrier every wave's part of k-unit u is in LDS (each wave waited for its own
     // copies above) and slot (u - 1) % kDwSlots is free (each wave consumed its LDS reads of it
     // in the MFMAs of iteration u - 1).  A plain s_barrier with compiler-only memory barriers:
@@ -1170,31 +1212,31 @@ __global__ __launch_bounds__(64 * kDwWaves) void mlp_dw_kernel(DwArgs a) {
     if (busy) {
       const uint8_t* fr = ring + (u % kDwSlots) * kDwSlot;
       if constexpr (P == kF32) {
-        f32x4 av[RM], bv[8];
+        f32x4 av[RM], bv[NC];
 #pragma unroll
         for (int i = 0; i < RM; ++i) av[i] = *reinterpret_cast<const f32x4*>(fr + (qm + i) * 1024 + lane * 16);
 #pragma unroll
-        for (int k = 0; k < 8; ++k) bv[k] = *reinterpret_cast<const f32x4*>(fr + (16 + qn + k) * 1024 + lane * 16);
+        for (int k = 0; k < NC; ++k) bv[k] = *reinterpret_cast<const f32x4*>(fr + (16 + qn + k) * 1024 + lane * 16);
 #pragma unroll
         for (int q = 0; q < 4; ++q)
 #pragma unroll
           for (int i = 0; i < RM; ++i)
 #pragma unroll
-            for (int k = 0; k < 8; ++k)
+            for (int k = 0; k < NC; ++k)
               if (mv[i] && nv[k]) acc[i][k] = mfma32(av[i][q], bv[k][q], acc[i][k]);
         if (do_db)
 #pragma unroll
           for (int i = 0; i < RM; ++i) rs[i] += (av[i][0] + av[i][1]) + (av[i][2] + av[i][3]);
       } else {
-        u32x4 av[RM], bv[8];
+        u32x4 av[RM], bv[NC];
 #pragma unroll
         for (int i = 0; i < RM; ++i) av[i] = *reinterpret_cast<const u32x4*>(fr + (qm + i) * 1024 + lane * 16);
 #pragma unroll
-        for (int k = 0; k < 8; ++k) bv[k] = *reinterpret_cast<const u32x4*>(fr + (16 + qn + k) * 1024 + lane * 16);
+        for (int k = 0; k < NC; ++k) bv[k] = *reinterpret_cast<const u32x4*>(fr + (16 + qn + k) * 1024 + lane * 16);
 #pragma unroll
         for (int i = 0; i < RM; ++i)
 #pragma unroll
-          for (int k = 0; k < 8; ++k)
+          for (int k = 0; k < NC; ++k)
             if (mv[i] && nv[k]) acc[i][k] = mfma16<P>(av[i], bv[k], acc[i][k]);
         if (do_db)
 #pragma unroll
@@ -1208,7 +1250,7 @@ __global__ __launch_bounds__(64 * kDwWaves) void mlp_dw_kernel(DwArgs a) {
   for (int i = 0; i < RM; ++i) {
     if (!mv[i]) continue;
 #pragma unroll
-    for (int k = 0; k < 8; ++k) {
+    for (int k = 0; k < NC; ++k) {
       if (!nv[k]) continue;
       const int c = n0 + 16 * (qn + k) + j;
 #pragma unroll
@@ -1246,9 +1288,9 @@ int launch_bwd(const BwdArgs& a, hipStream_t s) {
   return launch_status();
 }
 
-template <int P, int RM = 4>
+template <int P, int RM = 4, int NC = 8>
 int launch_dw(const DwArgs& d, hipStream_t s) {
-  hipLaunchKernelGGL((mlp_dw_kernel<P, RM>), dim3(d.total_work), dim3(64 * kDwWaves), 0, s, d);
+  hipLaunchKernelGGL((mlp_dw_kernel<P, RM, NC>), dim3(d.total_work), dim3(64 * kDwWaves), 0, s, d);
   return launch_status();
 }
 
@@ -1264,7 +1306,7 @@ int bwd_f32(const BwdArgs& a, bool vd, hipStream_t s);
 int bwd_bf16(const BwdArgs& a, bool vd, hipStream_t s);
 int dw_f32(const DwArgs& d, hipStream_t s);
 int dw_bf16(const DwArgs& d, hipStream_t s);
-int dw_f32_narrow(const DwArgs& d, hipStream_t s);  // 128-row tiles (jobs of at most 128 rows)
+int dw_f32_narrow(const DwArgs& d, hipStream_t s);  // 128 x 160 tiles (kDwNarrowRows x kDwNarrowCols)
 
 }  // namespace mlp
 }  // namespace srf
