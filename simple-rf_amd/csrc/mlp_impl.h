@@ -601,6 +601,24 @@ __device__ __forceinline__ float swap_adjacent(float v) {
   return __builtin_bit_cast(float, swap_adjacent(__builtin_bit_cast(uint32_t, v)));
 }
 
+// Lane index recomputed at the point of use (v_mbcnt; volatile so that it is neither hoisted nor
+// kept live across the layer loop, where register pressure would spill it to scratch and its
+// reload's vmcnt wait would drain the deferred stores).
+__device__ __forceinline__ uint32_t lane_here() {
+  uint32_t r;
+  asm volatile("v_mbcnt_lo_u32_b32 %0, -1, 0\n\tv_mbcnt_hi_u32_b32 %0, -1, %0" : "=v"(r));
+  return r;
+}
+// odd lanes ? if_odd : if_even, with the lane parity as a scalar lane mask (no per-lane flag).
+__device__ __forceinline__ uint32_t sel_odd(uint32_t if_odd, uint32_t if_even) {
+  uint32_t r;
+  asm("v_cndmask_b32_e64 %0, %1, %2, %3" : "=v"(r) : "v"(if_even), "v"(if_odd), "s"(0xaaaaaaaaaaaaaaaaull));
+  return r;
+}
+__device__ __forceinline__ float sel_odd(float if_odd, float if_even) {
+  return __builtin_bit_cast(float, sel_odd(__builtin_bit_cast(uint32_t, if_odd), __builtin_bit_cast(uint32_t, if_even)));
+}
+
 // Training stores go through raw buffer descriptors: the array base (the same in every lane) in
 // scalar registers, the wave's block offset as the 32-bit scalar offset, and only the lane's byte
 // offset in a vector register.  Per-lane 64-bit addresses for every tile would otherwise be kept
@@ -647,26 +665,24 @@ __device__ __forceinline__ uint32_t tile_block(int t, int ldP, int elem, int p0)
 template <int P>
 __device__ __forceinline__ void store_tile(Rsrc dst, uint32_t blk, const typename Tr<P>::Act (&v)[Tr<P>::CB]) {
   constexpr int CB = Tr<P>::CB;
-  const int lane = threadIdx.x & 63;
-  const bool odd = lane & 1;
-  const uint32_t rec = (uint32_t)lane * 32u;
+  const uint32_t rec = lane_here() * 32u;
   if constexpr (P == kF32) {
     // v[cb] = features 4g + 0..3 at point 16 cb + j; even lanes keep r0, r1 and take the
     // partner's; odd lanes keep r2, r3
     float rc[CB][2];
 #pragma unroll
     for (int cb = 0; cb < CB; ++cb) {
-      rc[cb][0] = swap_adjacent(odd ? v[cb][0] : v[cb][2]);
-      rc[cb][1] = swap_adjacent(odd ? v[cb][1] : v[cb][3]);
+      rc[cb][0] = swap_adjacent(sel_odd(v[cb][0], v[cb][2]));
+      rc[cb][1] = swap_adjacent(sel_odd(v[cb][1], v[cb][3]));
     }
     f32x4 c0, c1;
 #pragma unroll
     for (int cb = 0; cb < CB; ++cb) {
-      const float own0 = odd ? v[cb][2] : v[cb][0], own1 = odd ? v[cb][3] : v[cb][1];
-      c0[2 * cb + 0] = odd ? rc[cb][0] : own0;
-      c0[2 * cb + 1] = odd ? own0 : rc[cb][0];
-      c1[2 * cb + 0] = odd ? rc[cb][1] : own1;
-      c1[2 * cb + 1] = odd ? own1 : rc[cb][1];
+      const float own0 = sel_odd(v[cb][2], v[cb][0]), own1 = sel_odd(v[cb][3], v[cb][1]);
+      c0[2 * cb + 0] = sel_odd(rc[cb][0], own0);
+      c0[2 * cb + 1] = sel_odd(own0, rc[cb][0]);
+      c1[2 * cb + 0] = sel_odd(rc[cb][1], own1);
+      c1[2 * cb + 1] = sel_odd(own1, rc[cb][1]);
     }
     store_nt16(dst, rec, blk, __builtin_bit_cast(u32x4, c0));
     store_nt16(dst, rec, blk + 16, __builtin_bit_cast(u32x4, c1));
@@ -675,9 +691,9 @@ __device__ __forceinline__ void store_tile(Rsrc dst, uint32_t blk, const typenam
     u32x4 c0, c1;
 #pragma unroll
     for (int cb = 0; cb < CB; ++cb) {
-      const uint32_t recv = swap_adjacent(odd ? v[cb][0] : v[cb][1]);
-      const uint32_t own = odd ? v[cb][1] : v[cb][0];
-      const uint32_t first = odd ? recv : own, second = odd ? own : recv;  // points 2m, 2m + 1
+      const uint32_t recv = swap_adjacent(sel_odd(v[cb][0], v[cb][1]));
+      const uint32_t own = sel_odd(v[cb][1], v[cb][0]);
+      const uint32_t first = sel_odd(recv, own), second = sel_odd(own, recv);  // points 2m, 2m + 1
       c0[cb] = (first & 0xffffu) | (second << 16);
       c1[cb] = (first >> 16) | (second & 0xffff0000u);
     }
@@ -746,7 +762,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
   auto mask_store = [&](Rsrc m) SRF_AI {
 #pragma unroll
     for (int cb = 0; cb < CB; ++cb)
-      store_nt8(m, (uint32_t)lane * 8u, (uint32_t)(wave * CB + cb) * 512u, u32x2{mbits[cb][0], mbits[cb][1]});
+      store_nt8(m, lane_here() * 8u, (uint32_t)(wave * CB + cb) * 512u, u32x2{mbits[cb][0], mbits[cb][1]});
   };
   {
     const float* src = reinterpret_cast<const float*>(a.packed + pl.bias_off[0]);
