@@ -323,7 +323,8 @@ class SimpleNeRFHip(torch.nn.Module):
         merged = {}
         for key in data_chunks:
             if isinstance(data_chunks[key][0], torch.Tensor):
-                merged[key] = torch.cat(data_chunks[key], dim=0)
+                chunks = data_chunks[key]
+                merged[key] = chunks[0] if len(chunks) == 1 else torch.cat(chunks, dim=0)
             else:
                 raise NotImplementedError
         return merged

@@ -9,7 +9,7 @@ from dataclasses import dataclass
 import torch
 
 from . import _lib
-from ._checks import dev_tensor
+from ._checks import dev_tensor, zero_grads
 
 
 class _Desc(C.Structure):
@@ -73,7 +73,7 @@ class _ColorFunction(torch.autograd.Function):
         d = spec.desc()
         n = rgb.shape[0]
         d_app = torch.empty((n, spec.n_features), dtype=torch.float32, device=rgb.device)
-        grads = [torch.zeros_like(t) for t in (basis, w1, b1, w2, b2, w3, b3)]
+        grads = zero_grads([t.shape for t in (basis, w1, b1, w2, b2, w3, b3)], basis.device)
         _lib.call('srf_color_backward', C.byref(d), _lib.ptr(packed), n, _lib.ptr(rgb), _lib.ptr(g_rgb.contiguous()),
                   _lib.ptr(ws), _lib.ptr(d_app), *[_lib.ptr(g) for g in grads], _lib.stream_ptr(rgb.device))
         return (None, None, d_app, None, None, None, *grads)

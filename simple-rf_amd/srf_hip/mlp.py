@@ -14,7 +14,7 @@ from dataclasses import dataclass
 import torch
 
 from . import _lib
-from ._checks import dev_tensor
+from ._checks import dev_tensor, zero_grads
 
 PREC = {'fp32': 0, 'float32': 0, 'bf16': 1, 'bfloat16': 1, 'fp16': 2, 'float16': 2}
 
@@ -137,7 +137,7 @@ class _MLPFunction(torch.autograd.Function):
         n = sigma.shape[0]
         d_sigma = torch.zeros_like(sigma) if d_sigma is None else d_sigma.contiguous()
         d_rgb = torch.zeros_like(rgb) if d_rgb is None else d_rgb.contiguous()
-        grads = [None if s is None else torch.zeros(s, dtype=torch.float32, device=dev) for s in ctx.shapes]
+        grads = zero_grads(ctx.shapes, dev)  # srf_mlp_weight_grads accumulates
         d = spec.desc()
         stream = _lib.stream_ptr(dev)
         _lib.call('srf_mlp_backward_data', C.byref(d), packed.data_ptr(), n, sigma.data_ptr(), rgb.data_ptr(),
