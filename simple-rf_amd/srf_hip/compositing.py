@@ -34,6 +34,9 @@ class _Composite(torch.autograd.Function):
                   _lib.stream_ptr(dev))
         ctx.save_for_backward(sigma, rgb, z, rays_o, rays_d, rays_d_ndc, ray_offsets, acc, depth, depth_ndc)
         ctx.cfg = (n_rays, n_samples, ndc, white_bkgd, distance_scale)
+        # outputs the caller does not differentiate reach the backward as None (the kernel reads a
+        # null upstream gradient as zero) instead of as zero-filled tensors, one fill per output
+        ctx.set_materialize_grads(False)
         if rgb is None:
             ctx.mark_non_differentiable(rgb_map)
         if not ndc:
