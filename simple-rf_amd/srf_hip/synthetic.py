@@ -74,17 +74,19 @@ def train_loss(out: dict, batch: dict):
     """Synthetic objective of the benchmark: photometric MSE on the main coarse/fine and both
     augmented coarse renders (MSE14), 0.1 x sparse-depth MSE on metric depth (SparseDepthMSE14),
     and 0.1 x a plain MSE between augmented and main coarse depths standing in for the
-    reprojection-gated AugmentationsDepthLoss11."""
+    reprojection-gated AugmentationsDepthLoss11.  Each family of terms is evaluated as one stacked
+    expression (the sum of the per-term means), a handful of launches instead of several per term."""
     t = batch['target_rgb']
-    loss = 0.
-    for k in ('rgb_coarse', 'rgb_fine', 'points_augmentation_rgb_coarse', 'views_augmentation_rgb_coarse'):
-        if k in out:
-            loss = loss + torch.mean((out[k] - t) ** 2)
+    rgbs = [out[k] for k in ('rgb_coarse', 'rgb_fine', 'points_augmentation_rgb_coarse',
+                             'views_augmentation_rgb_coarse') if k in out]
+    loss = ((torch.stack(rgbs) - t) ** 2).mean(dim=(1, 2)).sum()
     m = batch['sd_mask']
-    for k in ('depth_coarse', 'points_augmentation_depth_coarse', 'views_augmentation_depth_coarse'):
-        if k in out:
-            loss = loss + 0.1 * torch.mean(((out[k] - batch['sd_depth']) ** 2) * m) * (m.numel() / m.sum().clamp_min(1))
-    for k in ('points_augmentation_depth_coarse', 'views_augmentation_depth_coarse'):
-        if k in out:
-            loss = loss + 0.1 * torch.mean((out[k] - out['depth_coarse'].detach()) ** 2)
+    depths = [out[k] for k in ('depth_coarse', 'points_augmentation_depth_coarse',
+                               'views_augmentation_depth_coarse') if k in out]
+    if depths:
+        scale = 0.1 * (m.numel() / m.sum().clamp_min(1))
+        loss = loss + scale * (((torch.stack(depths) - batch['sd_depth']) ** 2) * m).mean(dim=1).sum()
+    aug = [out[k] for k in ('points_augmentation_depth_coarse', 'views_augmentation_depth_coarse') if k in out]
+    if aug:
+        loss = loss + 0.1 * ((torch.stack(aug) - out['depth_coarse'].detach()) ** 2).mean(dim=1).sum()
     return loss
