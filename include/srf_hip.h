@@ -42,6 +42,20 @@ const char* srf_status_string(int status);
 
 /* ------------------------------------------------------------------ sampling (a) */
 
+/* Camera rays of n pixels in one pass.  Replaces get_rays_tr (nerf_synthetic=False),
+ * get_view_dirs_tr and get_ndc_rays_tr (src/utils/CommonUtils04.py:73-95, :147-149, :120-138)
+ * as called by SimpleNeRF.render_rays (src/models/SimpleNeRF17.py:159-199).
+ *   pixel_id [n,3] int32 (frame, x, y); intrinsics [n_frames,3,3] and camera-to-world extrinsics
+ *   [n_frames,4,4] per frame (FP32, row-major); mip_nerf: pixel centres at x + 0.5, y + 0.5.
+ * Outputs rays_o, rays_d, view_dirs [n,3]; with ndc, rays_o_ndc / rays_d_ndc [n,3] (resolution
+ * height x width and near as get_ndc_rays_tr; NULL otherwise).  A frame index outside
+ * [0, n_frames) gives NaN rays.  No backward: a model that learns its camera differentiates the
+ * PyTorch restatement. */
+int srf_rays(int n, const int32_t* pixel_id, int n_frames, const float* intrinsics,
+             const float* extrinsics, int mip_nerf, int ndc, int height, int width, float near,
+             float* rays_o, float* rays_d, float* view_dirs, float* rays_o_ndc, float* rays_d_ndc,
+             void* stream);
+
 /* Coarse depths z[n_rays, n_samples].
  * Replaces SimpleNeRF.get_z_vals_coarse (src/models/SimpleNeRF17.py:330-358) and the NDC
  * branch of SimpleTensoRF.get_z_vals_coarse (src/models/SimpleTensoRF09.py:364-381).

@@ -117,7 +117,8 @@ class SimpleNeRFHip(torch.nn.Module):
         input_batch['intrinsics'] = intrinsics
         k_cams = torch.arange(self.intrinsics_learner.initial_intrinsics.shape[0], device=image_id.device,
                               dtype=image_id.dtype)
-        input_batch['intrinsics_inv'] = torch.linalg.inv_ex(self.intrinsics_learner(k_cams))[0][image_id]
+        # per-frame cameras (a tuple, so that batchify_rays does not slice them as per-ray data)
+        input_batch['camera_frames'] = (self.intrinsics_learner(k_cams), cam_extrinsics)
         input_batch['extrinsics'] = extrinsics
         if mode == 'camera_params_only':
             out = {}
@@ -155,18 +156,32 @@ class SimpleNeRFHip(torch.nn.Module):
         ret = {}
         pixel_id = input_dict['pixel_id']
         intrinsics, extrinsics = input_dict['intrinsics'], input_dict['extrinsics']
-        rays_o, rays_d = geometry.get_rays_tr(pixel_id, intrinsics, extrinsics, mip_nerf_used=False,
-                                              intrinsic_inv=input_dict.get('intrinsics_inv'))
+        frames = input_dict.get('camera_frames')
+        fused = frames is not None and mode != 'static_camera' and \
+            not (torch.is_grad_enabled() and (frames[0].requires_grad or frames[1].requires_grad))
+        rays_o_ndc = rays_d_ndc = fused_view_dirs = None
+        if fused:
+            # rays, view directions and the NDC warp in one kernel (the camera carries no gradient)
+            rays_o, rays_d, fused_view_dirs, rays_o_ndc, rays_d_ndc = geometry.rays_fused(
+                pixel_id, frames[0], frames[1], mip_nerf_used=False, ndc=self.ndc,
+                resolution=self.model_configs['resolution'], near=self.model_configs['near'])
+        else:
+            intrinsics_inv = None if frames is None else \
+                torch.linalg.inv_ex(frames[0])[0][pixel_id[:, 0].long()]
+            rays_o, rays_d = geometry.get_rays_tr(pixel_id, intrinsics, extrinsics, mip_nerf_used=False,
+                                                  intrinsic_inv=intrinsics_inv)
+            if self.ndc:
+                rays_o_ndc, rays_d_ndc = geometry.get_ndc_rays_tr(rays_o, rays_d, self.model_configs['resolution'],
+                                                                  intrinsics, near=self.model_configs['near'])
         ret['rays_o'], ret['rays_d'] = rays_o, rays_d
-        rays_o_ndc = rays_d_ndc = None
         if self.ndc:
-            rays_o_ndc, rays_d_ndc = geometry.get_ndc_rays_tr(rays_o, rays_d, self.model_configs['resolution'],
-                                                              intrinsics, near=self.model_configs['near'])
             ret['rays_o_ndc'], ret['rays_d_ndc'] = rays_o_ndc, rays_d_ndc
         view_dirs = None
         if (self.coarse_model_needed and self.configs['model']['coarse_model']['use_view_dirs']) or \
                 (self.fine_model_needed and self.configs['model']['fine_model']['use_view_dirs']):
-            if mode == 'static_camera':
+            if fused:
+                view_dirs = fused_view_dirs
+            elif mode == 'static_camera':
                 cd = input_dict['common_data']
                 vi = cd['view_intrinsic'][None].repeat((pixel_id.shape[0], 1, 1)) if cd['view_intrinsic'] is not None \
                     else intrinsics

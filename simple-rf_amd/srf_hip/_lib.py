@@ -43,6 +43,7 @@ _PROTOS = {
     'srf_selftest_lds_dma': [_P, _P, _P],
     'srf_tune_set': [_I, _L],
     'srf_tune_get': [_I],
+    'srf_rays': [_I, _P, _I, _P, _P, _I, _I, _I, _I, _F, _P, _P, _P, _P, _P, _P],
     'srf_stratified_samples': [_I, _I, _F, _F, _I, _P, _P, _P],
     'srf_sample_pdf': [_I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P],
     'srf_fine_z_vals': [_I, _I, _I, _P, _P, _P, _P, _P, _P],

@@ -1,8 +1,8 @@
-"""Ray generation and NDC warping (host side of the drop-in, PyTorch ops on the GPU).
+"""Ray generation and NDC warping (src/utils/CommonUtils04.py:73-149).
 
-These produce the inputs of the hot path as the reference does (src/utils/CommonUtils04.py:73-149).
-They are small per-ray computations kept in PyTorch; fusing them into the sampler is the next step
-after the hot path.
+rays_fused runs get_rays_tr, get_view_dirs_tr and get_ndc_rays_tr as one HIP kernel (srf_rays) and
+is what the models use when the camera carries no gradient.  The PyTorch functions restate the
+reference op for op; they are the differentiable path for a model that learns its camera.
 """
 from __future__ import annotations
 
@@ -56,3 +56,30 @@ def get_ndc_rays_tr(rays_o, rays_d, resolution, intrinsic, near):
 def get_view_dirs_tr(rays_d):
     """src/utils/CommonUtils04.py:147-149."""
     return rays_d / torch.linalg.norm(rays_d, ord=2, dim=-1, keepdim=True)
+
+
+def rays_fused(pixel_id: torch.Tensor, intrinsics: torch.Tensor, extrinsics: torch.Tensor, *, mip_nerf_used: bool,
+               ndc: bool, resolution=None, near: float = 0.0):
+    """get_rays_tr, get_view_dirs_tr and (ndc) get_ndc_rays_tr in one HIP kernel (srf_rays).
+
+    pixel_id [N,3] (frame, x, y); intrinsics [F,3,3] and extrinsics (camera-to-world) [F,4,4] per
+    frame, indexed by pixel_id[:, 0].  Returns rays_o, rays_d, view_dirs and rays_o_ndc,
+    rays_d_ndc (None without ndc).  Not differentiable: callers that learn the camera use the
+    PyTorch functions above."""
+    from . import _lib
+    from ._checks import dev_tensor
+    n = pixel_id.shape[0]
+    dev = pixel_id.device
+    pid = dev_tensor(pixel_id.to(torch.int32), 'pixel_id', (n, 3), torch.int32)
+    n_frames = intrinsics.shape[0]
+    K = dev_tensor(intrinsics.detach().float(), 'intrinsics', (n_frames, 3, 3))
+    E = dev_tensor(extrinsics.detach().float(), 'extrinsics', (n_frames, 4, 4))
+    f = dict(dtype=torch.float32, device=dev)
+    rays_o, rays_d, view_dirs = (torch.empty((n, 3), **f) for _ in range(3))
+    o_ndc = torch.empty((n, 3), **f) if ndc else None
+    d_ndc = torch.empty((n, 3), **f) if ndc else None
+    h, w = resolution if ndc else (0, 0)
+    _lib.call('srf_rays', n, _lib.ptr(pid), n_frames, _lib.ptr(K), _lib.ptr(E), int(mip_nerf_used), int(ndc),
+              int(h), int(w), float(near), _lib.ptr(rays_o), _lib.ptr(rays_d), _lib.ptr(view_dirs),
+              _lib.ptr(o_ndc), _lib.ptr(d_ndc), _lib.stream_ptr(dev))
+    return rays_o, rays_d, view_dirs, o_ndc, d_ndc
