@@ -321,7 +321,7 @@ static int mlp_backward_impl(const srf_mlp_desc* desc, const void* packed, int n
   }
   d.n_jobs = nj;
   // points per work item: about 1024 blocks over all jobs, whole multiples of 256 points (so every
-  // item holds an even number of k-units), between 2048 and 16384 points
+  // item holds an even number of k-units), between 2048 and 262144 points
   long long target = ((long long)w.cols * nj / 1024 + 255) / 256 * 256;
   if (const int64_t v = tune_value(SRF_TUNE_DW_CHUNK)) target = v;  // autotuned schedule
   d.chunk = (int)(target < 2048 ? 2048 : (target > 262144 ? 262144 : target));
