@@ -677,7 +677,7 @@ extern "C" int srf_color_backward(const srf_color_desc* desc, const void* packed
   job(w.dz3, 16, w.h2, kHid, 3, kHid, d_w3, kHid, d_b3);
   job(w.dF, kMaxB, w.appT, g.Cp, g.NB, g.C, d_basis, g.C, nullptr);
   d.n_jobs = nj;
-  long long target = ((long long)w.cols * nj / 1024 + 255) / 256 * 256;
+  long long target = mlp::dw_default_chunk(w.cols, nj, mlp::device_compute_units());
   if (const int64_t v = tune_value(SRF_TUNE_COLOR_DW_CHUNK)) target = v;  // autotuned schedule
   d.chunk = (int)(target < 2048 ? 2048 : (target > 65536 ? 65536 : target));
   d.n_chunks = (w.cols + d.chunk - 1) / d.chunk;

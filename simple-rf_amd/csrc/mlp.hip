@@ -320,9 +320,9 @@ static int mlp_backward_impl(const srf_mlp_desc* desc, const void* packed, int n
     job(w.dz[8], 0, 16, w.h[7], 256, 4, 256, d_weights[9], 256, d_biases[9]);
   }
   d.n_jobs = nj;
-  // points per work item: about 1024 blocks over all jobs, whole multiples of 256 points (so every
-  // item holds an even number of k-units), between 2048 and 262144 points
-  long long target = ((long long)w.cols * nj / 1024 + 255) / 256 * 256;
+  // points per work item: whole rounds of resident blocks (dw_default_chunk), whole multiples of
+  // 256 points (so every item holds an even number of k-units), between 2048 and 262144 points
+  long long target = dw_default_chunk(w.cols, nj, device_compute_units());
   if (const int64_t v = tune_value(SRF_TUNE_DW_CHUNK)) target = v;  // autotuned schedule
   d.chunk = (int)(target < 2048 ? 2048 : (target > 262144 ? 262144 : target));
   d.n_chunks = (w.cols + d.chunk - 1) / d.chunk;

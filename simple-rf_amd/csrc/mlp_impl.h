@@ -1084,6 +1084,41 @@ struct DwJob {
   int mblocks, nblocks, work_begin, work_count;
 };
 
+// Points per work item of a grouped weight-gradient launch of nj jobs over cols points: whole
+// multiples of 256 points, at least 2048.  One block (160 KB of LDS) is resident per CU, so a
+// launch costs about (rounds of `cus` blocks) x (per-block time), and a last, partly filled round
+// costs as much as a full one.  The per-block time is modelled as the chunk plus a fixed
+// overhead worth kDwBlockOverhead points (the tile's FP32 atomics and the ring's fill); the
+// chunk minimising rounds x that time over 3..8 rounds is chosen.
+constexpr int kDwBlockOverhead = 640;
+inline int dw_default_chunk(int cols, int nj, int cus) {
+  long long best_cost = -1;
+  int best = 2048;
+  for (int rounds = 3; rounds <= 8; ++rounds) {
+    const int max_chunks = rounds * cus / (nj > 0 ? nj : 1);
+    if (max_chunks < 1) continue;
+    int chunk = round_up((cols + max_chunks - 1) / max_chunks, 256);
+    if (chunk < 2048) chunk = 2048;
+    const long long blocks = (long long)nj * ((cols + chunk - 1) / chunk);
+    const long long cost = (blocks + cus - 1) / cus * (long long)(chunk + kDwBlockOverhead);
+    if (best_cost < 0 || cost < best_cost) {
+      best_cost = cost;
+      best = chunk;
+    }
+  }
+  return best;
+}
+
+// Compute units of the current device (the launch-schedule heuristics above; 256 if the query
+// fails).
+inline int device_compute_units() {
+  int dev = 0, cus = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+    return 256;
+  return cus;
+}
+
 struct DwArgs {
   DwJob jobs[kMaxJobs];
   int n_jobs, ldP, cols, chunk, n_chunks, total_work;
