@@ -72,6 +72,26 @@ def profiled_traffic(kernel: str, rays: int):
     return None if entry is None else entry['hbm_bytes_per_launch']
 
 
+def device_copy_rate(dev, mib: int = 1024, reps: int = 5) -> float:
+    """Bytes read plus written per second by torch's device-to-device copy of a `mib` MiB buffer
+    (HIP events on torch's current stream, which the copy runs on), in GB/s."""
+    import torch
+    src = torch.rand(mib << 18, device=dev)
+    dst = torch.empty_like(src)
+    for _ in range(2):
+        dst.copy_(src)
+    t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0.record()
+    for _ in range(reps):
+        dst.copy_(src)
+    t1.record()
+    t1.synchronize()
+    rate = 2 * src.numel() * 4 * reps / (t0.elapsed_time(t1) * 1e-3) / 1e9
+    del src, dst
+    torch.cuda.empty_cache()
+    return round(rate, 1)
+
+
 class KernelTimer:
     """HIP events around the C-ABI entry points (each launches one kernel on the current
     stream, except srf_mlp_pack which launches a few small ones)."""
@@ -704,6 +724,11 @@ def main():
                      'flop_per_launch': int(dom['flop'] / dom['calls']),
                      'tflops': round(tflops, 1), 'mfma_frac': round(tflops / PEAK_BF16_TFLOPS, 4),
                      'avg_launch_ms': round(dom['ms'] / dom['calls'], 4)})
+        if roof['traffic']:
+            # the measured HBM traffic over the same launch time, beside what a plain device copy moves
+            # on this GPU (the practical ceiling of a streaming kernel, below the nominal peak)
+            roof['traffic_GBps'] = round(roof['traffic'] / (roof['avg_launch_ms'] * 1e-3) / 1e9, 1)
+        roof['copy_GBps'] = device_copy_rate(dev)
     per_kernel = {k: {'calls': v['calls'], 'ms_per_step': round(v['ms'] / args.steps, 3),
                       **({'tflops': round(v['flop'] / (v['ms'] * 1e-3) / 1e12, 1),
                           'GB/s': round(v['bytes'] / (v['ms'] * 1e-3) / 1e9, 1)} if v['flop'] else {})}
