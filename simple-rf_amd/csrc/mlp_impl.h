@@ -654,18 +654,20 @@ __device__ __forceinline__ uint32_t tile_block(int t, int ldP, int elem, int p0)
 // Writes one 16-feature tile of the wave's points in the paired tile layout read by the weight-
 // gradient GEMM: the 2 KB block at byte offset blk (tile_block) of the array described by dst.
 // A tile of the whole workspace array occupies the bytes of 16 feature-major rows;
-// inside it, the wave owning points p0 .. p0 + 16 CB - 1 writes one 2 KB block of 64 lane records
-// of 32 B.  The lanes of a pair (j = 2m, 2m + 1) first swap half of their D-tile values
-// (swap_adjacent), so that record (g, 2m + e) holds features 4 g + 2 e + c (c = 0, 1), each as one
-// 16 B chunk of that feature at the points 16 cb + 2 m and 16 cb + 2 m + 1 (cb = 0 .. CB - 1), in
-// point order.  A 16 B chunk is then exactly one lane's MFMA operand slice of one feature, so the
-// GEMM reads fragments with plain 16 B copies and every store here is a coalesced 32 B per lane.
+// inside it, the wave owning points p0 .. p0 + 16 CB - 1 writes one 2 KB block of 64 lane records,
+// each two 16 B chunks.  The lanes of a pair (j = 2m, 2m + 1) first swap half of their D-tile
+// values (swap_adjacent), so that record (g, 2m + e) holds features 4 g + 2 e + c (c = 0, 1), each
+// as one 16 B chunk of that feature at the points 16 cb + 2 m and 16 cb + 2 m + 1 (cb = 0 ..
+// CB - 1), in point order.  A 16 B chunk is then exactly one lane's MFMA operand slice of one
+// feature, so the GEMM reads fragments with plain 16 B copies.  The block is chunk-major: chunk c
+// of record r sits at byte 1024 c + 16 r, so each of the two stores writes 1 KB of whole
+// contiguous lines (a record-major block would leave every line half written between them).
 // The stores are non-temporal: a layer's tiles of all the CUs of an XCD (~4 MB) would otherwise
 // displace the streamed weights from that XCD's L2, and they are read back only by the backward.
 template <int P>
 __device__ __forceinline__ void store_tile(Rsrc dst, uint32_t blk, const typename Tr<P>::Act (&v)[Tr<P>::CB]) {
   constexpr int CB = Tr<P>::CB;
-  const uint32_t rec = lane_here() * 32u;
+  const uint32_t rec = lane_here() * 16u;
   if constexpr (P == kF32) {
     // v[cb] = features 4g + 0..3 at point 16 cb + j; even lanes keep r0, r1 and take the
     // partner's; odd lanes keep r2, r3
@@ -685,7 +687,7 @@ __device__ __forceinline__ void store_tile(Rsrc dst, uint32_t blk, const typenam
       c1[2 * cb + 1] = sel_odd(own1, rc[cb][1]);
     }
     store_nt16(dst, rec, blk, __builtin_bit_cast(u32x4, c0));
-    store_nt16(dst, rec, blk + 16, __builtin_bit_cast(u32x4, c1));
+    store_nt16(dst, rec, blk + 1024, __builtin_bit_cast(u32x4, c1));
   } else {
     // v[cb] = {bf16 (r0, r1), bf16 (r2, r3)} at point 16 cb + j
     u32x4 c0, c1;
@@ -698,7 +700,7 @@ __device__ __forceinline__ void store_tile(Rsrc dst, uint32_t blk, const typenam
       c1[cb] = (first >> 16) | (second & 0xffff0000u);
     }
     store_nt16(dst, rec, blk, c0);
-    store_nt16(dst, rec, blk + 16, c1);
+    store_nt16(dst, rec, blk + 1024, c1);
   }
 }
 
@@ -1213,7 +1215,7 @@ __global__ __launch_bounds__(64 * kDwWaves) void mlp_dw_kernel(DwArgs a) {
         constexpr int PPW = P == kF32 ? 32 : 64;  // points per stored wave block
         const int wb = p / PPW, u = (p % PPW) / KPTS;
         const int rec = 16 * (j >> 2) + 2 * (4 * u + g) + ((j >> 1) & 1);
-        src = base + (size_t)row0 * ldP * ELEM + (size_t)wb * 2048 + rec * 32 + (j & 1) * 16;
+        src = base + (size_t)row0 * ldP * ELEM + (size_t)wb * 2048 + (j & 1) * 1024 + rec * 16;
       } else {
         const int pg = P == kF32 ? 4 * g : 8 * g;
         src = base + (((size_t)(row0 + j)) * ldP + p + pg) * ELEM;
