@@ -155,10 +155,12 @@ __global__ __launch_bounds__(256) void pack_kernel(PackArgs a) {
 // for a reduction depth nu known only at run time.  The A fragments (1 KB per tile and k-unit, upr
 // k-units per packed tile row) are staged through LDS in chunks of kKC k-units, double-buffered;
 // fin may load or compute its operand (it is called once per k-unit and column block, so it must
-// not index register arrays).  Block-collective: every wave calls it with the same arguments.
-template <int OUT, typename FIn>
+// not index register arrays); keep(u, b) then sees the k-unit's operands of all column blocks (the
+// training copies of the layer input are stored there).  Block-collective: every wave calls it with
+// the same arguments.
+template <int OUT, typename FIn, typename Keep>
 __device__ __forceinline__ void staged_layer_rt(f32x4 (&acc)[OUT][kCB], const float* __restrict__ A, int upr,
-                                                int nu, FIn&& fin, uint8_t* stage) {
+                                                int nu, FIn&& fin, Keep&& keep, uint8_t* stage) {
   constexpr int kKC = mlp::kKC;
   constexpr int kHalf = kST * kKC * 1024;
   static_assert(OUT <= kST, "the stage buffer holds kST output tiles");
@@ -186,6 +188,7 @@ __device__ __forceinline__ void staged_layer_rt(f32x4 (&acc)[OUT][kCB], const fl
       f32x4 b[kCB];
 #pragma unroll
       for (int cb = 0; cb < kCB; ++cb) b[cb] = fin(c * kKC + kc, cb);
+      keep(c * kKC + kc, b);
 #pragma unroll
       for (int r = 0; r < 4; ++r)
 #pragma unroll
@@ -249,7 +252,27 @@ __device__ __forceinline__ void store_fm(uint8_t* base, size_t row_off, int ld, 
   __builtin_nontemporal_store(v, reinterpret_cast<float*>(base + row_off) + (size_t)row * ld + p);
 }
 
-__global__ __launch_bounds__(64 * kWaves) void color_fwd_kernel(FwdArgs a) {
+typedef float f32x2_t __attribute__((ext_vector_type(2)));
+
+// Stores the wave's 32 points (p0 a multiple of 32) of one feature row per 16-lane group: v0 is the
+// lane's value at point p0 + j and v1 at p0 + 16 + j (j = lane & 15, the two column blocks of an
+// MFMA tile).  The group's lanes exchange values (__shfl) so that lane j writes points 2j and
+// 2j + 1 as 8 B: each store then writes every row's 128 B as one whole line, where one store per
+// column block would leave each line half written between the two.  All lanes must call it (the
+// exchange); `keep` only masks the store.
+__device__ __forceinline__ void store_fm_row32(uint8_t* base, size_t row_off, int ld, int row, int p0, float v0,
+                                               float v1, bool keep) {
+  const int lane = threadIdx.x & 63, j = lane & 15, grp = lane & 48;
+  const int s0 = grp | ((2 * j) & 15), s1 = grp | ((2 * j + 1) & 15);
+  const float a0 = __shfl(v0, s0, 64), a1 = __shfl(v1, s0, 64);
+  const float b0 = __shfl(v0, s1, 64), b1 = __shfl(v1, s1, 64);
+  const f32x2_t o = j < 8 ? f32x2_t{a0, b0} : f32x2_t{a1, b1};
+  if (keep)
+    __builtin_nontemporal_store(
+        o, reinterpret_cast<f32x2_t*>(reinterpret_cast<float*>(base + row_off) + (size_t)row * ld + p0 + 2 * j));
+}
+
+__global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(2, 2))) void color_fwd_kernel(FwdArgs a) {
   __shared__ __attribute__((aligned(16))) uint8_t stage[kStage];
   __shared__ float sF[kWaves][kMaxB][kPW + 1];
   __shared__ float sD[kWaves][3][kPW];
@@ -269,26 +292,26 @@ __global__ __launch_bounds__(64 * kWaves) void color_fwd_kernel(FwdArgs a) {
     const int c = 16 * u + 4 * gq;
     float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
     if (p < a.n && c < g.C) v = *reinterpret_cast<const float4*>(a.app + (size_t)p * g.C + c);
-    if (train && c < g.C) {
-      store_fm(a.ws, a.w.appT, a.w.ld, c + 0, p, v.x);
-      store_fm(a.ws, a.w.appT, a.w.ld, c + 1, p, v.y);
-      store_fm(a.ws, a.w.appT, a.w.ld, c + 2, p, v.z);
-      store_fm(a.ws, a.w.appT, a.w.ld, c + 3, p, v.w);
-    }
     return f32x4{v.x, v.y, v.z, v.w};
   };
-  staged_layer_rt<2>(accF, a.pk + g.fB, g.Cp / 16, g.Cp / 16, fin_app, stage);
+  // training copy of the appearance features, feature-major (rows c .. c + 3 of the lane's group)
+  auto keep_app = [&](int u, const f32x4(&b)[kCB]) {
+    if (!train) return;
+    const int c = 16 * u + 4 * gq;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) store_fm_row32(a.ws, a.w.appT, a.w.ld, c + r, p0, b[0][r], b[1][r], c < g.C);
+  };
+  staged_layer_rt<2>(accF, a.pk + g.fB, g.Cp / 16, g.Cp / 16, fin_app, keep_app, stage);
   // F and the view directions to LDS (and F feature-major for the backward)
 #pragma unroll
   for (int T = 0; T < 2; ++T)
 #pragma unroll
-    for (int cb = 0; cb < kCB; ++cb)
+    for (int r = 0; r < 4; ++r) {
+      const int f = 16 * T + 4 * gq + r;
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int f = 16 * T + 4 * gq + r;
-        sF[wv][f][16 * cb + j] = accF[T][cb][r];
-        if (train) store_fm(a.ws, a.w.F, a.w.ld, f, p0 + 16 * cb + j, accF[T][cb][r]);
-      }
+      for (int cb = 0; cb < kCB; ++cb) sF[wv][f][16 * cb + j] = accF[T][cb][r];
+      if (train) store_fm_row32(a.ws, a.w.F, a.w.ld, f, p0, accF[T][0][r], accF[T][1][r], true);
+    }
   if (g.view && lane < kPW) {
     const int p = p0 + lane;
     float d0 = 0.f, d1 = 0.f, d2 = 0.f;
@@ -320,21 +343,26 @@ __global__ __launch_bounds__(64 * kWaves) void color_fwd_kernel(FwdArgs a) {
     for (int r = 0; r < 4; ++r) {
       const int i = 16 * u + 4 * gq + r;
       b[r] = pe_input(g, i, &sF[wv][0][col], kPW + 1, dir);
-      if (train && i < g.D1) store_fm(a.ws, a.w.x1, a.w.ld, i, p0 + col, b[r]);
     }
     return b;
   };
-  staged_layer_rt<kHT>(h1, a.pk + g.fW1, g.D1p / 16, g.D1p / 16, fin_pe, stage);
+  auto keep_pe = [&](int u, const f32x4(&b)[kCB]) {
+    if (!train) return;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int i = 16 * u + 4 * gq + r;
+      store_fm_row32(a.ws, a.w.x1, a.w.ld, i, p0, b[0][r], b[1][r], i < g.D1);
+    }
+  };
+  staged_layer_rt<kHT>(h1, a.pk + g.fW1, g.D1p / 16, g.D1p / 16, fin_pe, keep_pe, stage);
 #pragma unroll
   for (int T = 0; T < kHT; ++T)
 #pragma unroll
-    for (int cb = 0; cb < kCB; ++cb)
+    for (int r = 0; r < 4; ++r) {
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const float v = h1[T][cb][r] > 0.f ? h1[T][cb][r] : 0.f;
-        h1[T][cb][r] = v;
-        if (train) store_fm(a.ws, a.w.h1, a.w.ld, 16 * T + 4 * gq + r, p0 + 16 * cb + j, v);
-      }
+      for (int cb = 0; cb < kCB; ++cb) h1[T][cb][r] = h1[T][cb][r] > 0.f ? h1[T][cb][r] : 0.f;
+      if (train) store_fm_row32(a.ws, a.w.h1, a.w.ld, 16 * T + 4 * gq + r, p0, h1[T][0][r], h1[T][1][r], true);
+    }
   // 3) second layer
   f32x4 h2[kHT][kCB];
 #pragma unroll
@@ -347,13 +375,11 @@ __global__ __launch_bounds__(64 * kWaves) void color_fwd_kernel(FwdArgs a) {
 #pragma unroll
   for (int T = 0; T < kHT; ++T)
 #pragma unroll
-    for (int cb = 0; cb < kCB; ++cb)
+    for (int r = 0; r < 4; ++r) {
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const float v = h2[T][cb][r] > 0.f ? h2[T][cb][r] : 0.f;
-        h2[T][cb][r] = v;
-        if (train) store_fm(a.ws, a.w.h2, a.w.ld, 16 * T + 4 * gq + r, p0 + 16 * cb + j, v);
-      }
+      for (int cb = 0; cb < kCB; ++cb) h2[T][cb][r] = h2[T][cb][r] > 0.f ? h2[T][cb][r] : 0.f;
+      if (train) store_fm_row32(a.ws, a.w.h2, a.w.ld, 16 * T + 4 * gq + r, p0, h2[T][0][r], h2[T][1][r], true);
+    }
   // 4) output layer + sigmoid
   f32x4 o[1][kCB];
 #pragma unroll
@@ -413,9 +439,10 @@ __global__ __launch_bounds__(64 * kWaves) void color_bwd_kernel(BwdArgs a) {
         v = __fmul_rn(__fmul_rn(a.d_rgb[3 * (size_t)p + r], __fsub_rn(1.f, s)), s);
       }
       dz3[0][cb][r] = v;
-      store_fm(a.ws, a.w.dz3, a.w.ld, 4 * gq + r, p, v);
     }
   }
+#pragma unroll
+  for (int r = 0; r < 4; ++r) store_fm_row32(a.ws, a.w.dz3, a.w.ld, 4 * gq + r, p0, dz3[0][0][r], dz3[0][1][r], true);
   // 2) dz2 = (W3^T dz3) * [h2 > 0]
   f32x4 dz2[kHT][kCB];
 #pragma unroll
@@ -430,10 +457,13 @@ __global__ __launch_bounds__(64 * kWaves) void color_bwd_kernel(BwdArgs a) {
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int row = 16 * T + 4 * gq + r, p = p0 + 16 * cb + j;
-        const float v = load_fm(a.ws, a.w.h2, a.w.ld, row, p) > 0.f ? dz2[T][cb][r] : 0.f;
-        dz2[T][cb][r] = v;
-        store_fm(a.ws, a.w.dz2, a.w.ld, row, p, v);
+        dz2[T][cb][r] = load_fm(a.ws, a.w.h2, a.w.ld, row, p) > 0.f ? dz2[T][cb][r] : 0.f;
       }
+#pragma unroll
+  for (int T = 0; T < kHT; ++T)
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+      store_fm_row32(a.ws, a.w.dz2, a.w.ld, 16 * T + 4 * gq + r, p0, dz2[T][0][r], dz2[T][1][r], true);
   // 3) dz1 = (W2^T dz2) * [h1 > 0]
   f32x4 dz1[kHT][kCB];
 #pragma unroll
@@ -448,10 +478,13 @@ __global__ __launch_bounds__(64 * kWaves) void color_bwd_kernel(BwdArgs a) {
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int row = 16 * T + 4 * gq + r, p = p0 + 16 * cb + j;
-        const float v = load_fm(a.ws, a.w.h1, a.w.ld, row, p) > 0.f ? dz1[T][cb][r] : 0.f;
-        dz1[T][cb][r] = v;
-        store_fm(a.ws, a.w.dz1, a.w.ld, row, p, v);
+        dz1[T][cb][r] = load_fm(a.ws, a.w.h1, a.w.ld, row, p) > 0.f ? dz1[T][cb][r] : 0.f;
       }
+#pragma unroll
+  for (int T = 0; T < kHT; ++T)
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+      store_fm_row32(a.ws, a.w.dz1, a.w.ld, 16 * T + 4 * gq + r, p0, dz1[T][0][r], dz1[T][1][r], true);
   // 4) dx1 = W1^T dz1 into LDS, kST output tiles per staged pass and single tiles for the rest
   auto fin_dz1 = [&](int u, int cb) { return dz1[u][cb]; };
   auto dx_tiles = [&](auto OUTC, int T0) {
