@@ -59,6 +59,8 @@ def mlp_algorithmic_bytes(name, desc, n):
         return n * elem * sum(i + o for i, o in dims) + 4 * sum(i * o + o for i, o in dims)
     if name == 'srf_mlp_forward':
         return n * (elem * sum(i for i, _ in dims[1:]) + elem * desc.trunk_pe + masks + 24 + 16)
+    if name == 'srf_mlp_forward_rays':  # z per point instead of points and directions (rays: 36 B per ray)
+        return n * (elem * sum(i for i, _ in dims[1:]) + elem * desc.trunk_pe + masks + 4 + 16)
     return n * (elem * sum(o for _, o in dims) + masks + 32)
 # HBM bytes per launch of the dominant kernel from the rocprofv3 FETCH_SIZE / WRITE_SIZE passes
 # (scaled by a plain copy of known size) over this same train step.
@@ -129,11 +131,12 @@ class KernelTimer:
             d['calls'] += 1
             d['ms'] += ms
             d.setdefault('bytes', 0.0)
-            if name in ('srf_mlp_forward', 'srf_mlp_backward_data', 'srf_mlp_weight_grads'):
+            if name in ('srf_mlp_forward', 'srf_mlp_forward_rays', 'srf_mlp_backward_data', 'srf_mlp_weight_grads'):
                 desc = args[0]._obj
-                n = args[2] if name != 'srf_mlp_weight_grads' else args[1]
+                n = {'srf_mlp_weight_grads': lambda: args[1], 'srf_mlp_forward_rays': lambda: args[2] * args[3]}.get(
+                    name, lambda: args[2])()
                 d['flop'] += 2.0 * MACS[(desc.trunk_pe, bool(desc.view_dependent))] * n
-                d['bytes'] += mlp_algorithmic_bytes(name, desc, n)
+                d['bytes'] += mlp_algorithmic_bytes(name, desc, n) + (36 * args[2] if name == 'srf_mlp_forward_rays' else 0)
         return out
 
 

@@ -193,6 +193,22 @@ int srf_mlp_forward(const srf_mlp_desc* desc, const void* packed, int n_points, 
                     const float* dirs, const float* noise, float* sigma, float* rgb,
                     void* workspace, void* stream);
 
+/* srf_mlp_forward with the sample points generated in the kernel instead of read from a
+ * materialised [n_rays, n_samples, 3] array.  Replaces the point generation
+ * pts = rays_o[..., None, :] + rays_d[..., None, :] * z_vals[..., :, None]
+ * (src/models/SimpleNeRF17.py:213-215, :266-268; the product and the sum are separately rounded
+ * FP32 operations, so the points equal the reference's) and the view-direction expansion
+ * viewdirs[:, None].expand(pts.shape) of run_network (:429-433), then runs the same forward.
+ *   rays_o, rays_d [n_rays,3]; z_vals [n_rays, n_samples]; view_dirs [n_rays,3] (per ray, needed
+ *   when view_dependent); noise [n_rays * n_samples] or NULL.
+ * Outputs sigma [n_rays * n_samples], rgb [n_rays * n_samples, 3]; point p belongs to ray
+ * p / n_samples.  The workspace is the one of n_rays * n_samples points, and srf_mlp_backward /
+ * _backward_data / _weight_grads take it with n_points = n_rays * n_samples. */
+int srf_mlp_forward_rays(const srf_mlp_desc* desc, const void* packed, int n_rays, int n_samples,
+                         const float* rays_o, const float* rays_d, const float* z_vals,
+                         const float* view_dirs, const float* noise, float* sigma, float* rgb,
+                         void* workspace, void* stream);
+
 /* Backward of srf_mlp_forward: d_sigma [n], d_rgb [n,3] upstream gradients; sigma, rgb the
  * forward outputs.  ACCUMULATES (+=) the FP32 parameter gradients into d_weights / d_biases
  * (host arrays of SRF_MLP_NUM_PARAMS device pointers in the reference order and shapes). */

@@ -219,13 +219,15 @@ extern "C" int srf_mlp_pack(const srf_mlp_desc* desc, const float* const* weight
   return launch_status();
 }
 
-extern "C" int srf_mlp_forward(const srf_mlp_desc* desc, const void* packed, int n_points,
-                               const float* pts, const float* dirs, const float* noise, float* sigma,
-                               float* rgb, void* workspace, void* stream) {
+static int mlp_forward_impl(const srf_mlp_desc* desc, const void* packed, int n_points, const float* pts,
+                            const float* rays_o, const float* rays_d, const float* z, int n_samples,
+                            const float* dirs, const float* noise, float* sigma, float* rgb, void* workspace,
+                            void* stream) {
   Plan p;
   if (!make_plan(desc, &p) || n_points < 0 || !packed) return SRF_EINVAL;
   if (n_points == 0) return SRF_OK;
-  if (!pts || !sigma || !rgb || (p.view_dep && !dirs)) return SRF_EINVAL;
+  if (!sigma || !rgb || (p.view_dep && !dirs)) return SRF_EINVAL;
+  if (z ? (!rays_o || !rays_d || n_samples <= 0 || n_points % n_samples != 0) : !pts) return SRF_EINVAL;
   FwdArgs a;
   a.p = p;
   a.w = make_ws(p, n_points);
@@ -237,8 +239,29 @@ extern "C" int srf_mlp_forward(const srf_mlp_desc* desc, const void* packed, int
   a.sigma = sigma;
   a.rgb = rgb;
   a.ws = reinterpret_cast<uint8_t*>(workspace);
+  a.rays_o = rays_o;
+  a.rays_d = rays_d;
+  a.z = z;
+  a.n_samples = n_samples;
   if (workspace && !ws_addressable(p, a.w)) return SRF_EINVAL;
   return fwd_launch(p.prec, a, p.view_dep, workspace != nullptr, as_stream(stream));
+}
+
+extern "C" int srf_mlp_forward(const srf_mlp_desc* desc, const void* packed, int n_points,
+                               const float* pts, const float* dirs, const float* noise, float* sigma,
+                               float* rgb, void* workspace, void* stream) {
+  return mlp_forward_impl(desc, packed, n_points, pts, nullptr, nullptr, nullptr, 0, dirs, noise, sigma, rgb,
+                          workspace, stream);
+}
+
+extern "C" int srf_mlp_forward_rays(const srf_mlp_desc* desc, const void* packed, int n_rays, int n_samples,
+                                    const float* rays_o, const float* rays_d, const float* z_vals,
+                                    const float* view_dirs, const float* noise, float* sigma, float* rgb,
+                                    void* workspace, void* stream) {
+  if (n_rays < 0 || n_samples <= 0 || (long long)n_rays * n_samples > 0x7fffffffLL) return SRF_EINVAL;
+  if (!z_vals && n_rays > 0) return SRF_EINVAL;
+  return mlp_forward_impl(desc, packed, n_rays * n_samples, nullptr, rays_o, rays_d, z_vals, n_samples, view_dirs,
+                          noise, sigma, rgb, workspace, stream);
 }
 
 static int mlp_backward_impl(const srf_mlp_desc* desc, const void* packed, int n_points,

@@ -357,6 +357,12 @@ struct FwdArgs {
   float* sigma;
   float* rgb;
   uint8_t* ws;  // null for inference
+  // ray form (z non-null): point p is rays_o[r] + rays_d[r] * z[p] of ray r = p / n_samples, and
+  // dirs holds one view direction per ray
+  const float* rays_o;
+  const float* rays_d;
+  const float* z;
+  int n_samples;
 };
 
 struct BwdArgs {
@@ -747,9 +753,35 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
 #pragma unroll
     for (int c = 0; c < 3; ++c) v[c] = p < a.n ? src[3 * p + c] : 0.f;
   };
+  // the point's position: stored, or rays_o[..., None, :] + rays_d[..., None, :] * z[..., :, None]
+  // of its ray (src/models/SimpleNeRF17.py:213-215, :266-268; two rounded FP32 operations)
+  auto load_xyz = [&](float (&v)[3]) SRF_AI {
+    if (a.z) {
+      const int p = p0 + pe_pt;
+      const bool in = p < a.n;
+      const int r = in ? p / a.n_samples : 0;
+      const float zz = in ? a.z[p] : 0.f;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) v[c] = in ? __fadd_rn(a.rays_o[3 * r + c], __fmul_rn(a.rays_d[3 * r + c], zz)) : 0.f;
+    } else {
+      load_pt(a.pts, v);
+    }
+  };
+  // the point's view direction: stored per point, or its ray's
+  // (viewdirs[:, None].expand(pts.shape), src/models/SimpleNeRF17.py:429-433)
+  auto load_dir = [&](float (&v)[3]) SRF_AI {
+    if (a.z) {
+      const int p = p0 + pe_pt;
+      const int r = p < a.n ? p / a.n_samples : 0;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) v[c] = p < a.n ? a.dirs[3 * r + c] : 0.f;
+    } else {
+      load_pt(a.dirs, v);
+    }
+  };
   auto pe_trunk = [&]() SRF_AI {
     float x[3];
-    load_pt(a.pts, x);
+    load_xyz(x);
     PR::write(pe_rows + pe_pt * PR::kStride, x, pl.pts_deg, 0, pl.trunk_pe, 0, pe_part);
     wave_lds_sync();
     static_for<0, 4>([&](auto TI) SRF_AI {
@@ -888,8 +920,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
     {
       const int pts_dim = 3 * (2 * pl.pts_deg + 1), n_hi = pts_dim - pl.trunk_pe;
       float x[3], d[3];
-      load_pt(a.pts, x);
-      load_pt(a.dirs, d);
+      load_xyz(x);
+      load_dir(d);
       typename PR::E* row = pe_rows + pe_pt * PR::kStride;
       if (n_hi > 0) PR::write(row, x, pl.pts_deg, pl.trunk_pe, pts_dim, pl.trunk_pe, pe_part);
       PR::write(row, d, pl.dir_deg, 0, pl.extra - n_hi, -n_hi, pe_part);

@@ -202,7 +202,8 @@ class SimpleNeRFHip(torch.nn.Module):
         weights_coarse = z_coarse = None
         if self.coarse_model_needed:
             z_coarse = self.get_z_vals_coarse(input_dict)
-            net_in = {'pts': o_s[..., None, :] + d_s[..., None, :] * z_coarse[..., :, None]}
+            # the sample points o + d z are generated inside the MLP kernel (srf_mlp_forward_rays)
+            net_in = {'rays_o': o_s, 'rays_d': d_s, 'z_vals': z_coarse}
             if self.configs['model']['coarse_model']['use_view_dirs']:
                 net_in['view_dirs'] = view_dirs
             net_out = self.run_network(net_in, self.coarse_model)
@@ -229,7 +230,7 @@ class SimpleNeRFHip(torch.nn.Module):
                                 ret[f"{aug['name']}_raw_{k}_coarse"] = a_out[k]
         if self.fine_model_needed:
             z_fine = self.get_z_vals_fine(z_coarse, weights_coarse)
-            net_in = {'pts': o_s[..., None, :] + d_s[..., None, :] * z_fine[..., :, None]}
+            net_in = {'rays_o': o_s, 'rays_d': d_s, 'z_vals': z_fine}
             if self.configs['model']['fine_model']['use_view_dirs']:
                 net_in['view_dirs'] = view_dirs
             net_out = self.run_network(net_in, self.fine_model)
@@ -296,17 +297,25 @@ class SimpleNeRFHip(torch.nn.Module):
 
     # --------------------------------------------------------------- network
     def run_network(self, input_dict, nerf_mlp):
-        """src/models/SimpleNeRF17.py:419-448: flatten, one fused MLP launch, reshape."""
-        pts = input_dict['pts']
-        pts_flat = pts.reshape(-1, pts.shape[-1])
-        net_in = {'pts': pts_flat}
-        if nerf_mlp.mlp_configs['use_view_dirs']:
-            vd = input_dict['view_dirs']
-            if vd.ndim == 2:
-                vd = vd[:, None].expand(pts.shape)
-            net_in['view_dirs'] = vd.reshape(-1, vd.shape[-1])
+        """src/models/SimpleNeRF17.py:419-448: flatten, one fused MLP launch, reshape.  Besides the
+        reference's {'pts': [R,S,3]} it takes the ray form {'rays_o', 'rays_d': [R,3], 'z_vals': [R,S]},
+        whose points (src/models/SimpleNeRF17.py:213-215) the kernel generates itself."""
+        if 'z_vals' in input_dict:
+            lead = list(input_dict['z_vals'].shape)
+            net_in = {k: input_dict[k] for k in ('rays_o', 'rays_d', 'z_vals')}
+            if nerf_mlp.mlp_configs['use_view_dirs']:
+                net_in['view_dirs'] = input_dict['view_dirs']  # one per ray: expanded in the kernel
+        else:
+            pts = input_dict['pts']
+            lead = list(pts.shape[:-1])
+            net_in = {'pts': pts.reshape(-1, pts.shape[-1])}
+            if nerf_mlp.mlp_configs['use_view_dirs']:
+                vd = input_dict['view_dirs']
+                if vd.ndim == 2:
+                    vd = vd[:, None].expand(pts.shape)
+                net_in['view_dirs'] = vd.reshape(-1, vd.shape[-1])
         out = self.batchify(nerf_mlp)(net_in)
-        return {k: v.reshape(list(pts.shape[:-1]) + list(v.shape[1:])) for k, v in out.items()}
+        return {k: v.reshape(lead + list(v.shape[1:])) for k, v in out.items()}
 
     def batchify(self, nerf_mlp):
         """src/models/SimpleNeRF17.py:450-484.  The fused kernel streams any number of points,
@@ -411,12 +420,18 @@ class MLP(torch.nn.Module):
         return [None if l is None else l.weight for l in lins], [None if l is None else l.bias for l in lins]
 
     def forward(self, input_batch: dict):
-        pts = input_batch['pts']
+        ray_form = 'z_vals' in input_batch  # rays_o, rays_d [R,3], z_vals [R,S]: points made in the kernel
+        ref = input_batch['z_vals'] if ray_form else input_batch['pts']
+        n = ref.numel() if ray_form else ref.shape[0]
         noise = None
         if self.training and self.raw_noise_std > 0.:
-            noise = torch.randn(pts.shape[0], device=pts.device) * self.raw_noise_std
+            noise = torch.randn(n, device=ref.device) * self.raw_noise_std
         ws, bs = self.layer_params()
-        sigma, rgb = hip_mlp.mlp_forward(self.spec, ws, bs, pts, input_batch.get('view_dirs'), noise)
+        if ray_form:
+            sigma, rgb = hip_mlp.mlp_forward_rays(self.spec, ws, bs, input_batch['rays_o'], input_batch['rays_d'], ref,
+                                                  input_batch.get('view_dirs'), noise)
+        else:
+            sigma, rgb = hip_mlp.mlp_forward(self.spec, ws, bs, ref, input_batch.get('view_dirs'), noise)
         out = {'sigma': sigma[:, None]}
         out['rgb_view_dependent' if self.view_dep_rgb else 'rgb_view_independent'] = rgb
         out['rgb'] = rgb

@@ -57,6 +57,7 @@ _PROTOS = {
     'srf_mlp_pack': [_P, _P, _P, _P, _P],
     'srf_mlp_workspace_bytes': [_P, _I],
     'srf_mlp_forward': [_P, _P, _I, _P, _P, _P, _P, _P, _P, _P],
+    'srf_mlp_forward_rays': [_P, _P, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P],
     'srf_mlp_backward': [_P, _P, _I, _P, _P, _P, _P, _P, _P, _P, _P],
     'srf_mlp_backward_data': [_P, _P, _I, _P, _P, _P, _P, _P, _P],
     'srf_mlp_weight_grads': [_P, _I, _P, _P, _P, _P],

@@ -111,17 +111,27 @@ def pack(spec: MLPSpec, weights, biases, out: torch.Tensor | None = None) -> tor
 
 class _MLPFunction(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, spec, train, pts, dirs, noise, *params):
+    def forward(ctx, spec, train, pts, dirs, noise, rays, *params):
+        # pts [P,3] (dirs per point), or pts None and rays = (rays_o, rays_d, z_vals) with the points
+        # generated in the kernel (dirs per ray)
         weights, biases = params[:12], params[12:]
         packed = pack(spec, weights, biases)
-        n = pts.shape[0]
-        dev = pts.device
+        if rays is None:
+            n, dev = pts.shape[0], pts.device
+        else:
+            n, dev = rays[2].numel(), rays[2].device
         sigma = torch.empty((n,), dtype=torch.float32, device=dev)
         rgb = torch.empty((n, 3), dtype=torch.float32, device=dev)
         ws = torch.empty(workspace_bytes(spec, n), dtype=torch.uint8, device=dev) if train else None
         d = spec.desc()
-        _lib.call('srf_mlp_forward', C.byref(d), packed.data_ptr(), n, pts.data_ptr(), _lib.ptr(dirs),
-                  _lib.ptr(noise), sigma.data_ptr(), rgb.data_ptr(), _lib.ptr(ws), _lib.stream_ptr(dev))
+        if rays is None:
+            _lib.call('srf_mlp_forward', C.byref(d), packed.data_ptr(), n, pts.data_ptr(), _lib.ptr(dirs),
+                      _lib.ptr(noise), sigma.data_ptr(), rgb.data_ptr(), _lib.ptr(ws), _lib.stream_ptr(dev))
+        else:
+            o, dr, z = rays
+            _lib.call('srf_mlp_forward_rays', C.byref(d), packed.data_ptr(), z.shape[0], z.shape[1], o.data_ptr(),
+                      dr.data_ptr(), z.data_ptr(), _lib.ptr(dirs), _lib.ptr(noise), sigma.data_ptr(),
+                      rgb.data_ptr(), _lib.ptr(ws), _lib.stream_ptr(dev))
         ctx.spec = spec
         ctx.shapes = [None if p is None else p.shape for p in params]
         ctx.save_for_backward(packed, ws, sigma, rgb)
@@ -144,7 +154,7 @@ class _MLPFunction(torch.autograd.Function):
                   d_sigma.data_ptr(), d_rgb.data_ptr(), ws.data_ptr(), stream)
         _lib.call('srf_mlp_weight_grads', C.byref(d), n, ws.data_ptr(), _ptr_array(grads[:12]),
                   _ptr_array(grads[12:]), stream)
-        return (None, None, None, None, None, *grads)
+        return (None, None, None, None, None, None, *grads)
 
 
 def mlp_forward(spec: MLPSpec, weights, biases, pts: torch.Tensor, dirs: torch.Tensor | None,
@@ -168,4 +178,29 @@ def mlp_forward(spec: MLPSpec, weights, biases, pts: torch.Tensor, dirs: torch.T
     params = [None if spec.param_shapes()[i] is None else weights[i] for i in range(12)] + \
              [None if spec.param_shapes()[i] is None else biases[i] for i in range(12)]
     train = torch.is_grad_enabled() and any(p is not None and p.requires_grad for p in params)
-    return _MLPFunction.apply(spec, train, pts, dirs, noise, *params)
+    return _MLPFunction.apply(spec, train, pts, dirs, noise, None, *params)
+
+
+def mlp_forward_rays(spec: MLPSpec, weights, biases, rays_o: torch.Tensor, rays_d: torch.Tensor,
+                     z_vals: torch.Tensor, view_dirs: torch.Tensor | None, noise: torch.Tensor | None = None):
+    """mlp_forward at the points rays_o[:, None] + rays_d[:, None] * z_vals[..., None], generated in the
+    kernel (src/models/SimpleNeRF17.py:213-215, :266-268) instead of materialised: the same sigma
+    [R*S] and rgb [R*S,3] as mlp_forward on those points, bit for bit.  view_dirs [R,3] is one
+    direction per ray (run_network's expand, :429-433).  The rays get no gradient, as pts in
+    mlp_forward."""
+    z_vals = dev_tensor(z_vals, 'z_vals')
+    if z_vals.dim() != 2:
+        raise _lib.SrfError(f'z_vals must be [R,S], got {tuple(z_vals.shape)}')
+    r, s = z_vals.shape
+    rays_o = dev_tensor(rays_o, 'rays_o', (r, 3))
+    rays_d = dev_tensor(rays_d, 'rays_d', (r, 3))
+    if spec.view_dependent:
+        view_dirs = dev_tensor(view_dirs, 'view_dirs', (r, 3))
+    else:
+        view_dirs = None
+    if noise is not None:
+        noise = dev_tensor(noise.reshape(-1), 'noise', (r * s,))
+    params = [None if spec.param_shapes()[i] is None else weights[i] for i in range(12)] + \
+             [None if spec.param_shapes()[i] is None else biases[i] for i in range(12)]
+    train = torch.is_grad_enabled() and any(p is not None and p.requires_grad for p in params)
+    return _MLPFunction.apply(spec, train, None, view_dirs, noise, (rays_o, rays_d, z_vals), *params)

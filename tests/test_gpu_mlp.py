@@ -239,3 +239,40 @@ def test_bf16_backward_matches_rounding_model(cuda, variant):
                                       / ref.norm().clamp_min(1e-12)).item()
     print('bf16-model', variant, ' '.join(f'{k}={v:.1e}' for k, v in errs.items()))
     assert max(errs.values()) < 2e-2, errs
+
+
+@pytest.mark.parametrize('variant', list(VARIANTS))
+@pytest.mark.parametrize('prec', ['fp32', 'bf16'])
+def test_ray_form_matches_materialised_points(cuda, variant, prec):
+    """srf_mlp_forward_rays generates rays_o + rays_d * z (the reference's product and sum, each
+    rounded) and expands the per-ray view directions in the kernel: sigma, rgb and the weight
+    gradients equal those of srf_mlp_forward on the materialised points, bit for bit; also with a
+    ray count that leaves the last 256-point block partly empty, and with sigma noise."""
+    spec = hm.MLPSpec(prec=prec, **VARIANTS[variant]['spec'])
+    ws, bs = make_params(spec, 11)
+    n_rays, n_s = 333, 64
+    g = torch.Generator().manual_seed(12)
+    o = (torch.rand(n_rays, 3, generator=g) * 2 - 1).to(cuda)
+    d = torch.randn(n_rays, 3, generator=g).to(cuda)
+    z = torch.sort(torch.rand(n_rays, n_s, generator=g), dim=-1).values.to(cuda)
+    vd = torch.nn.functional.normalize(d, dim=-1)
+    noise = torch.randn(n_rays * n_s, generator=g).to(cuda)
+    pts = (o[..., None, :] + d[..., None, :] * z[..., :, None]).reshape(-1, 3)
+    dirs = vd[:, None].expand(n_rays, n_s, 3).reshape(-1, 3)
+
+    def run(ray_form):
+        w = [None if t is None else t.to(cuda).requires_grad_() for t in ws]
+        b = [None if t is None else t.to(cuda).requires_grad_() for t in bs]
+        if ray_form:
+            s, c = hm.mlp_forward_rays(spec, w, b, o, d, z, vd, noise)
+        else:
+            s, c = hm.mlp_forward(spec, w, b, pts, dirs, noise)
+        (s.square().sum() + (c * torch.linspace(-1, 1, 3, device=cuda)).sum()).backward()
+        return s.detach(), c.detach(), [t.grad for t in w + b if t is not None]
+
+    s0, c0, g0 = run(False)
+    s1, c1, g1 = run(True)
+    assert torch.equal(s0, s1) and torch.equal(c0, c1)
+    for a, b in zip(g0, g1):
+        # the same products, summed by FP32 atomics whose order may differ between the launches
+        assert (a - b).abs().max() <= 1e-5 * a.abs().max() + 1e-12
