@@ -679,8 +679,8 @@ def main():
                          'of known size for scaling rocprof FETCH_SIZE / WRITE_SIZE)')
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get('WORLD_SIZE', '1')) != 1:
-        print('bench.py measures one GPU: this build has no multi-GPU path (rays shard across GPUs with '
-              'one gradient all-reduce per step; see DESIGN.md)', file=sys.stderr)
+        print('bench.py measures one GPU: the library has no multi-GPU path (rays would shard across GPUs, '
+              'with one all-reduce of the parameter gradients per step)', file=sys.stderr)
         sys.exit(2)
     import torch
     dev = torch.device('cuda:0')
