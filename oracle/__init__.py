@@ -16,8 +16,10 @@ Pinning status (what checks each function beyond retyping; see tests/test_oracle
                                      (u=0, u=1, flat pdf -> linear, spike, denom rule)
   nerf.fine_z_vals                -- multiset + sortedness vs concatenation
   nerf.positional_encoding        -- closed form per element, dims 63/27/21
-  nerf.mlp_forward                -- golden vectors written by the reference's own
-                                     MLP class (tests/golden), FP64 gradcheck
+  nerf.mlp_forward                -- vectors written by the reference's own MLP class
+                                     (tests/golden/mlp_golden.npz: main, points-augmented
+                                     21/42 and views-augmented, FP32 to 1e-6, FP64 to 1e-12);
+                                     swapped concatenation orders miss them (test_oracle_golden)
   nerf.volume_rendering           -- closed forms (sigma=0, one opaque sample,
                                      constant sigma -> geometric weights), acc<=1,
                                      telescoping identity, gradcheck
@@ -28,5 +30,6 @@ Pinning status (what checks each function beyond retyping; see tests/test_oracle
                                      same factors, exact texel-centre values,
                                      rank-1 separable grids, gradcheck
   tensorf.alpha_mask_sample       -- grid_sample 3D (library) + texel-centre values
-  tensorf.color_mlp_forward       -- golden vectors from the reference class
+  tensorf.color_mlp_forward       -- vectors written by the reference's MlpFeaturesColorPredictor
+                                     (same fixture: feature/view PE 0/0 and 2/2)
 """
