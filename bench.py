@@ -19,6 +19,7 @@ import argparse
 import json
 import os
 import statistics
+import subprocess
 import sys
 import time
 from pathlib import Path
@@ -31,6 +32,11 @@ for _p in (REPO, REPO / 'simple-rf_amd'):
 METRIC = 'train rays/s (fwd+bwd) & render rays/s: fused BF16 vs plain FP32, 1 GPU'
 PEAK_BF16_TFLOPS = 2500.0
 PEAK_HBM_GBS = 8000.0
+PEAK_FP32_TFLOPS = 157.3
+# BF16 MFMA products per algorithmic product of each split-BF16 kernel: three operand planes in the
+# forward (six plane products), two in the backward chain and the weight gradients (three)
+SPLIT_PRODUCTS = {'srf_mlp_forward': 6, 'srf_mlp_forward_rays': 6, 'srf_mlp_backward_data': 3,
+                  'srf_mlp_weight_grads': 3}
 MACS = {(63, True): 593408, (21, True): 577280, (63, False): 492032}
 
 
@@ -53,7 +59,7 @@ def mlp_algorithmic_bytes(name, desc, n):
     gradients once; the training forward writes every layer input and the ReLU masks (9 x 32 B);
     the backward chain writes every output gradient and reads the masks."""
     dims = mlp_layer_dims(desc)
-    elem = 4 if desc.prec == 0 else 2
+    elem = 4 if desc.prec in (0, 3) else 2  # FP32 and split-BF16 store FP32 tiles
     masks = 9 * 32
     if name == 'srf_mlp_weight_grads':
         return n * elem * sum(i + o for i, o in dims) + 4 * sum(i * o + o for i, o in dims)
@@ -150,10 +156,10 @@ def build_model(precision, dev, seed=0):
     return configs, model
 
 
-def hip_train_leg(args, dev):
+def hip_train_leg(args, dev, precision='bf16'):
     import torch
     from srf_hip import _lib, synthetic
-    _, model = build_model('bf16', dev)
+    _, model = build_model(precision, dev)
     model.train()
     params = [p for p in model.parameters() if p.requires_grad]
     opt = torch.optim.Adam(params, lr=5e-4, betas=(0.9, 0.999), fused=True)
@@ -171,7 +177,7 @@ def hip_train_leg(args, dev):
     for _ in range(args.warmup):
         step()
     if TUNER is not None:   # the weight-gradient schedule, measured on this step (untimed)
-        TUNER.tune(f'nerf_train_{args.rays}rays', 'dw_chunk', step)
+        TUNER.tune(f'nerf_train_{args.rays}rays' + ('' if precision == 'bf16' else f'_{precision}'), 'dw_chunk', step)
     torch.cuda.synchronize()
     timer = KernelTimer(_lib)
     with timer:
@@ -187,12 +193,12 @@ def hip_train_leg(args, dev):
     return dict(seconds=t1 - t0, loss=float(loss.item()), kernels=kern, model=model)
 
 
-def hip_graph_train_leg(args, dev):
+def hip_graph_train_leg(args, dev, precision='bf16'):
     """The same train step captured once in a HIP graph (torch.cuda.graph: forward, hand-written
     backward kernels, capturable Adam) and replayed: one launch per step instead of hundreds."""
     import torch
     from srf_hip import synthetic
-    _, model = build_model('bf16', dev)
+    _, model = build_model(precision, dev)
     model.train()
     params = [p for p in model.parameters() if p.requires_grad]
     opt = torch.optim.Adam(params, lr=5e-4, betas=(0.9, 0.999), capturable=True, fused=True)
@@ -502,6 +508,19 @@ def _events_time(fn, reps):
     return s.elapsed_time(e) / reps * 1e-3
 
 
+TENSORF_COUNTERS = REPO / 'profiles' / 'tensorf_train_step_8192rays.json'
+
+
+def tensorf_counter_bytes() -> dict:
+    """Scaled FETCH_SIZE + WRITE_SIZE bytes per launch of the factor gathers of the 300^2 TensoRF step,
+    from the committed rocprof summary (empty when absent)."""
+    try:
+        d = json.loads(TENSORF_COUNTERS.read_text())
+    except (OSError, ValueError):
+        return {}
+    return {k: int(v['hbm_bytes_per_launch']) for k, v in d.get('kernels', {}).items() if 'hbm_bytes_per_launch' in v}
+
+
 def hbm_leg(dev, tmodel, out, reps=10):
     """HBM rooflines of the memory-bound kernels on the TensoRF workload: every kernel is timed
     alone (HIP events around `reps` back-to-back launches) and priced with its algorithmic bytes.
@@ -571,20 +590,28 @@ def hbm_leg(dev, tmodel, out, reps=10):
     t = _events_time(lambda: torch.autograd.grad(cw['weights'], sg, gw, retain_graph=True), reps)
     res['composite_bwd'] = (t, n * 16 + z.shape[0] * 72, f'{z.shape[0]} rays x {z.shape[1]}')
     table = {}
+    counters = tensorf_counter_bytes()
     for k, (sec, byts, what) in res.items():
         gbs = byts / sec / 1e9
-        table[k] = {'us': round(sec * 1e6, 1), 'algorithmic_bytes': int(byts), 'GB/s': round(gbs, 1),
-                    'frac_hbm': round(gbs / PEAK_HBM_GBS, 4), 'shape': what}
         if k.startswith('vm_'):
-            # every tap counted: the factors (69 MB here) largely stay in the L2 / 256 MB MALL, so
-            # the rate can exceed the HBM peak; rocprof's FETCH / WRITE give the HBM bytes
-            table[k]['bytes_kind'] = 'factor taps (L2 / MALL hits counted)'
+            # every factor tap counted: the factors (69 MB here) largely stay in the L2 / 256 MB MALL, so
+            # this is a cache-side rate, not an HBM fraction; the HBM rate comes from the rocprof
+            # FETCH_SIZE + WRITE_SIZE bytes of the same kernel (profiles/), over this launch time
+            table[k] = {'us': round(sec * 1e6, 1), 'tap_bytes': int(byts), 'tap_GB/s': round(gbs, 1), 'shape': what}
+            hbm = counters.get(k)
+            if hbm:
+                table[k].update({'counter_hbm_bytes': hbm, 'hbm_GB/s': round(hbm / sec / 1e9, 1),
+                                 'frac_hbm': round(hbm / sec / 1e9 / PEAK_HBM_GBS, 4)})
+        else:
+            table[k] = {'us': round(sec * 1e6, 1), 'algorithmic_bytes': int(byts), 'GB/s': round(gbs, 1),
+                        'frac_hbm': round(gbs / PEAK_HBM_GBS, 4), 'shape': what}
 
     return table
 
 
-def oracle_step_fn(model, dev, n_rays, seed=0):
-    """The same training step restated in plain PyTorch FP32 (oracle) on device `dev`."""
+def oracle_step_fn(model, dev, n_rays, seed=0, coarse_only=False):
+    """The same training step restated in plain PyTorch FP32 (oracle) on device `dev`.  coarse_only:
+    the plain coarse model alone with the photometric MSE (the 1024-ray CPU configuration)."""
     import torch
     import oracle.train_step as ots
     from srf_hip import synthetic
@@ -594,6 +621,8 @@ def oracle_step_fn(model, dev, n_rays, seed=0):
     names = {'coarse': model.coarse_model, 'fine': model.fine_model,
              'points_augmentation': model.augmented_models[0]['coarse_model'],
              'views_augmentation': model.augmented_models[1]['coarse_model']}
+    if coarse_only:
+        names = {'coarse': model.coarse_model}
     leaves = []
     for key, m in names.items():
         params[key] = {k: v.detach().to(dev, torch.float32).clone().requires_grad_(True) for k, v in m.state_dict().items()}
@@ -610,7 +639,10 @@ def oracle_step_fn(model, dev, n_rays, seed=0):
         u_f = torch.rand(n_rays, 128, device=dev)
         noise = {k: torch.randn(n_rays * (192 if k == 'fine' else 64), device=dev) for k in names}
         out = ots.render_train(params, pid, K, E, res, u_coarse=u_c, u_fine=u_f, noise=noise)
-        loss = synthetic.train_loss(out, batch)
+        if coarse_only:
+            loss = ((out['rgb_coarse'] - batch['target_rgb']) ** 2).mean()
+        else:
+            loss = synthetic.train_loss(out, batch)
         opt.zero_grad(set_to_none=True)
         loss.backward()
         opt.step()
@@ -618,39 +650,147 @@ def oracle_step_fn(model, dev, n_rays, seed=0):
 
 
 def plain_gpu_leg(args, dev, model):
+    """The plain form on this GPU: the oracle's PyTorch FP32 step (allow_tf32 off), 5 warm-ups, then
+    the median of args.plain_steps steps, each timed with HIP events."""
     import torch
     torch.backends.cuda.matmul.allow_tf32 = False
     torch.backends.cudnn.allow_tf32 = False
     step = oracle_step_fn(model, dev, args.rays)
-    for _ in range(2):
+    for _ in range(5):
         step()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    n = args.plain_steps
-    for _ in range(n):
-        step()
-    torch.cuda.synchronize()
-    return args.rays * n / (time.perf_counter() - t0)
-
-
-def cpu_baseline_leg(args, model):
-    """The oracle (plain PyTorch FP32 restatement) on the host cores, bounded sample."""
-    import torch
-    threads = os.cpu_count() or 1
-    torch.set_num_threads(threads)
-    n = args.cpu_rays
-    step = oracle_step_fn(model, 'cpu', n)
-    step()
     times = []
-    budget = time.perf_counter() + args.cpu_seconds
-    while len(times) < 3 and (not times or time.perf_counter() < budget):
+    for _ in range(args.plain_steps):
+        s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        s.record()
+        step()
+        e.record()
+        e.synchronize()
+        times.append(s.elapsed_time(e) * 1e-3)
+    med = statistics.median(times)
+    return {'value': round(args.rays / med, 1), 'unit': 'rays/s', 'ms_per_step': round(med * 1e3, 3),
+            'what': f'same step in plain PyTorch FP32 (oracle) on this GPU, allow_tf32=False; median of '
+                    f'{len(times)} steps after 5 warm-ups'}
+
+
+def cpu_model_name() -> str:
+    try:
+        out = subprocess.run(['lscpu'], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            if line.startswith('Model name:'):
+                return line.split(':', 1)[1].strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    try:
+        for line in open('/proc/cpuinfo'):
+            if line.startswith('model name'):
+                return line.split(':', 1)[1].strip()
+    except OSError:
+        pass
+    return 'unknown'
+
+
+def _cpu_median(step, warmup, steps):
+    for _ in range(warmup):
+        step()
+    times = []
+    for _ in range(steps):
         t0 = time.perf_counter()
         step()
         times.append(time.perf_counter() - t0)
-    med = statistics.median(times)
-    return {'value': round(n / med, 2), 'unit': 'rays/s', 'cores': threads, 'kind': 'port',
-            'sample': f'{n} rays x {len(times)} timed steps (median {med:.2f} s/step) of the same '
-                      f'train step (fwd+bwd+Adam, 4 MLPs) in the PyTorch FP32 oracle on CPU'}
+    return statistics.median(times), len(times)
+
+
+def cpu_baseline_leg(args, model):
+    """The oracle (plain PyTorch FP32 restatement) on the host cores: 3 warm-up steps, then the median
+    of at least 10.  `cpu_baseline` is the headline step (coarse + fine + augmented, fwd+bwd+Adam) on
+    a bounded sample of args.cpu_rays rays; `cfg1` is the plain coarse model on 1024 rays in full."""
+    import torch
+    threads = os.cpu_count() or 1
+    torch.set_num_threads(threads)
+    model_name = cpu_model_name()
+    n = args.cpu_rays
+    med, k = _cpu_median(oracle_step_fn(model, 'cpu', n), 3, args.cpu_steps)
+    base = {'value': round(n / med, 2), 'unit': 'rays/s', 'cores': threads, 'kind': 'port', 'cpu': model_name,
+            'sample': f'{n} of the 4096 rays per step, median of {k} timed steps after 3 warm-ups '
+                      f'({med:.3f} s/step) of the same train step (fwd+bwd+Adam, 4 MLPs) in the PyTorch '
+                      f'FP32 oracle on CPU'}
+    med1, k1 = _cpu_median(oracle_step_fn(model, 'cpu', 1024, coarse_only=True), 3, args.cpu_steps)
+    cfg1 = {'value': round(1024 / med1, 2), 'unit': 'rays/s', 'ms_per_step': round(med1 * 1e3, 1),
+            'cores': threads, 'cpu': model_name,
+            'workload': 'plain Simple-NeRF coarse model, FP32, 1024 rays x 64 stratified samples, MSE, fwd+bwd+Adam, '
+                        'in full on the CPU', 'steps': f'median of {k1} after 3 warm-ups',
+            'gflop_per_s': round(1024 * 227.9e6 / med1 / 1e9, 1)}
+    return base, cfg1
+
+
+def step_flop(rays: int) -> float:
+    """Algorithmic MLP FLOPs of one training step: 2 MAC per point forward, x3 for fwd+bwd, over the
+    coarse 64 samples through the main, points- and views-augmented models and the 192 merged fine
+    samples through the fine model (1.322 GFLOP per ray)."""
+    per_ray = 64 * (MACS[(63, True)] + MACS[(21, True)] + MACS[(63, False)]) + 192 * MACS[(63, True)]
+    return 3 * 2.0 * per_ray * rays
+
+
+def mfma_roofline(kern, ms_per_step, args, dev, split=False):
+    """MFMA roofline of the NeRF train step (the MLP products are the bound): the dominant kernel's
+    algorithmic FLOPs per launch (2 MAC per point) over its average launch time (HIP events on the
+    launch stream), and the whole step's FLOPs over the step time.  Split-BF16 kernels also report the
+    BF16 MFMA work they issue (six or three plane products per algorithmic product).  The HBM view of
+    the same kernel (algorithmic bytes over the same time) is a secondary field."""
+    single = {k: v for k, v in kern.items() if k not in ('srf_mlp_pack',)}
+    dom_name, dom = max(single.items(), key=lambda kv: kv[1]['ms'])
+    if dom['flop'] <= 0:
+        return None
+    sec = dom['ms'] * 1e-3
+    tflops = dom['flop'] / sec / 1e12
+    gbs = dom['bytes'] / sec / 1e9
+    flop_step = step_flop(args.rays)
+    roof = {'bound': 'mfma', 'kernel': dom_name, 'achieved': round(tflops, 1), 'peak': PEAK_BF16_TFLOPS,
+            'unit': 'TFLOP/s', 'frac': round(tflops / PEAK_BF16_TFLOPS, 4),
+            'traffic': None if split else profiled_traffic(dom_name, args.rays),
+            'flop_per_launch': int(dom['flop'] / dom['calls']),
+            'avg_launch_ms': round(dom['ms'] / dom['calls'], 4),
+            'step': {'flop': flop_step, 'ms': round(ms_per_step, 3),
+                     'tflops': round(flop_step / (ms_per_step * 1e-3) / 1e12, 1),
+                     'frac': round(flop_step / (ms_per_step * 1e-3) / 1e12 / PEAK_BF16_TFLOPS, 4)},
+            'hbm_view': {'algorithmic_bytes_per_launch': int(dom['bytes'] / dom['calls']),
+                         'achieved_GBps': round(gbs, 1), 'frac': round(gbs / PEAK_HBM_GBS, 4)}}
+    if split:
+        k = SPLIT_PRODUCTS.get(dom_name, 1)
+        roof['issued_bf16_tflops'] = round(k * tflops, 1)
+        roof['issued_frac'] = round(k * tflops / PEAK_BF16_TFLOPS, 4)
+        roof['vs_fp32_mfma_peak'] = round(tflops / PEAK_FP32_TFLOPS, 4)
+    if roof['traffic']:
+        roof['traffic_GBps'] = round(roof['traffic'] / (roof['avg_launch_ms'] * 1e-3) / 1e9, 1)
+    if not split:
+        roof['copy_GBps'] = device_copy_rate(dev)
+    return roof
+
+
+def split_train_leg(args, dev):
+    """The same train step with split-BF16 MLP operands (FP32-level forward, two-plane backward): the
+    mode whose gradients meet the 1e-3 bar (as the FP32 kernels' do), eager and graph-replayed."""
+    tr = hip_train_leg(args, dev, 'bf16_split')
+    eager = args.rays * args.steps / tr['seconds']
+    value, ms = eager, tr['seconds'] * 1000.0 / args.steps
+    mode = 'eager'
+    if not args.no_graph:
+        gr = hip_graph_train_leg(args, dev, 'bf16_split')
+        value, ms = args.rays * args.steps / gr['seconds'], gr['seconds'] * 1000.0 / args.steps
+        mode = 'hip_graph_replay'
+    kern = tr['kernels']
+    out = {'value': round(value, 1), 'unit': 'rays/s', 'ms_per_step': round(ms, 3), 'step_mode': mode,
+           'eager_rays_per_s': round(eager, 1),
+           'precision': 'split BF16: FP32 activations and stored tiles, products on BF16 MFMAs over operands '
+                        'split into BF16 planes (3 in the forward, 2 in the backward chain and weight gradients)',
+           'accuracy': 'rgb ~1e-6 and gradients as the FP32 kernels\' against FP64 (tests/test_gpu_model.py, '
+                       'tests/test_gpu_mlp.py)',
+           'roofline': mfma_roofline(kern, ms, args, dev, split=True),
+           'kernels': {k: {'calls': v['calls'], 'ms_per_step': round(v['ms'] / args.steps, 3),
+                           **({'tflops': round(v['flop'] / (v['ms'] * 1e-3) / 1e12, 1)} if v['flop'] else {})}
+                       for k, v in kern.items()}}
+    del tr
+    return out
 
 
 def main():
@@ -662,9 +802,9 @@ def main():
     ap.add_argument('--render-rays', type=int, default=65536)
     ap.add_argument('--render-chunk', type=int, default=65536)
     ap.add_argument('--no-image', action='store_true', help='skip the full-image render leg')
-    ap.add_argument('--plain-steps', type=int, default=5)
-    ap.add_argument('--cpu-rays', type=int, default=128)
-    ap.add_argument('--cpu-seconds', type=float, default=20.0)
+    ap.add_argument('--plain-steps', type=int, default=20)
+    ap.add_argument('--cpu-rays', type=int, default=512)
+    ap.add_argument('--cpu-steps', type=int, default=10, help='timed CPU steps (after 3 warm-ups) per CPU leg')
     ap.add_argument('--no-baselines', action='store_true')
     ap.add_argument('--tensorf-rays', type=int, default=8192)
     ap.add_argument('--tensorf-steps', type=int, default=5)
@@ -673,6 +813,7 @@ def main():
     ap.add_argument('--ragged-steps', type=int, default=3)
     ap.add_argument('--no-ragged', action='store_true', help='skip the uneven-batch TensoRF leg')
     ap.add_argument('--no-graph', action='store_true', help='report the eager step only')
+    ap.add_argument('--no-split', action='store_true', help='skip the split-BF16 train leg')
     ap.add_argument('--no-autotune', action='store_true', help="keep the library's built-in launch schedule")
     ap.add_argument('--copy-calibration-mb', type=int, default=0,
                     help='after the timed legs, run 3 device copies of this many MiB (a plain copy kernel '
@@ -703,35 +844,7 @@ def main():
         except Exception as exc:  # reported, not hidden: value is then the eager step
             graph_info = {'captured': False, 'error': f'{type(exc).__name__}: {exc}'[:300]}
     kern = tr['kernels']
-    # dominant kernel: the single-kernel entry point with the largest total time
-    single = {k: v for k, v in kern.items() if k not in ('srf_mlp_pack',)}
-    dom_name, dom = max(single.items(), key=lambda kv: kv[1]['ms'])
-    roof = None
-    if dom['flop'] > 0:
-        # the binding resource: the one whose peak gives the longer minimum time for the
-        # kernel's algorithmic FLOPs and bytes (the weight-gradient GEMM, ~120 FLOP per byte, sits
-        # below the 312 FLOP/B ridge of 2.5 PFLOP/s over 8 TB/s: HBM-bound)
-        sec = dom['ms'] * 1e-3
-        t_mfma = dom['flop'] / (PEAK_BF16_TFLOPS * 1e12)
-        t_hbm = dom['bytes'] / (PEAK_HBM_GBS * 1e9)
-        tflops = dom['flop'] / sec / 1e12
-        gbs = dom['bytes'] / sec / 1e9
-        if t_hbm >= t_mfma:
-            roof = {'bound': 'hbm', 'kernel': dom_name, 'achieved': round(gbs, 1), 'peak': PEAK_HBM_GBS,
-                    'unit': 'GB/s', 'frac': round(gbs / PEAK_HBM_GBS, 4)}
-        else:
-            roof = {'bound': 'mfma', 'kernel': dom_name, 'achieved': round(tflops, 2), 'peak': PEAK_BF16_TFLOPS,
-                    'unit': 'TFLOP/s', 'frac': round(tflops / PEAK_BF16_TFLOPS, 4)}
-        roof.update({'traffic': profiled_traffic(dom_name, args.rays),
-                     'algorithmic_bytes_per_launch': int(dom['bytes'] / dom['calls']),
-                     'flop_per_launch': int(dom['flop'] / dom['calls']),
-                     'tflops': round(tflops, 1), 'mfma_frac': round(tflops / PEAK_BF16_TFLOPS, 4),
-                     'avg_launch_ms': round(dom['ms'] / dom['calls'], 4)})
-        if roof['traffic']:
-            # the measured HBM traffic over the same launch time, beside what a plain device copy moves
-            # on this GPU (the practical ceiling of a streaming kernel, below the nominal peak)
-            roof['traffic_GBps'] = round(roof['traffic'] / (roof['avg_launch_ms'] * 1e-3) / 1e9, 1)
-        roof['copy_GBps'] = device_copy_rate(dev)
+    roof = mfma_roofline(kern, ms, args, dev)
     per_kernel = {k: {'calls': v['calls'], 'ms_per_step': round(v['ms'] / args.steps, 3),
                       **({'tflops': round(v['flop'] / (v['ms'] * 1e-3) / 1e12, 1),
                           'GB/s': round(v['bytes'] / (v['ms'] * 1e-3) / 1e9, 1)} if v['flop'] else {})}
@@ -745,6 +858,7 @@ def main():
         for _ in range(3):
             dst.copy_(src)
         torch.cuda.synchronize()
+    split = None if args.no_split else split_train_leg(args, dev)
     result = {
         'metric': METRIC, 'value': round(value, 1), 'unit': 'rays/s', 'n_gpus': 1, 'steps': args.steps,
         'warmup': args.warmup, 'ms_per_step': round(ms, 3), 'higher_is_better': True, 'scaling': 'weak',
@@ -759,6 +873,7 @@ def main():
         'render_rays_per_s': round(render, 1),
         **({'render_image': image} if image else {}),
         'kernels': per_kernel,
+        **({'train_bf16_split': split} if split else {}),
     }
     if not args.no_tensorf:
         tf = tensorf_train_leg(args, dev)
@@ -771,10 +886,11 @@ def main():
         result['autotune'] = TUNER.results
     if not args.no_baselines:
         plain = plain_gpu_leg(args, dev, tr['model'])
-        result['plain_fp32_gpu'] = {'value': round(plain, 1), 'unit': 'rays/s',
-                                    'what': 'same step in plain PyTorch FP32 (oracle) on this GPU, allow_tf32=False'}
-        result['speedup_vs_plain_fp32_gpu'] = round(value / plain, 2)
-        result['cpu_baseline'] = cpu_baseline_leg(args, tr['model'])
+        result['plain_fp32_gpu'] = plain
+        result['speedup_vs_plain_fp32_gpu'] = round(value / plain['value'], 2)
+        if split:
+            split['speedup_vs_plain_fp32_gpu'] = round(split['value'] / plain['value'], 2)
+        result['cpu_baseline'], result['cpu_cfg1'] = cpu_baseline_leg(args, tr['model'])
     print(json.dumps(result))
 
 

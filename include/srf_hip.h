@@ -145,10 +145,15 @@ int srf_composite_bwd(int n_rays, int n_samples, const int32_t* ray_offsets, con
  *   view_dependent: 1 -> feature_linear + views_linears + views_output_linear (rgb = sigmoid of
  *                   the view head); 0 -> 4-channel pts_output_linear (views-augmented model);
  *   sigma_softplus: 0 -> relu(raw + noise) (reference, :738-743); 1 -> softplus(raw + noise);
- *   prec:           SRF_PREC_F32 / SRF_PREC_BF16 / SRF_PREC_F16 operands of the MFMA tiles
- *                   (accumulation is always FP32).  SRF_PREC_F16 covers the inference forward
- *                   only (srf_mlp_forward with workspace NULL); its training entry points
- *                   return SRF_EUNSUPPORTED. */
+ *   prec:           SRF_PREC_F32 / SRF_PREC_BF16 / SRF_PREC_F16 / SRF_PREC_BF16_SPLIT operands of
+ *                   the MFMA tiles (accumulation is always FP32).  SRF_PREC_F16 covers the
+ *                   inference forward only (srf_mlp_forward with workspace NULL); its training
+ *                   entry points return SRF_EUNSUPPORTED.  SRF_PREC_BF16_SPLIT keeps FP32
+ *                   activations and stored tiles (the SRF_PREC_F32 workspace) and multiplies on
+ *                   BF16 MFMAs over split operands x = x_0 + x_1 (+ x_2), each plane the BF16
+ *                   rounding of what the previous planes leave: three planes (six plane
+ *                   products, FP32-level rounding) in the forward, two (three products) in the
+ *                   backward chain and the weight gradients. */
 typedef struct srf_mlp_desc {
   int trunk_pe;
   int pts_pe_degree;
@@ -161,6 +166,7 @@ typedef struct srf_mlp_desc {
 #define SRF_PREC_F32 0
 #define SRF_PREC_BF16 1
 #define SRF_PREC_F16 2
+#define SRF_PREC_BF16_SPLIT 3
 
 /* Reference parameter order for srf_mlp_pack / srf_mlp_backward (12 entries; NULL where the
  * variant has no such layer): pts_linears.0..7, feature_linear, pts_output_linear,
@@ -179,7 +185,7 @@ int srf_mlp_pack(const srf_mlp_desc* desc, const float* const* weights, const fl
 
 /* Bytes of the training workspace for n_points (saved activations, ReLU bit masks and
  * activation gradients, all feature-major).  Training launches (forward with a workspace,
- * backward) take up to about 3.9 M points (FP32) or 7.8 M points (16-bit operands) per call, so
+ * backward) take up to about 3.9 M points (FP32, split BF16) or 7.8 M points (16-bit operands) per call, so
  * that each workspace array stays below 4 GB; larger calls return SRF_EINVAL. */
 size_t srf_mlp_workspace_bytes(const srf_mlp_desc* desc, int n_points);
 
@@ -433,6 +439,59 @@ int srf_tune_set(int knob, int64_t value);
 
 /* Current value of knob (0 = built-in choice), -1 for an unknown knob. */
 int64_t srf_tune_get(int knob);
+
+/* ---------------------------------------------------------------- training losses (f1) */
+
+/* Weighted sum of MSE terms over the selected rays:
+ *   loss = sum_t weights[t] * mean_{i selected} mean_c (preds[t][i,c] - target[i,c])^2.
+ * Replaces MSE.compute_mse summed over models_to_regularize (src/loss_functions/MSE14.py:29-94; c = 3,
+ * target_rgb, indices_mask_nerf) and SparseDepthMSE.compute_depth_loss (src/loss_functions/
+ * SparseDepthMSE14.py:29-95; c = 1, the sparse depths, indices_mask_sparse_depth), and
+ * CoarseFineConsistencyLoss.compute_loss_sd (src/loss_functions/CoarseFineConsistencyLoss34.py:166-181;
+ * target = the detached fine depth).
+ *   preds: host array of n_terms (<= 8) device pointers to [n, channels]; weights: host array of n_terms;
+ *   select [n] bytes (nonzero = selected) or NULL for all rays.
+ * Outputs (device): loss [1], term_values [n_terms] or NULL (each term's mean), maps [n_terms, n] or NULL
+ * (each ray's channel mean, the reference's loss maps; 0 on rays not selected), count [1] (the number of
+ * selected rays; an empty selection gives 0, as the reference's `if numel() > 0` does).  One block,
+ * fixed-order sums: no host synchronisation, the same value on every run. */
+int srf_mse_terms_fwd(int n_terms, const float* const* preds, const float* weights, const float* target,
+                      const uint8_t* select, int n, int channels, float* loss, float* term_values, float* maps,
+                      int* count, void* stream);
+
+/* Backward of srf_mse_terms_fwd: d_preds[t][i,c] = g_loss weights[t] 2 (preds[t][i,c] - target[i,c]) /
+ * (channels count) on selected rays, 0 elsewhere.  g_loss [1] and count [1] are device scalars;
+ * d_preds: host array of n_terms device pointers (NULL entries are skipped). */
+int srf_mse_terms_bwd(int n_terms, const float* const* preds, const float* weights, const float* target,
+                      const uint8_t* select, int n, int channels, const int* count, const float* g_loss,
+                      float* const* d_preds, void* stream);
+
+/* Reprojection-gated depth consistency between two depth estimates of the same rays.  Replaces
+ * compute_loss_nerf of AugmentationsDepthLoss (src/loss_functions/AugmentationsDepthLoss11.py:108-181,
+ * both_invalid_rule = 1) and of CoarseFineConsistencyLoss (src/loss_functions/
+ * CoarseFineConsistencyLoss34.py:85-164, both_invalid_rule = 0), with CommonUtils.reproject
+ * (src/utils/CommonUtils04.py:227-253).  Per selected ray: both depths' points o + d z reprojected into
+ * the view with the nearest camera centre (rounded half to even), the patch_y x patch_x ground-truth
+ * patches around the source pixel and the two reprojections (zero padded as the reference pads), their
+ * RMSEs, the gate masks, and the loss maps map1 = mask2 (d1 - d2)^2, map2 = mask1 mask2 (d2 - d1)^2 (the
+ * reference zeroes its inputs in place, so its second term only counts rays where both masks hold).
+ *   depth1, depth2 [n]; rays_o, rays_d [n,3] (world); pixel_id [n,3] int32 (image, x, y);
+ *   select [n] bytes or NULL (indices_mask_nerf); poses [n_views,4,4] camera-to-world (extrinsics_all);
+ *   intrinsic [3,3] (the first view's, as the reference uses); images [n_views,height,width,channels].
+ * Outputs: map1, map2 [n] (0 on rays not selected), flags [n] (bit 0 mask1, bit 1 mask2, bit 2
+ * selected), loss [1] = mean(map1) + mean(map2) over the selected rays, count [1]. */
+int srf_depth_consistency_fwd(int n, const float* depth1, const float* depth2, const float* rays_o,
+                              const float* rays_d, const int32_t* pixel_id, const uint8_t* select, int n_views,
+                              const float* poses, const float* intrinsic, const float* images, int height,
+                              int width, int channels, int patch_x, int patch_y, float rmse_threshold,
+                              int both_invalid_rule, float* map1, float* map2, uint8_t* flags, float* loss,
+                              int* count, void* stream);
+
+/* Backward of srf_depth_consistency_fwd: d_depth1 = g 2 (d1 - d2) / count where mask2, d_depth2 =
+ * g 2 (d2 - d1) / count where mask1 and mask2, 0 elsewhere (either output may be NULL). */
+int srf_depth_consistency_bwd(int n, const float* depth1, const float* depth2, const uint8_t* flags,
+                              const int* count, const float* g_loss, float* d_depth1, float* d_depth2,
+                              void* stream);
 
 /* Lane-layout self-test of the MFMA building blocks (one 16x16 tile): d = a b with
  * a [16,K], b [K,16] row-major FP32 (rounded to the operand type), K = 4 for SRF_PREC_F32 and

@@ -21,9 +21,11 @@ AUG_CFG = {
 
 
 def render_train(params: dict, pixel_id, K, E, resolution, *, u_coarse=None, u_fine=None, noise=None,
-                 n_coarse=64, n_fine=128, near=1.0, white_bkgd=False, augment=True):
+                 n_coarse=64, n_fine=128, near=1.0, white_bkgd=False, augment=True, z_fine=None):
     """params: {'coarse'|'fine'|'points_augmentation'|'views_augmentation': reference-named tensors}.
-    K [N,3,3], E [N,4,4] per ray.  noise: dict name -> [N*S] scaled sigma noise (or None)."""
+    K [N,3,3], E [N,4,4] per ray.  noise: dict name -> [N*S] scaled sigma noise (or None).
+    z_fine: the merged [N, n_coarse + n_fine] fine depths to render instead of drawing them from
+    the coarse weights (a parity check conditioned on identical fine samples)."""
     rays_o, rays_d = on.get_rays(pixel_id, K, E)
     o_ndc, d_ndc = on.get_ndc_rays(rays_o, rays_d, resolution, K, near)
     view_dirs = on.get_view_dirs(rays_d)
@@ -50,7 +52,7 @@ def render_train(params: dict, pixel_id, K, E, resolution, *, u_coarse=None, u_f
                 for k, v in net_render(name, z_c).items():
                     out[f'{name}_{k}_coarse'] = v
     if 'fine' in params:
-        z_f, _ = on.fine_z_vals(z_c, vr['weights'], n_fine, u=u_fine)
+        z_f = on.fine_z_vals(z_c, vr['weights'], n_fine, u=u_fine)[0] if z_fine is None else z_fine.to(dt)
         for k, v in net_render('fine', z_f).items():
             out[f'{k}_fine'] = v
     return out

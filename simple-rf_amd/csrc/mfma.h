@@ -26,7 +26,17 @@ typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
 
-enum Prec { kF32 = 0, kBF16 = 1, kF16 = 2 };
+// kBFS3 / kBFS2: split-BF16 operands.  Activations, stored tiles and accumulators are FP32 (the
+// kF32 layouts); each MFMA operand x is split into BF16 planes x = x_0 + x_1 (+ x_2), every plane
+// the BF16 rounding of what the previous planes leave, and a product is the sum of the plane
+// products x_i y_j with i + j < planes (16x16x32 BF16 MFMAs into one FP32 accumulator).  Three
+// planes (six products) carry 24 significand bits, as FP32; two planes (three products) 16.
+enum Prec { kF32 = 0, kBF16 = 1, kF16 = 2, kBFS3 = 3, kBFS2 = 4 };
+
+template <int P>
+__host__ __device__ constexpr bool f32_layout() { return P == kF32 || P == kBFS3 || P == kBFS2; }
+template <int P>
+__host__ __device__ constexpr int split_planes() { return P == kBFS3 ? 3 : (P == kBFS2 ? 2 : 1); }
 
 // Round-to-nearest-even float -> bf16 bits (NaN kept quiet).
 __device__ __forceinline__ uint32_t f32_to_bf16_bits(float f) {
@@ -85,6 +95,23 @@ __device__ __forceinline__ f32x4 mfma16(const u32x4& a, const u32x4& b, const f3
 
 __device__ __forceinline__ f32x4 mfma32(float a, float b, const f32x4& c) {
   return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+
+// BF16 planes of two floats (a -> low half, b -> high half of each packed dword): plane 0 is the
+// round-to-nearest BF16 pair of (a, b), plane i + 1 the BF16 pair of what planes 0..i leave.  Each
+// remainder is exact in FP32 (x minus its own BF16 rounding drops only the bits that rounding
+// kept), so after three planes |x - sum| <= 2^-27 |x|, after two 2^-18 |x|.
+template <int NP>
+__device__ __forceinline__ void split_pair(float a, float b, uint32_t (&out)[NP]) {
+#pragma unroll
+  for (int i = 0; i < NP; ++i) {
+    const uint32_t h = pack2<kBF16>(a, b);
+    out[i] = h;
+    if (i + 1 < NP) {
+      a = a - __uint_as_float(h << 16);
+      b = b - __uint_as_float(h & 0xffff0000u);
+    }
+  }
 }
 
 // Direct global -> LDS copy of one 1 KB fragment by a whole wave (gfx950

@@ -71,6 +71,7 @@ __device__ inline int slot_k(int s, int g, int i) {
 __device__ void pack_element(const PackArgs& a, int l, int bwd, long idx) {
   const Plan& p = a.p;
   const bool f32 = p.prec == SRF_PREC_F32;
+  const bool split = p.prec == SRF_PREC_BF16_SPLIT;
   const int kpl = f32 ? 1 : 8;
   int rows_t, k_tiles;
   if (!bwd) {
@@ -97,7 +98,19 @@ __device__ void pack_element(const PackArgs& a, int l, int bwd, long idx) {
     v = ref_weight(a, l, kp, ref_col(p, l, 16 * T + jj));
   }
   uint8_t* dst = a.out + (bwd ? p.bwd_off[l] : p.fwd_off[l]);
-  if (f32) {
+  if (split) {
+    // BF16 planes of v, each 1 KB fragment (512 values of one (tile, k-step)) followed by its
+    // other planes: plane pl of fragment f at byte (f planes + pl) KB
+    const int np = bwd ? 2 : 3;
+    const long f = idx / 512, within = idx % 512;
+    uint16_t* d16 = reinterpret_cast<uint16_t*>(dst);
+    float r = v;
+    for (int pl = 0; pl < np; ++pl) {
+      const uint32_t h = f32_to_bf16_bits(r);
+      d16[(f * np + pl) * 512 + within] = (uint16_t)h;
+      r = r - bf16_bits_to_f32(h);
+    }
+  } else if (f32) {
     reinterpret_cast<float*>(dst)[idx] = v;
   } else if (p.prec == SRF_PREC_BF16) {
     reinterpret_cast<uint16_t*>(dst)[idx] = (uint16_t)f32_to_bf16_bits(v);
@@ -140,6 +153,7 @@ int fwd_launch(int prec, const FwdArgs& a, bool vd, bool tr, hipStream_t s) {
     case SRF_PREC_F32: return fwd_f32(a, vd, tr, s);
     case SRF_PREC_BF16: return fwd_bf16(a, vd, tr, s);
     case SRF_PREC_F16: return fwd_f16(a, vd, tr, s);  // inference only (tr -> SRF_EUNSUPPORTED)
+    case SRF_PREC_BF16_SPLIT: return fwd_bf16s(a, vd, tr, s);
     default: return SRF_EUNSUPPORTED;
   }
 }
@@ -148,6 +162,7 @@ int bwd_launch(int prec, const BwdArgs& a, bool vd, hipStream_t s) {
   switch (prec) {
     case SRF_PREC_F32: return bwd_f32(a, vd, s);
     case SRF_PREC_BF16: return bwd_bf16(a, vd, s);
+    case SRF_PREC_BF16_SPLIT: return bwd_bf16s(a, vd, s);
     default: return SRF_EUNSUPPORTED;
   }
 }
@@ -156,6 +171,7 @@ int dw_launch(int prec, const DwArgs& d, hipStream_t s) {
   switch (prec) {
     case SRF_PREC_F32: return dw_f32(d, s);
     case SRF_PREC_BF16: return dw_bf16(d, s);
+    case SRF_PREC_BF16_SPLIT: return dw_bf16s(d, s);
     default: return SRF_EUNSUPPORTED;
   }
 }

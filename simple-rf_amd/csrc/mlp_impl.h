@@ -57,6 +57,9 @@ struct Plan {
   int prec, view_dep, softplus, trunk_pe, pts_deg, dir_deg;
   int extra;  // computed inputs of the view layer: [PE_xyz[trunk_pe:], PE_dir]
   int elem;   // bytes per stored value
+  // bytes of the forward and of the transposed (backward) operand image per (output tile, input
+  // tile): 16 x 16 values, times the BF16 planes of the split precision
+  int fwd_tile_bytes, bwd_tile_bytes;
   int out_rows[kLayers], out_tiles[kLayers], reg_tiles[kLayers], comp_feats[kLayers],
       comp_tiles[kLayers], bwd_tiles[kLayers], present[kLayers];
   int bias_floats;  // padded biases of every present layer, contiguous from bias_off[0]
@@ -68,7 +71,7 @@ __host__ __device__ inline int round_up(int a, int b) { return (a + b - 1) / b *
 inline bool make_plan(const srf_mlp_desc* d, Plan* p) {
   if (!d) return false;
   memset(p, 0, sizeof(Plan));
-  if (d->prec < SRF_PREC_F32 || d->prec > SRF_PREC_F16) return false;
+  if (d->prec < SRF_PREC_F32 || d->prec > SRF_PREC_BF16_SPLIT) return false;
   if (d->pts_pe_degree < 0 || d->pts_pe_degree > 16 || d->dirs_pe_degree < 0 || d->dirs_pe_degree > 16)
     return false;
   const int pts_dim = 3 * (2 * d->pts_pe_degree + 1);
@@ -79,7 +82,11 @@ inline bool make_plan(const srf_mlp_desc* d, Plan* p) {
   p->trunk_pe = d->trunk_pe;
   p->pts_deg = d->pts_pe_degree;
   p->dir_deg = d->dirs_pe_degree;
-  p->elem = d->prec == SRF_PREC_F32 ? 4 : 2;
+  const bool split = d->prec == SRF_PREC_BF16_SPLIT;
+  p->elem = (d->prec == SRF_PREC_F32 || split) ? 4 : 2;
+  // split: forward image in three BF16 planes, transposed image in two
+  p->fwd_tile_bytes = split ? 3 * 512 : 256 * p->elem;
+  p->bwd_tile_bytes = split ? 2 * 512 : 256 * p->elem;
   p->extra = p->view_dep ? (pts_dim - d->trunk_pe) + 3 * (2 * d->dirs_pe_degree + 1) : 0;
   if (p->extra > 96) return false;
   if (!p->view_dep && pts_dim != d->trunk_pe) return false;
@@ -109,12 +116,12 @@ inline bool make_plan(const srf_mlp_desc* d, Plan* p) {
     p->bwd_tiles[l] = round_up(p->out_tiles[l], 2);
     const int in_tiles = p->reg_tiles[l] + p->comp_tiles[l];
     p->fwd_off[l] = off;
-    off += (size_t)p->out_tiles[l] * in_tiles * 256 * p->elem;
+    off += (size_t)p->out_tiles[l] * in_tiles * p->fwd_tile_bytes;
   }
   for (int l = 1; l < kLayers; ++l) {
     if (!p->present[l]) continue;
     p->bwd_off[l] = off;
-    off += (size_t)p->reg_tiles[l] * p->bwd_tiles[l] * 256 * p->elem;
+    off += (size_t)p->reg_tiles[l] * p->bwd_tiles[l] * p->bwd_tile_bytes;
   }
   for (int l = 0; l < kLayers; ++l) {
     if (!p->present[l]) continue;
@@ -196,10 +203,14 @@ struct Tr<kF32> {
   static constexpr int CB = 2;
   using Act = f32x4;
 };
+template <>
+struct Tr<kBFS3> : Tr<kF32> {};
+template <>
+struct Tr<kBFS2> : Tr<kF32> {};
 
 template <int P>
 __device__ __forceinline__ typename Tr<P>::Act to_act(const f32x4& v) {
-  if constexpr (P == kF32) {
+  if constexpr (f32_layout<P>()) {
     return v;
   } else {
     u32x2 a;
@@ -212,14 +223,14 @@ __device__ __forceinline__ typename Tr<P>::Act to_act(const f32x4& v) {
 template <int P>
 __device__ __forceinline__ typename Tr<P>::Act zero_act() {
   typename Tr<P>::Act a;
-  if constexpr (P == kF32) a = f32x4{0.f, 0.f, 0.f, 0.f};
+  if constexpr (f32_layout<P>()) a = f32x4{0.f, 0.f, 0.f, 0.f};
   else a = u32x2{0u, 0u};
   return a;
 }
 
 template <int P>
 __device__ __forceinline__ uint32_t act_elem_bits(const typename Tr<P>::Act& a, int r) {
-  if constexpr (P == kF32) return __float_as_uint(a[r]);
+  if constexpr (f32_layout<P>()) return __float_as_uint(a[r]);
   else return (r < 2 ? a.x : a.y) >> (16 * (r & 1)) & 0xffffu;
 }
 
@@ -256,6 +267,10 @@ struct ActQuads {
 };
 template <>
 struct ActQuads<kF32> : ActTiles<kF32> {};
+template <>
+struct ActQuads<kBFS3> : ActTiles<kBFS3> {};
+template <>
+struct ActQuads<kBFS2> : ActTiles<kBFS2> {};
 
 // sin and cos of x for |x| up to ~1e4 (positional-encoding arguments are |2^m x| < 1e3):
 // Cody-Waite reduction x = k pi/2 + r with pi/2 split into three floats applied by fma, then
@@ -292,11 +307,11 @@ struct PeRows {
   static constexpr int kParts = 64 / kNpt;     // lanes per point (FP32: two, levels split)
   static constexpr int kStride = 100;          // elements per row: up to 96 features, and
                                                // conflict-free 8 (16)-byte gathers
-  static constexpr int kBytes = kNpt * kStride * (P == kF32 ? 4 : 2);
-  using E = std::conditional_t<P == kF32, float, uint16_t>;
+  static constexpr int kBytes = kNpt * kStride * (f32_layout<P>() ? 4 : 2);
+  using E = std::conditional_t<f32_layout<P>(), float, uint16_t>;
 
   __device__ static SRF_AI E cvt(float v) {
-    if constexpr (P == kF32) return v;
+    if constexpr (f32_layout<P>()) return v;
     else return (uint16_t)(pack2<P>(v, 0.f) & 0xffffu);
   }
   // Features f in [f_lo, f_hi) of PE_L(x), written at row[f - shift]; this lane computes the
@@ -324,7 +339,7 @@ struct PeRows {
     const int lane = threadIdx.x & 63, g = lane >> 4, j = lane & 15;
     const int c0 = 16 * t + 4 * g;
     const E* src = rows + (16 * cb + j) * kStride + c0;
-    if constexpr (P == kF32) {
+    if constexpr (f32_layout<P>()) {
       f32x4 v = *reinterpret_cast<const f32x4*>(src);
 #pragma unroll
       for (int r = 0; r < 4; ++r) v[r] = c0 + r < n_valid ? v[r] : 0.f;
@@ -396,6 +411,13 @@ __device__ __forceinline__ void static_for(F&& f) {
 // traffic out of L2 by the number of waves per block and hides its latency.
 constexpr int kKC = 2;
 constexpr int kStageBytes = 16 * kKC * 1024;
+// Split-BF16 operands: a k-unit's fragment is one 1 KB plane per BF16 plane (3 KB forward, 2 KB
+// backward), and a chunk holds one k-unit when there are three planes, so that one buffer of the
+// stage is 48 KB (forward) or 64 KB (backward, two k-units).
+template <int P>
+constexpr int kc_of() { return split_planes<P>() == 3 ? 1 : kKC; }
+template <int P>
+constexpr int stage_half_bytes() { return 16 * kc_of<P>() * split_planes<P>() * 1024; }
 
 template <int P>
 __host__ __device__ constexpr int units(int tiles) { return P == kF32 ? tiles : tiles / 2; }
@@ -409,15 +431,18 @@ constexpr int vmcnt_only(int n) { return 0x0F70 | (n & 0xF) | ((n >> 4) << 14); 
 template <int P, int NT, int OUT, int TB, int C>
 __device__ __forceinline__ void stage_chunk(const uint8_t* __restrict__ A, int upr, int u0, uint8_t* buf) {
   constexpr int NU = NT / (P == kF32 ? 1 : 2);
-  constexpr int kn = (NU - C * kKC) < kKC ? (NU - C * kKC) : kKC;
+  constexpr int KC = kc_of<P>(), NPL = split_planes<P>();
+  constexpr int kn = (NU - C * KC) < KC ? (NU - C * KC) : KC;
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
 #pragma unroll
-  for (int i = 0; i < (OUT * kn + kWaves - 1) / kWaves; ++i) {
+  for (int i = 0; i < (OUT * kn * NPL + kWaves - 1) / kWaves; ++i) {
     const int f = w + kWaves * i;
-    if (f < OUT * kn) {
-      const int to = f / kn, kc = f - (f / kn) * kn;
-      const int frag = __builtin_amdgcn_readfirstlane((TB + to) * upr + u0 + C * kKC + kc);
-      const int dst = __builtin_amdgcn_readfirstlane((to * kKC + kc) * 1024);
+    if (f < OUT * kn * NPL) {
+      // fragment f: plane f % NPL of k-unit kc of output tile to
+      const int fk = f / NPL, pl = f - fk * NPL;
+      const int to = fk / kn, kc = fk - (fk / kn) * kn;
+      const int frag = __builtin_amdgcn_readfirstlane(((TB + to) * upr + u0 + C * KC + kc) * NPL + pl);
+      const int dst = __builtin_amdgcn_readfirstlane(((to * KC + kc) * NPL + pl) * 1024);
       lds_dma_1k(A + (size_t)frag * 1024, buf + dst);
     }
   }
@@ -470,8 +495,9 @@ __device__ __forceinline__ void mma_tiles(f32x4 (&acc)[OUT][Tr<P>::CB], const ui
   constexpr int CB = Tr<P>::CB;
   constexpr int TPU = P == kF32 ? 1 : 2;
   constexpr int NU = NT / TPU;
-  constexpr int NCH = (NU + kKC - 1) / kKC;
-  constexpr int kHalf = ST * kKC * 1024;  // one buffer of the double-buffered stage (2 kHalf bytes)
+  constexpr int KC = kc_of<P>(), NPL = split_planes<P>();
+  constexpr int NCH = (NU + KC - 1) / KC;
+  constexpr int kHalf = ST * KC * NPL * 1024;  // one buffer of the double-buffered stage (2 kHalf bytes)
   static_assert(OUT <= ST, "the stage buffer holds ST output tiles");
   const int lane = threadIdx.x & 63;
   stage_chunk<P, NT, OUT, TB, 0>(A, upr, u0, stage);
@@ -495,10 +521,10 @@ __device__ __forceinline__ void mma_tiles(f32x4 (&acc)[OUT][Tr<P>::CB], const ui
       });
       asm volatile("" ::: "memory");
     }
-    constexpr int kn = (NU - c * kKC) < kKC ? (NU - c * kKC) : kKC;
+    constexpr int kn = (NU - c * KC) < KC ? (NU - c * KC) : KC;
     static_for<0, kn>([&](auto KI) SRF_AI {
       constexpr int kc = decltype(KI)::value;
-      constexpr int u = c * kKC + kc;
+      constexpr int u = c * KC + kc;
       if constexpr (P == kF32) {
         f32x4 b[CB];
 #pragma unroll
@@ -507,11 +533,64 @@ __device__ __forceinline__ void mma_tiles(f32x4 (&acc)[OUT][Tr<P>::CB], const ui
         for (int r = 0; r < 4; ++r) {
 #pragma unroll
           for (int to = 0; to < OUT; ++to) {
-            const float a = *reinterpret_cast<const float*>(cur + (to * kKC + kc) * 1024 + (r * 64 + lane) * 4);
+            const float a = *reinterpret_cast<const float*>(cur + (to * KC + kc) * 1024 + (r * 64 + lane) * 4);
 #pragma unroll
             for (int cb = 0; cb < CB; ++cb) acc[to][cb] = mfma32(a, b[cb][r], acc[to][cb]);
           }
         }
+      } else if constexpr (NPL > 1) {
+        // Split BF16: the FP32 input tiles 2u, 2u + 1 of each column block become NPL BF16 B
+        // operands (slot order of the 16-bit map), the weight fragments are already split into
+        // NPL planes (stage layout: plane pl of (to, kc) at ((to KC + kc) NPL + pl) KB), and each
+        // output tile accumulates the NPL (NPL + 1) / 2 plane products i + j < NPL.
+        u32x4 b[NPL][CB];
+#pragma unroll
+        for (int cb = 0; cb < CB; ++cb) {
+          const f32x4 x0 = fin(2 * u, cb), x1 = fin(2 * u + 1, cb);
+          uint32_t q0[NPL], q1[NPL], q2[NPL], q3[NPL];
+          split_pair<NPL>(x0[0], x0[1], q0);
+          split_pair<NPL>(x0[2], x0[3], q1);
+          split_pair<NPL>(x1[0], x1[1], q2);
+          split_pair<NPL>(x1[2], x1[3], q3);
+#pragma unroll
+          for (int i = 0; i < NPL; ++i) b[i][cb] = u32x4{q0[i], q1[i], q2[i], q3[i]};
+        }
+        // output tiles in groups of two (the A planes of the next group are read while this
+        // group's MFMAs run; two tiles x two column blocks keep four independent accumulators
+        // between dependent MFMAs)
+        constexpr int kG = 2;
+        constexpr int NG = (OUT + kG - 1) / kG;
+        auto read_a = [&](auto GI, u32x4 (&a)[kG][NPL]) SRF_AI {
+          constexpr int gi = decltype(GI)::value;
+#pragma unroll
+          for (int q = 0; q < kG; ++q)
+            if (gi * kG + q < OUT)
+#pragma unroll
+              for (int pl = 0; pl < NPL; ++pl)
+                a[q][pl] = *reinterpret_cast<const u32x4*>(cur + (((gi * kG + q) * KC + kc) * NPL + pl) * 1024 + lane * 16);
+        };
+        u32x4 a_cur[kG][NPL], a_nxt[kG][NPL];
+        read_a(std::integral_constant<int, 0>{}, a_cur);
+        static_for<0, NG>([&](auto GI) SRF_AI {
+          constexpr int gi = decltype(GI)::value;
+          if constexpr (gi + 1 < NG) read_a(std::integral_constant<int, gi + 1>{}, a_nxt);
+          // smallest products first: (i, j) ordered by i + j descending
+#pragma unroll
+          for (int sum = NPL - 1; sum >= 0; --sum)
+#pragma unroll
+            for (int i = 0; i <= sum; ++i)
+#pragma unroll
+              for (int q = 0; q < kG; ++q)
+                if (gi * kG + q < OUT)
+#pragma unroll
+                  for (int cb = 0; cb < CB; ++cb)
+                    acc[gi * kG + q][cb] = mfma16<kBF16>(a_cur[q][i], b[sum - i][cb], acc[gi * kG + q][cb]);
+          __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+          for (int q = 0; q < kG; ++q)
+#pragma unroll
+            for (int pl = 0; pl < NPL; ++pl) a_cur[q][pl] = a_nxt[q][pl];
+        });
       } else {
         u32x4 b[CB];
 #pragma unroll
@@ -529,7 +608,7 @@ __device__ __forceinline__ void mma_tiles(f32x4 (&acc)[OUT][Tr<P>::CB], const ui
 #pragma unroll
           for (int q = 0; q < kG; ++q)
             if (gi * kG + q < OUT)
-              a[q] = *reinterpret_cast<const u32x4*>(cur + ((gi * kG + q) * kKC + kc) * 1024 + lane * 16);
+              a[q] = *reinterpret_cast<const u32x4*>(cur + ((gi * kG + q) * KC + kc) * 1024 + lane * 16);
         };
         u32x4 a_cur[kG], a_nxt[kG];
         read_a(std::integral_constant<int, 0>{}, a_cur);
@@ -674,7 +753,7 @@ template <int P>
 __device__ __forceinline__ void store_tile(Rsrc dst, uint32_t blk, const typename Tr<P>::Act (&v)[Tr<P>::CB]) {
   constexpr int CB = Tr<P>::CB;
   const uint32_t rec = lane_here() * 16u;
-  if constexpr (P == kF32) {
+  if constexpr (f32_layout<P>()) {
     // v[cb] = features 4g + 0..3 at point 16 cb + j; even lanes keep r0, r1 and take the
     // partner's; odd lanes keep r2, r3
     float rc[CB][2];
@@ -720,7 +799,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
     FwdArgs a) {
   constexpr int CB = Tr<P>::CB;
   using Act = typename Tr<P>::Act;
-  __shared__ __attribute__((aligned(16))) uint8_t stage[2 * kStageBytes];
+  __shared__ __attribute__((aligned(16))) uint8_t stage[2 * stage_half_bytes<P>()];
   // computed input features of each wave's points (trunk PE for layers 0 and 5, then the view
   // layer's [PE_xyz[trunk_pe:], PE_dir]) as B-operand tiles: 6 tiles x CB column blocks x 64
   // lanes.  Each lane reads back only its own entries, so no barrier is needed.
@@ -745,7 +824,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
   // layer 0 and between the feature and view layers; a block barrier follows the gathers,
   // before the next product's DMA refills the stage.
   using PR = PeRows<P>;
-  static_assert(kWaves * PR::kBytes <= 2 * kStageBytes, "PE rows fit in the stage");
+  static_assert(kWaves * PR::kBytes <= 2 * stage_half_bytes<P>(), "PE rows fit in the stage");
   typename PR::E* pe_rows = reinterpret_cast<typename PR::E*>(stage + wv * PR::kBytes);
   const int pe_pt = lane % PR::kNpt, pe_part = lane / PR::kNpt;
   auto load_pt = [&](const float* src, float (&v)[3]) SRF_AI {
@@ -967,7 +1046,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
     BwdArgs a) {
   constexpr int CB = Tr<P>::CB;
   using Act = typename Tr<P>::Act;
-  __shared__ __attribute__((aligned(16))) uint8_t stage[2 * kStageBytes];
+  __shared__ __attribute__((aligned(16))) uint8_t stage[2 * stage_half_bytes<P>()];
   const int lane = threadIdx.x & 63, g = lane >> 4, j = lane & 15;
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform: scalar store offsets
   const int wave = blockIdx.x * kWaves + wv;
@@ -1204,8 +1283,9 @@ __global__ __launch_bounds__(64 * kDwWaves) void mlp_dw_kernel(DwArgs a) {
   const int pb = chunk * a.chunk;
   const int pe = min(pb + a.chunk, a.cols);
   const size_t ldP = a.ldP;
-  constexpr int ELEM = P == kF32 ? 4 : 2;
-  constexpr int KPTS = P == kF32 ? 16 : 32;  // points per k-unit
+  constexpr bool F32L = f32_layout<P>();
+  constexpr int ELEM = F32L ? 4 : 2;
+  constexpr int KPTS = F32L ? 16 : 32;  // points per k-unit (split BF16: two units per MFMA k-step)
   const int qm = kRowWaves ? w : RM * (w >> 1), qn = kRowWaves ? 0 : 8 * (w & 1);
   bool mv[RM], nv[NC];
 #pragma unroll
@@ -1244,12 +1324,12 @@ __global__ __launch_bounds__(64 * kDwWaves) void mlp_dw_kernel(DwArgs a) {
       if (a.paired) {
         // lane (g, j) takes feature j's chunk of the record pair m = 4 u + g of the 2 KB block of
         // the points' wave (see store_tile): record (j >> 2, 2 m + ((j >> 1) & 1)), chunk j & 1
-        constexpr int PPW = P == kF32 ? 32 : 64;  // points per stored wave block
+        constexpr int PPW = F32L ? 32 : 64;  // points per stored wave block
         const int wb = p / PPW, u = (p % PPW) / KPTS;
         const int rec = 16 * (j >> 2) + 2 * (4 * u + g) + ((j >> 1) & 1);
         src = base + (size_t)row0 * ldP * ELEM + (size_t)wb * 2048 + (j & 1) * 1024 + rec * 16;
       } else {
-        const int pg = P == kF32 ? 4 * g : 8 * g;
+        const int pg = F32L ? 4 * g : 8 * g;
         src = base + (((size_t)(row0 + j)) * ldP + p + pg) * ELEM;
       }
       __builtin_amdgcn_global_load_lds((const void*)src,
@@ -1354,6 +1434,147 @@ __global__ __launch_bounds__(64 * kDwWaves) void mlp_dw_kernel(DwArgs a) {
   }
 }
 
+// Split-BF16 weight gradients: the FP32 dZ and X tiles (the FP32 layouts, DMA'd through the same
+// five-slot ring as the FP32 GEMM, 16 points per k-unit) are split into two BF16 planes and reduced
+// with 32x32x16 BF16 MFMAs, dW += dZ_hi X_hi^T + dZ_hi X_lo^T + dZ_lo X_hi^T.  One 16-point unit is
+// exactly one K = 16 step: lane (h, i) of an A operand holds row i of a 32-row block at the eight
+// points of lane groups 2h and 2h + 1 of the FP32 fragments (records 16 (2h) + (i mod 16) and
+// 16 (2h + 1) + (i mod 16)), and the B operand takes the same points of its columns, so the two
+// agree slot for slot.  Wave w owns rows 64 (w >> 1) .. + 63 (two 32-row blocks) and columns
+// 128 (w & 1) .. + 127 (four 32-column blocks) of a 256 x 256 tile.  32x32 D layout: lane l, register
+// r holds row 8 (r / 4) + 4 (l / 32) + (r mod 4), column l mod 32.
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+template <int P>
+__global__ __launch_bounds__(64 * kDwWaves) void mlp_dw_split_kernel(DwArgs a) {
+  static_assert(split_planes<P>() == 2, "two BF16 planes");
+  __shared__ __attribute__((aligned(16))) uint8_t ring[kDwSlots * kDwSlot];
+  const int lane = threadIdx.x & 63, g = lane >> 4, j = lane & 15;
+  const int h = lane >> 5, i32 = lane & 31;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int item = blockIdx.x;
+  int ji = 0;
+#pragma unroll 1
+  while (ji + 1 < a.n_jobs && item >= a.jobs[ji + 1].work_begin) ++ji;
+  const DwJob& jb = a.jobs[ji];
+  const int local = item - jb.work_begin;
+  const int nblk = jb.mblocks * jb.nblocks;
+  const int tile = local % nblk, chunk = local / nblk;
+  const int m0 = kDwTile * (tile % jb.mblocks), n0 = kDwTile * (tile / jb.mblocks);
+  const int pb = chunk * a.chunk;
+  const int pe = min(pb + a.chunk, a.cols);
+  const size_t ldP = a.ldP;
+  constexpr int KPTS = 16;
+  const int qm = 4 * (w >> 1), qn = 8 * (w & 1);  // first 16-row / 16-column tile of the wave
+  bool mv[2], nv[4];  // 32-row / 32-column blocks with data (their first 16-row tile inside the job)
+#pragma unroll
+  for (int b = 0; b < 2; ++b) mv[b] = m0 + 16 * (qm + 2 * b) < jb.dz_rows;
+#pragma unroll
+  for (int b = 0; b < 4; ++b) nv[b] = n0 + 16 * (qn + 2 * b) < jb.x_rows;
+  // DMA of one 16-point unit: the 16 dZ and 16 X fragments of the tile, as in mlp_dw_kernel (a row
+  // tile beyond the job's rows re-reads its first tile, so every wave issues four loads per unit)
+  auto dma_unit = [&](int p, uint8_t* buf) SRF_AI {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int f = w + kDwWaves * q;
+      const bool is_a = f < 16;
+      const int rt = is_a ? f : f - 16;
+      int row0 = (is_a ? m0 : n0) + 16 * rt;
+      if (row0 >= (is_a ? jb.dz_rows : jb.x_rows)) row0 = is_a ? m0 : n0;
+      const uint8_t* base = is_a ? jb.dz : jb.x;
+      const int wb = p / 32, u = (p % 32) / KPTS;
+      const int rec = 16 * (j >> 2) + 2 * (4 * u + g) + ((j >> 1) & 1);
+      const uint8_t* src = base + (size_t)row0 * ldP * 4 + (size_t)wb * 2048 + (j & 1) * 1024 + rec * 16;
+      __builtin_amdgcn_global_load_lds((const void*)src, (__attribute__((address_space(3))) void*)(buf + f * 1024),
+                                       16, 0, 0);
+    }
+  };
+  f32x16 acc[2][4];
+  float rs[2] = {0.f, 0.f};
+#pragma unroll
+  for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+    for (int cb = 0; cb < 4; ++cb)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) acc[rb][cb][r] = 0.f;
+  const bool do_db = jb.db != nullptr && n0 == 0 && qn == 0;
+  bool any_m = mv[0] || mv[1], any_n = false;
+#pragma unroll
+  for (int b = 0; b < 4; ++b) any_n |= nv[b];
+  const bool busy = any_m && (any_n || do_db);
+  const int n_units = (pe - pb) / KPTS;
+  // two BF16 planes of the eight FP32 values lane (h, i) takes from fragment f: records 2h, 2h + 1
+  // of its lane group pair
+  auto planes = [&](const uint8_t* fr, int f, u32x4& hi, u32x4& lo, float* sum) SRF_AI {
+    const int jj = i32 & 15;
+    const f32x4 v0 = *reinterpret_cast<const f32x4*>(fr + f * 1024 + ((2 * h) * 16 + jj) * 16);
+    const f32x4 v1 = *reinterpret_cast<const f32x4*>(fr + f * 1024 + ((2 * h + 1) * 16 + jj) * 16);
+    uint32_t t[4][2];
+    split_pair<2>(v0[0], v0[1], t[0]);
+    split_pair<2>(v0[2], v0[3], t[1]);
+    split_pair<2>(v1[0], v1[1], t[2]);
+    split_pair<2>(v1[2], v1[3], t[3]);
+    hi = u32x4{t[0][0], t[1][0], t[2][0], t[3][0]};
+    lo = u32x4{t[0][1], t[1][1], t[2][1], t[3][1]};
+    if (sum) *sum += ((v0[0] + v0[1]) + (v0[2] + v0[3])) + ((v1[0] + v1[1]) + (v1[2] + v1[3]));
+  };
+#pragma unroll
+  for (int u = 0; u < kDwSlots - 1; ++u)
+    if (u < n_units) dma_unit(pb + u * KPTS, ring + u * kDwSlot);
+#pragma unroll 1
+  for (int u = 0; u < n_units; ++u) {
+    const int after = min(kDwSlots - 2, n_units - 1 - u);
+    if (after >= 3) __builtin_amdgcn_s_waitcnt(vmcnt_only(12));
+    else if (after == 2) __builtin_amdgcn_s_waitcnt(vmcnt_only(8));
+    else if (after == 1) __builtin_amdgcn_s_waitcnt(vmcnt_only(4));
+    else __builtin_amdgcn_s_waitcnt(vmcnt_only(0));
+    // every wave's part of unit u is in LDS and slot (u - 1) % kDwSlots is free (see mlp_dw_kernel)
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_s_barrier();
+    asm volatile("" ::: "memory");
+    if (u + kDwSlots - 1 < n_units)
+      dma_unit(pb + (u + kDwSlots - 1) * KPTS, ring + ((u + kDwSlots - 1) % kDwSlots) * kDwSlot);
+    if (busy) {
+      const uint8_t* fr = ring + (u % kDwSlots) * kDwSlot;
+      u32x4 ah[2], al[2], bh[4], bl[4];
+#pragma unroll
+      for (int rb = 0; rb < 2; ++rb)
+        planes(fr, qm + 2 * rb + (i32 >> 4), ah[rb], al[rb], do_db ? &rs[rb] : nullptr);
+#pragma unroll
+      for (int cb = 0; cb < 4; ++cb) planes(fr, 16 + qn + 2 * cb + (i32 >> 4), bh[cb], bl[cb], nullptr);
+#pragma unroll
+      for (int pr = 0; pr < 3; ++pr)
+#pragma unroll
+        for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+          for (int cb = 0; cb < 4; ++cb)
+            if (mv[rb] && nv[cb])
+              acc[rb][cb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
+                  __builtin_bit_cast(bf16x8, pr == 0 ? al[rb] : ah[rb]),
+                  __builtin_bit_cast(bf16x8, pr == 1 ? bl[cb] : bh[cb]), acc[rb][cb], 0, 0, 0);
+    }
+  }
+#pragma unroll
+  for (int rb = 0; rb < 2; ++rb) {
+    if (!mv[rb]) continue;
+#pragma unroll
+    for (int cb = 0; cb < 4; ++cb) {
+      if (!nv[cb]) continue;
+      const int c = n0 + 16 * qn + 32 * cb + i32;
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int o = m0 + 16 * qm + 32 * rb + 8 * (r >> 2) + 4 * h + (r & 3);
+        if (o < jb.M && c < jb.N) unsafeAtomicAdd(jb.dw + (size_t)o * jb.ld_dw + c, acc[rb][cb][r]);
+      }
+    }
+    if (do_db) {
+      const float sr = rs[rb] + __shfl_xor(rs[rb], 32, 64);
+      const int o = m0 + 16 * qm + 32 * rb + i32;
+      if (h == 0 && o < jb.M) unsafeAtomicAdd(jb.db + o, sr);
+    }
+  }
+}
+
 // --------------------------------------------------------------------- host glue
 template <int P, bool VD, bool TRAIN>
 int launch_fwd(const FwdArgs& a, hipStream_t s) {
@@ -1379,8 +1600,8 @@ int launch_dw(const DwArgs& d, hipStream_t s) {
   return launch_status();
 }
 
-// Per-precision launchers, one translation unit each (mlp_f32.hip, mlp_bf16.hip; mlp_f16.hip holds
-// the inference forward only).
+// Per-precision launchers, one translation unit each (mlp_f32.hip, mlp_bf16.hip, mlp_bf16s.hip;
+// mlp_f16.hip holds the inference forward only).
 int fwd_launch(int prec, const FwdArgs& a, bool view_dep, bool train, hipStream_t s);
 int bwd_launch(int prec, const BwdArgs& a, bool view_dep, hipStream_t s);
 int dw_launch(int prec, const DwArgs& d, hipStream_t s);
@@ -1392,6 +1613,9 @@ int bwd_bf16(const BwdArgs& a, bool vd, hipStream_t s);
 int dw_f32(const DwArgs& d, hipStream_t s);
 int dw_bf16(const DwArgs& d, hipStream_t s);
 int dw_f32_narrow(const DwArgs& d, hipStream_t s);  // 128 x 160 tiles (kDwNarrowRows x kDwNarrowCols)
+int fwd_bf16s(const FwdArgs& a, bool vd, bool tr, hipStream_t s);  // split BF16 (mlp_bf16s.hip)
+int bwd_bf16s(const BwdArgs& a, bool vd, hipStream_t s);
+int dw_bf16s(const DwArgs& d, hipStream_t s);  // mlp_dw_split_kernel
 
 }  // namespace mlp
 }  // namespace srf

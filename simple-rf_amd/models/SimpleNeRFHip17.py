@@ -14,7 +14,8 @@ interchange and Trainer10/Tester07 drive it unchanged.  The hot path runs on the
 Random draws the reference makes on the CPU generator inside the path (t_rand :355, u :397,
 sigma noise :740) are drawn on the GPU (torch.rand / torch.randn on the ray device, from
 ``self.generator`` when set) and passed to the kernels as explicit inputs.  Extra config key:
-``configs['model']['mlp_precision']`` in {'bf16', 'fp32'} (default 'bf16').
+``configs['model']['mlp_precision']`` in {'bf16', 'bf16_split', 'fp32'} (default 'bf16'; 'bf16_split'
+rounds the forward like FP32 and the backward with two BF16 planes, on BF16 MFMAs).
 """
 from __future__ import annotations
 
@@ -176,6 +177,11 @@ class SimpleNeRFHip(torch.nn.Module):
         ret['rays_o'], ret['rays_d'] = rays_o, rays_d
         if self.ndc:
             ret['rays_o_ndc'], ret['rays_d_ndc'] = rays_o_ndc, rays_d_ndc
+        if torch.is_grad_enabled() and (rays_o.requires_grad or rays_d.requires_grad):
+            # a learned camera (learn_camera_* or camera optimisation at test time) differentiates the
+            # loss through the positional encoding into the rays (src/models/SimpleNeRF17.py:831-842);
+            # the fused kernels give no gradient to points or rays
+            raise NotImplementedError('camera gradients through the rays are not supported by the HIP kernels')
         view_dirs = None
         if (self.coarse_model_needed and self.configs['model']['coarse_model']['use_view_dirs']) or \
                 (self.fine_model_needed and self.configs['model']['fine_model']['use_view_dirs']):

@@ -4,7 +4,9 @@ Mirrors class MLP (src/models/SimpleNeRF17.py:616-785): the same parameter names
 (so state_dicts interchange), the same three variants (main, points-augmented with the 21/42
 PE split, views-augmented with the 4-channel head).  The FP32 master parameters are repacked
 into MFMA operand order on every call (srf_mlp_pack, ~1 MB), and the MLP runs with BF16,
-FP16 or FP32 MFMA operands and FP32 accumulation.
+FP16, FP32 or split-BF16 ('bf16_split': FP32 activations, products on BF16 MFMAs over operands
+split into BF16 planes, three in the forward and two in the backward) MFMA operands and FP32
+accumulation.
 """
 from __future__ import annotations
 
@@ -16,7 +18,7 @@ import torch
 from . import _lib
 from ._checks import dev_tensor, zero_grads
 
-PREC = {'fp32': 0, 'float32': 0, 'bf16': 1, 'bfloat16': 1, 'fp16': 2, 'float16': 2}
+PREC = {'fp32': 0, 'float32': 0, 'bf16': 1, 'bfloat16': 1, 'fp16': 2, 'float16': 2, 'bf16_split': 3}
 
 PARAM_NAMES = [f'pts_linears.{i}' for i in range(8)] + ['feature_linear', 'pts_output_linear', 'views_linears.0',
                                                         'views_output_linear']
