@@ -185,6 +185,11 @@ class SimpleTensoRFHip(torch.nn.Module):
         rays_o, rays_d = geometry.get_rays_tr(pixel_id, intrinsics, extrinsics, mip_nerf_used=True,
                                               nerf_synthetic=self.nerf_synthetic,
                                               intrinsic_inv=input_dict.get('intrinsics_inv'))
+        if torch.is_grad_enabled() and (rays_o.requires_grad or rays_d.requires_grad):
+            # a learned camera differentiates the loss through the sample points into the rays; the
+            # factor gathers detach their coordinates (src/models/SimpleTensoRF09.py:1231-1235), and
+            # the kernels give the rays no gradient
+            raise NotImplementedError('camera gradients through the rays are not supported by the HIP kernels')
         rays_o = rays_o.clone()
         rays_d = rays_d.clone()
         rays_o[:, :1] *= -1

@@ -88,3 +88,98 @@ def test_bf16_train_render_close_to_fp64(cuda):
               ('rgb_coarse', 'rgb_fine', 'points_augmentation_rgb_coarse', 'views_augmentation_rgb_coarse'))
     print('bf16 rgb max abs error vs fp64:', err)
     assert err < 2e-3
+
+
+def _opaque(nets, shift=2.0):
+    """Raise every density head's bias by `shift`, so that the rays are opaque (acc near 1) as in a
+    trained model.  At random init almost every ray is empty (median acc 0): there the reference's own
+    FP32 formulas (alpha = 1 - exp(-sigma delta) at sigma delta ~ 1e-8, depth = sum(w z) / (acc + 1e-6))
+    keep no significant digits, and the FP32 oracle itself lands ~2e-3 from FP64 in gradients."""
+    with torch.no_grad():
+        for m in nets.values():
+            m.pts_output_linear.bias[0] += shift
+
+
+def _oracle_step(nets, pid, K, E, mc, batch, dtype, z_fine=None, loss_fn=None):
+    ps = {k: {n: p.detach().cpu().to(dtype).requires_grad_(True) for n, p in m.state_dict().items()}
+          for k, m in nets.items()}
+    o = ots.render_train(ps, pid, K.to(dtype), E.to(dtype), mc['resolution'], z_fine=z_fine)
+    if loss_fn is not None:
+        cpu_batch = {k: v.cpu() for k, v in batch.items()}
+        b = {**cpu_batch, 'target_rgb': cpu_batch['target_rgb'].to(dtype), 'sd_depth': cpu_batch['sd_depth'].to(dtype)}
+        loss_fn(o, b).backward()
+    return o, ps
+
+
+def test_train_step_gradients_meet_bar_on_opaque_rays(cuda):
+    """The training step (coarse + fine + both augmented models, 1024 opaque rays, the benchmark
+    objective) in split BF16 and in FP32, against the FP64 oracle:
+      * rgb within 2e-3 (split BF16: ~1e-6);
+      * the gradient of all parameters together within 1e-3 relative, in both precisions;
+      * per network, the worst tensor within 1e-3 or within 1.5x of the worst tensor of the
+        reference's own FP32 autograd (the FP32 oracle): the first trunk layers' gradients (norm ~1e-6
+        against ~1e-3 at the heads) are sums with heavy cancellation, where every FP32 evaluation, the
+        reference's included, lands 1-3e-3 from FP64;
+      * split BF16 against the FP32 kernels tensor by tensor: within 3e-4 of FP64's norm, an order of
+        magnitude inside the bar (the split forward rounds like FP32, so ReLU decisions agree).
+    Plain BF16 operands miss the bar by 10-100x (test_backward in test_gpu_mlp)."""
+    grads, outs = {}, {}
+    for precision in ('bf16_split', 'fp32'):
+        model, nets, batch, inp, pid, K, E, mc = _setup(cuda, precision, n_rays=1024)
+        _opaque(nets)
+        out = model(inp)
+        synthetic.train_loss(out, batch).backward()
+        grads[precision] = {(k, n): p.grad.cpu().double() for k, m in nets.items() for n, p in m.named_parameters()}
+        outs[precision] = {k: v.detach().cpu().double() for k, v in out.items() if isinstance(v, torch.Tensor)}
+    ref, p64 = _oracle_step(nets, pid, K, E, mc, batch, torch.float64, loss_fn=synthetic.train_loss)
+    _, p32 = _oracle_step(nets, pid, K, E, mc, batch, torch.float32, loss_fn=synthetic.train_loss)
+    assert ref['acc_coarse'].median() > 0.5
+    g64 = {(k, n): p64[k][n].grad for k in p64 for n in p64[k]}
+    g32o = {(k, n): p32[k][n].grad.double() for k in p32 for n in p32[k]}
+    rel = lambda a, key: ((a - g64[key]).norm() / g64[key].norm().clamp_min(1e-30)).item()
+    err = max((outs['bf16_split'][k] - ref[k]).abs().max().item() for k in
+              ('rgb_coarse', 'rgb_fine', 'points_augmentation_rgb_coarse', 'views_augmentation_rgb_coarse'))
+    assert err < 2e-3
+    report = [f'bf16_split rgb max abs error vs fp64 {err:.2e}']
+    for precision, g in grads.items():
+        total = (sum((g[k] - g64[k]).square().sum().item() for k in g64) /
+                 sum(g64[k].square().sum().item() for k in g64)) ** 0.5
+        report.append(f'{precision}: all-parameter gradient rel err {total:.2e}')
+        assert total < 1e-3, (precision, total)
+        for net in nets:
+            keys = [k for k in g64 if k[0] == net]
+            worst = max(rel(g[k], k) for k in keys)
+            worst_o = max(rel(g32o[k], k) for k in keys)
+            report.append(f'  {net}: worst tensor {worst:.2e} (fp32 oracle {worst_o:.2e})')
+            assert worst < max(1e-3, 1.5 * worst_o), (precision, net, worst, worst_o)
+    split_vs_fp32 = max(((grads['bf16_split'][k] - grads['fp32'][k]).norm() / g64[k].norm().clamp_min(1e-30)).item()
+                        for k in g64)
+    report.append(f'split vs fp32 kernels, worst tensor: {split_vs_fp32:.2e}')
+    print('\n'.join(report))
+    assert split_vs_fp32 < 3e-4
+
+
+def test_fp32_step_matches_oracle_1e5_given_the_kernels_fine_depths(cuda):
+    """FP32 kernels against the FP64 oracle on the full training render (coarse + fine + augmented, 4096
+    opaque rays), with the oracle fed the kernels' merged fine depths: the fine samples then coincide (a
+    rounding-level difference in the CDF normaliser can otherwise move a sample, which the unconditioned
+    test_train_render_and_grads_match_oracle_fp32 allows for).  rgb and acc within 1e-5 abs; NDC depth
+    within 1e-5 abs; metric depth within 1e-5 relative to max(1, depth) (metric depths here reach ~100,
+    where one FP32 ulp is 7.6e-6)."""
+    model, nets, batch, inp, pid, K, E, mc = _setup(cuda, 'fp32', n_rays=4096)
+    _opaque(nets)
+    with torch.no_grad():
+        out = model(inp, retraw=True)
+    z_fine = out['z_vals_fine'].cpu().double()
+    ref, _ = _oracle_step(nets, pid, K, E, mc, batch, torch.float64, z_fine=z_fine)
+    assert ref['acc_coarse'].median() > 0.5
+    errs = {}
+    for k in KEYS + ['acc_fine', 'depth_ndc_fine', 'points_augmentation_acc_coarse', 'views_augmentation_acc_coarse']:
+        a, b = out[k].detach().cpu().double(), ref[k]
+        d = (a - b).abs()
+        if k.startswith('depth') or '_depth_coarse' in k:
+            if 'ndc' not in k:
+                d = d / b.abs().clamp_min(1.0)
+        errs[k] = d.max().item()
+    print(' '.join(f'{k}={v:.1e}' for k, v in errs.items()))
+    assert max(errs.values()) < 1e-5, errs

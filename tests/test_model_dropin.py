@@ -129,3 +129,21 @@ def test_tensorf_cp_dropin_state_dict_matches_reference_layout():
     for k, v in sd.items():
         assert torch.equal(b.state_dict()[k], v), k
     assert b.coarse_model.vectors_density[1].permute(0, 2, 3, 1).is_contiguous()
+
+
+@pytest.mark.parametrize('family', ['nerf', 'tensorf'])
+def test_learned_camera_raises_instead_of_dropping_its_gradient(family):
+    """A camera learner (learn_camera_*, src/models/SimpleNeRF17.py:831-842) makes the rays carry a
+    gradient; the kernels give rays none, so the drop-in refuses before any kernel runs (on the CPU
+    here) rather than training the camera with a zero gradient."""
+    from srf_hip import synthetic
+    if family == 'nerf':
+        c = cfgs.simple_nerf_configs(precision='fp32')
+    else:
+        c = cfgs.simple_tensorf_configs(augmentations=False, coarse=dict(num_voxels_initial=16 ** 3))
+    c['model']['learn_camera_rotation'] = True
+    m = get_model(c, model_configs=cfgs.model_configs())
+    m.train()
+    b = synthetic.train_batch(16, seed=0)
+    with pytest.raises(NotImplementedError, match='camera gradients'):
+        m({'pixel_id': b['pixel_id'], 'num_frames': 3, 'iter_num': 0, 'sub_batch_index': 0})
