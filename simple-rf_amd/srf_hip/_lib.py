@@ -80,6 +80,11 @@ _PROTOS = {
     'srf_color_pack': [_P, _P, _P, _P, _P, _P, _P, _P, _P, _P],
     'srf_color_forward': [_P, _P, _I, _P, _P, _I, _P, _P, _P, _P],
     'srf_color_backward': [_P, _P, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P],
+    'srf_mse_terms_fwd': [_I, _P, _P, _P, _P, _I, _I, _P, _P, _P, _P, _P],
+    'srf_mse_terms_bwd': [_I, _P, _P, _P, _P, _I, _I, _P, _P, _P, _P],
+    'srf_depth_consistency_fwd': [_I, _P, _P, _P, _P, _P, _P, _I, _P, _P, _P, _I, _I, _I, _I, _I, _F, _I,
+                                  _P, _P, _P, _P, _P, _P],
+    'srf_depth_consistency_bwd': [_I, _P, _P, _P, _P, _P, _P, _P, _P],
 }
 _RESTYPES = {'srf_status_string': C.c_char_p, 'srf_mlp_packed_bytes': C.c_size_t,
              'srf_mlp_workspace_bytes': C.c_size_t, 'srf_compact_workspace_bytes': C.c_size_t,

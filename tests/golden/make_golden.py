@@ -1,5 +1,7 @@
-"""Writes tests/golden/mlp_golden.npz: input/output vectors of the reference's own NeRF MLP and TensoRF
-colour network, for the oracle functions that no closed form pins (their concatenation orders).
+"""Writes tests/golden/mlp_golden.npz and tests/golden/loss_golden.npz: input/output vectors of the
+reference's own NeRF MLP, TensoRF colour network and depth-consistency losses, for the oracle functions
+that no closed form pins (the MLPs' concatenation orders; the losses' reprojection, patch gating and
+in-place masking).
 
 The reference classes are imported from the reference tree in THIS container only, instantiated with
 seeded weights (narrower than the shipped 256/128 widths, so the fixture stays small; the wiring under
@@ -15,6 +17,11 @@ Variants (reference classes and configs):
   color_shipped    MlpFeaturesColorPredictor, src/models/SimpleTensoRF09.py:1370-1421, features 27,
                    features PE 0, views PE 0 (src/NerfLlffTrainerTester10.py:958-961)
   color_pe         same class with features PE 2 and views PE 2 (both encoders and their order)
+  aug_depth        AugmentationsDepthLoss.compute_loss_nerf, src/loss_functions/AugmentationsDepthLoss11.py:108-181
+  coarse_fine      CoarseFineConsistencyLoss.compute_loss_nerf, src/loss_functions/CoarseFineConsistencyLoss34.py:85-164
+                   (loss, both loss maps and the gradients of the loss w.r.t. both depths, on a seeded scene
+                   of four forward-facing views with smooth images so that the patch gate takes both
+                   decisions)
 
     python tests/golden/make_golden.py [/path/to/reference]
 """
@@ -102,6 +109,84 @@ def main(ref_root: str = '/root/reference') -> None:
     out_path = HERE / 'mlp_golden.npz'
     np.savez_compressed(out_path, **arrays)
     print('wrote', out_path, sum(a.nbytes for a in arrays.values()), 'bytes of arrays')
+    write_loss_golden()
+
+
+def loss_scene(seed: int = 7, n_rays: int = 256, n_views: int = 4, h: int = 36, w: int = 44, plane: float = 4.0):
+    """Four forward-facing views (distinct centres, so the nearest view is unique) of one textured plane
+    z = -plane, each image rendered by intersecting its pixel rays (the reference's convention,
+    CommonUtils04.py:73-95) with the plane, so that the views agree where the plane is seen.  Each ray
+    carries two depth estimates near the true one (2 % and 6 % noise): their reprojections land inside
+    and outside the image, and the 5x5 patch RMSE falls on both sides of the 0.1 threshold."""
+    g = torch.Generator().manual_seed(seed)
+    f = 0.9 * w
+    K = torch.tensor([[f, 0., w / 2.], [0., f, h / 2.], [0., 0., 1.]], dtype=torch.float64)
+    poses = torch.zeros(n_views, 4, 4, dtype=torch.float64)
+    for v in range(n_views):
+        th = (torch.rand((), generator=g, dtype=torch.float64) - 0.5) * 0.1
+        c, s_ = torch.cos(th), torch.sin(th)
+        poses[v, :3, :3] = torch.tensor([[c, 0., s_], [0., 1., 0.], [-s_, 0., c]], dtype=torch.float64)
+        poses[v, :3, 3] = torch.tensor([0.12 * v, 0.05 * (v % 2), 0.02 * v], dtype=torch.float64)
+        poses[v, 3, 3] = 1.
+
+    def rays(v, x, y):
+        d_cam = torch.stack([(x - w / 2.) / f, -(y - h / 2.) / f, -torch.ones_like(x)], -1)
+        return poses[v, :3, 3].expand_as(d_cam), (poses[v, :3, :3] @ d_cam[..., None])[..., 0]
+
+    def texture(p):
+        return torch.stack([0.5 + 0.4 * torch.sin(2.3 * p[..., 0] + 1.1 * p[..., 1]),
+                            0.5 + 0.4 * torch.cos(1.7 * p[..., 1] - 0.6 * p[..., 0]),
+                            0.5 + 0.3 * torch.sin(1.3 * (p[..., 0] + p[..., 1]))], -1)
+
+    yy, xx = torch.meshgrid(torch.arange(h, dtype=torch.float64), torch.arange(w, dtype=torch.float64), indexing='ij')
+    imgs = []
+    for v in range(n_views):
+        o, d = rays(v, xx, yy)
+        t = (-plane - o[..., 2]) / d[..., 2]
+        imgs.append(texture(o + d * t[..., None]))
+    imgs = torch.stack(imgs)
+    img = torch.randint(0, n_views, (n_rays,), generator=g)
+    x = torch.randint(0, w, (n_rays,), generator=g).double()
+    y = torch.randint(0, h, (n_rays,), generator=g).double()
+    pixel_id = torch.stack([img, x.long(), y.long()], -1)
+    rays_o, rays_d = torch.stack([poses[i, :3, 3] for i in img.tolist()]), None
+    rays_d = torch.stack([rays(int(i), x[k], y[k])[1] for k, i in enumerate(img.tolist())])
+    t_true = (-plane - rays_o[:, 2]) / rays_d[:, 2]
+    depth1 = t_true * (1 + 0.02 * torch.randn(n_rays, generator=g, dtype=torch.float64))
+    depth2 = t_true * (1 + 0.06 * torch.randn(n_rays, generator=g, dtype=torch.float64))
+    mask = torch.rand(n_rays, generator=g) < 0.85
+    return dict(intrinsics=K[None].repeat(n_views, 1, 1), extrinsics=poses, images=imgs, pixel_id=pixel_id,
+                rays_o=rays_o, rays_d=rays_d, depth1=depth1, depth2=depth2, indices_mask_nerf=mask,
+                resolution=np.array([h, w]))
+
+
+def write_loss_golden(ref_root: str = '/root/reference') -> None:
+    from loss_functions.AugmentationsDepthLoss11 import AugmentationsDepthLoss
+    from loss_functions.CoarseFineConsistencyLoss34 import CoarseFineConsistencyLoss
+    sc = loss_scene()
+    arrays = {k: (v.numpy() if isinstance(v, torch.Tensor) else v) for k, v in sc.items()}
+    configs = {'model': {'coarse_model': {}, 'fine_model': {}, 'augmentations': []}, 'data_loader': {}}
+    loss_configs = {'rmse_threshold': 0.1, 'patch_size': [5, 5]}
+    classes = {'aug_depth': AugmentationsDepthLoss, 'coarse_fine': CoarseFineConsistencyLoss}
+    for name, cls in classes.items():
+        obj = cls(configs, loss_configs)
+        # FP32 only: reproject builds its axis permutation in the default dtype (CommonUtils04.py:247-248)
+        for dtype, tag in ((torch.float32, 'f32'),):
+            d1 = sc['depth1'].to(dtype).clone().requires_grad_(True)
+            d2 = sc['depth2'].to(dtype).clone().requires_grad_(True)
+            loss, m1, m2 = obj.compute_loss_nerf(
+                d1, d2, sc['indices_mask_nerf'], sc['rays_o'].to(dtype), sc['rays_d'].to(dtype),
+                sc['extrinsics'].to(dtype), sc['images'].to(dtype), sc['pixel_id'], sc['intrinsics'].to(dtype),
+                tuple(int(v) for v in sc['resolution']))
+            loss.backward()
+            arrays[f'{name}/{tag}/loss'] = np.array(loss.item())
+            arrays[f'{name}/{tag}/map1'] = m1.detach().numpy()
+            arrays[f'{name}/{tag}/map2'] = m2.detach().numpy()
+            arrays[f'{name}/{tag}/grad1'] = d1.grad.numpy()
+            arrays[f'{name}/{tag}/grad2'] = d2.grad.numpy()
+    out_path = HERE / 'loss_golden.npz'
+    np.savez_compressed(out_path, **arrays)
+    print('wrote', out_path, sum(np.asarray(a).nbytes for a in arrays.values()), 'bytes of arrays')
 
 
 if __name__ == '__main__':
