@@ -642,7 +642,7 @@ def oracle_step_fn(model, dev, n_rays, seed=0, coarse_only=False):
         if coarse_only:
             loss = ((out['rgb_coarse'] - batch['target_rgb']) ** 2).mean()
         else:
-            loss = synthetic.train_loss(out, batch)
+            loss = ots.reference_objective(out, batch, K)
         opt.zero_grad(set_to_none=True)
         loss.backward()
         opt.step()

@@ -24,14 +24,14 @@ def mse(pred_rgb: torch.Tensor, target_rgb: torch.Tensor, indices_mask) -> torch
     p = pred_rgb[indices_mask]
     t = target_rgb[indices_mask]
     m = torch.mean(torch.square(p - t), dim=1)
-    return torch.mean(m) if p.numel() > 0 else torch.zeros((), dtype=pred_rgb.dtype)
+    return torch.mean(m) if p.numel() > 0 else torch.zeros((), dtype=pred_rgb.dtype, device=pred_rgb.device)
 
 
 def sparse_depth_mse(pred_depth: torch.Tensor, gt_depth: torch.Tensor, indices_mask) -> torch.Tensor:
     """SparseDepthMSE.compute_depth_loss, src/loss_functions/SparseDepthMSE14.py:82-95."""
     p = pred_depth[indices_mask]
     t = gt_depth[indices_mask]
-    return torch.mean(torch.square(p - t)) if p.numel() > 0 else torch.zeros((), dtype=pred_depth.dtype)
+    return torch.mean(torch.square(p - t)) if p.numel() > 0 else torch.zeros((), dtype=pred_depth.dtype, device=pred_depth.device)
 
 
 # -------------------------------------------------------------------- reprojection
@@ -41,7 +41,7 @@ def reproject(points: torch.Tensor, poses: torch.Tensor, intrinsics: torch.Tenso
     origins = poses[:, :3, 3]
     rotations = poses[:, :3, :3]
     rel = points - origins
-    permuter = torch.eye(3, dtype=points.dtype)
+    permuter = torch.eye(3, dtype=points.dtype, device=points.device)
     permuter[1:] *= -1
     k = intrinsics[:1]
     pos = (k @ permuter[None] @ rotations.transpose(1, 2) @ rel[..., None]).squeeze()
@@ -98,9 +98,9 @@ def depth_consistency_nerf(depth1, depth2, indices_mask_nerf, rays_o, rays_d, ex
     x2c, y2c = torch.clip(x2b, 0, w - 1), torch.clip(y2b, 0, h - 1)
     n = image_ids_a.shape[0]
     c = gt_images.shape[3]
-    pa = torch.zeros(n, py, px, c, dtype=gt_images.dtype)
-    p1 = torch.zeros(n, py, px, c, dtype=gt_images.dtype)
-    p2 = torch.zeros(n, py, px, c, dtype=gt_images.dtype)
+    pa = torch.zeros(n, py, px, c, dtype=gt_images.dtype, device=gt_images.device)
+    p1 = torch.zeros(n, py, px, c, dtype=gt_images.dtype, device=gt_images.device)
+    p2 = torch.zeros(n, py, px, c, dtype=gt_images.dtype, device=gt_images.device)
     # zero padding after the last row and column only (negative offsets wrap to the far end,
     # whose padding is zero, as F.pad(..., (0, 0, 0, hpy, 0, hpx)) then negative indexing does)
     padded = F.pad(gt_images, (0, 0, 0, hpy, 0, hpx), mode='constant', value=0)
@@ -126,7 +126,7 @@ def depth_consistency_nerf(depth1, depth2, indices_mask_nerf, rays_o, rays_d, ex
     d2 = torch.where(both, depth2, torch.zeros_like(depth2))
     d1_gt = torch.where(both, depth1.detach(), torch.zeros_like(depth1))
     map2 = torch.square(d2 - d1_gt)
-    zero = torch.zeros((), dtype=depth1.dtype)
+    zero = torch.zeros((), dtype=depth1.dtype, device=depth1.device)
     loss = (torch.mean(map1) if n > 0 else zero) + (torch.mean(map2) if n > 0 else zero)
     return loss, map1, map2, mask1, mask2
 
@@ -136,4 +136,4 @@ def coarse_fine_sparse_depth(depth_coarse, depth_fine, indices_mask_sd) -> torch
     coarse depth pulled towards the detached fine depth on the sparse-depth rays."""
     dc = depth_coarse[indices_mask_sd]
     df = depth_fine[indices_mask_sd].detach()
-    return torch.mean(torch.square(dc - df)) if dc.numel() > 0 else torch.zeros((), dtype=depth_coarse.dtype)
+    return torch.mean(torch.square(dc - df)) if dc.numel() > 0 else torch.zeros((), dtype=depth_coarse.dtype, device=depth_coarse.device)

@@ -42,6 +42,7 @@ def render_train(params: dict, pixel_id, K, E, resolution, *, u_coarse=None, u_f
         return on.volume_rendering(net['sigma'].reshape(n, s), net['rgb'].reshape(n, s, 3), z_vals_ndc=z,
                                    rays_o=rays_o, rays_d=rays_d, rays_d_ndc=d_ndc, ndc=True, white_bkgd=white_bkgd)
 
+    out['rays_o'], out['rays_d'] = rays_o, rays_d
     z_c = on.stratified_z_vals(n, n_coarse, 0., 1., u=u_coarse, dtype=dt, device=K.device)
     vr = net_render('coarse', z_c)
     for k, v in vr.items():
@@ -56,3 +57,41 @@ def render_train(params: dict, pixel_id, K, E, resolution, *, u_coarse=None, u_f
         for k, v in net_render('fine', z_f).items():
             out[f'{k}_fine'] = v
     return out
+
+
+LOSS_WEIGHTS = {'MSE14': 1.0, 'SparseDepthMSE14': 0.1, 'AugmentationsDepthLoss11': 0.1,
+                'CoarseFineConsistencyLoss34': 0.1}
+
+
+def reference_objective(out: dict, batch: dict, K):
+    """The Simple-NeRF training objective of src/NerfLlffTrainerTester10.py:381-417 (LossComputer03
+    weighting the losses, the depth-consistency ones at their weight from iteration 10000 on), on the
+    oracle's losses: MSE14 on the NeRF rays, SparseDepthMSE14 on the coarse renders' metric depths over
+    the sparse-depth rays, AugmentationsDepthLoss11 (main against each augmented coarse depth) and
+    CoarseFineConsistencyLoss34 (coarse against fine, plus its sparse-depth term).  out: render_train's
+    output; batch: pixel_id, target_rgb, indices_mask_nerf, indices_mask_sparse_depth, sd_depth, images,
+    poses (per frame); K: per-ray intrinsics."""
+    from . import losses as ol
+    w = LOSS_WEIGHTS
+    nerf, sd = batch['indices_mask_nerf'], batch['indices_mask_sparse_depth']
+    loss = w['MSE14'] * sum(ol.mse(out[k], batch['target_rgb'], nerf) for k in
+                            ('rgb_coarse', 'rgb_fine', 'points_augmentation_rgb_coarse', 'views_augmentation_rgb_coarse')
+                            if k in out)
+    loss = loss + w['SparseDepthMSE14'] * sum(ol.sparse_depth_mse(out[k], batch['sd_depth'], sd) for k in
+                                              ('depth_coarse', 'points_augmentation_depth_coarse',
+                                               'views_augmentation_depth_coarse') if k in out)
+    images = batch['images'].to(out['rgb_coarse'].dtype)
+    res = tuple(images.shape[1:3])
+    poses = batch['poses'].to(out['rgb_coarse'].dtype)
+
+    def gated(d1, d2, rule):
+        return ol.depth_consistency_nerf(d1, d2, nerf, out['rays_o'], out['rays_d'], poses, images, batch['pixel_id'],
+                                         K, res, both_invalid_rule=rule)[0]
+    for a in ('points_augmentation', 'views_augmentation'):
+        if f'{a}_depth_coarse' in out:
+            loss = loss + w['AugmentationsDepthLoss11'] * gated(out['depth_coarse'], out[f'{a}_depth_coarse'], True)
+    if 'depth_fine' in out:
+        loss = loss + w['CoarseFineConsistencyLoss34'] * (
+            gated(out['depth_coarse'], out['depth_fine'], False) +
+            ol.coarse_fine_sparse_depth(out['depth_coarse'], out['depth_fine'], sd))
+    return loss

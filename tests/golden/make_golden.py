@@ -22,6 +22,12 @@ Variants (reference classes and configs):
                    (loss, both loss maps and the gradients of the loss w.r.t. both depths, on a seeded scene
                    of four forward-facing views with smooth images so that the patch gate takes both
                    decisions)
+and tests/golden/tensorf_mods_golden.npz: the TensoRF model modifications of VmDecomposedTensor
+(src/models/SimpleTensoRF09.py:820-944, 1274-1321) on a 20^3-voxel grid whose density is occupied in a
+sub-box only: the alpha-mask update (dense evaluation, 3x3x3 max pool, threshold, new box), the shrink,
+the factor upsampling and a second alpha-mask update through the existing mask; and the TV loss
+(src/loss_functions/TotalVariationLoss04.py:97-116) and the mass-concentration entropy
+(src/loss_functions/MassConcentrationLoss06.py:63-83) with their gradients.
 
     python tests/golden/make_golden.py [/path/to/reference]
 """
@@ -110,6 +116,7 @@ def main(ref_root: str = '/root/reference') -> None:
     np.savez_compressed(out_path, **arrays)
     print('wrote', out_path, sum(a.nbytes for a in arrays.values()), 'bytes of arrays')
     write_loss_golden()
+    write_tensorf_mods_golden()
 
 
 def loss_scene(seed: int = 7, n_rays: int = 256, n_views: int = 4, h: int = 36, w: int = 44, plane: float = 4.0):
@@ -191,3 +198,83 @@ def write_loss_golden(ref_root: str = '/root/reference') -> None:
 
 if __name__ == '__main__':
     main(*sys.argv[1:])
+
+
+TENSORF_CFG = {'decomposition_type': 'VectorMatrix', 'num_samples_max': 1e6, 'num_components_density': [4, 2, 2],
+               'num_components_color': [6, 3, 3], 'bounding_box': [[-1.5, -1.67, -1.0], [1.5, 1.67, 1.0]],
+               'num_voxels_initial': 20 ** 3, 'num_voxels_final': 40 ** 3, 'tensor_upsampling_iters': [2000, 3000],
+               'num_voxels_per_sample': 0.5, 'alpha_mask_update_iters': [2500], 'alpha_mask_threshold': 0.0001,
+               'ray_marching_weight_threshold': 0.0001, 'use_view_dirs': True, 'view_dependent_color': True,
+               'views_positional_encoding_degree': 0, 'features_positional_encoding_degree': 0,
+               'features_dimension_color': 27, 'density_offset': -10, 'distance_scale': 25, 'density_predictor': 'ReLU',
+               'color_predictor': 'MLP_Features', 'num_units_color_predictor': 32, 'predict_visibility': False}
+
+
+def write_tensorf_mods_golden() -> None:
+    from models.SimpleTensoRF09 import VmDecomposedTensor
+    from loss_functions.TotalVariationLoss04 import TotalVariationLoss
+    from loss_functions.MassConcentrationLoss06 import MassConcentrationLoss
+    arrays = {}
+    m = VmDecomposedTensor('coarse_model', {'data_loader': {'ndc': True}, 'model': {}}, TENSORF_CFG,
+                           {'bounding_box': TENSORF_CFG['bounding_box']})
+    m.train()
+    g = torch.Generator().manual_seed(31)
+    with torch.no_grad():
+        # density occupied in a sub-box only: the xy plane of component 0 positive on a rectangle, its z
+        # line positive on a range, every other density factor zero; colour factors random
+        for p in list(m.matrices_density) + list(m.vectors_density):
+            p.zero_()
+        pl, ln = m.matrices_density[0], m.vectors_density[0]
+        pl[0, 0, 6:15, 4:13] = 0.5 + torch.rand(9, 9, generator=g)
+        ln[0, 0, 3:9, 0] = 0.5 + torch.rand(6, generator=g)
+        for p in list(m.matrices_color) + list(m.vectors_color):
+            p.copy_(0.1 * torch.randn(p.shape, generator=g))
+
+    def snap(tag):
+        arrays[f'{tag}/resolution'] = m.resolution.numpy().astype(np.int64)
+        arrays[f'{tag}/bounding_box'] = m.bounding_box.numpy()
+        arrays[f'{tag}/step_size'] = np.array(float(m.step_size))
+        arrays[f'{tag}/num_samples'] = np.array(int(m.num_samples))
+        for kind in ('density', 'color'):
+            for i in range(3):
+                arrays[f'{tag}/matrices_{kind}.{i}'] = getattr(m, f'matrices_{kind}')[i].detach().numpy()
+                arrays[f'{tag}/vectors_{kind}.{i}'] = getattr(m, f'vectors_{kind}')[i].detach().numpy()
+    snap('initial')
+    with torch.no_grad():
+        bb = m.update_alpha_mask(2500)
+    arrays['alpha1/alpha_volume'] = m.alpha_mask.alpha_volume.numpy().astype(np.uint8)  # 0 / 1
+    arrays['alpha1/bounding_box_new'] = bb.numpy()
+    with torch.no_grad():
+        m.shrink_tensor(bb)
+    snap('shrunk')
+    with torch.no_grad():
+        m.upsample_model_resolution(3000)
+    snap('upsampled')
+    with torch.no_grad():
+        bb2 = m.update_alpha_mask(2500)
+    arrays['alpha2/alpha_volume'] = m.alpha_mask.alpha_volume.numpy().astype(np.uint8)
+    arrays['alpha2/bounding_box_new'] = bb2.numpy()
+    # TV on the upsampled planes (VM) and lines (as CP components), at iteration weight 0.7
+    for kind in ('density', 'color'):
+        for comps, tag in ((getattr(m, f'matrices_{kind}'), 'planes'), (getattr(m, f'vectors_{kind}'), 'lines')):
+            leaves = [p.detach().clone().requires_grad_(True) for p in comps]
+            tv = TotalVariationLoss.compute_tv_loss(leaves, 0.7, False)['loss_value']
+            tv.backward()
+            arrays[f'tv/{kind}_{tag}/loss'] = np.array(tv.item())
+            for i, p in enumerate(leaves):
+                arrays[f'tv/{kind}_{tag}/grad.{i}'] = p.grad.numpy()
+    # mass concentration on seeded weights of 40 rays x 443 samples, 11 bins
+    w = torch.rand(40, 443, generator=g) ** 4 / 40
+    mask = torch.rand(40, generator=g) < 0.8
+    mc = MassConcentrationLoss({'model': {}}, {'num_bins': 11})
+    wl = w.clone().requires_grad_(True)
+    ent = mc.compute_entropy_loss(wl, mask, False)['loss_value']
+    ent.backward()
+    arrays['entropy/weights'] = w.numpy()
+    arrays['entropy/mask'] = mask.numpy()
+    arrays['entropy/num_bins'] = np.array(11)
+    arrays['entropy/loss'] = np.array(ent.item())
+    arrays['entropy/grad'] = wl.grad.numpy()
+    out_path = HERE / 'tensorf_mods_golden.npz'
+    np.savez_compressed(out_path, **arrays)
+    print('wrote', out_path, sum(np.asarray(a).nbytes for a in arrays.values()), 'bytes of arrays')

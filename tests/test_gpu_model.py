@@ -58,7 +58,7 @@ def test_train_render_and_grads_match_oracle_fp32(cuda):
               for k, m in nets.items()}
         o = ots.render_train(ps, pid, K.to(dtype), E.to(dtype), mc['resolution'])
         b = {**cpu_batch, 'target_rgb': cpu_batch['target_rgb'].to(dtype), 'sd_depth': cpu_batch['sd_depth'].to(dtype)}
-        lo = synthetic.train_loss(o, b)
+        lo = ots.reference_objective(o, b, K.to(dtype))
         lo.backward()
         return lo, ps
 
@@ -100,14 +100,15 @@ def _opaque(nets, shift=2.0):
             m.pts_output_linear.bias[0] += shift
 
 
-def _oracle_step(nets, pid, K, E, mc, batch, dtype, z_fine=None, loss_fn=None):
+def _oracle_step(nets, pid, K, E, mc, batch, dtype, z_fine=None, objective=False):
+    """The oracle's training render (and, with objective, the backward of the reference's objective)."""
     ps = {k: {n: p.detach().cpu().to(dtype).requires_grad_(True) for n, p in m.state_dict().items()}
           for k, m in nets.items()}
     o = ots.render_train(ps, pid, K.to(dtype), E.to(dtype), mc['resolution'], z_fine=z_fine)
-    if loss_fn is not None:
+    if objective:
         cpu_batch = {k: v.cpu() for k, v in batch.items()}
         b = {**cpu_batch, 'target_rgb': cpu_batch['target_rgb'].to(dtype), 'sd_depth': cpu_batch['sd_depth'].to(dtype)}
-        loss_fn(o, b).backward()
+        ots.reference_objective(o, b, K.to(dtype)).backward()
     return o, ps
 
 
@@ -131,8 +132,8 @@ def test_train_step_gradients_meet_bar_on_opaque_rays(cuda):
         synthetic.train_loss(out, batch).backward()
         grads[precision] = {(k, n): p.grad.cpu().double() for k, m in nets.items() for n, p in m.named_parameters()}
         outs[precision] = {k: v.detach().cpu().double() for k, v in out.items() if isinstance(v, torch.Tensor)}
-    ref, p64 = _oracle_step(nets, pid, K, E, mc, batch, torch.float64, loss_fn=synthetic.train_loss)
-    _, p32 = _oracle_step(nets, pid, K, E, mc, batch, torch.float32, loss_fn=synthetic.train_loss)
+    ref, p64 = _oracle_step(nets, pid, K, E, mc, batch, torch.float64, objective=True)
+    _, p32 = _oracle_step(nets, pid, K, E, mc, batch, torch.float32, objective=True)
     assert ref['acc_coarse'].median() > 0.5
     g64 = {(k, n): p64[k][n].grad for k in p64 for n in p64[k]}
     g32o = {(k, n): p32[k][n].grad.double() for k in p32 for n in p32[k]}
