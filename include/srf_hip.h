@@ -493,6 +493,56 @@ int srf_depth_consistency_bwd(int n, const float* depth1, const float* depth2, c
                               const int* count, const float* g_loss, float* d_depth1, float* d_depth2,
                               void* stream);
 
+/* ------------------------------------------------ TensoRF model modifications (f3) */
+
+/* Points of the dense evaluation grid of LowRankTensor.update_alpha_mask (src/models/SimpleTensoRF09.py:
+ * 851-857): pts [gx*gy*gz, 3], point (i, j, k) at row (i gy + j) gz + k (meshgrid 'ij'), each coordinate
+ * bb0 (1 - s) + bb1 s with s the torch.linspace(0, 1, g) position on its axis.  bbox: host array of 6
+ * floats (min xyz, max xyz). */
+int srf_dense_grid_points(int gx, int gy, int gz, const float* bbox, float* pts, void* stream);
+
+/* The new occupancy volume of update_alpha_mask (:858-868) from the dense grid's densities: alpha =
+ * clamp(1 - exp(-sigma length), 0, 1), transposed to (z, y, x), max-pooled over 3x3x3 (stride 1, padding
+ * 1), then 1 where >= threshold and 0 elsewhere.  sigma [gx*gy*gz] in srf_dense_grid_points' order;
+ * alpha_out [gz, gy, gx]; bounds int32 [6]: min x, y, z and max x, y, z index of the occupied voxels (the
+ * box of :870-875; min > max when no voxel is occupied). */
+int srf_alpha_mask_build(int gx, int gy, int gz, const float* sigma, float length, float threshold,
+                         float* alpha_out, int* bounds, void* stream);
+
+/* F.interpolate(mode='bilinear', align_corners=True) of one texel-major factor [height, width, channels]
+ * to [height_new, width_new, channels] (VmDecomposedTensor.upsample_vectors_and_matrices, :1281-1302;
+ * lines are factors of width 1). */
+int srf_factor_upsample(int channels, int height, int width, const float* src, int height_new, int width_new,
+                        float* dst, void* stream);
+
+/* Workspace bytes of srf_tv_fwd for n_planes (<= 8) planes (0 otherwise). */
+size_t srf_tv_workspace_bytes(int n_planes);
+
+/* TotalVariationLoss.compute_tv_loss (src/loss_functions/TotalVariationLoss04.py:97-116) over n_planes
+ * texel-major components [heights[i], widths[i], channels[i]]: loss [1] = scale sum_i 2 (sum of squared
+ * differences along H / (c (H - 1) W) + along W / (c H (W - 1))), each count at least 1.  planes,
+ * channels, heights, widths: host arrays; scale = the iteration weight times the loss weight. */
+int srf_tv_fwd(int n_planes, const int* channels, const int* heights, const int* widths,
+               const float* const* planes, float scale, void* workspace, float* loss, void* stream);
+
+/* Backward of srf_tv_fwd: d_planes[i] (texel-major, overwritten) = g_loss times the derivative. */
+int srf_tv_bwd(int n_planes, const int* channels, const int* heights, const int* widths,
+               const float* const* planes, float scale, const float* g_loss, float* const* d_planes,
+               void* stream);
+
+/* MassConcentrationLoss.compute_entropy_loss (src/loss_functions/MassConcentrationLoss06.py:63-83) on
+ * weights [n_rays, n_samples]: per selected ray, the weights summed over n_bins (<= 64) consecutive bins of
+ * bin_sizes[b] samples (host array, summing to n_samples), entr(bin + 1e-3) = -x log x summed over the
+ * bins; loss [1] = their mean over the selected rays (select [n_rays] bytes or NULL), count [1].
+ * binned [n_rays, n_bins] (bin + 1e-3) and ray_loss [n_rays] are outputs the backward reads. */
+int srf_entropy_fwd(int n_rays, int n_samples, const float* weights, const uint8_t* select, int n_bins,
+                    const int* bin_sizes, float* binned, float* ray_loss, float* loss, int* count, void* stream);
+
+/* Backward of srf_entropy_fwd: d_weights [n_rays, n_samples] = g_loss / count * -(log(binned) + 1) of
+ * the sample's bin on selected rays, 0 elsewhere. */
+int srf_entropy_bwd(int n_rays, int n_samples, const uint8_t* select, int n_bins, const int* bin_sizes,
+                    const float* binned, const int* count, const float* g_loss, float* d_weights, void* stream);
+
 /* Lane-layout self-test of the MFMA building blocks (one 16x16 tile): d = a b with
  * a [16,K], b [K,16] row-major FP32 (rounded to the operand type), K = 4 for SRF_PREC_F32 and
  * 32 for SRF_PREC_BF16 / SRF_PREC_F16.  No counterpart in the reference: a unit check of the

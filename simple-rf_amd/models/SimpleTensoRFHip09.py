@@ -23,8 +23,10 @@ The factor
 parameters keep the reference's shapes but are stored texel-major (component axis innermost),
 which is the layout the gather kernels read.  Random draws of the path (t_rand :376, the
 per-ray offset :396, sample_pdf's u) come from the GPU generator and are passed to the kernels.
-Options outside the path raise NotImplementedError: visibility prediction and the periodic
-model modifications (alpha-mask update, shrink, upsampling).
+The periodic model modifications (alpha-mask update with its 3x3x3 max pool, shrink to the
+occupied box, factor upsampling, optimiser reconfiguration; :820-944) run on
+srf_dense_grid_points, the density gather, srf_alpha_mask_build and srf_factor_upsample.
+Options outside the path raise NotImplementedError: visibility prediction.
 """
 from __future__ import annotations
 
@@ -33,7 +35,7 @@ import torch
 import torch.nn.functional as F
 from torch.nn import ModuleDict, ModuleList
 
-from srf_hip import color as hip_color, compositing, geometry, sampling, tensorf
+from srf_hip import color as hip_color, compositing, geometry, sampling, tensorf, tensorf_mods
 
 from .SimpleNeRFHip17 import ExtrinsicsLearner, IntrinsicsLearner
 
@@ -533,14 +535,93 @@ class LowRankTensor(torch.nn.Module):
         return out
 
     def run_model_modifications(self, iter_num):
-        """src/models/SimpleTensoRF09.py:820-829.  The alpha-mask update, shrink and upsampling
-        are the next rows after the hot path; reaching one of their iterations is an error here."""
+        """src/models/SimpleTensoRF09.py:820-829: the alpha-mask update (and, at its first iteration, the
+        shrink of the grid to the occupied box), and the upsampling of the factors with the optimiser
+        reconfigured for the new parameters."""
         if not self.training or iter_num is None:
             return
-        if iter_num in self.tensor_configs.get('alpha_mask_update_iters', []) or \
-                iter_num in self.tensor_configs.get('tensor_upsampling_iters', []):
-            raise NotImplementedError(f'{self.name}: TensoRF model modification at iteration {iter_num} '
-                                      '(alpha-mask update / upsampling) is not on the HIP path yet')
+        if iter_num in self.tensor_configs['alpha_mask_update_iters']:
+            bounding_box_new = self.update_alpha_mask(iter_num)
+            if iter_num == self.tensor_configs['alpha_mask_update_iters'][0]:
+                self.shrink_tensor(bounding_box_new)
+        if iter_num in self.tensor_configs['tensor_upsampling_iters']:
+            self.upsample_model_resolution(iter_num)
+            self.reconfigure_optimizer()
+
+    def upsample_model_resolution(self, iter_num):
+        """src/models/SimpleTensoRF09.py:831-835 (the subclasses resample their factors)."""
+        num_voxels_new = self.get_new_num_voxels(iter_num)
+        self.resolution = self.compute_resolution_in_voxels(num_voxels_new, self.bounding_box.cpu()).to(
+            self.bounding_box.device)
+        self.update_tensor_params(self.resolution, self.bounding_box)
+
+    def get_new_num_voxels(self, iter_num):
+        """src/models/SimpleTensoRF09.py:837-847: the voxel count interpolated geometrically between the
+        initial and final counts over the upsampling iterations."""
+        iters = self.tensor_configs['tensor_upsampling_iters']
+        if iter_num not in iters:
+            raise RuntimeError('LowRankTensor.get_new_num_voxels() called at invalid iteration number')
+        k = iters.index(iter_num) + 1
+        lo = numpy.log(self.tensor_configs['num_voxels_initial'])
+        hi = numpy.log(self.tensor_configs['num_voxels_final'])
+        return int(numpy.round(numpy.exp(lo + (hi - lo) * k / len(iters))))
+
+    def update_alpha_mask(self, iter_num):
+        """src/models/SimpleTensoRF09.py:849-870 on the kernels: the densities of the dense grid
+        (srf_dense_grid_points, then this decomposition's density gather through the current mask), the
+        alpha, transpose, 3x3x3 max pool and threshold in srf_alpha_mask_build; returns the box of the
+        occupied voxels."""
+        grid = tuple(int(v) for v in self.resolution.tolist())
+        hp = self.host_params()
+        dev = self.bounding_box.device
+        pts = tensorf_mods.dense_grid_points(grid, hp['bbox'], dev)
+        mask = self.alpha_mask.kernel_mask() if self.alpha_mask is not None else None
+        with torch.no_grad():
+            sigma = self._density(pts, hp['bbox'], mask, float(self.tensor_configs.get('density_offset', 0.0)))
+        alpha, bounds = tensorf_mods.alpha_mask_build(sigma, grid, hp['step_size'],
+                                                      self.tensor_configs['alpha_mask_threshold'])
+        self.alpha_mask = AlphaGridMask(alpha, self.bounding_box)
+        b = bounds.tolist()
+        if b[0] > b[3]:
+            raise RuntimeError(f'{self.name}: the alpha-mask update found no occupied voxel')
+        gx, gy, gz = grid
+        lo = pts[(b[0] * gy + b[1]) * gz + b[2]]
+        hi = pts[(b[3] * gy + b[4]) * gz + b[5]]
+        return torch.stack((lo, hi))
+
+    def shrink_tensor(self, bounding_box_new):
+        """src/models/SimpleTensoRF09.py:899-915: the voxel range [t_l, b_r) that keeps the new box on each
+        axis (the box snapped to voxel positions when the mask's resolution differs), the new geometry;
+        the subclasses crop their factors.  Returns (t_l, b_r) as host lists."""
+        bbox, res = self.bounding_box, self.resolution
+        xyz_min, xyz_max = bounding_box_new
+        t_l, b_r = (xyz_min - bbox[0]) / self.voxel_length, (xyz_max - bbox[0]) / self.voxel_length
+        t_l, b_r = torch.round(torch.round(t_l)).long(), torch.round(b_r).long() + 1
+        b_r = torch.stack([b_r, res]).amin(0)
+        if not torch.equal(self.alpha_mask.resolution.to(res.device), res):
+            t_l_r, b_r_r = t_l / (res - 1), (b_r - 1) / (res - 1)
+            corrected = torch.zeros_like(bounding_box_new)
+            corrected[0] = (1 - t_l_r) * bbox[0] + t_l_r * bbox[1]
+            corrected[1] = (1 - b_r_r) * bbox[0] + b_r_r * bbox[1]
+            bounding_box_new = corrected
+        self.update_tensor_params(b_r - t_l, bounding_box_new)
+        return t_l.tolist(), b_r.tolist()
+
+    def reconfigure_optimizer(self):
+        """src/models/SimpleTensoRF09.py:917-944: replace this model's parameter groups in the main
+        optimiser by groups of the new (resampled) parameters, dropping the old parameters' state."""
+        if not self.optimizers:
+            return
+        optimizer = self.optimizers['optimizer_nerf']
+        opt_cfg = next(c for c in self.configs['optimizers'] if c['name'] == 'optimizer_main')
+        groups = self.get_trainable_parameters(opt_cfg)
+        names = {g['name'] for g in groups}
+        for old in [g for g in optimizer.param_groups if g.get('name') in names]:
+            for q in old['params']:
+                optimizer.state.pop(q, None)
+            optimizer.param_groups.remove(old)
+        for g in groups:
+            optimizer.add_param_group(g)
 
     def normalize_points(self, pts: torch.Tensor):
         """src/models/SimpleTensoRF09.py:763-765."""
@@ -689,6 +770,30 @@ class VmDecomposedTensor(LowRankTensor):
              'lr': optimizer_configs['lr_initial_network']},
         ]
 
+    def upsample_model_resolution(self, iter_num):
+        """src/models/SimpleTensoRF09.py:1274-1279: new geometry, then every plane and line resampled
+        (srf_factor_upsample)."""
+        super().upsample_model_resolution(iter_num)
+        res = self.resolution.tolist()
+        for kind in ('density', 'color'):
+            planes, lines = getattr(self, f'matrices_{kind}'), getattr(self, f'vectors_{kind}')
+            for i in range(3):
+                a0, a1 = MATRIX_AXES[i]
+                planes[i] = tensorf_mods.upsample_factor(planes[i], res[a1], res[a0])
+                lines[i] = tensorf_mods.upsample_factor(lines[i], res[VECTOR_AXES[i]], 1)
+
+    def shrink_tensor(self, bounding_box_new):
+        """src/models/SimpleTensoRF09.py:1304-1321: crop every plane and line to the kept voxel range."""
+        t_l, b_r = super().shrink_tensor(bounding_box_new)
+        for kind in ('density', 'color'):
+            planes, lines = getattr(self, f'matrices_{kind}'), getattr(self, f'vectors_{kind}')
+            for i in range(3):
+                v = VECTOR_AXES[i]
+                lines[i] = tensorf_mods.crop_factor(lines[i], slice(t_l[v], b_r[v]), slice(None))
+                a0, a1 = MATRIX_AXES[i]
+                planes[i] = tensorf_mods.crop_factor(planes[i], slice(t_l[a1], b_r[a1]), slice(t_l[a0], b_r[a0]))
+        return t_l, b_r
+
     def _density(self, pts, bbox, alpha, offset):
         sigma, _ = tensorf.vm_density(pts, list(self.matrices_density), list(self.vectors_density), bbox,
                                       alpha=alpha, predictor=self.density_kind, offset=offset)
@@ -738,6 +843,25 @@ class CpDecomposedTensor(LowRankTensor):
             {'name': f'{self.name}_network_params', 'params': network_params,
              'lr': optimizer_configs['lr_initial_network']},
         ]
+
+    def upsample_model_resolution(self, iter_num):
+        """src/models/SimpleTensoRF09.py:1093-1098: new geometry, then every line resampled."""
+        super().upsample_model_resolution(iter_num)
+        res = self.resolution.tolist()
+        for kind in ('density', 'color'):
+            lines = getattr(self, f'vectors_{kind}')
+            for i in range(3):
+                lines[i] = tensorf_mods.upsample_factor(lines[i], res[VECTOR_AXES[i]], 1)
+
+    def shrink_tensor(self, bounding_box_new):
+        """src/models/SimpleTensoRF09.py:1112-1123: crop every line to the kept voxel range."""
+        t_l, b_r = super().shrink_tensor(bounding_box_new)
+        for kind in ('density', 'color'):
+            lines = getattr(self, f'vectors_{kind}')
+            for i in range(3):
+                v = VECTOR_AXES[i]
+                lines[i] = tensorf_mods.crop_factor(lines[i], slice(t_l[v], b_r[v]), slice(None))
+        return t_l, b_r
 
     def _density(self, pts, bbox, alpha, offset):
         sigma, _ = tensorf.cp_density(pts, list(self.vectors_density), bbox, alpha=alpha, predictor=self.density_kind,

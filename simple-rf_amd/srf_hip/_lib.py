@@ -85,11 +85,19 @@ _PROTOS = {
     'srf_depth_consistency_fwd': [_I, _P, _P, _P, _P, _P, _P, _I, _P, _P, _P, _I, _I, _I, _I, _I, _F, _I,
                                   _P, _P, _P, _P, _P, _P],
     'srf_depth_consistency_bwd': [_I, _P, _P, _P, _P, _P, _P, _P, _P],
+    'srf_dense_grid_points': [_I, _I, _I, _P, _P, _P],
+    'srf_alpha_mask_build': [_I, _I, _I, _P, _F, _F, _P, _P, _P],
+    'srf_factor_upsample': [_I, _I, _I, _P, _I, _I, _P, _P],
+    'srf_tv_workspace_bytes': [_I],
+    'srf_tv_fwd': [_I, _P, _P, _P, _P, _F, _P, _P, _P],
+    'srf_tv_bwd': [_I, _P, _P, _P, _P, _F, _P, _P, _P],
+    'srf_entropy_fwd': [_I, _I, _P, _P, _I, _P, _P, _P, _P, _P, _P],
+    'srf_entropy_bwd': [_I, _I, _P, _I, _P, _P, _P, _P, _P, _P],
 }
 _RESTYPES = {'srf_status_string': C.c_char_p, 'srf_mlp_packed_bytes': C.c_size_t,
              'srf_mlp_workspace_bytes': C.c_size_t, 'srf_compact_workspace_bytes': C.c_size_t,
              'srf_color_packed_bytes': C.c_size_t, 'srf_color_workspace_bytes': C.c_size_t,
-             'srf_tune_get': C.c_int64}
+             'srf_tune_get': C.c_int64, 'srf_tv_workspace_bytes': C.c_size_t}
 
 _lib = None
 

@@ -53,6 +53,15 @@ def check_fwd(out, ref, keys, ref64=None):
             ek = ((a.double() - t).abs() / t.abs().clamp_min(1e-5)).max().item()
             eo = ((b.double() - t).abs() / t.abs().clamp_min(1e-5)).max().item()
             assert ek <= 2 * eo + 1e-5, (k, ek, eo)
+        elif k == 'depth' and ref64 is not None:
+            # metric depth grows like 1/(1-z_ndc), so sum(w z) adds terms up to ~1e3 and FP32 keeps
+            # ~1e-4 absolute (the FP32 oracle itself, at 443 samples): the kernel must be within
+            # 2e-5 relative of FP64 or as close to it as the FP32 oracle
+            t = ref64[k]
+            ek = (a.double() - t).abs()
+            eo = (b.double() - t).abs()
+            bound = torch.maximum(1e-5 + 2e-5 * t.abs(), 2 * eo.max())
+            assert (ek <= bound).all(), (k, ek.max().item(), eo.max().item())
         elif k == 'depth':
             # metric depth grows like 1/(1-z_ndc); compare with a relative floor
             assert torch.allclose(a, b, atol=1e-5, rtol=2e-5), (k, (a - b).abs().max())

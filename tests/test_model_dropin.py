@@ -91,11 +91,9 @@ def test_tensorf_unsupported_options_raise():
     c = cfgs.simple_tensorf_configs(augmentations=False, coarse=dict(decomposition_type='Tucker'))
     with pytest.raises(RuntimeError):   # SimpleTensoRF09.py:542-543
         get_model(c, model_configs=cfgs.model_configs())
-    c = cfgs.simple_tensorf_configs(augmentations=False, coarse=dict(num_voxels_initial=16 ** 3))
-    m = get_model(c, model_configs=cfgs.model_configs())
-    m.train()
+    c = cfgs.simple_tensorf_configs(augmentations=False, coarse=dict(num_voxels_initial=16 ** 3, predict_visibility=True))
     with pytest.raises(NotImplementedError):
-        m.coarse_model.run_model_modifications(2500)
+        get_model(c, model_configs=cfgs.model_configs())
 
 
 def _cp_configs(**kw):
