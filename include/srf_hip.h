@@ -245,8 +245,8 @@ int srf_mlp_weight_grads(const srf_mlp_desc* desc, int n_points, void* workspace
  * [1, n_comp[p], line_len[p], 1], stored texel-major: plane p as [plane_h][plane_w][n_comp]
  * and line p as [line_len][n_comp] (the component axis innermost, 16-byte aligned bases).
  * Plane p is sampled at (x = pts[a0], y = pts[a1]) with (a0, a1) = (0,1), (0,2), (1,2) and
- * line p at pts[2 - p] (matrix_axes / vector_axes at :1131-1132).  n_comp[p] % 4 == 0 and
- * sum n_comp <= 256. */
+ * line p at pts[2 - p] (matrix_axes / vector_axes at :1131-1132).  n_comp[p] % 4 == 0 (else
+ * SRF_EUNSUPPORTED) and sum n_comp <= 256. */
 typedef struct srf_vm_factors {
   int n_comp[3];
   int plane_h[3];

@@ -773,9 +773,9 @@ int make_vm_args(const srf_vm_factors* f, const float* bbox, VmArgs* a, bool nee
   memset(a, 0, sizeof(*a));
   int q = 0;
   for (int p = 0; p < kPlanes; ++p) {
-    if (f->n_comp[p] < 0 || f->n_comp[p] % 4 != 0 || f->plane_h[p] < 1 || f->plane_w[p] < 1 ||
-        f->line_len[p] < 1)
-      return SRF_EINVAL;
+    if (f->n_comp[p] < 0 || f->plane_h[p] < 1 || f->plane_w[p] < 1 || f->line_len[p] < 1) return SRF_EINVAL;
+    // components are gathered four at a time (float4 taps)
+    if (f->n_comp[p] % 4 != 0) return SRF_EUNSUPPORTED;
     if (need_values && f->n_comp[p] > 0 && (!f->planes[p] || !f->lines[p])) return SRF_EINVAL;
     // taps are read as float4: the factor bases must be 16-byte aligned
     if ((reinterpret_cast<uintptr_t>(f->planes[p]) | reinterpret_cast<uintptr_t>(f->lines[p])) & 15)
@@ -800,7 +800,8 @@ int make_vm_args(const srf_vm_factors* f, const float* bbox, VmArgs* a, bool nee
 int make_cp_args(const srf_cp_factors* f, const float* bbox, CpArgs* a) {
   if (!f || !bbox) return SRF_EINVAL;
   memset(a, 0, sizeof(*a));
-  if (f->n_comp < 4 || f->n_comp % 4 != 0) return SRF_EINVAL;
+  if (f->n_comp < 4) return SRF_EINVAL;
+  if (f->n_comp % 4 != 0) return SRF_EUNSUPPORTED;
   if (f->n_comp / 4 > kMaxQuads) return SRF_EUNSUPPORTED;
   for (int k = 0; k < kPlanes; ++k) {
     if (f->line_len[k] < 1 || !f->lines[k]) return SRF_EINVAL;

@@ -432,6 +432,11 @@ class LowRankTensor(torch.nn.Module):
         self.predict_visibility = self.tensor_configs['predict_visibility']
         if self.predict_visibility:
             raise NotImplementedError('visibility prediction is outside the HIP hot path')
+        counts = list(self.tensor_configs['num_components_density']) + list(self.tensor_configs['num_components_color'])
+        if any(int(c) % 4 for c in counts):
+            # the reference's configurations use 4, 12, 16 and 48 components per factor
+            raise NotImplementedError(f'the factor gathers read components in groups of 4: {counts} are not all '
+                                      'multiples of 4')
         self.register_buffer('bounding_box', torch.zeros(size=(2, 3)))
         self.register_buffer('resolution', torch.zeros(size=(3,)))
         self.register_buffer('num_samples', torch.tensor(0))

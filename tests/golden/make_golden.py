@@ -196,18 +196,15 @@ def write_loss_golden(ref_root: str = '/root/reference') -> None:
     print('wrote', out_path, sum(np.asarray(a).nbytes for a in arrays.values()), 'bytes of arrays')
 
 
-if __name__ == '__main__':
-    main(*sys.argv[1:])
 
-
-TENSORF_CFG = {'decomposition_type': 'VectorMatrix', 'num_samples_max': 1e6, 'num_components_density': [4, 2, 2],
-               'num_components_color': [6, 3, 3], 'bounding_box': [[-1.5, -1.67, -1.0], [1.5, 1.67, 1.0]],
+TENSORF_CFG = {'decomposition_type': 'VectorMatrix', 'num_samples_max': 1e6, 'num_components_density': [4, 4, 4],
+               'num_components_color': [8, 4, 4], 'bounding_box': [[-1.5, -1.67, -1.0], [1.5, 1.67, 1.0]],
                'num_voxels_initial': 20 ** 3, 'num_voxels_final': 40 ** 3, 'tensor_upsampling_iters': [2000, 3000],
                'num_voxels_per_sample': 0.5, 'alpha_mask_update_iters': [2500], 'alpha_mask_threshold': 0.0001,
                'ray_marching_weight_threshold': 0.0001, 'use_view_dirs': True, 'view_dependent_color': True,
                'views_positional_encoding_degree': 0, 'features_positional_encoding_degree': 0,
                'features_dimension_color': 27, 'density_offset': -10, 'distance_scale': 25, 'density_predictor': 'ReLU',
-               'color_predictor': 'MLP_Features', 'num_units_color_predictor': 32, 'predict_visibility': False}
+               'color_predictor': 'MLP_Features', 'num_units_color_predictor': 128, 'predict_visibility': False}
 
 
 def write_tensorf_mods_golden() -> None:
@@ -278,3 +275,7 @@ def write_tensorf_mods_golden() -> None:
     out_path = HERE / 'tensorf_mods_golden.npz'
     np.savez_compressed(out_path, **arrays)
     print('wrote', out_path, sum(np.asarray(a).nbytes for a in arrays.values()), 'bytes of arrays')
+
+
+if __name__ == '__main__':
+    main(*sys.argv[1:])

@@ -63,11 +63,14 @@ def test_train_render_and_grads_match_oracle_fp32(cuda):
         return lo, ps
 
     ref_loss, p64 = oracle_grads(torch.float64)
-    loss32, p32 = oracle_grads(torch.float32)
-    # the objective's depth terms inherit the conditioning of depth = sum(w z) / (acc + 1e-6): the
-    # kernels' loss must be as close to FP64 as the FP32 oracle's (or within 1e-4 relative)
-    ek, eo = abs(loss.item() - ref_loss.item()), abs(loss32.item() - ref_loss.item())
-    assert ek < max(1e-4 * max(1.0, abs(ref_loss.item())), 3 * eo), (ek, eo)
+    _, p32 = oracle_grads(torch.float32)
+    # At random init the objective's depth terms inherit the conditioning of depth = sum(w z) /
+    # (acc + 1e-6) on empty rays, so its value is checked given the kernels' renders (which match the
+    # oracle above): the loss kernels against the oracle's objective in FP64 on the same renders.
+    on_k = {k: v.detach().cpu().double() for k, v in out.items() if k in ref}
+    b64 = {**cpu_batch, 'target_rgb': cpu_batch['target_rgb'].double(), 'sd_depth': cpu_batch['sd_depth'].double()}
+    ref_on_k = ots.reference_objective(on_k, b64, K.double()).item()
+    assert abs(loss.item() - ref_on_k) < 1e-5 * max(1.0, abs(ref_on_k)), (loss.item(), ref_on_k, ref_loss.item())
     # The depth losses differentiate sum(w z)/(acc + 1e-6), which is ill-conditioned where acc is
     # ~1e-6: the kernels' gradients must be as close to FP64 as the FP32 oracle's own (and within
     # 1e-3 wherever the FP32 oracle is).

@@ -15,15 +15,14 @@ from srf_hip import tensorf_mods
 pytestmark = pytest.mark.gpu
 
 GOLDEN = pathlib.Path(__file__).resolve().parent / 'golden' / 'tensorf_mods_golden.npz'
-CFG = {'decomposition_type': 'VectorMatrix', 'num_samples_max': 1e6, 'num_components_density': [4, 2, 2],
-       'num_components_color': [6, 3, 3], 'bounding_box': [[-1.5, -1.67, -1.0], [1.5, 1.67, 1.0]],
+CFG = {'decomposition_type': 'VectorMatrix', 'num_samples_max': 1e6, 'num_components_density': [4, 4, 4],
+       'num_components_color': [8, 4, 4], 'bounding_box': [[-1.5, -1.67, -1.0], [1.5, 1.67, 1.0]],
        'num_voxels_initial': 20 ** 3, 'num_voxels_final': 40 ** 3, 'tensor_upsampling_iters': [2000, 3000],
        'num_voxels_per_sample': 0.5, 'alpha_mask_update_iters': [2500], 'alpha_mask_threshold': 0.0001,
        'ray_marching_weight_threshold': 0.0001, 'use_view_dirs': True, 'view_dependent_color': True,
        'views_positional_encoding_degree': 0, 'features_positional_encoding_degree': 0,
        'features_dimension_color': 27, 'density_offset': -10, 'distance_scale': 25, 'density_predictor': 'ReLU',
        'color_predictor': 'MLP_Features', 'num_units_color_predictor': 128, 'predict_visibility': False}
-# the colour network plays no part in the modifications; 128 units is the width the colour kernels take
 
 
 @pytest.fixture(scope='module')
@@ -146,3 +145,55 @@ def test_upsample_reproduces_linear_ramps_and_identity(cuda):
     assert (up - want).abs().max().item() < 1e-5
     same = tensorf_mods.upsample_factor(p, h, w).detach().cpu()
     assert torch.equal(same, ramp)
+
+
+def test_regulariser_plugins_match_oracle(cuda):
+    """The drop-in TotalVariationLossHip04 and MassConcentrationLossHip06 plugins on a training forward
+    of the Simple-TensoRF demo (one points-augmented VM model) against the oracle's restatement of the
+    two losses in FP64: loss values, the augmented factors' TV gradients, the weights' entropy gradient
+    and the per-ray map (selected rays only, as the reference's map)."""
+    import oracle.tensorf_mods as om
+    from loss_functions.MassConcentrationLossHip06 import MassConcentrationLossHip
+    from loss_functions.TotalVariationLossHip04 import TotalVariationLossHip
+    from models.ModelFactory02 import get_model
+    from srf_hip import configs as cfgs, synthetic
+    torch.manual_seed(0)
+    c = cfgs.simple_tensorf_configs(perturb=False, coarse=dict(num_voxels_initial=32 ** 3),
+                                    augmented=dict(num_voxels_initial=16 ** 3))
+    c['optimizers'] = [{'name': 'optimizer_main', 'lr_decay_ratio': 0.1, 'lr_decay_iters': None}]
+    c['num_iterations'] = 25000
+    model = get_model(c, model_configs=cfgs.model_configs()).to(cuda)
+    model.train()
+    batch = synthetic.train_batch(512, seed=3, device=cuda)
+    out = model({'pixel_id': batch['pixel_id'], 'num_frames': 3, 'iter_num': 10}, retraw=True)
+    inp = {'target_rgb': batch['target_rgb'], 'indices_mask_nerf': batch['indices_mask_nerf'], 'iter_num': 10}
+    tv = TotalVariationLossHip(c, {'name': 'TotalVariationLossHip04', 'weight': 1e-2, 'weight_density': 1.0,
+                                   'weight_color': 0.5})
+    mc = MassConcentrationLossHip(c, {'name': 'MassConcentrationLossHip06', 'num_bins': 5})
+    res_tv = tv.compute_loss(inp, out, model, return_loss_maps=True)
+    res_mc = mc.compute_loss(inp, out, model, return_loss_maps=True)
+    aug = model.augmented_models[0]['coarse_model']
+    comps = list(aug.matrices_density) + list(aug.matrices_color)
+    g_tv = torch.autograd.grad(res_tv['loss_value'], comps, retain_graph=True)
+    w = out['points_augmentation_weights_coarse']
+    g_w, = torch.autograd.grad(res_mc['loss_value'], w)
+
+    iw = 0.1 ** (11 / 25000)
+    c64 = [p.detach().cpu().double().requires_grad_(True) for p in comps]
+    ref_tv = om.tv_loss(c64[:3], iw) * 1.0 + om.tv_loss(c64[3:], iw) * 0.5
+    ref_tv.backward()
+    assert abs(res_tv['loss_value'].item() - ref_tv.item()) <= 1e-5 * abs(ref_tv.item())
+    for a, b in zip(g_tv, c64):
+        assert (a.cpu().double() - b.grad).abs().max().item() <= 1e-4 * b.grad.abs().max().item()
+
+    sel = batch['indices_mask_nerf'].cpu()
+    w64 = w.detach().cpu().double().requires_grad_(True)
+    ref_mc = om.entropy_loss(w64, sel, 5)
+    ref_mc.backward()
+    assert abs(res_mc['loss_value'].item() - ref_mc.item()) <= 1e-5 * abs(ref_mc.item())
+    assert (g_w.cpu().double() - w64.grad).abs().max().item() <= 1e-5 * w64.grad.abs().max().item()
+    m = res_mc['loss_maps']['MassConcentrationLoss06_points_augmentation_coarse']
+    assert m.shape[0] == int(sel.sum())
+    sizes = om.entropy_bins(w64.shape[1], 5)
+    binned = torch.stack([b.sum(1) for b in torch.split(w64.detach()[sel], sizes, 1)], 1)
+    assert torch.allclose(m.cpu().double(), torch.special.entr(binned + 1e-3).sum(1), rtol=1e-5, atol=1e-7)

@@ -1,0 +1,37 @@
+"""The loss plugins load by the reference's rule (src/loss_functions/LossComputer03.py:20-31: module
+loss_functions.<name>, class <name>[:-2]) and construct from the reference's loss configs, without a GPU."""
+import importlib
+import pathlib
+
+import pytest
+
+PLUGIN_DIR = pathlib.Path(__file__).resolve().parents[1] / 'simple-rf_amd' / 'loss_functions'
+NAMES = sorted(p.stem for p in PLUGIN_DIR.glob('*Hip[0-9][0-9].py'))
+
+CONFIGS = {'data_loader': {'ndc': True},
+           'model': {'coarse_model': {}, 'fine_model': {},
+                     'augmentations': [{'name': 'points_augmentation', 'coarse_model': {}}]},
+           'optimizers': [{'name': 'optimizer_main', 'lr_decay_ratio': 0.1, 'lr_decay_iters': None}],
+           'num_iterations': 25000}
+LOSS_CONFIGS = {'models_to_regularize': {'main_coarse': 1, 'main_fine': 1, 'points_augmentation_coarse': 1},
+                'weight_density': 1, 'weight_color': 1, 'num_bins': 5, 'rmse_threshold': 0.1, 'patch_size': [5, 5]}
+
+
+def test_every_reference_loss_on_the_path_has_a_plugin():
+    assert NAMES == ['AugmentationsDepthLossHip11', 'CoarseFineConsistencyLossHip34', 'MSEHip14',
+                     'MassConcentrationLossHip06', 'SparseDepthMSEHip14', 'TotalVariationLossHip04']
+
+
+@pytest.mark.parametrize('name', NAMES)
+def test_plugin_loads_by_the_reference_rule(name):
+    module = importlib.import_module(f'loss_functions.{name}')
+    cls = getattr(module, name[:-2])
+    obj = cls(CONFIGS, LOSS_CONFIGS)
+    assert callable(obj.compute_loss)
+
+
+def test_tv_iteration_weight_follows_the_main_optimiser():
+    from loss_functions.TotalVariationLossHip04 import TotalVariationLossHip
+    tv = TotalVariationLossHip(CONFIGS, LOSS_CONFIGS)
+    assert tv.get_iter_weight(0) == 0.1 ** (1 / 25000)
+    assert abs(tv.get_iter_weight(24999) - 0.1) < 1e-12

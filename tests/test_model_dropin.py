@@ -94,6 +94,10 @@ def test_tensorf_unsupported_options_raise():
     c = cfgs.simple_tensorf_configs(augmentations=False, coarse=dict(num_voxels_initial=16 ** 3, predict_visibility=True))
     with pytest.raises(NotImplementedError):
         get_model(c, model_configs=cfgs.model_configs())
+    c = cfgs.simple_tensorf_configs(augmentations=False, coarse=dict(num_voxels_initial=16 ** 3,
+                                                                      num_components_density=(4, 2, 2)))
+    with pytest.raises(NotImplementedError, match='groups of 4'):
+        get_model(c, model_configs=cfgs.model_configs())
 
 
 def _cp_configs(**kw):
