@@ -31,7 +31,8 @@ enum srf_status {
   SRF_OK = 0,
   SRF_EHIP = 1,        /* a HIP launch/runtime error */
   SRF_EINVAL = 2,      /* bad size, null required pointer, unsupported option */
-  SRF_EUNSUPPORTED = 3 /* shape outside what the specialised kernels support */
+  SRF_EUNSUPPORTED = 3, /* shape outside what the specialised kernels support */
+  SRF_ETOOLARGE = 4     /* input beyond the fixed scratch of an allocation-free entry point */
 };
 
 /* ABI version of the loaded library (SRF_ABI_VERSION). */
@@ -350,6 +351,18 @@ int srf_alpha_mask_sample(const srf_alpha_grid* alpha, int64_t n_points, const f
 size_t srf_compact_workspace_bytes(int64_t n);
 int srf_threshold_compact(int64_t n, const float* values, float threshold, int32_t* idx,
                           int32_t* count, void* workspace, void* stream);
+
+/* srf_threshold_compact without a workspace argument and without any allocation: the per-tile
+ * counts live in a fixed device array of the library (1 MiB per GPU, held until the process
+ * ends), so n <= SRF_COMPACT_NOALLOC_MAX (else SRF_ETOOLARGE).  No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (code under HIP graph capture, serving
+ * loops, harnesses that forbid hipMalloc).  Call it with the stream's device current (else
+ * SRF_EINVAL).  Every call on one GPU shares that GPU's array: calls that overlap on one GPU
+ * (on different streams, or from different host threads) race and must be serialised by the
+ * caller.  Same results as srf_threshold_compact. */
+#define SRF_COMPACT_NOALLOC_MAX (1LL << 28)
+int srf_threshold_compact_noalloc(int64_t n, const float* values, float threshold, int32_t* idx,
+                                  int32_t* count, void* stream);
 
 /* out[r, c] = sum_k weights[k] * values[k, c] over the samples k of ray r (dense or ragged,
  * as srf_composite_fwd).  Replaces LowRankTensor.volume_render (src/models/SimpleTensoRF09.py:

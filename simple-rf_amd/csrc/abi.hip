@@ -100,6 +100,7 @@ extern "C" const char* srf_status_string(int status) {
     case SRF_EHIP: return "HIP runtime or launch error";
     case SRF_EINVAL: return "invalid argument (size, null pointer or option)";
     case SRF_EUNSUPPORTED: return "shape outside what the specialised kernels support";
+    case SRF_ETOOLARGE: return "input beyond the fixed scratch of an allocation-free entry point";
     default: return "unknown status";
   }
 }

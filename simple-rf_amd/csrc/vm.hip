@@ -23,6 +23,8 @@
 // compiled with -ffp-contract=off so no product is fused into a sum.
 #include <string.h>
 
+#include <mutex>
+
 #include "common.h"
 
 namespace srf {
@@ -1032,21 +1034,77 @@ extern "C" size_t srf_compact_workspace_bytes(int64_t n) {
   return (size_t)((tiles * 4 + 255) / 256 * 256 + 256);
 }
 
+namespace srf {
+namespace {
+// The three compaction passes on tile counts at `tile` (tiles = ceil(n / kCompactTile) entries).
+void launch_compact(int64_t n, const float* values, float threshold, int32_t* idx, int32_t* count, int32_t* tile,
+                    hipStream_t s) {
+  const int64_t tiles = (n + kCompactTile - 1) / kCompactTile;
+  if (tiles > 0) {
+    hipLaunchKernelGGL(compact_count_kernel, dim3((unsigned)tiles), dim3(256), 0, s, n, values, threshold, tile);
+  }
+  // the scan also writes *count (0 for an empty input), so no memset is needed
+  hipLaunchKernelGGL(compact_scan_kernel, dim3(1), dim3(1024), 0, s, tiles, tile, count);
+  if (tiles > 0) {
+    hipLaunchKernelGGL(compact_scatter_kernel, dim3((unsigned)tiles), dim3(256), 0, s, n, values, threshold, tile,
+                       idx);
+  }
+}
+}  // namespace
+}  // namespace srf
+
 extern "C" int srf_threshold_compact(int64_t n, const float* values, float threshold, int32_t* idx,
                                      int32_t* count, void* workspace, void* stream) {
   if (n < 0 || n > 0x7fffffffLL || !count) return SRF_EINVAL;
   if (n > 0 && (!values || !idx || !workspace)) return SRF_EINVAL;
-  const int64_t tiles = (n + kCompactTile - 1) / kCompactTile;
-  int32_t* tile = reinterpret_cast<int32_t*>(workspace);
-  if (tiles > 0) {
-    hipLaunchKernelGGL(compact_count_kernel, dim3((unsigned)tiles), dim3(256), 0, as_stream(stream), n, values,
-                       threshold, tile);
-  }
-  // the scan also writes *count (0 for an empty input), so no memset is needed
-  hipLaunchKernelGGL(compact_scan_kernel, dim3(1), dim3(1024), 0, as_stream(stream), tiles, tile, count);
-  if (tiles > 0) {
-    hipLaunchKernelGGL(compact_scatter_kernel, dim3((unsigned)tiles), dim3(256), 0, as_stream(stream), n, values,
-                       threshold, tile, idx);
-  }
+  launch_compact(n, values, threshold, idx, count, reinterpret_cast<int32_t*>(workspace), as_stream(stream));
   return launch_status();
 }
+
+// ------------------------------------------------- allocation-free compaction (fixed scratch)
+// The tile counts of srf_threshold_compact_noalloc.  Each GPU holds its own copy of this array
+// at its own address; host code never passes the array's name to a kernel (on the host it
+// names a stand-in, not device memory) but the per-device address from hipGetSymbolAddress.
+// Every count the passes read is written first by compact_count_kernel in the same call, so
+// nothing needs clearing between calls.
+constexpr int64_t kNoAllocTiles = SRF_COMPACT_NOALLOC_MAX / srf::kCompactTile;  // 262144 ints = 1 MiB
+static __device__ __align__(256) int32_t scratch[kNoAllocTiles];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+int32_t* g_scratch_ptr[kMaxDevices];  // device address of `scratch` on each GPU, looked up once
+
+// The scratch address on the current device (looked up on the first call there).
+int scratch_on_current_device(hipStream_t stream, int32_t** out) {
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess) return SRF_EHIP;
+  if (dev < 0 || dev >= kMaxDevices) return SRF_EINVAL;
+  if (stream != nullptr) {
+    hipDevice_t sdev = -1;
+    if (hipStreamGetDevice(stream, &sdev) != hipSuccess) return SRF_EHIP;
+    if (sdev != dev) return SRF_EINVAL;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (g_scratch_ptr[dev] == nullptr) {
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || p == nullptr) return SRF_EHIP;
+    g_scratch_ptr[dev] = static_cast<int32_t*>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return SRF_OK;
+}
+}  // namespace
+
+extern "C" int srf_threshold_compact_noalloc(int64_t n, const float* values, float threshold, int32_t* idx,
+                                             int32_t* count, void* stream) {
+  if (n < 0 || !count) return SRF_EINVAL;
+  if (n > SRF_COMPACT_NOALLOC_MAX) return SRF_ETOOLARGE;
+  if (n > 0 && (!values || !idx)) return SRF_EINVAL;
+  int32_t* tile = nullptr;
+  const int st = scratch_on_current_device(as_stream(stream), &tile);
+  if (st != SRF_OK) return st;
+  launch_compact(n, values, threshold, idx, count, tile, as_stream(stream));
+  return launch_status();
+}
+
