@@ -74,23 +74,29 @@ class _MSETerms(torch.autograd.Function):
     def backward(ctx, g_loss, _g_maps):
         target, select, count, *preds = ctx.saved_tensors
         select = select if ctx.has_select else None
-        d = [torch.empty_like(p) if ctx.needs_input_grad[4 + i] else None for i, p in enumerate(preds)]
+        need_t = ctx.needs_input_grad[0]
+        d = [torch.empty_like(p) if (ctx.needs_input_grad[4 + i] or need_t) else None for i, p in enumerate(preds)]
         if any(x is not None for x in d):
             _lib.call('srf_mse_terms_bwd', len(preds), _ptrs(preds), _floats(ctx.weights), target.data_ptr(),
                       _lib.ptr(select), ctx.n, ctx.c, count.data_ptr(), g_loss.contiguous().data_ptr(), _ptrs(d),
                       _lib.stream_ptr(target.device))
-        return (None, None, None, None, *d)
+        # the target enters every term as (pred - target): its gradient is minus the sum of the terms'
+        d_t = None
+        if need_t:
+            d_t = -d[0] if len(d) == 1 else -torch.stack(d).sum(0)
+        d = [x if ctx.needs_input_grad[4 + i] else None for i, x in enumerate(d)]
+        return (d_t, None, None, None, *d)
 
 
-def mse_terms(preds, target, indices_mask=None, weights=None, return_maps: bool = False):
+def mse_terms(preds, target, indices_mask=None, weights=None, return_maps: bool = False, target_grad: bool = False):
     """sum_t weights[t] * mean over the selected rays of the per-ray channel mean of (preds[t] - target)^2.
 
-    preds: up to 8 tensors shaped like target ([n, 3] colours or [n] depths); the target gets no
-    gradient (it is data, or a detached depth).  Returns the loss (0-dim) and, with return_maps, the
-    per-ray maps [n_terms, n] (0 on rays not selected)."""
+    preds: up to 8 tensors shaped like target ([n, 3] colours or [n] depths).  The target gets no
+    gradient (it is data, or a detached depth) unless target_grad.  Returns the loss (0-dim) and, with
+    return_maps, the per-ray maps [n_terms, n] (0 on rays not selected)."""
     if not preds:
         raise _lib.SrfError('mse_terms needs at least one prediction')
-    target = dev_tensor(target.detach(), 'target')
+    target = dev_tensor(target if target_grad else target.detach(), 'target')
     shape = target.shape
     t2 = target.reshape(shape[0], -1)
     ps = [dev_tensor(p, f'preds[{i}]', tuple(shape)).reshape(t2.shape) for i, p in enumerate(preds)]

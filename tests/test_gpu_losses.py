@@ -193,3 +193,29 @@ def test_loss_plugins_match_reference_formulas(cuda, scene):
                                      'AugmentationsDepthLoss11_points_augmentation_coarse_augmented'}
     cf = mod('CoarseFineConsistencyLossHip34')(configs, lc).compute_loss(gi, go, None)['loss_value']
     assert abs(cf.item() - float(scene['coarse_fine/f32/loss'])) <= 1e-6 * float(scene['coarse_fine/f32/loss'])
+
+
+@pytest.mark.parametrize('ndc', [True, False])
+def test_coarse_fine_consistency_33_plugin(cuda, ndc):
+    """CoarseFineConsistencyLossHip33 against CoarseFineConsistencyLoss33.py:35-57 in FP64: the mean of
+    (coarse - fine)^2 over all rays on the NDC (or metric) depths, both depths differentiated, and its map."""
+    import importlib
+    g = torch.Generator().manual_seed(8)
+    n = 3001
+    key = 'depth_ndc' if ndc else 'depth'
+    c0, f0 = torch.rand(n, generator=g), torch.rand(n, generator=g)
+    c, f = c0.to(cuda).requires_grad_(True), f0.to(cuda).requires_grad_(True)
+    configs = {'model': {'coarse_model': {}, 'fine_model': {}}, 'data_loader': {'ndc': ndc}}
+    cls = importlib.import_module('loss_functions.CoarseFineConsistencyLossHip33').CoarseFineConsistencyLossHip
+    res = cls(configs, {}).compute_loss({'target_rgb': torch.zeros(n, 3, device=cuda)},
+                                        {f'{key}_coarse': c, f'{key}_fine': f}, None, return_loss_maps=True)
+    res['loss_value'].backward()
+    c64, f64 = c0.double().requires_grad_(True), f0.double().requires_grad_(True)
+    m64 = torch.square(c64 - f64)
+    ref = m64.mean()
+    ref.backward()
+    assert abs(res['loss_value'].item() - ref.item()) <= 1e-6 * ref.item()
+    assert torch.allclose(c.grad.cpu().double(), c64.grad, rtol=1e-5, atol=1e-12)
+    assert torch.allclose(f.grad.cpu().double(), f64.grad, rtol=1e-5, atol=1e-12)
+    assert torch.allclose(res['loss_maps']['CoarseFineConsistencyLoss33'].detach().cpu().double(), m64.detach(),
+                          rtol=1e-6, atol=1e-12)

@@ -18,8 +18,8 @@ LOSS_CONFIGS = {'models_to_regularize': {'main_coarse': 1, 'main_fine': 1, 'poin
 
 
 def test_every_reference_loss_on_the_path_has_a_plugin():
-    assert NAMES == ['AugmentationsDepthLossHip11', 'CoarseFineConsistencyLossHip34', 'MSEHip14',
-                     'MassConcentrationLossHip06', 'SparseDepthMSEHip14', 'TotalVariationLossHip04']
+    assert NAMES == ['AugmentationsDepthLossHip11', 'CoarseFineConsistencyLossHip33', 'CoarseFineConsistencyLossHip34',
+                     'MSEHip14', 'MassConcentrationLossHip06', 'SparseDepthMSEHip14', 'TotalVariationLossHip04']
 
 
 @pytest.mark.parametrize('name', NAMES)
