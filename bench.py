@@ -234,6 +234,32 @@ def hip_graph_train_leg(args, dev, precision='bf16'):
     return dict(seconds=t1 - t0, loss=float(loss.item()))
 
 
+def graph_replay_time(step, opt, warmup, steps):
+    """Seconds for `steps` replays of `step` (forward, backward, optimiser step) captured once in a
+    HIP graph: warm-up on a side stream, capture with the gradients unset (the graph allocates and
+    writes them), two untimed replays, then the timed ones."""
+    import torch
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        for _ in range(max(warmup, 2)):
+            opt.zero_grad(set_to_none=True)
+            step()
+    torch.cuda.current_stream().wait_stream(side)
+    graph = torch.cuda.CUDAGraph()
+    opt.zero_grad(set_to_none=True)
+    with torch.cuda.graph(graph):
+        step()
+    for _ in range(2):
+        graph.replay()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        graph.replay()
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0
+
+
 def hip_render_leg(args, dev, model):
     """Forward-only eval render of one chunk (deterministic sampling, no augmentations)."""
     import torch
@@ -388,17 +414,20 @@ def tensorf_train_leg(args, dev):
     _, model = tensorf_model(dev)
     model.train()
     groups = model.get_trainable_parameters({'lr_initial_tensor': 2e-2, 'lr_initial_network': 1e-3})
-    opt = torch.optim.Adam(groups, betas=(0.9, 0.99), fused=True)
+    opt = torch.optim.Adam(groups, betas=(0.9, 0.99), fused=True, capturable=True)
     batch = synthetic.train_batch(args.tensorf_rays, seed=5, device=dev)
     inp = {'pixel_id': batch['pixel_id'], 'num_frames': 3, 'iter_num': 1, 'sub_batch_index': 0}
 
-    def step():
+    def fwd_bwd():
         out = model(inp)
         loss = synthetic.train_loss(out, batch)
-        opt.zero_grad(set_to_none=True)
         loss.backward()
         opt.step()
         return out
+
+    def step():
+        opt.zero_grad(set_to_none=True)
+        return fwd_bwd()
 
     for _ in range(args.warmup):
         step()
@@ -417,8 +446,16 @@ def tensorf_train_leg(args, dev):
         timer.enabled = False
     kern = timer.summary()
     n = args.tensorf_steps
-    return {'value': round(args.tensorf_rays * n / (t1 - t0), 1), 'unit': 'rays/s',
-            'ms_per_step': round((t1 - t0) * 1e3 / n, 3),
+    # keep only values of the last eager step: its autograd graph (and the per-parameter gradient
+    # accumulators it references, tied to the stream they were created on) must not outlive it into
+    # the capture on another stream
+    out = {k: (v.detach() if isinstance(v, torch.Tensor) else v) for k, v in out.items()}
+    # the step has no host synchronisation (device-side surface count): capture it once and replay
+    g_sec = graph_replay_time(fwd_bwd, opt, args.warmup, n)
+    return {'value': round(args.tensorf_rays * n / g_sec, 1), 'unit': 'rays/s',
+            'ms_per_step': round(g_sec * 1e3 / n, 3), 'step_mode': 'hip_graph_replay',
+            'eager_rays_per_s': round(args.tensorf_rays * n / (t1 - t0), 1),
+            'eager_ms_per_step': round((t1 - t0) * 1e3 / n, 3),
             'workload': f'{args.tensorf_rays} rays x 443 samples, VM 300^2 density 3x16 / appearance 3x48, '
                         '64^3 alpha mask (40 % occupied), low-rank augmented model, fwd+bwd+Adam, FP32',
             'kernels': {k: {'calls': v['calls'], 'ms_per_step': round(v['ms'] / n, 3)} for k, v in kern.items()},
@@ -434,7 +471,7 @@ def tensorf_ragged_leg(args, dev):
     import torch
     import torch.nn.functional as F
     from models.ModelFactory02 import get_model
-    from srf_hip import _lib, configs as cfgs
+    from srf_hip import _lib, configs as cfgs, losses
     torch.manual_seed(0)
     c = cfgs.simple_tensorf_configs(
         augmentations=False, ndc=False, perturb=False,
@@ -457,15 +494,18 @@ def tensorf_ragged_leg(args, dev):
            'rays_o': o.to(dev), 'rays_d': d.to(dev), 'view_dirs': d.to(dev), 'max_samples': int(counts.max())}
     target = torch.rand(n, 3, generator=g).to(dev)
     groups = tm.get_trainable_parameters({'lr_initial_tensor': 2e-2, 'lr_initial_network': 1e-3})
-    opt = torch.optim.Adam(groups, betas=(0.9, 0.99), fused=True)
+    opt = torch.optim.Adam(groups, betas=(0.9, 0.99), fused=True, capturable=True)
 
-    def train_step():
+    def fwd_bwd():
         out = tm.forward_ragged(inp)
-        loss = ((out['rgb'] - target) ** 2).mean()
-        opt.zero_grad(set_to_none=True)
+        loss = losses.mse_terms([out['rgb']], target)
         loss.backward()
         opt.step()
         return out
+
+    def train_step():
+        opt.zero_grad(set_to_none=True)
+        return fwd_bwd()
 
     for _ in range(2):
         out = train_step()
@@ -485,11 +525,16 @@ def tensorf_ragged_leg(args, dev):
     kern = timer.summary()
     s = args.ragged_steps
     n_samples = int(off[-1])
-    return {'value': round(n * s / (t1 - t0), 1), 'unit': 'rays/s', 'ms_per_step': round((t1 - t0) * 1e3 / s, 3),
-            'samples_per_step': n_samples, 'surface_samples': int((out['weights'] > 1e-4).sum()),
+    surface = int((out['weights'] > 1e-4).sum())
+    del out  # no autograd graph of an eager step may outlive it into the capture (see tensorf_train_leg)
+    g_sec = graph_replay_time(fwd_bwd, opt, 2, s)
+    return {'value': round(n * s / g_sec, 1), 'unit': 'rays/s', 'ms_per_step': round(g_sec * 1e3 / s, 3),
+            'step_mode': 'hip_graph_replay', 'eager_rays_per_s': round(n * s / (t1 - t0), 1),
+            'eager_ms_per_step': round((t1 - t0) * 1e3 / s, 3),
+            'samples_per_step': n_samples, 'surface_samples': surface,
             'workload': f'{n} rays, ragged counts (truncated log-normal, median 96, max 512; {n_samples:,} '
                         'samples), VM 512^2 density 3x24 / appearance 3x64 (277 MB), features PE L=2, '
-                        'fwd+bwd+Adam, FP32',
+                        'MSE, fwd+bwd+Adam, FP32',
             'kernels': {k: {'calls': v['calls'], 'ms_per_step': round(v['ms'] / s, 3)} for k, v in kern.items()}}
 
 
@@ -793,6 +838,11 @@ def split_train_leg(args, dev):
     return out
 
 
+def progress(msg):
+    """One line per finished stage on stderr (the JSON result alone goes to stdout)."""
+    print(f'[bench] {msg}', file=sys.stderr, flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -832,12 +882,14 @@ def main():
         from srf_hip import autotune
         TUNER = autotune.Autotuner()
 
+    progress('NeRF train step, eager')
     tr = hip_train_leg(args, dev)
     eager_value = args.rays * args.steps / tr['seconds']
     graph_info = None
     value, ms = eager_value, tr['seconds'] * 1000.0 / args.steps
     if not args.no_graph:
         try:
+            progress('NeRF train step, graph capture and replay')
             gr = hip_graph_train_leg(args, dev)
             value, ms = args.rays * args.steps / gr['seconds'], gr['seconds'] * 1000.0 / args.steps
             graph_info = {'captured': True}
@@ -849,6 +901,7 @@ def main():
                       **({'tflops': round(v['flop'] / (v['ms'] * 1e-3) / 1e12, 1),
                           'GB/s': round(v['bytes'] / (v['ms'] * 1e-3) / 1e9, 1)} if v['flop'] else {})}
                   for k, v in kern.items()}
+    progress('render legs')
     render = hip_render_leg(args, dev, tr['model'])
     image = None if args.no_image else render_image_leg(args, dev)
     if args.copy_calibration_mb:
@@ -858,6 +911,7 @@ def main():
         for _ in range(3):
             dst.copy_(src)
         torch.cuda.synchronize()
+    progress('split-BF16 train step')
     split = None if args.no_split else split_train_leg(args, dev)
     result = {
         'metric': METRIC, 'value': round(value, 1), 'unit': 'rays/s', 'n_gpus': 1, 'steps': args.steps,
@@ -876,15 +930,18 @@ def main():
         **({'train_bf16_split': split} if split else {}),
     }
     if not args.no_tensorf:
+        progress('TensoRF train step (eager, then graph capture and replay)')
         tf = tensorf_train_leg(args, dev)
         result['tensorf_train'] = {k: v for k, v in tf.items() if k not in ('model', 'out')}
         result['hbm_kernels'] = hbm_leg(dev, tf['model'], tf['out'])
         del tf
     if not args.no_ragged:
+        progress('ragged TensoRF train step (eager, then graph capture and replay)')
         result['tensorf_ragged_train'] = tensorf_ragged_leg(args, dev)
     if TUNER is not None:
         result['autotune'] = TUNER.results
     if not args.no_baselines:
+        progress('plain FP32 PyTorch step on the GPU, then the CPU baselines')
         plain = plain_gpu_leg(args, dev, tr['model'])
         result['plain_fp32_gpu'] = plain
         result['speedup_vs_plain_fp32_gpu'] = round(value / plain['value'], 2)

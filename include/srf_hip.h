@@ -25,7 +25,7 @@
 extern "C" {
 #endif
 
-#define SRF_ABI_VERSION 1
+#define SRF_ABI_VERSION 2
 
 enum srf_status {
   SRF_OK = 0,
@@ -286,19 +286,23 @@ int srf_vm_density_bwd(const srf_vm_factors* factors, float* const* d_planes,
                        const float* pts, const uint8_t* valid, const float* feat, int predictor,
                        float offset, const float* d_sigma, void* stream);
 
-/* Appearance features of n_rows samples: features[j, :] = cat_p(plane_p * line_p) at sample
+/* Appearance features of the listed samples: features[j, :] = cat_p(plane_p * line_p) at sample
  * idx[j] (idx NULL: sample j), [n_rows, sum n_comp] row-major.  Replaces the gather part of
  * VmDecomposedTensor.get_color (src/models/SimpleTensoRF09.py:1241-1263, before
- * basis_matrix_color). */
+ * basis_matrix_color).  n_rows is the capacity of idx and features; row_count (device int32,
+ * NULL: n_rows) is the number of rows processed, e.g. the count srf_threshold_compact wrote, so
+ * that a caller needs no host synchronisation (the reference's boolean indexing at :1249 has
+ * one). */
 int srf_vm_appearance_fwd(const srf_vm_factors* factors, const float* bbox, int64_t n_rows,
-                          const int32_t* idx, const float* pts, float* features, void* stream);
+                          const int32_t* row_count, const int32_t* idx, const float* pts,
+                          float* features, void* stream);
 
-/* Backward of srf_vm_appearance_fwd: d_features [n_rows, sum n_comp] scattered into
- * d_planes / d_lines with FP32 atomics (accumulating). */
+/* Backward of srf_vm_appearance_fwd: d_features [n_rows, sum n_comp] (rows below the count)
+ * scattered into d_planes / d_lines with FP32 atomics (accumulating). */
 int srf_vm_appearance_bwd(const srf_vm_factors* factors, float* const* d_planes,
                           float* const* d_lines, const float* bbox, int64_t n_rows,
-                          const int32_t* idx, const float* pts, const float* d_features,
-                          void* stream);
+                          const int32_t* row_count, const int32_t* idx, const float* pts,
+                          const float* d_features, void* stream);
 
 /* The CANDECOMP/PARAFAC factors of one CpDecomposedTensor field (src/models/SimpleTensoRF09.py:
  * 964-1127): three lines of the same rank n_comp, line i being the reference's vectors_*[i] of
@@ -328,15 +332,16 @@ int srf_cp_density_bwd(const srf_cp_factors* factors, float* const* d_lines, con
 /* CP appearance features: features[j, r] = line_0[r] * line_1[r] * line_2[r] at sample idx[j]
  * (idx NULL: sample j), [n_rows, n_comp] row-major.  Replaces the gather part of
  * CpDecomposedTensor.get_color (src/models/SimpleTensoRF09.py:1064-1078, before
- * basis_matrix_color). */
+ * basis_matrix_color).  n_rows and row_count as for srf_vm_appearance_fwd. */
 int srf_cp_appearance_fwd(const srf_cp_factors* factors, const float* bbox, int64_t n_rows,
-                          const int32_t* idx, const float* pts, float* features, void* stream);
+                          const int32_t* row_count, const int32_t* idx, const float* pts,
+                          float* features, void* stream);
 
-/* Backward of srf_cp_appearance_fwd: d_features [n_rows, n_comp] scattered into d_lines[3]
- * with FP32 atomics (accumulating). */
+/* Backward of srf_cp_appearance_fwd: d_features [n_rows, n_comp] (rows below the count)
+ * scattered into d_lines[3] with FP32 atomics (accumulating). */
 int srf_cp_appearance_bwd(const srf_cp_factors* factors, float* const* d_lines, const float* bbox,
-                          int64_t n_rows, const int32_t* idx, const float* pts,
-                          const float* d_features, void* stream);
+                          int64_t n_rows, const int32_t* row_count, const int32_t* idx,
+                          const float* pts, const float* d_features, void* stream);
 
 /* Trilinear occupancy value of each point (AlphaGridMask.sample_alpha,
  * src/models/SimpleTensoRF09.py:1342-1345): out [n_points]. */
@@ -414,19 +419,26 @@ int srf_color_pack(const srf_color_desc* desc, const float* basis, const float* 
                    const float* w2, const float* b2, const float* w3, const float* b3, void* packed,
                    void* stream);
 
-/* rgb [n,3] of the n_points appearance rows app [n, n_features] (16-byte aligned).  View
- * direction of row j: view_dirs[r] with r = (idx ? idx[j] : j) / n_samples (the sample's ray in
- * the dense [rays, n_samples] layout).  Replaces basis_matrix_color + color_predictor in
- * VmDecomposedTensor.get_color (src/models/SimpleTensoRF09.py:1262-1270).  workspace NULL:
- * inference; otherwise the layer inputs are kept for srf_color_backward. */
-int srf_color_forward(const srf_color_desc* desc, const void* packed, int n_points, const float* app,
-                      const int32_t* idx, int n_samples, const float* view_dirs, float* rgb,
-                      void* workspace, void* stream);
+/* rgb of the appearance rows app [n, n_features] (16-byte aligned).  View direction of row j:
+ * view_dirs[r] with r = (idx ? idx[j] : j) / n_samples (the sample's ray in the dense
+ * [rays, n_samples] layout).  n_points is the capacity (rows of app and idx, and the workspace
+ * size); point_count (device int32, NULL: n_points) the rows processed.  rgb_at_idx 0: rgb row j
+ * of [n, 3]; 1: rgb row idx[j] of the dense [rays * n_samples, 3] sample buffer, whose other rows
+ * the caller zero-fills (the reference's rgb = zeros(...), rgb[validity_mask] = surface_rgb at
+ * :1246, :1271).  Replaces
+ * basis_matrix_color + color_predictor in VmDecomposedTensor.get_color
+ * (src/models/SimpleTensoRF09.py:1262-1270).  workspace NULL: inference; otherwise the layer
+ * inputs are kept for srf_color_backward. */
+int srf_color_forward(const srf_color_desc* desc, const void* packed, int n_points,
+                      const int32_t* point_count, const float* app, const int32_t* idx, int n_samples,
+                      const float* view_dirs, float* rgb, int rgb_at_idx, void* workspace, void* stream);
 
-/* Backward of srf_color_forward (same workspace): d_app [n, n_features] (overwritten, 16-byte
- * aligned) for srf_vm_appearance_bwd, and the parameter gradients in the reference layouts,
- * accumulated with FP32 atomics (zero them first). */
-int srf_color_backward(const srf_color_desc* desc, const void* packed, int n_points, const float* rgb,
+/* Backward of srf_color_forward (same workspace, capacity, count and rgb addressing): d_app
+ * [n, n_features] (rows below the count overwritten, 16-byte aligned) for
+ * srf_vm_appearance_bwd, and the parameter gradients in the reference layouts, accumulated with
+ * FP32 atomics (zero them first). */
+int srf_color_backward(const srf_color_desc* desc, const void* packed, int n_points,
+                       const int32_t* point_count, const int32_t* idx, int rgb_at_idx, const float* rgb,
                        const float* d_rgb, void* workspace, float* d_app, float* d_basis, float* d_w1,
                        float* d_b1, float* d_w2, float* d_b2, float* d_w3, float* d_b3, void* stream);
 

@@ -18,6 +18,8 @@
 // column = point) for the weight-gradient GEMM, which is the grouped dW kernel of the MLP.
 #include <string.h>
 
+#include <mutex>
+
 #include "mlp_impl.h"
 
 namespace srf {
@@ -238,14 +240,28 @@ struct FwdArgs {
   Geo g;
   Ws w;
   const float* pk;
-  int n;
+  int n;                 // capacity (rows of app, idx and the workspace)
+  const int32_t* count;  // device-side row count, or null (n rows)
   const float* app;  // [n, C]
   const int32_t* idx;
   int n_samples;
   const float* view_dirs;
   float* rgb;
+  int rgb_at_idx;  // rgb row idx[p] of the dense sample buffer instead of row p
   uint8_t* ws;  // null: inference
 };
+
+// Rows a launch processes: the capacity, or the device-side count below it.
+__device__ __forceinline__ int active_rows(int capacity, const int32_t* count) {
+  return count ? min(capacity, *count) : capacity;
+}
+
+// Whether block b has work: blocks up to the count rounded to whole 256-point workspace blocks run
+// (their points past the count as padding: zero features, finite activations, dZ = 0), so every
+// column the weight-gradient GEMM reads (it rounds the count the same way) was written this call.
+__device__ __forceinline__ bool block_has_rows(int b, int n) {
+  return (int64_t)b * kWaves * kPW < (int64_t)(n + 255) / 256 * 256;
+}
 
 __device__ __forceinline__ void store_fm(uint8_t* base, size_t row_off, int ld, int row, int p, float v) {
   // streamed past L2 (non-temporal): the rows are read back only by the backward
@@ -277,6 +293,8 @@ __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(2, 
   __shared__ float sF[kWaves][kMaxB][kPW + 1];
   __shared__ float sD[kWaves][3][kPW];
   const Geo& g = a.g;
+  const int n = active_rows(a.n, a.count);
+  if (!block_has_rows(blockIdx.x, n)) return;  // uniform over the block
   const int lane = threadIdx.x & 63, gq = lane >> 4, j = lane & 15;
   const int wv = threadIdx.x >> 6;
   const int p0 = (blockIdx.x * kWaves + wv) * kPW;
@@ -291,7 +309,7 @@ __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(2, 
     const int p = p0 + 16 * cb + j;
     const int c = 16 * u + 4 * gq;
     float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (p < a.n && c < g.C) v = *reinterpret_cast<const float4*>(a.app + (size_t)p * g.C + c);
+    if (p < n && c < g.C) v = *reinterpret_cast<const float4*>(a.app + (size_t)p * g.C + c);
     return f32x4{v.x, v.y, v.z, v.w};
   };
   // training copy of the appearance features, feature-major (rows c .. c + 3 of the lane's group)
@@ -315,7 +333,7 @@ __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(2, 
   if (g.view && lane < kPW) {
     const int p = p0 + lane;
     float d0 = 0.f, d1 = 0.f, d2 = 0.f;
-    if (p < a.n) {
+    if (p < n) {
       const int64_t smp = a.idx ? (int64_t)a.idx[p] : (int64_t)p;
       const int64_t ray = smp / a.n_samples;
       d0 = a.view_dirs[3 * ray + 0];
@@ -391,9 +409,11 @@ __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(2, 
 #pragma unroll
     for (int cb = 0; cb < kCB; ++cb) {
       const int p = p0 + 16 * cb + j;
-      if (p < a.n)
+      if (p < n) {
+        const size_t dst = a.rgb_at_idx ? (size_t)a.idx[p] : (size_t)p;
 #pragma unroll
-        for (int r = 0; r < 3; ++r) a.rgb[3 * (size_t)p + r] = 1.f / (1.f + expf(-o[0][cb][r]));
+        for (int r = 0; r < 3; ++r) a.rgb[3 * dst + r] = 1.f / (1.f + expf(-o[0][cb][r]));
+      }
     }
   }
 }
@@ -402,7 +422,10 @@ struct BwdArgs {
   Geo g;
   Ws w;
   const float* pk;
-  int n;
+  int n;                 // capacity
+  const int32_t* count;  // device-side row count, or null
+  const int32_t* idx;    // rows of rgb / d_rgb when rgb_at_idx
+  int rgb_at_idx;
   const float* rgb;
   const float* d_rgb;
   float* d_app;
@@ -420,6 +443,8 @@ __global__ __launch_bounds__(64 * kWaves) void color_bwd_kernel(BwdArgs a) {
   __shared__ __attribute__((aligned(16))) uint8_t stage[kStage];
   extern __shared__ float lds_dyn[];
   const Geo& g = a.g;
+  const int n = active_rows(a.n, a.count);
+  if (!block_has_rows(blockIdx.x, n)) return;  // uniform over the block
   const int lane = threadIdx.x & 63, gq = lane >> 4, j = lane & 15;
   const int wv = threadIdx.x >> 6;
   const int xrows = bwd_rows(g);
@@ -434,9 +459,10 @@ __global__ __launch_bounds__(64 * kWaves) void color_bwd_kernel(BwdArgs a) {
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       float v = 0.f;
-      if (gq == 0 && r < 3 && p < a.n) {
-        const float s = a.rgb[3 * (size_t)p + r];
-        v = __fmul_rn(__fmul_rn(a.d_rgb[3 * (size_t)p + r], __fsub_rn(1.f, s)), s);
+      if (gq == 0 && r < 3 && p < n) {
+        const size_t src = a.rgb_at_idx ? (size_t)a.idx[p] : (size_t)p;
+        const float s = a.rgb[3 * src + r];
+        v = __fmul_rn(__fmul_rn(a.d_rgb[3 * src + r], __fsub_rn(1.f, s)), s);
       }
       dz3[0][cb][r] = v;
     }
@@ -561,7 +587,7 @@ __global__ __launch_bounds__(64 * kWaves) void color_bwd_kernel(BwdArgs a) {
 #pragma unroll
         for (int cb = 0; cb < kCB; ++cb) {
           const int p = p0 + 16 * cb + j;
-          if (p < a.n)
+          if (p < n)
             *reinterpret_cast<float4*>(a.d_app + (size_t)p * g.C + c) =
                 make_float4(da[T][cb][0], da[T][cb][1], da[T][cb][2], da[T][cb][3]);
         }
@@ -578,6 +604,30 @@ __global__ __launch_bounds__(64 * kWaves) void color_bwd_kernel(BwdArgs a) {
 
 using namespace srf;
 using namespace srf::color;
+
+namespace {
+// The backward's per-wave LDS tile can take the block past the 64 KB default of dynamic LDS.  The
+// kernel's limit is raised once per device, to what the largest supported geometry needs, so that
+// every later call (a call under stream capture included, where no such runtime call belongs)
+// launches without touching it.
+int raise_bwd_lds_limit() {
+  constexpr int kMaxDevices = 64;
+  static std::mutex mutex;
+  static bool raised[kMaxDevices];
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return SRF_EHIP;
+  std::lock_guard<std::mutex> lock(mutex);
+  if (!raised[dev]) {
+    constexpr int kMaxLds = kWaves * (kMaxD1 > kMaxB ? kMaxD1 : kMaxB) * (kPW + 1) * (int)sizeof(float);
+    static_assert(kMaxLds + kStage <= 160 * 1024, "backward LDS within a CU's 160 KB");
+    if (hipFuncSetAttribute(reinterpret_cast<const void*>(&color_bwd_kernel),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, kMaxLds) != hipSuccess)
+      return SRF_EHIP;
+    raised[dev] = true;
+  }
+  return SRF_OK;
+}
+}  // namespace
 
 extern "C" size_t srf_color_packed_bytes(const srf_color_desc* desc) {
   Geo g;
@@ -630,38 +680,42 @@ extern "C" int srf_color_pack(const srf_color_desc* desc, const float* basis, co
   return launch_status();
 }
 
-extern "C" int srf_color_forward(const srf_color_desc* desc, const void* packed, int n_points, const float* app,
-                                 const int32_t* idx, int n_samples, const float* view_dirs, float* rgb,
-                                 void* workspace, void* stream) {
+extern "C" int srf_color_forward(const srf_color_desc* desc, const void* packed, int n_points,
+                                 const int32_t* point_count, const float* app, const int32_t* idx, int n_samples,
+                                 const float* view_dirs, float* rgb, int rgb_at_idx, void* workspace,
+                                 void* stream) {
   Geo g;
   if (!make_geo(desc, &g) || n_points < 0 || (g.view && n_samples < 1)) return SRF_EINVAL;
   if (n_points == 0) return SRF_OK;
-  if (!packed || !app || !rgb || (g.view && !view_dirs)) return SRF_EINVAL;
+  if (!packed || !app || !rgb || (g.view && !view_dirs) || (rgb_at_idx && !idx)) return SRF_EINVAL;
   if (reinterpret_cast<uintptr_t>(app) & 15) return SRF_EINVAL;
   FwdArgs a;
   a.g = g;
   a.w = make_ws(g, n_points);
   a.pk = reinterpret_cast<const float*>(packed);
   a.n = n_points;
+  a.count = point_count;
   a.app = app;
   a.idx = idx;
   a.n_samples = n_samples > 0 ? n_samples : 1;
   a.view_dirs = view_dirs;
   a.rgb = rgb;
+  a.rgb_at_idx = rgb_at_idx != 0;
   a.ws = reinterpret_cast<uint8_t*>(workspace);
   const int blocks = a.w.cols / (kPW * kWaves);
   hipLaunchKernelGGL(color_fwd_kernel, dim3(blocks), dim3(64 * kWaves), 0, as_stream(stream), a);
   return launch_status();
 }
 
-extern "C" int srf_color_backward(const srf_color_desc* desc, const void* packed, int n_points, const float* rgb,
+extern "C" int srf_color_backward(const srf_color_desc* desc, const void* packed, int n_points,
+                                  const int32_t* point_count, const int32_t* idx, int rgb_at_idx, const float* rgb,
                                   const float* d_rgb, void* workspace, float* d_app, float* d_basis, float* d_w1,
                                   float* d_b1, float* d_w2, float* d_b2, float* d_w3, float* d_b3, void* stream) {
   Geo g;
   if (!make_geo(desc, &g) || n_points < 0) return SRF_EINVAL;
   if (n_points == 0) return SRF_OK;
   if (!packed || !rgb || !d_rgb || !workspace || !d_app || !d_basis || !d_w1 || !d_b1 || !d_w2 || !d_b2 || !d_w3 ||
-      !d_b3)
+      !d_b3 || (rgb_at_idx && !idx))
     return SRF_EINVAL;
   if (reinterpret_cast<uintptr_t>(d_app) & 15) return SRF_EINVAL;
   hipStream_t s = as_stream(stream);
@@ -670,15 +724,15 @@ extern "C" int srf_color_backward(const srf_color_desc* desc, const void* packed
   b.w = make_ws(g, n_points);
   b.pk = reinterpret_cast<const float*>(packed);
   b.n = n_points;
+  b.count = point_count;
+  b.idx = idx;
+  b.rgb_at_idx = rgb_at_idx != 0;
   b.rgb = rgb;
   b.d_rgb = d_rgb;
   b.d_app = d_app;
   b.ws = reinterpret_cast<uint8_t*>(workspace);
   const size_t lds = (size_t)kWaves * bwd_rows(g) * (kPW + 1) * sizeof(float);
-  if (lds + kStage > 65536 &&
-      hipFuncSetAttribute(reinterpret_cast<const void*>(&color_bwd_kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
-                          (int)lds) != hipSuccess)
-    return SRF_EHIP;
+  if (const int st = raise_bwd_lds_limit(); st != SRF_OK) return st;
   hipLaunchKernelGGL(color_bwd_kernel, dim3(b.w.cols / (kPW * kWaves)), dim3(64 * kWaves), lds, s, b);
   int st = launch_status();
   if (st != SRF_OK) return st;
@@ -689,6 +743,7 @@ extern "C" int srf_color_backward(const srf_color_desc* desc, const void* packed
   memset(&d, 0, sizeof(d));
   d.ldP = w.ld;
   d.cols = w.cols;
+  d.cols_dev = point_count;
   int nj = 0;
   auto job = [&](size_t dz, int dz_rows, size_t x, int x_rows, int M, int N, float* dw, int ld, float* db) {
     DwJob& jb = d.jobs[nj++];

@@ -1236,7 +1236,16 @@ struct DwArgs {
   DwJob jobs[kMaxJobs];
   int n_jobs, ldP, cols, chunk, n_chunks, total_work;
   int paired;  // 1: dZ / X in the paired tile layout of store_tile; 0: plain feature-major rows
+  // optional device-side point count: only the columns below it, rounded up to whole 256-point
+  // blocks, are reduced (the chunks past it exit); the columns up to that bound must hold finite
+  // values with dZ = 0 beyond the count, as the forward and backward kernels leave them
+  const int32_t* cols_dev;
 };
+
+// Columns of a weight-gradient launch: a.cols, or fewer when a device-side count is given.
+__device__ __forceinline__ int dw_cols(const DwArgs& a) {
+  return a.cols_dev ? min(a.cols, (*a.cols_dev + 255) / 256 * 256) : a.cols;
+}
 
 // One block of 8 waves reduces a 256 x 256 tile of dW = dZ X^T (a whole layer of the MLP) over a
 // chunk of points, so every dZ and X row of the chunk is read from HBM once; the reduction over
@@ -1281,7 +1290,8 @@ __global__ __launch_bounds__(64 * kDwWaves) void mlp_dw_kernel(DwArgs a) {
   const int tile = local % nblk, chunk = local / nblk;
   const int m0 = kTM * (tile % jb.mblocks), n0 = kTN * (tile / jb.mblocks);
   const int pb = chunk * a.chunk;
-  const int pe = min(pb + a.chunk, a.cols);
+  const int pe = min(pb + a.chunk, dw_cols(a));
+  if (pb >= pe) return;  // the chunk lies past the device-side count (uniform over the block)
   const size_t ldP = a.ldP;
   constexpr bool F32L = f32_layout<P>();
   constexpr int ELEM = F32L ? 4 : 2;
@@ -1462,7 +1472,8 @@ __global__ __launch_bounds__(64 * kDwWaves) void mlp_dw_split_kernel(DwArgs a) {
   const int tile = local % nblk, chunk = local / nblk;
   const int m0 = kDwTile * (tile % jb.mblocks), n0 = kDwTile * (tile / jb.mblocks);
   const int pb = chunk * a.chunk;
-  const int pe = min(pb + a.chunk, a.cols);
+  const int pe = min(pb + a.chunk, dw_cols(a));
+  if (pb >= pe) return;  // the chunk lies past the device-side count (uniform over the block)
   const size_t ldP = a.ldP;
   constexpr int KPTS = 16;
   const int qm = 4 * (w >> 1), qn = 8 * (w & 1);  // first 16-row / 16-column tile of the wave

@@ -441,10 +441,12 @@ __global__ __launch_bounds__(256) void density_bwd_kernel(VmArgs a, int64_t n, i
 // Appearance features of the listed samples: out[j, c] = plane_c * line_c for the sample
 // idx[j] (or j when idx is null), components concatenated over the planes.
 template <int G>
-__global__ __launch_bounds__(256) void appearance_fwd_kernel(VmArgs a, int64_t n, const int32_t* __restrict__ idx,
+__global__ __launch_bounds__(256) void appearance_fwd_kernel(VmArgs a, int64_t n, const int32_t* __restrict__ n_dev,
+                                                             const int32_t* __restrict__ idx,
                                                              const float* __restrict__ pts,
                                                              float* __restrict__ out) {
   const GroupPos gp = group_pos<G>();
+  if (n_dev) n = min(n, (int64_t)*n_dev);  // device-side row count (n is the capacity)
   for (int64_t j = gp.first; j < n; j += gp.stride) {
     const int64_t i = idx ? (int64_t)idx[j] : j;
     float pn[3];
@@ -460,10 +462,12 @@ __global__ __launch_bounds__(256) void appearance_fwd_kernel(VmArgs a, int64_t n
 }
 
 template <int G>
-__global__ __launch_bounds__(256) void appearance_bwd_kernel(VmArgs a, int64_t n, int run_len, const int32_t* __restrict__ idx,
+__global__ __launch_bounds__(256) void appearance_bwd_kernel(VmArgs a, int64_t n, const int32_t* __restrict__ n_dev,
+                                                             int run_len, const int32_t* __restrict__ idx,
                                                              const float* __restrict__ pts,
                                                              const float* __restrict__ d_out) {
   const GroupPos gp = group_pos<G>();
+  if (n_dev) n = min(n, (int64_t)*n_dev);  // device-side row count (n is the capacity)
   const int64_t n_runs = (n + run_len - 1) / run_len;
   for (int64_t run = gp.first; run < n_runs; run += gp.stride) {
     const int64_t j0 = run * run_len;
@@ -626,10 +630,12 @@ __global__ __launch_bounds__(256) void cp_density_bwd_kernel(CpArgs a, int64_t n
 }
 
 template <int G>
-__global__ __launch_bounds__(256) void cp_appearance_fwd_kernel(CpArgs a, int64_t n, const int32_t* __restrict__ idx,
+__global__ __launch_bounds__(256) void cp_appearance_fwd_kernel(CpArgs a, int64_t n, const int32_t* __restrict__ n_dev,
+                                                                const int32_t* __restrict__ idx,
                                                                 const float* __restrict__ pts,
                                                                 float* __restrict__ out) {
   const GroupPos gp = group_pos<G>();
+  if (n_dev) n = min(n, (int64_t)*n_dev);  // device-side row count (n is the capacity)
   for (int64_t j = gp.first; j < n; j += gp.stride) {
     const int64_t i = idx ? (int64_t)idx[j] : j;
     float pn[3];
@@ -645,10 +651,12 @@ __global__ __launch_bounds__(256) void cp_appearance_fwd_kernel(CpArgs a, int64_
 }
 
 template <int G>
-__global__ __launch_bounds__(256) void cp_appearance_bwd_kernel(CpArgs a, int64_t n, int run_len, const int32_t* __restrict__ idx,
+__global__ __launch_bounds__(256) void cp_appearance_bwd_kernel(CpArgs a, int64_t n, const int32_t* __restrict__ n_dev,
+                                                                int run_len, const int32_t* __restrict__ idx,
                                                                 const float* __restrict__ pts,
                                                                 const float* __restrict__ d_out) {
   const GroupPos gp = group_pos<G>();
+  if (n_dev) n = min(n, (int64_t)*n_dev);  // device-side row count (n is the capacity)
   const int64_t n_runs = (n + run_len - 1) / run_len;
   for (int64_t run = gp.first; run < n_runs; run += gp.stride) {
     const int64_t j0 = run * run_len;
@@ -911,8 +919,8 @@ extern "C" int srf_vm_density_bwd(const srf_vm_factors* factors, float* const* d
 }
 
 extern "C" int srf_vm_appearance_fwd(const srf_vm_factors* factors, const float* bbox, int64_t n_rows,
-                                     const int32_t* idx, const float* pts, float* features,
-                                     void* stream) {
+                                     const int32_t* row_count, const int32_t* idx, const float* pts,
+                                     float* features, void* stream) {
   if (n_rows < 0) return SRF_EINVAL;
   if (n_rows == 0) return SRF_OK;
   VmArgs a;
@@ -921,14 +929,14 @@ extern "C" int srf_vm_appearance_fwd(const srf_vm_factors* factors, const float*
   if (!pts || !features) return SRF_EINVAL;
   const int G = next_pow2(a.n_quads);
   SRF_VM_DISPATCH(G, appearance_fwd_kernel, dim3(grid_for(n_rows, G)), dim3(256), 0, as_stream(stream), a,
-                  n_rows, idx, pts, features);
+                  n_rows, row_count, idx, pts, features);
   return launch_status();
 }
 
 extern "C" int srf_vm_appearance_bwd(const srf_vm_factors* factors, float* const* d_planes,
                                      float* const* d_lines, const float* bbox, int64_t n_rows,
-                                     const int32_t* idx, const float* pts, const float* d_features,
-                                     void* stream) {
+                                     const int32_t* row_count, const int32_t* idx, const float* pts,
+                                     const float* d_features, void* stream) {
   if (n_rows < 0) return SRF_EINVAL;
   if (n_rows == 0) return SRF_OK;
   VmArgs a;
@@ -943,7 +951,7 @@ extern "C" int srf_vm_appearance_bwd(const srf_vm_factors* factors, float* const
   const int G = next_pow2(a.n_quads);
   const int run = run_length();
   SRF_VM_DISPATCH(G, appearance_bwd_kernel, dim3(grid_for((n_rows + run - 1) / run, G)), dim3(256), 0,
-                  as_stream(stream), a, n_rows, run, idx, pts, d_features);
+                  as_stream(stream), a, n_rows, row_count, run, idx, pts, d_features);
   return launch_status();
 }
 
@@ -983,7 +991,8 @@ extern "C" int srf_cp_density_bwd(const srf_cp_factors* factors, float* const* d
 }
 
 extern "C" int srf_cp_appearance_fwd(const srf_cp_factors* factors, const float* bbox, int64_t n_rows,
-                                     const int32_t* idx, const float* pts, float* features, void* stream) {
+                                     const int32_t* row_count, const int32_t* idx, const float* pts,
+                                     float* features, void* stream) {
   if (n_rows < 0) return SRF_EINVAL;
   if (n_rows == 0) return SRF_OK;
   CpArgs a;
@@ -992,13 +1001,13 @@ extern "C" int srf_cp_appearance_fwd(const srf_cp_factors* factors, const float*
   if (!pts || !features || (reinterpret_cast<uintptr_t>(features) & 15)) return SRF_EINVAL;
   const int G = next_pow2(a.n_quads);
   SRF_VM_DISPATCH(G, cp_appearance_fwd_kernel, dim3(grid_for(n_rows, G)), dim3(256), 0, as_stream(stream), a,
-                  n_rows, idx, pts, features);
+                  n_rows, row_count, idx, pts, features);
   return launch_status();
 }
 
 extern "C" int srf_cp_appearance_bwd(const srf_cp_factors* factors, float* const* d_lines, const float* bbox,
-                                     int64_t n_rows, const int32_t* idx, const float* pts,
-                                     const float* d_features, void* stream) {
+                                     int64_t n_rows, const int32_t* row_count, const int32_t* idx,
+                                     const float* pts, const float* d_features, void* stream) {
   if (n_rows < 0) return SRF_EINVAL;
   if (n_rows == 0) return SRF_OK;
   CpArgs a;
@@ -1009,7 +1018,7 @@ extern "C" int srf_cp_appearance_bwd(const srf_cp_factors* factors, float* const
   const int G = next_pow2(a.n_quads);
   const int run = run_length();
   SRF_VM_DISPATCH(G, cp_appearance_bwd_kernel, dim3(grid_for((n_rows + run - 1) / run, G)), dim3(256), 0,
-                  as_stream(stream), a, n_rows, run, idx, pts, d_features);
+                  as_stream(stream), a, n_rows, row_count, run, idx, pts, d_features);
   return launch_status();
 }
 

@@ -518,8 +518,9 @@ class LowRankTensor(torch.nn.Module):
         """sigma [N, S] of every sample (0 where invalid)."""
         raise NotImplementedError
 
-    def _appearance(self, pts, bbox, idx, n_rows):
-        """Appearance features [n_rows, n_features] of the samples idx[:n_rows]."""
+    def _appearance(self, pts, bbox, idx, n_rows, count=None):
+        """Appearance features [n_rows, n_features] of the samples idx[:n_rows] (idx[:count] with a
+        device-side count, n_rows then being the capacity)."""
         raise NotImplementedError
 
     def _apply(self, fn, recurse=True):
@@ -643,27 +644,15 @@ class LowRankTensor(torch.nn.Module):
         sigma = self._density(pts, hp['bbox'], alpha, float(self.tensor_configs.get('density_offset', 0.0)))
         out = self.get_volume_rendering_weights(input_dict, sigma)
         weights = out['weights']
-        idx, count = tensorf.threshold_compact(weights, self.tensor_configs['ray_marching_weight_threshold'])
-        m = int(count.item())  # the reference's boolean indexing synchronises here too (:1249)
-        rgb_flat = torch.zeros((n_rays * n_samples, 3), dtype=torch.float32, device=pts.device)
-        if m > 0:
-            sidx = idx[:m]
-            app = self._appearance(pts, hp['bbox'], sidx, m)
-            vd, per_ray = None, n_samples
-            if self.tensor_configs['view_dependent_color']:
-                vd = input_dict['view_dirs']
-                if vd.ndim != 2:          # per-sample directions [N, S, 3]
-                    vd, per_ray = vd.reshape(-1, 3), 1
-                vd = vd.contiguous()
-            mlp = self.color_predictor.mlp
-            color = hip_color.color_forward(self.color_spec, app, self.basis_matrix_color.weight,
-                                            (mlp[0].weight, mlp[0].bias, mlp[2].weight, mlp[2].bias, mlp[4].weight,
-                                             mlp[4].bias), view_dirs=vd, idx=sidx, n_samples=per_ray)
-            rgb_flat = rgb_flat.index_put((sidx.long(),), color)
+        vd, per_ray = None, n_samples
+        if self.tensor_configs['view_dependent_color']:
+            vd = input_dict['view_dirs']
+            if vd.ndim != 2:          # per-sample directions [N, S, 3]
+                vd, per_ray = vd.reshape(-1, 3), 1
+            vd = vd.contiguous()
+        rgb_flat = self._surface_colours(pts, hp['bbox'], weights, vd, per_ray)
         rgb = rgb_flat.view(n_rays, n_samples, 3)
-        rgb_map = compositing.volume_render(rgb, weights)
-        if self.configs['model']['white_bkgd'] or (self.training and torch.rand((1,)) < 0.5):
-            rgb_map = rgb_map + (1 - out['acc'][..., None])
+        rgb_map = self._background(compositing.volume_render(rgb, weights), out['acc'])
         out['rgb'] = rgb_map
         if retraw:
             out['raw_sigma'] = sigma[..., None]
@@ -688,30 +677,46 @@ class LowRankTensor(torch.nn.Module):
                                     max_samples=int(input_dict['max_samples']))
         out.pop('rgb')
         weights = out['weights']
-        idx, count = tensorf.threshold_compact(weights, self.tensor_configs['ray_marching_weight_threshold'])
-        m = int(count.item())  # the reference's boolean indexing synchronises here too (:1249)
-        rgb_flat = torch.zeros((n_pts, 3), dtype=torch.float32, device=pts.device)
-        if m > 0:
-            sidx = idx[:m]
-            app = self._appearance(pts, hp['bbox'], sidx, m)
-            vd = None
-            if self.tensor_configs['view_dependent_color']:
-                counts = (off[1:] - off[:-1]).long()
-                ray_of = torch.repeat_interleave(torch.arange(n_rays, device=pts.device), counts, output_size=n_pts)
-                vd = input_dict['view_dirs'][ray_of].contiguous()
-            mlp = self.color_predictor.mlp
-            color = hip_color.color_forward(self.color_spec, app, self.basis_matrix_color.weight,
-                                            (mlp[0].weight, mlp[0].bias, mlp[2].weight, mlp[2].bias, mlp[4].weight,
-                                             mlp[4].bias), view_dirs=vd, idx=sidx, n_samples=1)
-            rgb_flat = rgb_flat.index_put((sidx.long(),), color)
-        rgb_map = compositing.volume_render(rgb_flat, weights, ray_offsets=off)
-        if self.configs['model']['white_bkgd'] or (self.training and torch.rand((1,)) < 0.5):
-            rgb_map = rgb_map + (1 - out['acc'][..., None])
+        vd = None
+        if self.tensor_configs['view_dependent_color']:
+            counts = (off[1:] - off[:-1]).long()
+            ray_of = torch.repeat_interleave(torch.arange(n_rays, device=pts.device), counts, output_size=n_pts)
+            vd = input_dict['view_dirs'][ray_of].contiguous()
+        rgb_flat = self._surface_colours(pts, hp['bbox'], weights, vd, 1)
+        rgb_map = self._background(compositing.volume_render(rgb_flat, weights, ray_offsets=off), out['acc'])
         out['rgb'] = rgb_map
         if retraw:
             out['raw_sigma'] = sigma[..., None]
             out['raw_rgb'] = rgb_flat
         return out
+
+    def _surface_colours(self, pts, bbox, weights, view_dirs, n_samples):
+        """rgb of every sample [P, 3]: the colour network's output on the samples whose weight passes
+        ray_marching_weight_threshold, 0 elsewhere (src/models/SimpleTensoRF09.py:726, :1241-1272).
+        The surface list and its count stay on the device: the appearance gather and the colour
+        network run over the capacity P and stop at the count, so nothing here synchronises the
+        host (the reference's pts[validity_mask] and validity_mask.any() at :1247-1250 do) and the
+        step can be graph-captured."""
+        n_all = weights.numel()
+        idx, count = tensorf.threshold_compact(weights, self.tensor_configs['ray_marching_weight_threshold'])
+        app = self._appearance(pts, bbox, idx, n_all, count)
+        mlp = self.color_predictor.mlp
+        return hip_color.color_forward(self.color_spec, app, self.basis_matrix_color.weight,
+                                       (mlp[0].weight, mlp[0].bias, mlp[2].weight, mlp[2].bias, mlp[4].weight,
+                                        mlp[4].bias), view_dirs=view_dirs, idx=idx, n_samples=n_samples, count=count,
+                                       dense_rows=n_all)
+
+    def _background(self, rgb_map, acc):
+        """White background (src/models/SimpleTensoRF09.py:746-747): always with white_bkgd, else on a coin
+        flip per training call.  The reference flips the coin on the host (torch.rand((1,)) < 0.5); here it is drawn on the
+        device and applied arithmetically (rgb + coin (1 - acc), exact for coin 0 and 1), so that the
+        step has no host-side branch and replays correctly from a captured graph."""
+        if self.configs['model']['white_bkgd']:
+            return rgb_map + (1 - acc[..., None])
+        if self.training:
+            coin = (torch.rand((1,), device=rgb_map.device) < 0.5).to(rgb_map.dtype)
+            return rgb_map + coin * (1 - acc[..., None])
+        return rgb_map
 
     def get_volume_rendering_weights(self, input_dict: dict, sigma: torch.Tensor):
         """src/models/SimpleTensoRF09.py:767-811 on srf_composite_fwd (colours composited later)."""
@@ -804,9 +809,9 @@ class VmDecomposedTensor(LowRankTensor):
                                       alpha=alpha, predictor=self.density_kind, offset=offset)
         return sigma
 
-    def _appearance(self, pts, bbox, idx, n_rows):
+    def _appearance(self, pts, bbox, idx, n_rows, count=None):
         return tensorf.vm_appearance(pts, list(self.matrices_color), list(self.vectors_color), bbox, idx=idx,
-                                     n_rows=n_rows)
+                                     n_rows=n_rows, count=count)
 
 
 class CpDecomposedTensor(LowRankTensor):
@@ -873,5 +878,5 @@ class CpDecomposedTensor(LowRankTensor):
                                       offset=offset)
         return sigma
 
-    def _appearance(self, pts, bbox, idx, n_rows):
-        return tensorf.cp_appearance(pts, list(self.vectors_color), bbox, idx=idx, n_rows=n_rows)
+    def _appearance(self, pts, bbox, idx, n_rows, count=None):
+        return tensorf.cp_appearance(pts, list(self.vectors_color), bbox, idx=idx, n_rows=n_rows, count=count)

@@ -11,7 +11,7 @@ from pathlib import Path
 import torch
 
 LIB_PATH = Path(__file__).resolve().parent / 'libsrf_hip.so'
-ABI_VERSION = 1
+ABI_VERSION = 2
 
 _P = C.c_void_p
 _I = C.c_int
@@ -63,12 +63,12 @@ _PROTOS = {
     'srf_mlp_weight_grads': [_P, _I, _P, _P, _P, _P],
     'srf_vm_density_fwd': [_P, _P, _P, _L, _P, _I, _F, _P, _P, _P, _P],
     'srf_vm_density_bwd': [_P, _P, _P, _P, _L, _P, _P, _P, _I, _F, _P, _P],
-    'srf_vm_appearance_fwd': [_P, _P, _L, _P, _P, _P, _P],
-    'srf_vm_appearance_bwd': [_P, _P, _P, _P, _L, _P, _P, _P, _P],
+    'srf_vm_appearance_fwd': [_P, _P, _L, _P, _P, _P, _P, _P],
+    'srf_vm_appearance_bwd': [_P, _P, _P, _P, _L, _P, _P, _P, _P, _P],
     'srf_cp_density_fwd': [_P, _P, _P, _L, _P, _I, _F, _P, _P, _P, _P],
     'srf_cp_density_bwd': [_P, _P, _P, _L, _P, _P, _P, _I, _F, _P, _P],
-    'srf_cp_appearance_fwd': [_P, _P, _L, _P, _P, _P, _P],
-    'srf_cp_appearance_bwd': [_P, _P, _P, _L, _P, _P, _P, _P],
+    'srf_cp_appearance_fwd': [_P, _P, _L, _P, _P, _P, _P, _P],
+    'srf_cp_appearance_bwd': [_P, _P, _P, _L, _P, _P, _P, _P, _P],
     'srf_alpha_mask_sample': [_P, _L, _P, _P, _P],
     'srf_compact_workspace_bytes': [_L],
     'srf_threshold_compact': [_L, _P, _F, _P, _P, _P, _P],
@@ -79,8 +79,8 @@ _PROTOS = {
     'srf_color_packed_bytes': [_P],
     'srf_color_workspace_bytes': [_P, _I],
     'srf_color_pack': [_P, _P, _P, _P, _P, _P, _P, _P, _P, _P],
-    'srf_color_forward': [_P, _P, _I, _P, _P, _I, _P, _P, _P, _P],
-    'srf_color_backward': [_P, _P, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P],
+    'srf_color_forward': [_P, _P, _I, _P, _P, _P, _I, _P, _P, _I, _P, _P],
+    'srf_color_backward': [_P, _P, _I, _P, _P, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P],
     'srf_mse_terms_fwd': [_I, _P, _P, _P, _P, _I, _I, _P, _P, _P, _P, _P],
     'srf_mse_terms_bwd': [_I, _P, _P, _P, _P, _I, _I, _P, _P, _P, _P],
     'srf_depth_consistency_fwd': [_I, _P, _P, _P, _P, _P, _P, _I, _P, _P, _P, _I, _I, _I, _I, _I, _F, _I,

@@ -49,7 +49,7 @@ def pack(spec: ColorSpec, basis, w1, b1, w2, b2, w3, b3) -> torch.Tensor:
 
 class _ColorFunction(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, spec, train, app, idx, n_samples, view_dirs, basis, w1, b1, w2, b2, w3, b3):
+    def forward(ctx, spec, train, app, idx, count, dense_rows, n_samples, view_dirs, basis, w1, b1, w2, b2, w3, b3):
         d = spec.desc()
         n = app.shape[0]
         dev = app.device
@@ -57,34 +57,46 @@ class _ColorFunction(torch.autograd.Function):
         ws = None
         if train:
             ws = torch.empty((_lib.lib().srf_color_workspace_bytes(C.byref(d), n),), dtype=torch.uint8, device=dev)
-        rgb = torch.empty((n, 3), dtype=torch.float32, device=dev)
-        _lib.call('srf_color_forward', C.byref(d), _lib.ptr(packed), n, _lib.ptr(app), _lib.ptr(idx), int(n_samples),
-                  _lib.ptr(view_dirs), _lib.ptr(rgb), _lib.ptr(ws), _lib.stream_ptr(dev))
+        at_idx = dense_rows is not None
+        if at_idx:
+            rgb = torch.zeros((dense_rows, 3), dtype=torch.float32, device=dev)
+        else:
+            rgb = torch.empty((n, 3), dtype=torch.float32, device=dev)
+        _lib.call('srf_color_forward', C.byref(d), _lib.ptr(packed), n, _lib.ptr(count), _lib.ptr(app), _lib.ptr(idx),
+                  int(n_samples), _lib.ptr(view_dirs), _lib.ptr(rgb), int(at_idx), _lib.ptr(ws), _lib.stream_ptr(dev))
         ctx.spec = spec
-        ctx.save_for_backward(packed, ws, rgb, basis, w1, b1, w2, b2, w3, b3)
+        ctx.n = n
+        ctx.at_idx = at_idx
+        ctx.save_for_backward(packed, ws, rgb, idx, count, basis, w1, b1, w2, b2, w3, b3)
         return rgb
 
     @staticmethod
     def backward(ctx, g_rgb):
-        packed, ws, rgb, basis, w1, b1, w2, b2, w3, b3 = ctx.saved_tensors
+        packed, ws, rgb, idx, count, basis, w1, b1, w2, b2, w3, b3 = ctx.saved_tensors
         if ws is None:
             raise _lib.SrfError('colour backward needs a training forward (workspace)')
         spec = ctx.spec
         d = spec.desc()
-        n = rgb.shape[0]
+        n = ctx.n
         d_app = torch.empty((n, spec.n_features), dtype=torch.float32, device=rgb.device)
         grads = zero_grads([t.shape for t in (basis, w1, b1, w2, b2, w3, b3)], basis.device)
-        _lib.call('srf_color_backward', C.byref(d), _lib.ptr(packed), n, _lib.ptr(rgb), _lib.ptr(g_rgb.contiguous()),
-                  _lib.ptr(ws), _lib.ptr(d_app), *[_lib.ptr(g) for g in grads], _lib.stream_ptr(rgb.device))
-        return (None, None, d_app, None, None, None, *grads)
+        _lib.call('srf_color_backward', C.byref(d), _lib.ptr(packed), n, _lib.ptr(count), _lib.ptr(idx),
+                  int(ctx.at_idx), _lib.ptr(rgb), _lib.ptr(g_rgb.contiguous()), _lib.ptr(ws), _lib.ptr(d_app),
+                  *[_lib.ptr(g) for g in grads], _lib.stream_ptr(rgb.device))
+        return (None, None, d_app, None, None, None, None, None, *grads)
 
 
 def color_forward(spec: ColorSpec, app: torch.Tensor, basis, mlp_params, *, view_dirs: torch.Tensor | None = None,
-                  idx: torch.Tensor | None = None, n_samples: int = 1) -> torch.Tensor:
+                  idx: torch.Tensor | None = None, n_samples: int = 1, count: torch.Tensor | None = None,
+                  dense_rows: int | None = None) -> torch.Tensor:
     """rgb [M,3] = sigmoid(MLP([PE(basis app), PE(dir)])) for appearance rows app [M, n_features].
 
     ``mlp_params``: (w1, b1, w2, b2, w3, b3) of color_predictor.mlp.{0,2,4}.  The view direction of
-    row j is view_dirs[(idx[j] if idx is not None else j) // n_samples] (view_dirs [rays, 3])."""
+    row j is view_dirs[(idx[j] if idx is not None else j) // n_samples] (view_dirs [rays, 3]).
+    ``count`` (device int32 [1]): only the rows below it are computed (M is then a capacity), with no
+    host synchronisation.  ``dense_rows``: return the dense sample buffer [dense_rows, 3] instead,
+    row idx[j] holding row j's colour and the other rows 0 (the reference's rgb = zeros(...),
+    rgb[validity_mask] = surface_rgb, SimpleTensoRF09.py:1246, :1271)."""
     app = dev_tensor(app, 'app')
     if app.dim() != 2 or app.shape[1] != spec.n_features:
         raise _lib.SrfError(f'app must be [M, {spec.n_features}], got {tuple(app.shape)}')
@@ -92,6 +104,10 @@ def color_forward(spec: ColorSpec, app: torch.Tensor, basis, mlp_params, *, view
         app = app.clone()
     if idx is not None:
         idx = dev_tensor(idx, 'idx', dtype=torch.int32)
+    if count is not None:
+        count = dev_tensor(count, 'count', (1,), dtype=torch.int32)
+    if dense_rows is not None and idx is None:
+        raise _lib.SrfError('dense_rows needs idx')
     if spec.use_view_dirs:
         view_dirs = dev_tensor(view_dirs, 'view_dirs')
     else:
@@ -99,4 +115,5 @@ def color_forward(spec: ColorSpec, app: torch.Tensor, basis, mlp_params, *, view
     w1, b1, w2, b2, w3, b3 = mlp_params
     params = (basis, w1, b1, w2, b2, w3, b3)
     train = torch.is_grad_enabled() and (app.requires_grad or any(p.requires_grad for p in params))
-    return _ColorFunction.apply(spec, train, app, idx, int(n_samples), view_dirs, *params)
+    return _ColorFunction.apply(spec, train, app, idx, count, None if dense_rows is None else int(dense_rows),
+                                int(n_samples), view_dirs, *params)

@@ -184,46 +184,56 @@ def vm_density(pts: torch.Tensor, planes, lines, bbox, *, alpha: AlphaMask | Non
 
 class _VmAppearance(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, pts, idx, n_rows, bbox, *factors):
+    def forward(ctx, pts, idx, n_rows, count, bbox, *factors):
         planes, lines = factors[:3], factors[3:]
         f, keep = _factors_struct(planes, lines)
         total = sum(f.n_comp)
         out = torch.empty((n_rows, total), dtype=torch.float32, device=pts.device)
-        _lib.call('srf_vm_appearance_fwd', C.byref(f), _bbox_arg(bbox), n_rows, _lib.ptr(idx), _lib.ptr(pts),
-                  _lib.ptr(out), _lib.stream_ptr(pts.device))
-        ctx.save_for_backward(pts, idx, *factors)
+        _lib.call('srf_vm_appearance_fwd', C.byref(f), _bbox_arg(bbox), n_rows, _lib.ptr(count), _lib.ptr(idx),
+                  _lib.ptr(pts), _lib.ptr(out), _lib.stream_ptr(pts.device))
+        ctx.save_for_backward(pts, idx, count, *factors)
         ctx.cfg = (n_rows, bbox)
         return out
 
     @staticmethod
     def backward(ctx, g_out):
-        pts, idx, *factors = ctx.saved_tensors
+        pts, idx, count, *factors = ctx.saved_tensors
         n_rows, bbox = ctx.cfg
-        needs = ctx.needs_input_grad[4:]
+        needs = ctx.needs_input_grad[5:]
         if not any(needs):
-            return (None,) * 10
+            return (None,) * 11
         planes, lines = factors[:3], factors[3:]
         f, keep = _factors_struct(planes, lines)
         dp, dl, grads = _grad_buffers(planes, lines, needs)
         g = g_out.contiguous()
         if g.data_ptr() % 16:   # rows are read as float4
             g = g.clone()
-        _lib.call('srf_vm_appearance_bwd', C.byref(f), dp, dl, _bbox_arg(bbox), n_rows, _lib.ptr(idx),
-                  _lib.ptr(pts), _lib.ptr(g), _lib.stream_ptr(pts.device))
-        return (None, None, None, None, *grads)
+        _lib.call('srf_vm_appearance_bwd', C.byref(f), dp, dl, _bbox_arg(bbox), n_rows, _lib.ptr(count),
+                  _lib.ptr(idx), _lib.ptr(pts), _lib.ptr(g), _lib.stream_ptr(pts.device))
+        return (None, None, None, None, None, *grads)
 
 
-def vm_appearance(pts: torch.Tensor, planes, lines, bbox, idx: torch.Tensor | None = None, n_rows: int | None = None):
-    """Appearance features cat_p(plane_p * line_p) [n_rows, sum c] of the samples idx[:n_rows]
-    of pts [..., 3] (all samples when idx is None): the gather of VmDecomposedTensor.get_color
-    before basis_matrix_color (SimpleTensoRF09.py:1241-1263)."""
-    p = dev_tensor(pts.detach(), 'pts').reshape(-1, 3)
+def _rows_and_count(p, idx, n_rows, count):
     if idx is None:
         n_rows = p.shape[0]
     else:
         idx = dev_tensor(idx, 'idx', dtype=torch.int32)
         n_rows = idx.shape[0] if n_rows is None else int(n_rows)
-    return _VmAppearance.apply(p, idx, n_rows, tuple(float(x) for x in bbox), *planes, *lines)
+    if count is not None:
+        count = dev_tensor(count, 'count', (1,), dtype=torch.int32)
+    return idx, n_rows, count
+
+
+def vm_appearance(pts: torch.Tensor, planes, lines, bbox, idx: torch.Tensor | None = None, n_rows: int | None = None,
+                  count: torch.Tensor | None = None):
+    """Appearance features cat_p(plane_p * line_p) [n_rows, sum c] of the samples idx[:n_rows]
+    of pts [..., 3] (all samples when idx is None): the gather of VmDecomposedTensor.get_color
+    before basis_matrix_color (SimpleTensoRF09.py:1241-1263).  With ``count`` (device int32 [1], e.g.
+    threshold_compact's) only the rows below it are gathered, n_rows being the capacity; the rows
+    past it are left unwritten, and no host synchronisation is needed."""
+    p = dev_tensor(pts.detach(), 'pts').reshape(-1, 3)
+    idx, n_rows, count = _rows_and_count(p, idx, n_rows, count)
+    return _VmAppearance.apply(p, idx, n_rows, count, tuple(float(x) for x in bbox), *planes, *lines)
 
 
 def _cp_struct(lines):
@@ -299,43 +309,41 @@ def cp_density(pts: torch.Tensor, lines, bbox, *, alpha: AlphaMask | None = None
 
 class _CpAppearance(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, pts, idx, n_rows, bbox, *lines):
+    def forward(ctx, pts, idx, n_rows, count, bbox, *lines):
         f, keep = _cp_struct(lines)
         out = torch.empty((n_rows, f.n_comp), dtype=torch.float32, device=pts.device)
-        _lib.call('srf_cp_appearance_fwd', C.byref(f), _bbox_arg(bbox), n_rows, _lib.ptr(idx), _lib.ptr(pts),
-                  _lib.ptr(out), _lib.stream_ptr(pts.device))
-        ctx.save_for_backward(pts, idx, *lines)
+        _lib.call('srf_cp_appearance_fwd', C.byref(f), _bbox_arg(bbox), n_rows, _lib.ptr(count), _lib.ptr(idx),
+                  _lib.ptr(pts), _lib.ptr(out), _lib.stream_ptr(pts.device))
+        ctx.save_for_backward(pts, idx, count, *lines)
         ctx.cfg = (n_rows, bbox)
         return out
 
     @staticmethod
     def backward(ctx, g_out):
-        pts, idx, *lines = ctx.saved_tensors
+        pts, idx, count, *lines = ctx.saved_tensors
         n_rows, bbox = ctx.cfg
-        needs = ctx.needs_input_grad[4:]
+        needs = ctx.needs_input_grad[5:]
         if not any(needs):
-            return (None,) * 7
+            return (None,) * 8
         f, keep = _cp_struct(lines)
         dl, grads = _cp_grad_buffers(lines, needs)
         g = g_out.contiguous()
         if g.data_ptr() % 16:
             g = g.clone()
-        _lib.call('srf_cp_appearance_bwd', C.byref(f), dl, _bbox_arg(bbox), n_rows, _lib.ptr(idx), _lib.ptr(pts),
-                  _lib.ptr(g), _lib.stream_ptr(pts.device))
-        return (None, None, None, None, *grads)
+        _lib.call('srf_cp_appearance_bwd', C.byref(f), dl, _bbox_arg(bbox), n_rows, _lib.ptr(count), _lib.ptr(idx),
+                  _lib.ptr(pts), _lib.ptr(g), _lib.stream_ptr(pts.device))
+        return (None, None, None, None, None, *grads)
 
 
-def cp_appearance(pts: torch.Tensor, lines, bbox, idx: torch.Tensor | None = None, n_rows: int | None = None):
+def cp_appearance(pts: torch.Tensor, lines, bbox, idx: torch.Tensor | None = None, n_rows: int | None = None,
+                  count: torch.Tensor | None = None):
     """CP appearance features [n_rows, R] (per-component products of the three lines) of the
     samples idx[:n_rows] of pts [..., 3] (all samples when idx is None): the gather of
-    CpDecomposedTensor.get_color before basis_matrix_color (SimpleTensoRF09.py:1064-1078)."""
+    CpDecomposedTensor.get_color before basis_matrix_color (SimpleTensoRF09.py:1064-1078).  ``count``
+    as for vm_appearance."""
     p = dev_tensor(pts.detach(), 'pts').reshape(-1, 3)
-    if idx is None:
-        n_rows = p.shape[0]
-    else:
-        idx = dev_tensor(idx, 'idx', dtype=torch.int32)
-        n_rows = idx.shape[0] if n_rows is None else int(n_rows)
-    return _CpAppearance.apply(p, idx, n_rows, tuple(float(x) for x in bbox), *lines)
+    idx, n_rows, count = _rows_and_count(p, idx, n_rows, count)
+    return _CpAppearance.apply(p, idx, n_rows, count, tuple(float(x) for x in bbox), *lines)
 
 
 def threshold_compact(values: torch.Tensor, threshold: float):
