@@ -127,7 +127,7 @@ def bilinear_align_corners(img: torch.Tensor, x: torch.Tensor, y: torch.Tensor) 
     iy = (y + 1) / 2 * (H - 1)
     x0 = torch.floor(ix)
     y0 = torch.floor(iy)
-    out = torch.zeros((C, x.shape[0]), dtype=img.dtype)
+    out = torch.zeros((C, x.shape[0]), dtype=img.dtype, device=img.device)
     for dx in (0, 1):
         for dy in (0, 1):
             xi = x0 + dx
@@ -170,11 +170,11 @@ def aabb_z_vals(rays_o: torch.Tensor, rays_d: torch.Tensor, bounding_box: torch.
     rate_a = (bounding_box[1] - rays_o) / vec
     rate_b = (bounding_box[0] - rays_o) / vec
     t_min = torch.minimum(rate_a, rate_b).amax(-1).clamp(min=near, max=far)
-    rng = torch.arange(n_samples, dtype=rays_o.dtype)[None]
+    rng = torch.arange(n_samples, dtype=rays_o.dtype, device=rays_o.device)[None]
     if u is not None:
         rng = rng.repeat(rays_d.shape[0], 1)
         rng = rng + u.to(rays_o.dtype)[:, None]
-    st = torch.tensor(step, dtype=rays_o.dtype) * rng
+    st = torch.tensor(step, dtype=rays_o.dtype, device=rays_o.device) * rng
     return t_min[..., None] + st
 
 
@@ -211,11 +211,11 @@ def tensorf_render(model: dict, pts: torch.Tensor, z_vals_ndc: torch.Tensor, ray
         alphas = alpha_mask_sample(alpha_volume, bbox, pts[valid])
         valid[valid.clone()] &= alphas > 0
     pts_n = normalize_points(pts, bbox)
-    sigma = torch.zeros(pts.shape[:-1], dtype=pts.dtype)
+    sigma = torch.zeros(pts.shape[:-1], dtype=pts.dtype, device=pts.device)
     if valid.any():
         feat = density_features(pts_n[valid])
         sigma[valid] = density_predictor(feat, density_kind, density_offset)
-    zero_rgb = torch.zeros(*pts.shape[:-1], 3, dtype=pts.dtype)
+    zero_rgb = torch.zeros(*pts.shape[:-1], 3, dtype=pts.dtype, device=pts.device)
     if ndc:
         out = volume_rendering(sigma, zero_rgb, z_vals_ndc=z_vals_ndc, rays_o=rays_o, rays_d=rays_d,
                                rays_d_ndc=rays_d_ndc, ndc=True, distance_scale=distance_scale)
@@ -223,7 +223,7 @@ def tensorf_render(model: dict, pts: torch.Tensor, z_vals_ndc: torch.Tensor, ray
         out = volume_rendering(sigma, zero_rgb, z_vals=z_vals_ndc, rays_o=rays_o, rays_d=rays_d, ndc=False,
                                distance_scale=distance_scale)
     surface = out['weights'] > weight_threshold
-    rgb = torch.zeros(*pts.shape[:-1], 3, dtype=pts.dtype)
+    rgb = torch.zeros(*pts.shape[:-1], 3, dtype=pts.dtype, device=pts.device)
     if surface.any():
         app = appearance_features(pts_n[surface])
         feats = F.linear(app, model['basis'])
@@ -256,7 +256,7 @@ def tensorf_render_ragged(model: dict, pts: torch.Tensor, z_vals: torch.Tensor, 
         alphas = alpha_mask_sample(alpha_volume, bbox, pts[valid])
         valid[valid.clone()] &= alphas > 0
     pts_n = normalize_points(pts, bbox)
-    sigma = torch.zeros(pts.shape[:-1], dtype=pts.dtype)
+    sigma = torch.zeros(pts.shape[:-1], dtype=pts.dtype, device=pts.device)
     if valid.any():
         if cp:
             feat = cp_density_features(model['lines_density'], pts_n[valid])
@@ -265,19 +265,20 @@ def tensorf_render_ragged(model: dict, pts: torch.Tensor, z_vals: torch.Tensor, 
         sigma[valid] = density_predictor(feat, density_kind, density_offset)
     off = [int(x) for x in ray_offsets]
     n = len(off) - 1
-    counts = torch.tensor([off[r + 1] - off[r] for r in range(n)])
-    ray_of = torch.repeat_interleave(torch.arange(n), counts)
+    counts = torch.tensor([off[r + 1] - off[r] for r in range(n)], device=pts.device)
+    ray_of = torch.repeat_interleave(torch.arange(n, device=pts.device), counts)
     per = {'acc': [], 'depth': [], 'depth_var': [], 'weights': []}
     for r in range(n):
         a, b = off[r], off[r + 1]
-        o = volume_rendering(sigma[a:b][None], torch.zeros(1, b - a, 3, dtype=pts.dtype), z_vals=z_vals[a:b][None],
-                             rays_o=rays_o[r:r + 1], rays_d=rays_d[r:r + 1], ndc=False, distance_scale=distance_scale)
+        o = volume_rendering(sigma[a:b][None], torch.zeros(1, b - a, 3, dtype=pts.dtype, device=pts.device),
+                             z_vals=z_vals[a:b][None], rays_o=rays_o[r:r + 1], rays_d=rays_d[r:r + 1], ndc=False,
+                             distance_scale=distance_scale)
         for k in per:
             per[k].append(o[k][0] if k == 'weights' else o[k])
     res = {k: torch.cat(v) for k, v in per.items()}
     weights = res['weights']
     surface = weights > weight_threshold
-    rgb = torch.zeros(*pts.shape[:-1], 3, dtype=pts.dtype)
+    rgb = torch.zeros(*pts.shape[:-1], 3, dtype=pts.dtype, device=pts.device)
     if surface.any():
         if cp:
             app = cp_appearance_features(model['lines_color'], pts_n[surface])
@@ -286,7 +287,7 @@ def tensorf_render_ragged(model: dict, pts: torch.Tensor, z_vals: torch.Tensor, 
         feats = F.linear(app, model['basis'])
         rgb[surface] = color_mlp_forward(model['color_mlp'], feats, view_dirs[ray_of][surface], features_pe_degree,
                                          views_pe_degree)
-    rgb_map = torch.zeros(n, 3, dtype=pts.dtype).index_add(0, ray_of, weights[:, None] * rgb)
+    rgb_map = torch.zeros(n, 3, dtype=pts.dtype, device=pts.device).index_add(0, ray_of, weights[:, None] * rgb)
     if white_bkgd:
         rgb_map = rgb_map + (1 - res['acc'][..., None])
     res['rgb'] = rgb_map

@@ -5,8 +5,9 @@
   443 samples) on 8192 rays: a 64-ray subset of the full batch against the FP64 oracle, and the factor and
   network gradients of a 256-ray batch on the full-size grids;
 * the uneven-batch TensoRF model (512^2 planes, 24 / 64 components, ragged log-normal sample counts) on
-  32,768 rays: a 64-ray subset against the oracle rendering each ray on its own, and gradients of a 256-ray
-  ragged batch.
+  32,768 rays: a 64-ray subset against the oracle rendering each ray on its own, gradients of a 256-ray
+  ragged batch, the gradients of all 32,768 rays against the FP64 oracle's autograd (the oracle run in
+  FP64 on the GPU), and the kernels' gradients of a 64-ray batch against FP64 central finite differences.
 """
 import pytest
 import torch
@@ -28,7 +29,7 @@ def _rel(a, b):
 
 def _assert_grads(pairs, what, tol=1e-3):
     for name, a, b in pairs:
-        e = _rel(a.cpu().double(), b)
+        e = _rel(a.cpu().double(), b.cpu())
         print(f'{what} {name}: grad rel err {e:.2e}')
         assert e < tol, (name, e)
 
@@ -99,9 +100,9 @@ def _vm300(cuda):
     return model, vol
 
 
-def _vm_params(tm, grad):
-    f64 = lambda t: t.detach().cpu().double().contiguous().requires_grad_(grad)
-    return {'bbox': tm.bounding_box.detach().cpu().double(),
+def _vm_params(tm, grad, device='cpu'):
+    f64 = lambda t: t.detach().to(device).double().contiguous().requires_grad_(grad)
+    return {'bbox': tm.bounding_box.detach().to(device).double(),
             'planes_density': [f64(p) for p in tm.matrices_density], 'lines_density': [f64(p) for p in tm.vectors_density],
             'planes_color': [f64(p) for p in tm.matrices_color], 'lines_color': [f64(p) for p in tm.vectors_color],
             'basis': f64(tm.basis_matrix_color.weight),
@@ -250,3 +251,137 @@ def test_vm512_ragged_full_batch_subset_and_gradients(cuda):
         pairs += [(f'{nm}.{i}', x.grad, y.grad) for i, (x, y) in enumerate(zip(getattr(tm, nm), prm2[key]))]
     pairs += [(k, dict(tm.color_predictor.named_parameters())[k].grad, v.grad) for k, v in prm2['color_mlp'].items()]
     _assert_grads(pairs, 'vm512 ragged')
+
+
+def _ragged_dev(b, device, dtype):
+    """The batch on `device`, floating-point entries in `dtype` (the offsets stay int32)."""
+    return {k: ((v.to(device=device, dtype=dtype) if v.is_floating_point() else v.to(device))
+                if torch.is_tensor(v) else v) for k, v in b.items()}
+
+
+def _grad_pairs(tm, prm):
+    pairs = []
+    for nm, key in (('matrices_density', 'planes_density'), ('vectors_density', 'lines_density'),
+                    ('matrices_color', 'planes_color'), ('vectors_color', 'lines_color')):
+        pairs += [(f'{nm}.{i}', x.grad, y.grad) for i, (x, y) in enumerate(zip(getattr(tm, nm), prm[key]))]
+    pairs.append(('basis', tm.basis_matrix_color.weight.grad, prm['basis'].grad))
+    pairs += [(k, dict(tm.color_predictor.named_parameters())[k].grad, v.grad) for k, v in prm['color_mlp'].items()]
+    return pairs
+
+
+def _density_kink(prm, pts):
+    """Samples whose density feature sits at the ReLU kink (|feature| < 1e-6), as a mask over pts."""
+    valid = ot.bbox_validity(pts, prm['bbox'])
+    feat = torch.zeros(pts.shape[0], dtype=pts.dtype, device=pts.device)
+    feat[valid] = ot.vm_density_features([p.detach() for p in prm['planes_density']],
+                                         [p.detach() for p in prm['lines_density']],
+                                         ot.normalize_points(pts[valid], prm['bbox']))
+    return valid & (feat.abs() < 1e-6)
+
+
+def test_vm512_ragged_all_rays_against_fp64_autograd(cuda):
+    """The 32,768-ray uneven batch, forward and hand-written backward on the kernels, against the
+    oracle's autograd in FP64 (run on the GPU for speed): rgb of every stable ray within 1e-5 and the
+    gradients of all factors and networks within 1e-3 relative."""
+    tm = _vm512(cuda)
+    b = _ragged(32768, float(tm.step_size), 21)
+    out = tm.forward_ragged(_ragged_dev(b, cuda, torch.float32), retraw=True)
+    prm = _vm_params(tm, True, device=cuda)
+    sub = _ragged_dev(b, cuda, torch.float64)
+    ref = ot.tensorf_render_ragged(prm, sub['pts'], sub['z_vals'], b['ray_offsets'], sub['rays_o'], sub['rays_d'],
+                                   sub['view_dirs'], distance_scale=25.0, weight_threshold=THR, white_bkgd=True,
+                                   features_pe_degree=2)
+    counts = (b['ray_offsets'][1:] - b['ray_offsets'][:-1]).long()
+    ray_of = torch.arange(32768).repeat_interleave(counts)
+    # Besides the samples near the thresholds, leave out the rays where the kernels' FP32 arithmetic and
+    # FP64 take a decision differently: the surface test (where alpha = 1 - exp(-sigma delta) is tiny its
+    # FP32 rounding is relative to sigma delta, far wider than the band) and the density ReLU (a density
+    # feature is a sum of 3 x 24 products with cancellation; its FP32 value can be a few 1e-6 off, beyond
+    # the 1e-6 kink band).  The reference's own FP32 arithmetic flips the same samples.
+    w64 = ref['weights'].detach().cpu()
+    flips = (out['weights'].detach().cpu() > THR) != (w64 > THR)
+    relu_flips = (out['raw_sigma'].detach().cpu().flatten() > 0) != (ref['raw_sigma'].detach().cpu().flatten() > 0)
+    bad = _unstable_rays(w64, ray_of, 32768, feat=_density_kink(prm, sub['pts']).cpu(),
+                         valid_mismatch=flips | relu_flips)
+    print('vm512 all rays: rays left out:', int(bad.sum()), 'with a flipped surface decision:',
+          int(torch.unique(ray_of[flips]).numel()), 'with a flipped density ReLU:',
+          int(torch.unique(ray_of[relu_flips]).numel()))
+    assert bad.sum() <= 32768 // 50
+    keep = ~bad
+    # rgb against FP64, ray by ray: within 1e-5, or as close as the reference's own FP32 arithmetic
+    # (the oracle in FP32) gets on that ray; the transmittance of a 512-sample ray accumulates ~512
+    # FP32 roundings
+    with torch.no_grad():
+        p32 = {k: ([t.float() for t in v] if isinstance(v, list) else
+                   ({n: t.float() for n, t in v.items()} if isinstance(v, dict) else v.float()))
+               for k, v in prm.items()}
+        s32 = _ragged_dev(b, cuda, torch.float32)
+        ref32 = ot.tensorf_render_ragged(p32, s32['pts'], s32['z_vals'], b['ray_offsets'], s32['rays_o'],
+                                         s32['rays_d'], s32['view_dirs'], distance_scale=25.0, weight_threshold=THR,
+                                         white_bkgd=True, features_pe_degree=2)
+    r64 = ref['rgb'].detach().cpu()
+    ek = (out['rgb'].detach().cpu().double() - r64).abs().amax(-1)[keep]
+    eo = (ref32['rgb'].cpu().double() - r64).abs().amax(-1)[keep]
+    worst = ek.argmax()
+    print(f'vm512 all rays: rgb max abs err {ek.max().item():.2e} (FP32 oracle {eo.max().item():.2e}); worst ray '
+          f'{ek[worst].item():.2e} against the FP32 oracle\'s {eo[worst].item():.2e} there; '
+          f'{int((ek > 1e-5).sum())} rays above 1e-5')
+    assert bool((ek <= torch.clamp(2 * eo, min=1e-5)).all())
+    gr = torch.randn(32768, 3, generator=torch.Generator().manual_seed(26))
+    gr[bad] = 0
+    (out['rgb'] * gr.to(cuda)).sum().backward()
+    (ref['rgb'] * gr.to(cuda).double()).sum().backward()
+    _assert_grads(_grad_pairs(tm, prm), 'vm512 all rays')
+
+
+def test_vm512_ragged_gradients_against_fp64_finite_differences(cuda):
+    """The kernels' FP32 gradients on a 64-ray uneven batch against FP64 central differences of the
+    oracle's loss: for each of seven parameter tensors (density and appearance planes and lines, the
+    basis, the colour network's first weight and last bias) the four entries with the largest gradient."""
+    tm = _vm512(cuda)
+    b = _ragged(64, float(tm.step_size), 25)
+    prm = _vm_params(tm, False)
+    sub = _ragged_dev(b, 'cpu', torch.float64)
+    render = lambda p: ot.tensorf_render_ragged(p, sub['pts'], sub['z_vals'], b['ray_offsets'], sub['rays_o'],
+                                                sub['rays_d'], sub['view_dirs'], distance_scale=25.0,
+                                                weight_threshold=THR, white_bkgd=True, features_pe_degree=2)
+    ref = render(prm)
+    counts = (b['ray_offsets'][1:] - b['ray_offsets'][:-1]).long()
+    bad = _unstable_rays(ref['weights'], torch.arange(64).repeat_interleave(counts), 64,
+                         feat=_density_kink(prm, sub['pts']))
+    assert bad.sum() <= 2
+    gr = torch.randn(64, 3, generator=torch.Generator().manual_seed(27), dtype=torch.float64)
+    gr[bad] = 0
+    out = tm.forward_ragged(_ragged_dev(b, cuda, torch.float32))
+    (out['rgb'] * gr.to(cuda).float()).sum().backward()
+    loss = lambda p: (render(p)['rgb'] * gr).sum().item()
+    cases = [('planes_density', 0, tm.matrices_density[0]), ('lines_density', 1, tm.vectors_density[1]),
+             ('planes_color', 2, tm.matrices_color[2]), ('lines_color', 0, tm.vectors_color[0]),
+             ('basis', None, tm.basis_matrix_color.weight), ('color_mlp', 'mlp.0.weight', None),
+             ('color_mlp', 'mlp.4.bias', None)]
+    mlp = dict(tm.color_predictor.named_parameters())
+    for key, i, param in cases:
+        if key == 'color_mlp':
+            param = mlp[i]
+            target = prm['color_mlp'][i]
+        elif i is None:
+            target = prm[key]
+        else:
+            target = prm[key][i]
+        g = param.grad.detach().cpu().double().contiguous()
+        top = torch.topk(g.abs().flatten(), min(4, g.numel())).indices
+        scale = g.abs().max().item()
+        assert scale > 0, key
+        for flat in top.tolist():
+            pos = target.view(-1)
+            v0 = pos[flat].item()
+            h = 1e-6 * max(1.0, abs(v0))
+            pos[flat] = v0 + h
+            lp = loss(prm)
+            pos[flat] = v0 - h
+            lm = loss(prm)
+            pos[flat] = v0
+            fd = (lp - lm) / (2 * h)
+            e = abs(g.view(-1)[flat].item() - fd) / scale
+            print(f'fd {key}[{i}] entry {flat}: kernel {g.view(-1)[flat].item():.6e} fd {fd:.6e} rel {e:.1e}')
+            assert e < 1e-3, (key, i, flat, e)
