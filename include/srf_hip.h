@@ -278,13 +278,21 @@ int srf_vm_density_fwd(const srf_vm_factors* factors, const float* bbox,
                        int predictor, float offset, float* sigma, float* feat, uint8_t* valid,
                        void* stream);
 
+/* Bytes of the workspace of the cell-sorted factor backwards (srf_vm_density_bwd,
+ * srf_vm_appearance_bwd) for n_rows rows (samples or appearance rows, the capacity). */
+size_t srf_vm_bwd_workspace_bytes(const srf_vm_factors* factors, int64_t n_rows);
+
 /* Backward of srf_vm_density_fwd into the factor gradients d_planes[3] / d_lines[3] (same
  * texel-major layout as the factors; accumulated with FP32 atomics, so the caller zeroes them
- * first).  Coordinates carry no gradient (the reference detaches them, :1224-1228). */
+ * first).  Coordinates carry no gradient (the reference detaches them, :1224-1228) -- the
+ * scatter of grid_sample's backward.  With a workspace (srf_vm_bwd_workspace_bytes) the
+ * contributing samples are first ordered by plane cell, so that each texel is flushed about once
+ * per run of samples and the line gradients are summed in LDS; workspace NULL walks the samples in
+ * their own order (more atomics).  Both give the same sums up to the order of FP32 additions. */
 int srf_vm_density_bwd(const srf_vm_factors* factors, float* const* d_planes,
                        float* const* d_lines, const float* bbox, int64_t n_points,
                        const float* pts, const uint8_t* valid, const float* feat, int predictor,
-                       float offset, const float* d_sigma, void* stream);
+                       float offset, const float* d_sigma, void* workspace, void* stream);
 
 /* Appearance features of the listed samples: features[j, :] = cat_p(plane_p * line_p) at sample
  * idx[j] (idx NULL: sample j), [n_rows, sum n_comp] row-major.  Replaces the gather part of
@@ -298,11 +306,12 @@ int srf_vm_appearance_fwd(const srf_vm_factors* factors, const float* bbox, int6
                           float* features, void* stream);
 
 /* Backward of srf_vm_appearance_fwd: d_features [n_rows, sum n_comp] (rows below the count)
- * scattered into d_planes / d_lines with FP32 atomics (accumulating). */
+ * scattered into d_planes / d_lines with FP32 atomics (accumulating); workspace as for
+ * srf_vm_density_bwd. */
 int srf_vm_appearance_bwd(const srf_vm_factors* factors, float* const* d_planes,
                           float* const* d_lines, const float* bbox, int64_t n_rows,
                           const int32_t* row_count, const int32_t* idx, const float* pts,
-                          const float* d_features, void* stream);
+                          const float* d_features, void* workspace, void* stream);
 
 /* The CANDECOMP/PARAFAC factors of one CpDecomposedTensor field (src/models/SimpleTensoRF09.py:
  * 964-1127): three lines of the same rank n_comp, line i being the reference's vectors_*[i] of

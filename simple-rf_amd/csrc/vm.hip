@@ -495,6 +495,282 @@ __global__ __launch_bounds__(256) void appearance_bwd_kernel(VmArgs a, int64_t n
   }
 }
 
+// ------------------------------------------------ cell-sorted backward (privatised scatter)
+// The run walk above flushes a texel whenever a ray's consecutive samples leave its cell.  A plane
+// texel is touched by many rays, so the scatter still issues tens of atomics per texel and
+// component.  With a caller workspace the backward instead orders the contributing rows by the
+// plane cell of their nw corner (a counting sort per plane: histogram, one scan, a scatter of row
+// numbers), so that a lane group walking the sorted rows meets each cell's samples in one run and
+// flushes each texel about once per run; the line gradients of the walk (the third coordinate of
+// the rows of one cell is arbitrary) go to a copy of the line in LDS, flushed once per block.
+// The order within a cell comes from integer atomics and varies between calls, as the order of
+// the FP32 atomics does anyway.
+struct BwdRows {
+  int64_t n;               // capacity
+  const int32_t* n_dev;    // device-side row count (appearance) or null
+  const int32_t* idx;      // sample of row j (appearance) or null
+  const float* pts;
+  const uint8_t* valid;    // density: rows are samples, valid and with a non-zero gradient
+  const float* feat;
+  int kind;
+  float offset;
+  const float* d_sigma;
+  const float* d_out;      // appearance: d_features [rows, total_comp]
+};
+
+constexpr int kMaxChunks = 1024;  // scan chunks per plane
+
+struct SortWs {
+  int32_t* hist[kPlanes];    // cells + 1: counts, then exclusive offsets
+  int32_t* cursor[kPlanes];  // cells: scatter positions
+  int32_t* order[kPlanes];   // capacity: rows in cell order
+  int32_t* total;            // [kPlanes] rows per plane
+  int32_t* partial;          // [kPlanes][kMaxChunks] chunk sums, then chunk offsets
+  int64_t chunk_cells;       // cells per scan chunk
+  int cells[kPlanes];
+  int32_t* zero_begin;       // the histograms, zeroed before each call
+  int64_t zero_count;
+};
+
+__device__ __forceinline__ int64_t bwd_rows_count(const BwdRows& r) {
+  return r.n_dev ? min(r.n, (int64_t)*r.n_dev) : r.n;
+}
+
+// Whether row j contributes, its sample i and (density) its scalar gradient.
+template <bool kDensity>
+__device__ __forceinline__ bool bwd_row(const BwdRows& r, int64_t j, int64_t& i, float& g) {
+  if (kDensity) {
+    i = j;
+    if (!r.valid[j]) return false;
+    g = __fmul_rn(r.d_sigma[j], predict_density_grad(r.feat[j], r.kind, r.offset));
+    return g != 0.f;
+  }
+  i = r.idx ? (int64_t)r.idx[j] : j;
+  g = 0.f;
+  return true;
+}
+
+__device__ __forceinline__ void normalized_point(const VmArgs& a, const float* pts, int64_t i, float pn[3]) {
+#pragma unroll
+  for (int k = 0; k < 3; ++k) pn[k] = normalize_coord(pts[3 * i + k], a.lo[k], a.size[k]);
+}
+
+// Sort key of a sample for plane p: the cell of its nw corner, clamped into the plane.
+__device__ __forceinline__ int plane_cell(const VmArgs& a, int p, const float pn[3]) {
+  const int W = sel3(p, a.pw[0], a.pw[1], a.pw[2]), H = sel3(p, a.ph[0], a.ph[1], a.ph[2]);
+  const float x = sel3(p, pn[axis_a0(0)], pn[axis_a0(1)], pn[axis_a0(2)]);
+  const float y = sel3(p, pn[axis_a1(0)], pn[axis_a1(1)], pn[axis_a1(2)]);
+  int xi = (int)floorf(source_index(x, W)), yi = (int)floorf(source_index(y, H));
+  xi = xi < 0 ? 0 : (xi > W - 1 ? W - 1 : xi);
+  yi = yi < 0 ? 0 : (yi > H - 1 ? H - 1 : yi);
+  return yi * W + xi;
+}
+
+__global__ __launch_bounds__(256) void fill_zero_kernel(int32_t* __restrict__ p, int64_t n) {
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x)
+    p[e] = 0;
+}
+
+// Runs of equal keys among the active lanes of a wave (consecutive rows of a ray mostly share a
+// cell): the run's first lane adds the run length with one atomic, instead of one per row.
+struct WaveRun {
+  bool head;      // this lane starts a run
+  int head_lane;  // first lane of this lane's run
+  int length;     // run length (meaningful on heads)
+};
+
+__device__ __forceinline__ WaveRun wave_run(int key, bool active) {
+  const int lane = threadIdx.x & (SRF_WAVE - 1);
+  const int prev_key = __shfl_up(key, 1, SRF_WAVE);
+  const int prev_active = __shfl_up(active ? 1 : 0, 1, SRF_WAVE);
+  WaveRun r;
+  r.head = active && (lane == 0 || !prev_active || prev_key != key);
+  const uint64_t heads = __ballot(r.head);
+  const uint64_t breaks = heads | ~__ballot(active);  // a run ends before the next head or inactive lane
+  const uint64_t upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
+  const uint64_t h = heads & upto;
+  r.head_lane = h ? 63 - __clzll(h) : lane;
+  const uint64_t after = lane == 63 ? 0ull : (breaks >> (lane + 1));
+  r.length = after ? __ffsll((long long)after) : 64 - lane;
+  return r;
+}
+
+// Rows are visited wave by wave (every lane of a wave runs the same iterations, so the wave-wide
+// operations of wave_run see the whole wave).
+template <bool kDensity, bool kScatter>
+__global__ __launch_bounds__(256) void cell_count_kernel(VmArgs a, BwdRows r, SortWs w) {
+  const int64_t n = bwd_rows_count(r);
+  const int lane = threadIdx.x & (SRF_WAVE - 1);
+  const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / SRF_WAVE;
+  const int64_t n_waves = (int64_t)gridDim.x * blockDim.x / SRF_WAVE;
+  for (int64_t base = wave * SRF_WAVE; base < n; base += n_waves * SRF_WAVE) {
+    const int64_t j = base + lane;
+    int64_t i = 0;
+    float g = 0.f;
+    const bool active = j < n && bwd_row<kDensity>(r, j, i, g);
+    float pn[3] = {0.f, 0.f, 0.f};
+    if (active) normalized_point(a, r.pts, i, pn);
+#pragma unroll
+    for (int p = 0; p < kPlanes; ++p) {
+      if (a.comp[p] == 0) continue;  // uniform
+      const int key = active ? plane_cell(a, p, pn) : -1;
+      const WaveRun run = wave_run(key, active);
+      if (!kScatter) {
+        if (run.head) atomicAdd(w.hist[p] + key, run.length);
+      } else {
+        int pos = 0;
+        if (run.head) pos = atomicAdd(w.cursor[p] + key, run.length);
+        const int first = __shfl(pos, run.head_lane, SRF_WAVE);
+        if (active) w.order[p][first + (lane - run.head_lane)] = (int32_t)j;
+      }
+    }
+  }
+}
+
+// Exclusive prefix of v over a block of kScanThreads threads, and the block total.
+constexpr int kScanThreads = 1024;
+
+__device__ __forceinline__ int32_t block_exclusive_scan(int32_t v, int32_t* total) {
+  __shared__ int32_t wave_tot[kScanThreads / SRF_WAVE];
+  const int lane = threadIdx.x & (SRF_WAVE - 1), wv = threadIdx.x / SRF_WAVE;
+  int32_t incl = v;
+#pragma unroll
+  for (int d = 1; d < SRF_WAVE; d <<= 1) {
+    const int32_t o = __shfl_up(incl, d, SRF_WAVE);
+    if (lane >= d) incl += o;
+  }
+  if (lane == SRF_WAVE - 1) wave_tot[wv] = incl;
+  __syncthreads();
+  int32_t before = 0, all = 0;
+#pragma unroll
+  for (int k = 0; k < kScanThreads / SRF_WAVE; ++k) {
+    const int32_t t = wave_tot[k];
+    if (k < wv) before += t;
+    all += t;
+  }
+  __syncthreads();  // wave_tot may be rewritten by the next call
+  *total = all;
+  return before + incl - v;
+}
+
+// The cell scan in three launches: per-chunk sums, a scan of the chunk sums (one block per plane,
+// which also writes the plane's row total), then each chunk scanned with its carry-in, writing the
+// exclusive offsets and the scatter cursors.  Chunks are kScanThreads-cell tiles read coalesced.
+__global__ __launch_bounds__(kScanThreads) void cell_chunk_sum_kernel(SortWs w) {
+  const int p = blockIdx.y;
+  const int64_t c0 = (int64_t)blockIdx.x * w.chunk_cells;
+  if (c0 >= w.cells[p]) return;  // uniform
+  const int64_t c1 = c0 + w.chunk_cells < w.cells[p] ? c0 + w.chunk_cells : w.cells[p];
+  int32_t v = 0;
+  for (int64_t c = c0 + threadIdx.x; c < c1; c += kScanThreads) v += w.hist[p][c];
+  int32_t total;
+  block_exclusive_scan(v, &total);
+  if (threadIdx.x == 0) w.partial[p * kMaxChunks + blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(kScanThreads) void cell_chunk_scan_kernel(SortWs w) {
+  const int p = blockIdx.x;
+  const int n_chunks = (int)((w.cells[p] + w.chunk_cells - 1) / w.chunk_cells);
+  const int t = threadIdx.x;
+  const int32_t v = t < n_chunks ? w.partial[p * kMaxChunks + t] : 0;
+  int32_t total;
+  const int32_t ex = block_exclusive_scan(v, &total);
+  if (t < n_chunks) w.partial[p * kMaxChunks + t] = ex;
+  if (t == 0) w.total[p] = total;
+}
+
+__global__ __launch_bounds__(kScanThreads) void cell_chunk_apply_kernel(SortWs w) {
+  const int p = blockIdx.y;
+  const int64_t c0 = (int64_t)blockIdx.x * w.chunk_cells;
+  if (c0 >= w.cells[p]) return;  // uniform
+  const int64_t c1 = c0 + w.chunk_cells < w.cells[p] ? c0 + w.chunk_cells : w.cells[p];
+  int32_t carry = w.partial[p * kMaxChunks + blockIdx.x];
+  for (int64_t t0 = c0; t0 < c1; t0 += kScanThreads) {  // uniform trip count
+    const int64_t c = t0 + threadIdx.x;
+    const int32_t v = c < c1 ? w.hist[p][c] : 0;
+    int32_t total;
+    const int32_t ex = block_exclusive_scan(v, &total);
+    if (c < c1) {
+      w.hist[p][c] = carry + ex;
+      w.cursor[p][c] = carry + ex;
+    }
+    carry += total;
+  }
+}
+
+constexpr int kSortedThreads = 512;
+constexpr int kMaxLineLds = 160 * 1024;  // a CU's LDS
+
+// Plane p's gradients from its cell-sorted rows: a lane group walks run_len consecutive sorted
+// rows with the corner slots of the run walk (flushing a plane texel when the walk leaves it) and
+// adds the line gradients into s_line (the line copy in LDS, line_lds != 0) or, when the line is
+// too large for LDS, through the line slots of the run walk.
+template <int G, bool kDensity>
+__global__ __launch_bounds__(kSortedThreads) void sorted_bwd_kernel(VmArgs a, BwdRows r, SortWs w, int p,
+                                                                    int run_len, int line_lds) {
+  extern __shared__ float s_line[];
+  const int C = sel3(p, a.comp[0], a.comp[1], a.comp[2]);
+  const int L = sel3(p, a.ll[0], a.ll[1], a.ll[2]);
+  const int64_t m = w.total[p];
+  const int groups = kSortedThreads / G;
+  const int64_t b0 = (int64_t)blockIdx.x * groups * run_len;
+  if (b0 >= m) return;  // uniform over the block
+  if (line_lds) {
+    for (int e = threadIdx.x; e < L * C; e += kSortedThreads) s_line[e] = 0.f;
+    __syncthreads();
+  }
+  const int grp = threadIdx.x / G, ql = threadIdx.x % G;
+  const int64_t s0 = b0 + (int64_t)grp * run_len;
+  const int64_t s1 = s0 + run_len < m ? s0 + run_len : m;
+  const int q0 = sel3(p, a.qoff[0], a.qoff[1], a.qoff[2]);
+  float* dP = sel3(p, a.dplane[0], a.dplane[1], a.dplane[2]);
+  float* dL = sel3(p, a.dline[0], a.dline[1], a.dline[2]);
+  const int32_t* order = sel3(p, w.order[0], w.order[1], w.order[2]);
+  for (int q = q0 + ql; q < q0 + C / 4; q += G) {
+    RunAcc acc;
+    run_init(acc);
+    int c = 0;
+    for (int64_t s = s0; s < s1; ++s) {
+      const int64_t j = order[s];
+      int64_t i;
+      float gs;
+      bwd_row<kDensity>(r, j, i, gs);
+      float pn[3];
+      normalized_point(a, r.pts, i, pn);
+      const float4 g = kDensity ? make_float4(gs, gs, gs, gs) : ld4(r.d_out + j * a.total_comp + 4 * q);
+      const QuadSite site = quad_site(a, q, pn);
+      float4 pv, lv;
+      quad_values(a, site, pv, lv);
+      plane_accum(acc.pidx, acc.pg, site.pt.idx, site.pt.w, mul4(g, lv), dP, C, site.c);
+      const float4 gl = mul4(g, pv);
+      if (line_lds) {
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+          if (site.lt.idx[t] < 0) continue;
+          float* d = s_line + site.lt.idx[t] * C + site.c;
+          const float wt = site.lt.w[t];
+          atomicAdd(d + 0, __fmul_rn(gl.x, wt));
+          atomicAdd(d + 1, __fmul_rn(gl.y, wt));
+          atomicAdd(d + 2, __fmul_rn(gl.z, wt));
+          atomicAdd(d + 3, __fmul_rn(gl.w, wt));
+        }
+      } else {
+        line_accum(acc.lidx, acc.lg, site.lt.idx, site.lt.w, gl, dL, C, site.c);
+      }
+      c = site.c;
+    }
+    run_flush_plane(a, p, c, acc);
+    if (!line_lds) run_flush_line(a, p, c, acc);
+  }
+  if (line_lds) {
+    __syncthreads();
+    for (int e = threadIdx.x; e < L * C; e += kSortedThreads) {
+      const float v = s_line[e];
+      if (v != 0.f) unsafeAtomicAdd(dL + e, v);
+    }
+  }
+}
+
 // ------------------------------------------------------------ CP decomposition (3 lines)
 // CpDecomposedTensor (SimpleTensoRF09.py:1043-1089): every component r is the product of the
 // three lines at pts[2], pts[1], pts[0], multiplied in that order.  The lines share one rank, so
@@ -873,6 +1149,124 @@ int run_length() {
     default: hipLaunchKernelGGL(KERNEL<64>, __VA_ARGS__); break;                          \
   }
 
+#define SRF_VM_DISPATCH_KD(G_, KD, KERNEL, ...)                                            \
+  switch (G_) {                                                                           \
+    case 1: hipLaunchKernelGGL((KERNEL<1, KD>), __VA_ARGS__); break;                      \
+    case 2: hipLaunchKernelGGL((KERNEL<2, KD>), __VA_ARGS__); break;                      \
+    case 4: hipLaunchKernelGGL((KERNEL<4, KD>), __VA_ARGS__); break;                      \
+    case 8: hipLaunchKernelGGL((KERNEL<8, KD>), __VA_ARGS__); break;                      \
+    case 16: hipLaunchKernelGGL((KERNEL<16, KD>), __VA_ARGS__); break;                    \
+    case 32: hipLaunchKernelGGL((KERNEL<32, KD>), __VA_ARGS__); break;                    \
+    default: hipLaunchKernelGGL((KERNEL<64, KD>), __VA_ARGS__); break;                    \
+  }
+
+// Workspace of the cell-sorted backward for `rows` rows (capacity), and its carving.
+size_t align256(size_t b) { return (b + 255) / 256 * 256; }
+
+size_t sorted_ws_bytes(const VmArgs& a, int64_t rows) {
+  size_t b = align256(kPlanes * sizeof(int32_t)) + align256((size_t)kPlanes * kMaxChunks * sizeof(int32_t));
+  for (int p = 0; p < kPlanes; ++p) {
+    const int64_t cells = (int64_t)a.ph[p] * a.pw[p];
+    b += align256((size_t)(cells + 1) * sizeof(int32_t)) + align256((size_t)cells * sizeof(int32_t)) +
+         align256((size_t)rows * sizeof(int32_t));
+  }
+  return b;
+}
+
+SortWs carve_sorted_ws(const VmArgs& a, int64_t rows, void* ws) {
+  SortWs w;
+  memset(&w, 0, sizeof(w));
+  uint8_t* base = reinterpret_cast<uint8_t*>(ws);
+  size_t off = 0;
+  w.total = reinterpret_cast<int32_t*>(base + off);
+  off += align256(kPlanes * sizeof(int32_t));
+  w.partial = reinterpret_cast<int32_t*>(base + off);
+  off += align256((size_t)kPlanes * kMaxChunks * sizeof(int32_t));
+  // the histograms first and contiguous, so one fill zeroes them
+  w.zero_begin = reinterpret_cast<int32_t*>(base + off);
+  for (int p = 0; p < kPlanes; ++p) {
+    w.cells[p] = a.ph[p] * a.pw[p];
+    w.hist[p] = reinterpret_cast<int32_t*>(base + off);
+    off += align256((size_t)(w.cells[p] + 1) * sizeof(int32_t));
+  }
+  w.zero_count = (int64_t)((base + off) - reinterpret_cast<uint8_t*>(w.zero_begin)) / (int64_t)sizeof(int32_t);
+  for (int p = 0; p < kPlanes; ++p) {
+    w.cursor[p] = reinterpret_cast<int32_t*>(base + off);
+    off += align256((size_t)w.cells[p] * sizeof(int32_t));
+  }
+  for (int p = 0; p < kPlanes; ++p) {
+    w.order[p] = reinterpret_cast<int32_t*>(base + off);
+    off += align256((size_t)rows * sizeof(int32_t));
+  }
+  int64_t most = 1;
+  for (int p = 0; p < kPlanes; ++p) most = w.cells[p] > most ? w.cells[p] : most;
+  const int64_t per = (most + kMaxChunks - 1) / kMaxChunks;
+  w.chunk_cells = (per + kScanThreads - 1) / kScanThreads * kScanThreads;  // whole tiles, <= kMaxChunks chunks
+  return w;
+}
+
+// The dynamic LDS of the sorted kernels (the line copy) can exceed the 64 KB default: raise the
+// limit of every instantiation once per device, before any call that could be under capture.
+template <bool kDensity>
+int raise_sorted_lds_limit() {
+  constexpr int kMaxDevices = 64;
+  static std::mutex mutex;
+  static bool raised[kMaxDevices];
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return SRF_EHIP;
+  std::lock_guard<std::mutex> lock(mutex);
+  if (raised[dev]) return SRF_OK;
+  const void* fns[] = {
+      reinterpret_cast<const void*>(&sorted_bwd_kernel<1, kDensity>),
+      reinterpret_cast<const void*>(&sorted_bwd_kernel<2, kDensity>),
+      reinterpret_cast<const void*>(&sorted_bwd_kernel<4, kDensity>),
+      reinterpret_cast<const void*>(&sorted_bwd_kernel<8, kDensity>),
+      reinterpret_cast<const void*>(&sorted_bwd_kernel<16, kDensity>),
+      reinterpret_cast<const void*>(&sorted_bwd_kernel<32, kDensity>),
+      reinterpret_cast<const void*>(&sorted_bwd_kernel<64, kDensity>)};
+  for (const void* f : fns)
+    if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, kMaxLineLds) != hipSuccess)
+      return SRF_EHIP;
+  raised[dev] = true;
+  return SRF_OK;
+}
+
+template <bool kDensity>
+int launch_sorted_bwd(const VmArgs& a, const BwdRows& r, void* ws, hipStream_t s) {
+  if (const int st = raise_sorted_lds_limit<kDensity>(); st != SRF_OK) return st;
+  const SortWs w = carve_sorted_ws(a, r.n, ws);
+  const int64_t zb = (w.zero_count + 255) / 256;
+  hipLaunchKernelGGL(fill_zero_kernel, dim3((unsigned)(zb < 2048 ? zb : 2048)), dim3(256), 0, s, w.zero_begin,
+                     w.zero_count);
+  const int64_t rb = (r.n + 255) / 256;
+  const unsigned row_blocks = (unsigned)(rb < 8192 ? rb : 8192);
+  hipLaunchKernelGGL((cell_count_kernel<kDensity, false>), dim3(row_blocks), dim3(256), 0, s, a, r, w);
+  int64_t most = 1;
+  for (int p = 0; p < kPlanes; ++p) most = w.cells[p] > most ? w.cells[p] : most;
+  const unsigned chunks = (unsigned)((most + w.chunk_cells - 1) / w.chunk_cells);
+  hipLaunchKernelGGL(cell_chunk_sum_kernel, dim3(chunks, kPlanes), dim3(kScanThreads), 0, s, w);
+  hipLaunchKernelGGL(cell_chunk_scan_kernel, dim3(kPlanes), dim3(kScanThreads), 0, s, w);
+  hipLaunchKernelGGL(cell_chunk_apply_kernel, dim3(chunks, kPlanes), dim3(kScanThreads), 0, s, w);
+  hipLaunchKernelGGL((cell_count_kernel<kDensity, true>), dim3(row_blocks), dim3(256), 0, s, a, r, w);
+  for (int p = 0; p < kPlanes; ++p) {
+    if (a.comp[p] == 0) continue;
+    const int G = next_pow2(a.comp[p] / 4);
+    const int groups = kSortedThreads / G;
+    // about 1024 blocks over the capacity (fewer run when the device count is lower), unless the
+    // gather run knob sets the walk length
+    int64_t run = (r.n + 1024LL * groups - 1) / (1024LL * groups);
+    run = run < 16 ? 16 : (run > 256 ? 256 : run);
+    if (const int64_t v = tune_value(SRF_TUNE_GATHER_RUN)) run = v;
+    const int64_t blocks = (r.n + (int64_t)groups * run - 1) / ((int64_t)groups * run);
+    const size_t line_bytes = (size_t)a.ll[p] * a.comp[p] * sizeof(float);
+    const int line_lds = line_bytes <= (size_t)kMaxLineLds ? 1 : 0;
+    SRF_VM_DISPATCH_KD(G, kDensity, sorted_bwd_kernel, dim3((unsigned)(blocks > 0 ? blocks : 1)),
+                       dim3(kSortedThreads), line_lds ? line_bytes : 0, s, a, r, w, p, (int)run, line_lds);
+  }
+  return launch_status();
+}
+
+
 }  // namespace
 }  // namespace srf
 
@@ -896,10 +1290,19 @@ extern "C" int srf_vm_density_fwd(const srf_vm_factors* factors, const float* bb
   return launch_status();
 }
 
+extern "C" size_t srf_vm_bwd_workspace_bytes(const srf_vm_factors* factors, int64_t n_rows) {
+  VmArgs a;
+  if (n_rows < 0 || !factors) return 0;
+  const float box[6] = {-1.f, -1.f, -1.f, 1.f, 1.f, 1.f};
+  if (make_vm_args(factors, box, &a, false) != SRF_OK) return 0;
+  return sorted_ws_bytes(a, n_rows);
+}
+
 extern "C" int srf_vm_density_bwd(const srf_vm_factors* factors, float* const* d_planes,
                                   float* const* d_lines, const float* bbox, int64_t n_points,
                                   const float* pts, const uint8_t* valid, const float* feat,
-                                  int predictor, float offset, const float* d_sigma, void* stream) {
+                                  int predictor, float offset, const float* d_sigma, void* workspace,
+                                  void* stream) {
   if (n_points < 0 || (predictor != SRF_DENSITY_RELU && predictor != SRF_DENSITY_SOFTPLUS)) return SRF_EINVAL;
   if (n_points == 0) return SRF_OK;
   VmArgs a;
@@ -910,6 +1313,18 @@ extern "C" int srf_vm_density_bwd(const srf_vm_factors* factors, float* const* d
     if (a.comp[p] > 0 && (!d_planes[p] || !d_lines[p])) return SRF_EINVAL;
     a.dplane[p] = d_planes[p];
     a.dline[p] = d_lines[p];
+  }
+  if (workspace) {
+    BwdRows r;
+    memset(&r, 0, sizeof(r));
+    r.n = n_points;
+    r.pts = pts;
+    r.valid = valid;
+    r.feat = feat;
+    r.kind = predictor;
+    r.offset = offset;
+    r.d_sigma = d_sigma;
+    return launch_sorted_bwd<true>(a, r, workspace, as_stream(stream));
   }
   const int G = next_pow2(a.n_quads);
   const int run = run_length();
@@ -936,7 +1351,7 @@ extern "C" int srf_vm_appearance_fwd(const srf_vm_factors* factors, const float*
 extern "C" int srf_vm_appearance_bwd(const srf_vm_factors* factors, float* const* d_planes,
                                      float* const* d_lines, const float* bbox, int64_t n_rows,
                                      const int32_t* row_count, const int32_t* idx, const float* pts,
-                                     const float* d_features, void* stream) {
+                                     const float* d_features, void* workspace, void* stream) {
   if (n_rows < 0) return SRF_EINVAL;
   if (n_rows == 0) return SRF_OK;
   VmArgs a;
@@ -947,6 +1362,16 @@ extern "C" int srf_vm_appearance_bwd(const srf_vm_factors* factors, float* const
     if (a.comp[p] > 0 && (!d_planes[p] || !d_lines[p])) return SRF_EINVAL;
     a.dplane[p] = d_planes[p];
     a.dline[p] = d_lines[p];
+  }
+  if (workspace) {
+    BwdRows r;
+    memset(&r, 0, sizeof(r));
+    r.n = n_rows;
+    r.n_dev = row_count;
+    r.idx = idx;
+    r.pts = pts;
+    r.d_out = d_features;
+    return launch_sorted_bwd<false>(a, r, workspace, as_stream(stream));
   }
   const int G = next_pow2(a.n_quads);
   const int run = run_length();

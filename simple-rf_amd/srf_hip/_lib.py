@@ -62,9 +62,10 @@ _PROTOS = {
     'srf_mlp_backward_data': [_P, _P, _I, _P, _P, _P, _P, _P, _P],
     'srf_mlp_weight_grads': [_P, _I, _P, _P, _P, _P],
     'srf_vm_density_fwd': [_P, _P, _P, _L, _P, _I, _F, _P, _P, _P, _P],
-    'srf_vm_density_bwd': [_P, _P, _P, _P, _L, _P, _P, _P, _I, _F, _P, _P],
+    'srf_vm_density_bwd': [_P, _P, _P, _P, _L, _P, _P, _P, _I, _F, _P, _P, _P],
     'srf_vm_appearance_fwd': [_P, _P, _L, _P, _P, _P, _P, _P],
-    'srf_vm_appearance_bwd': [_P, _P, _P, _P, _L, _P, _P, _P, _P, _P],
+    'srf_vm_appearance_bwd': [_P, _P, _P, _P, _L, _P, _P, _P, _P, _P, _P],
+    'srf_vm_bwd_workspace_bytes': [_P, _L],
     'srf_cp_density_fwd': [_P, _P, _P, _L, _P, _I, _F, _P, _P, _P, _P],
     'srf_cp_density_bwd': [_P, _P, _P, _L, _P, _P, _P, _I, _F, _P, _P],
     'srf_cp_appearance_fwd': [_P, _P, _L, _P, _P, _P, _P, _P],
@@ -98,7 +99,8 @@ _PROTOS = {
 _RESTYPES = {'srf_status_string': C.c_char_p, 'srf_mlp_packed_bytes': C.c_size_t,
              'srf_mlp_workspace_bytes': C.c_size_t, 'srf_compact_workspace_bytes': C.c_size_t,
              'srf_color_packed_bytes': C.c_size_t, 'srf_color_workspace_bytes': C.c_size_t,
-             'srf_tune_get': C.c_int64, 'srf_tv_workspace_bytes': C.c_size_t}
+             'srf_tune_get': C.c_int64, 'srf_tv_workspace_bytes': C.c_size_t,
+             'srf_vm_bwd_workspace_bytes': C.c_size_t}
 
 _lib = None
 

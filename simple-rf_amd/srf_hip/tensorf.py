@@ -132,6 +132,19 @@ def alpha_mask_sample(mask: AlphaMask, pts: torch.Tensor) -> torch.Tensor:
     return out
 
 
+# The VM factor backwards order the contributing samples by plane cell before the scatter (a
+# workspace from srf_vm_bwd_workspace_bytes); False walks them in sample order instead (same sums up
+# to the order of the FP32 atomics, more atomics).
+SORTED_BACKWARD = True
+
+
+def _sorted_ws(f, rows: int, device):
+    if not SORTED_BACKWARD:
+        return None
+    nbytes = _lib.lib().srf_vm_bwd_workspace_bytes(C.byref(f), int(rows))
+    return torch.empty((max(int(nbytes), 1),), dtype=torch.uint8, device=device)
+
+
 class _VmDensity(torch.autograd.Function):
     @staticmethod
     def forward(ctx, pts, bbox, alpha, predictor, offset, *factors):
@@ -161,8 +174,9 @@ class _VmDensity(torch.autograd.Function):
         planes, lines = factors[:3], factors[3:]
         f, keep = _factors_struct(planes, lines)
         dp, dl, grads = _grad_buffers(planes, lines, needs)
+        ws = _sorted_ws(f, pts.shape[0], pts.device)
         _lib.call('srf_vm_density_bwd', C.byref(f), dp, dl, _bbox_arg(bbox), pts.shape[0], _lib.ptr(pts),
-                  _lib.ptr(valid), _lib.ptr(feat), predictor, offset, _lib.ptr(g_sigma.contiguous()),
+                  _lib.ptr(valid), _lib.ptr(feat), predictor, offset, _lib.ptr(g_sigma.contiguous()), _lib.ptr(ws),
                   _lib.stream_ptr(pts.device))
         return (None, None, None, None, None, *grads)
 
@@ -208,8 +222,9 @@ class _VmAppearance(torch.autograd.Function):
         g = g_out.contiguous()
         if g.data_ptr() % 16:   # rows are read as float4
             g = g.clone()
+        ws = _sorted_ws(f, n_rows, pts.device)
         _lib.call('srf_vm_appearance_bwd', C.byref(f), dp, dl, _bbox_arg(bbox), n_rows, _lib.ptr(count),
-                  _lib.ptr(idx), _lib.ptr(pts), _lib.ptr(g), _lib.stream_ptr(pts.device))
+                  _lib.ptr(idx), _lib.ptr(pts), _lib.ptr(g), _lib.ptr(ws), _lib.stream_ptr(pts.device))
         return (None, None, None, None, None, *grads)
 
 

@@ -103,8 +103,15 @@ def test_density_alpha_mask_and_softplus(cuda):
     assert (a.cpu() - ref32).abs().max() < 1e-5
 
 
+@pytest.fixture(params=[True, False], ids=['cell_sorted', 'sample_order'])
+def sorted_bwd(request, monkeypatch):
+    """Both factor-backward schedules: samples ordered by plane cell (workspace) and in sample order."""
+    monkeypatch.setattr(tensorf, 'SORTED_BACKWARD', request.param)
+    return request.param
+
+
 @pytest.mark.parametrize('comps,kind', [([16, 4, 4], 'ReLU'), ([8, 8, 8], 'SoftPlus')])
-def test_density_backward_matches_autograd(cuda, comps, kind):
+def test_density_backward_matches_autograd(cuda, comps, kind, sorted_bwd):
     planes, lines = factors(comps, RES, 5)
     pts = points(8000, BBOX, 6)
     g = torch.Generator().manual_seed(7)
@@ -127,7 +134,7 @@ def test_density_backward_matches_autograd(cuda, comps, kind):
 
 
 @pytest.mark.parametrize('comps', [[48, 12, 12], [16, 16, 16]])
-def test_appearance_forward_backward(cuda, comps):
+def test_appearance_forward_backward(cuda, comps, sorted_bwd):
     planes, lines = factors(comps, RES, 8)
     pts = points(6000, BBOX, 9, outside=0.0)
     g = torch.Generator().manual_seed(10)
@@ -211,3 +218,32 @@ def test_composite_without_colour(cuda):
         assert (out[k].detach().cpu().double() - ref[k]).abs().max() < 1e-5, k
     out['weights'].sum().backward()
     assert torch.isfinite(sd.grad).all()
+
+
+def test_sorted_backward_at_full_size_matches_sample_order(cuda):
+    """At the 300^2 shape (line copies of 300 x 48 floats in LDS) and the 512^2 one (512 x 64, the
+    largest LDS copy), with a device-side row count below the capacity: the cell-sorted appearance
+    and density backwards equal the sample-order ones up to the order of FP32 additions."""
+    for res, comps, n in ((300, [48, 48, 48], 200000), (512, [64, 64, 64], 120000)):
+        planes, lines = factors(comps, [res, res, res], 31)
+        pts = points(n, BBOX, 32, outside=0.0).to(cuda)
+        P = to_dev(planes, cuda, True)
+        L = to_dev(lines, cuda, True)
+        idx = torch.randperm(n, generator=torch.Generator().manual_seed(33))[: n // 2].to(torch.int32).to(cuda)
+        count = torch.tensor([n // 3], dtype=torch.int32, device=cuda)
+        gout = torch.randn(n // 2, sum(comps), generator=torch.Generator().manual_seed(34)).to(cuda)
+        grads = {}
+        for mode in (True, False):
+            tensorf.SORTED_BACKWARD = mode
+            try:
+                for t in P + L:
+                    t.grad = None
+                f = tensorf.vm_appearance(pts, P, L, BBOX.flatten().tolist(), idx=idx, n_rows=n // 2, count=count)
+                (f[: n // 3] * gout[: n // 3]).sum().backward()
+                sig, _ = tensorf.vm_density(pts, P, L, BBOX.flatten().tolist())
+                (sig * gout[:, 0].repeat(2)[:n]).sum().backward()
+                grads[mode] = [t.grad.detach().clone() for t in P + L]
+            finally:
+                tensorf.SORTED_BACKWARD = True
+        for a, b in zip(grads[True], grads[False]):
+            assert (a - b).abs().max().item() <= 1e-5 * b.abs().max().item()

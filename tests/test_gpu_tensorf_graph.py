@@ -64,17 +64,19 @@ def test_train_step_has_no_host_synchronisation(cuda):
     assert model.coarse_model.matrices_color[0].grad.abs().sum().item() > 0
 
 
-# The FP32 atomics of the factor gathers and of the weight-gradient GEMM make two runs differ at
-# rounding level: SGD passes that through; Adam's m / sqrt(v) can turn a rounding-level gradient of
-# either sign into a full +-lr step per element and step.
-# features PE 2: the colour backward's LDS tile then exceeds the 64 KB default of dynamic LDS
-@pytest.mark.parametrize('kind,cp,features_pe,param_tol', [('sgd', False, 0, None), ('adam', False, 0, 1e-3 * 5),
+# Each replay is compared with an eager step from the same state (the captured model's parameters
+# and optimiser state are set to the eager model's before every replay), so that the comparison
+# sees one step's summation-order differences only: over several steps those would grow wherever a
+# density ReLU or the surface threshold flips, and the trajectories would part for that reason alone.
+@pytest.mark.parametrize('kind,cp,features_pe,param_tol', [('sgd', False, 0, None), ('adam', False, 0, 1e-3),
                                                           ('sgd', True, 0, None), ('sgd', False, 2, None)])
 def test_graph_replays_match_eager_steps(cuda, kind, cp, features_pe, param_tol):
     a = _model(cuda, cp, features_pe)
     b = copy.deepcopy(a)
     batch, inp = _inputs(cuda)
     opt_a, opt_b = _opt(kind, a), _opt(kind, b)
+    pa = [p for p in a.parameters() if p.requires_grad]
+    pb = [p for p in b.parameters() if p.requires_grad]
 
     def step(model, opt):
         out = model(inp)
@@ -83,30 +85,36 @@ def test_graph_replays_match_eager_steps(cuda, kind, cp, features_pe, param_tol)
         opt.step()
         return loss
 
-    ref = []
-    for _ in range(5):
+    for _ in range(2):   # the eager model's optimiser state exists from here on, as the captured one's
         opt_a.zero_grad(set_to_none=True)
-        ref.append(step(a, opt_a).item())
+        step(a, opt_a)
     side = torch.cuda.Stream()
     side.wait_stream(torch.cuda.current_stream())
     with torch.cuda.stream(side):
-        got = []
         for _ in range(2):
             opt_b.zero_grad(set_to_none=True)
-            got.append(step(b, opt_b).item())
+            step(b, opt_b)
     torch.cuda.current_stream().wait_stream(side)
     graph = torch.cuda.CUDAGraph()
     opt_b.zero_grad(set_to_none=True)
     with torch.cuda.graph(graph):
-        loss = step(b, opt_b)
-    for _ in range(3):
+        loss_b = step(b, opt_b)
+    for k in range(3):
+        with torch.no_grad():
+            for x, y in zip(pb, pa):
+                x.copy_(y)
+            for x, y in zip(pb, pa):
+                for key, v in opt_a.state.get(y, {}).items():
+                    if isinstance(v, torch.Tensor):
+                        opt_b.state[x][key].copy_(v)
+        opt_a.zero_grad(set_to_none=True)
+        la = step(a, opt_a).item()
         graph.replay()
         torch.cuda.synchronize()
-        got.append(loss.item())
-        _ = sum(float(p.detach().abs().sum()) for p in b.parameters())   # eager work between replays
-    for i, (x, y) in enumerate(zip(ref, got)):
-        assert abs(x - y) <= 1e-4 * max(1.0, abs(x)), (i, x, y)
-    for (na, pa), (nb, pb) in zip(a.named_parameters(), b.named_parameters()):
-        err = (pa.detach() - pb.detach()).abs().max().item()
-        tol = param_tol if param_tol is not None else 1e-5 * max(1.0, pa.detach().abs().max().item())
-        assert err <= tol, (na, err)
+        lb = loss_b.item()
+        _ = sum(float(p.detach().abs().sum()) for p in pb)   # eager work between replays
+        assert abs(la - lb) <= 1e-5 * max(1.0, abs(la)), (k, la, lb)
+        for (na, xa), xb in zip(a.named_parameters(), b.parameters()):
+            err = (xa.detach() - xb.detach()).abs().max().item()
+            tol = param_tol if param_tol is not None else 1e-5 * max(1.0, xa.detach().abs().max().item())
+            assert err <= tol, (k, na, err)
