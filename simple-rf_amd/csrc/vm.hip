@@ -347,6 +347,19 @@ __device__ __forceinline__ void flush_slot(float* grad, int idx, float4 acc, int
   if (idx >= 0) atomic_add4v(grad + (int64_t)idx * C + c, acc);
 }
 
+// The block's LDS copy of a line (dynamic shared memory of the cell-sorted backward).  Named at
+// file scope so that every access is known to be an LDS access (ds_add_f32), not a flat one.
+extern __shared__ float g_line_lds[];
+
+__device__ __forceinline__ void flush_slot_lds(int idx, float4 acc, int C, int c) {
+  if (idx < 0) return;
+  const int o = idx * C + c;
+  atomicAdd(&g_line_lds[o + 0], acc.x);
+  atomicAdd(&g_line_lds[o + 1], acc.y);
+  atomicAdd(&g_line_lds[o + 2], acc.z);
+  atomicAdd(&g_line_lds[o + 3], acc.w);
+}
+
 __device__ __forceinline__ float4 pick4(bool b, float4 x, float4 y) {
   return make_float4(b ? x.x : y.x, b ? x.y : y.y, b ? x.z : y.z, b ? x.w : y.w);
 }
@@ -375,13 +388,19 @@ __device__ __forceinline__ void plane_accum(int (&si)[4], float4 (&sa)[4], const
   for (int k = 0; k < 4; ++k) si[k] = ci[k];
 }
 
+template <bool kLds = false>
 __device__ __forceinline__ void line_accum(int (&si)[2], float4 (&sa)[2], const int (&ci)[2], const float (&cw)[2],
                                            float4 g, float* grad, int C, int c) {
   const bool same = ci[0] == si[0] && ci[1] == si[1];
   const bool up = !same && ci[0] == si[1];          // lower <- upper
   const bool down = !same && !up && ci[1] == si[0];  // upper <- lower
-  if (!(same || down)) flush_slot(grad, si[0], sa[0], C, c);
-  if (!(same || up)) flush_slot(grad, si[1], sa[1], C, c);
+  if (kLds) {  // grad unused: the slots flush into the LDS line copy
+    if (!(same || down)) flush_slot_lds(si[0], sa[0], C, c);
+    if (!(same || up)) flush_slot_lds(si[1], sa[1], C, c);
+  } else {
+    if (!(same || down)) flush_slot(grad, si[0], sa[0], C, c);
+    if (!(same || up)) flush_slot(grad, si[1], sa[1], C, c);
+  }
   const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
   const float4 n0 = pick4(same, sa[0], pick4(up, sa[1], z));
   const float4 n1 = pick4(same, sa[1], pick4(down, sa[0], z));
@@ -523,7 +542,8 @@ constexpr int kMaxChunks = 1024;  // scan chunks per plane
 struct SortWs {
   int32_t* hist[kPlanes];    // cells + 1: counts, then exclusive offsets
   int32_t* cursor[kPlanes];  // cells: scatter positions
-  int32_t* order[kPlanes];   // capacity: rows in cell order
+  float4* order[kPlanes];    // capacity: per row in cell order, its normalised point and either its
+                             // scalar gradient (density) or its row number as int bits (appearance)
   int32_t* total;            // [kPlanes] rows per plane
   int32_t* partial;          // [kPlanes][kMaxChunks] chunk sums, then chunk offsets
   int64_t chunk_cells;       // cells per scan chunk
@@ -621,7 +641,9 @@ __global__ __launch_bounds__(256) void cell_count_kernel(VmArgs a, BwdRows r, So
         int pos = 0;
         if (run.head) pos = atomicAdd(w.cursor[p] + key, run.length);
         const int first = __shfl(pos, run.head_lane, SRF_WAVE);
-        if (active) w.order[p][first + (lane - run.head_lane)] = (int32_t)j;
+        if (active)
+          w.order[p][first + (lane - run.head_lane)] =
+              make_float4(pn[0], pn[1], pn[2], kDensity ? g : __int_as_float((int)j));
       }
     }
   }
@@ -703,12 +725,11 @@ constexpr int kMaxLineLds = 160 * 1024;  // a CU's LDS
 
 // Plane p's gradients from its cell-sorted rows: a lane group walks run_len consecutive sorted
 // rows with the corner slots of the run walk (flushing a plane texel when the walk leaves it) and
-// adds the line gradients into s_line (the line copy in LDS, line_lds != 0) or, when the line is
+// adds the line gradients into g_line_lds (the line copy in LDS, line_lds != 0) or, when the line is
 // too large for LDS, through the line slots of the run walk.
 template <int G, bool kDensity>
 __global__ __launch_bounds__(kSortedThreads) void sorted_bwd_kernel(VmArgs a, BwdRows r, SortWs w, int p,
                                                                     int run_len, int line_lds) {
-  extern __shared__ float s_line[];
   const int C = sel3(p, a.comp[0], a.comp[1], a.comp[2]);
   const int L = sel3(p, a.ll[0], a.ll[1], a.ll[2]);
   const int64_t m = w.total[p];
@@ -716,7 +737,7 @@ __global__ __launch_bounds__(kSortedThreads) void sorted_bwd_kernel(VmArgs a, Bw
   const int64_t b0 = (int64_t)blockIdx.x * groups * run_len;
   if (b0 >= m) return;  // uniform over the block
   if (line_lds) {
-    for (int e = threadIdx.x; e < L * C; e += kSortedThreads) s_line[e] = 0.f;
+    for (int e = threadIdx.x; e < L * C; e += kSortedThreads) g_line_lds[e] = 0.f;
     __syncthreads();
   }
   const int grp = threadIdx.x / G, ql = threadIdx.x % G;
@@ -725,47 +746,41 @@ __global__ __launch_bounds__(kSortedThreads) void sorted_bwd_kernel(VmArgs a, Bw
   const int q0 = sel3(p, a.qoff[0], a.qoff[1], a.qoff[2]);
   float* dP = sel3(p, a.dplane[0], a.dplane[1], a.dplane[2]);
   float* dL = sel3(p, a.dline[0], a.dline[1], a.dline[2]);
-  const int32_t* order = sel3(p, w.order[0], w.order[1], w.order[2]);
+  const float4* order = sel3(p, w.order[0], w.order[1], w.order[2]);
   for (int q = q0 + ql; q < q0 + C / 4; q += G) {
     RunAcc acc;
     run_init(acc);
     int c = 0;
     for (int64_t s = s0; s < s1; ++s) {
-      const int64_t j = order[s];
-      int64_t i;
-      float gs;
-      bwd_row<kDensity>(r, j, i, gs);
-      float pn[3];
-      normalized_point(a, r.pts, i, pn);
-      const float4 g = kDensity ? make_float4(gs, gs, gs, gs) : ld4(r.d_out + j * a.total_comp + 4 * q);
+      const float4 e = order[s];
+      const float pn[3] = {e.x, e.y, e.z};
+      const float4 g = kDensity ? make_float4(e.w, e.w, e.w, e.w)
+                                : ld4(r.d_out + (int64_t)__float_as_int(e.w) * a.total_comp + 4 * q);
       const QuadSite site = quad_site(a, q, pn);
       float4 pv, lv;
       quad_values(a, site, pv, lv);
       plane_accum(acc.pidx, acc.pg, site.pt.idx, site.pt.w, mul4(g, lv), dP, C, site.c);
       const float4 gl = mul4(g, pv);
-      if (line_lds) {
-#pragma unroll
-        for (int t = 0; t < 2; ++t) {
-          if (site.lt.idx[t] < 0) continue;
-          float* d = s_line + site.lt.idx[t] * C + site.c;
-          const float wt = site.lt.w[t];
-          atomicAdd(d + 0, __fmul_rn(gl.x, wt));
-          atomicAdd(d + 1, __fmul_rn(gl.y, wt));
-          atomicAdd(d + 2, __fmul_rn(gl.z, wt));
-          atomicAdd(d + 3, __fmul_rn(gl.w, wt));
-        }
-      } else {
-        line_accum(acc.lidx, acc.lg, site.lt.idx, site.lt.w, gl, dL, C, site.c);
-      }
+      // the line slots carry a texel while consecutive rows (one ray's samples within the cell) stay
+      // on it, and flush into the LDS copy (or HBM) when the walk leaves it
+      if (line_lds)
+        line_accum<true>(acc.lidx, acc.lg, site.lt.idx, site.lt.w, gl, nullptr, C, site.c);
+      else
+        line_accum<false>(acc.lidx, acc.lg, site.lt.idx, site.lt.w, gl, dL, C, site.c);
       c = site.c;
     }
     run_flush_plane(a, p, c, acc);
-    if (!line_lds) run_flush_line(a, p, c, acc);
+    for (int t = 0; t < 2; ++t) {
+      if (line_lds)
+        flush_slot_lds(acc.lidx[t], acc.lg[t], C, c);
+      else
+        flush_slot(dL, acc.lidx[t], acc.lg[t], C, c);
+    }
   }
   if (line_lds) {
     __syncthreads();
     for (int e = threadIdx.x; e < L * C; e += kSortedThreads) {
-      const float v = s_line[e];
+      const float v = g_line_lds[e];
       if (v != 0.f) unsafeAtomicAdd(dL + e, v);
     }
   }
@@ -1168,7 +1183,7 @@ size_t sorted_ws_bytes(const VmArgs& a, int64_t rows) {
   for (int p = 0; p < kPlanes; ++p) {
     const int64_t cells = (int64_t)a.ph[p] * a.pw[p];
     b += align256((size_t)(cells + 1) * sizeof(int32_t)) + align256((size_t)cells * sizeof(int32_t)) +
-         align256((size_t)rows * sizeof(int32_t));
+         align256((size_t)rows * sizeof(float4));
   }
   return b;
 }
@@ -1195,8 +1210,8 @@ SortWs carve_sorted_ws(const VmArgs& a, int64_t rows, void* ws) {
     off += align256((size_t)w.cells[p] * sizeof(int32_t));
   }
   for (int p = 0; p < kPlanes; ++p) {
-    w.order[p] = reinterpret_cast<int32_t*>(base + off);
-    off += align256((size_t)rows * sizeof(int32_t));
+    w.order[p] = reinterpret_cast<float4*>(base + off);
+    off += align256((size_t)rows * sizeof(float4));
   }
   int64_t most = 1;
   for (int p = 0; p < kPlanes; ++p) most = w.cells[p] > most ? w.cells[p] : most;
