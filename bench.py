@@ -162,7 +162,7 @@ def hip_train_leg(args, dev, precision='bf16'):
     _, model = build_model(precision, dev)
     model.train()
     params = [p for p in model.parameters() if p.requires_grad]
-    opt = torch.optim.Adam(params, lr=5e-4, betas=(0.9, 0.999), fused=True)
+    opt = make_adam(params, lr=5e-4, betas=(0.9, 0.999))
     batch = synthetic.train_batch(args.rays, seed=0, device=dev)
     inp = {'pixel_id': batch['pixel_id'], 'num_frames': 3, 'iter_num': 0, 'sub_batch_index': 0}
 
@@ -201,7 +201,7 @@ def hip_graph_train_leg(args, dev, precision='bf16'):
     _, model = build_model(precision, dev)
     model.train()
     params = [p for p in model.parameters() if p.requires_grad]
-    opt = torch.optim.Adam(params, lr=5e-4, betas=(0.9, 0.999), capturable=True, fused=True)
+    opt = make_adam(params, lr=5e-4, betas=(0.9, 0.999))
     batch = synthetic.train_batch(args.rays, seed=0, device=dev)
     inp = {'pixel_id': batch['pixel_id'], 'num_frames': 3, 'iter_num': 0, 'sub_batch_index': 0}
 
@@ -232,6 +232,19 @@ def hip_graph_train_leg(args, dev, precision='bf16'):
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     return dict(seconds=t1 - t0, loss=float(loss.item()))
+
+
+ADAM = 'hip'
+
+
+def make_adam(params, **kw):
+    """The step's optimiser: the fused HIP Adam (srf_hip.adam.AdamHip, the drop-in for the reference's
+    torch.optim.Adam) or, with --adam torch, torch's fused capturable Adam."""
+    import torch
+    if ADAM == 'hip':
+        from srf_hip.adam import AdamHip
+        return AdamHip(params, **kw)
+    return torch.optim.Adam(params, fused=True, capturable=True, **kw)
 
 
 def graph_replay_time(step, opt, warmup, steps):
@@ -414,7 +427,7 @@ def tensorf_train_leg(args, dev):
     _, model = tensorf_model(dev)
     model.train()
     groups = model.get_trainable_parameters({'lr_initial_tensor': 2e-2, 'lr_initial_network': 1e-3})
-    opt = torch.optim.Adam(groups, betas=(0.9, 0.99), fused=True, capturable=True)
+    opt = make_adam(groups, betas=(0.9, 0.99))
     batch = synthetic.train_batch(args.tensorf_rays, seed=5, device=dev)
     inp = {'pixel_id': batch['pixel_id'], 'num_frames': 3, 'iter_num': 1, 'sub_batch_index': 0}
 
@@ -494,7 +507,7 @@ def tensorf_ragged_leg(args, dev):
            'rays_o': o.to(dev), 'rays_d': d.to(dev), 'view_dirs': d.to(dev), 'max_samples': int(counts.max())}
     target = torch.rand(n, 3, generator=g).to(dev)
     groups = tm.get_trainable_parameters({'lr_initial_tensor': 2e-2, 'lr_initial_network': 1e-3})
-    opt = torch.optim.Adam(groups, betas=(0.9, 0.99), fused=True, capturable=True)
+    opt = make_adam(groups, betas=(0.9, 0.99))
 
     def fwd_bwd():
         out = tm.forward_ragged(inp)
@@ -865,10 +878,14 @@ def main():
     ap.add_argument('--no-graph', action='store_true', help='report the eager step only')
     ap.add_argument('--no-split', action='store_true', help='skip the split-BF16 train leg')
     ap.add_argument('--no-autotune', action='store_true', help="keep the library's built-in launch schedule")
+    ap.add_argument('--adam', choices=('hip', 'torch'), default='hip',
+                    help="optimiser of the train steps: the fused HIP Adam (default) or torch's fused Adam")
     ap.add_argument('--copy-calibration-mb', type=int, default=0,
                     help='after the timed legs, run 3 device copies of this many MiB (a plain copy kernel '
                          'of known size for scaling rocprof FETCH_SIZE / WRITE_SIZE)')
     args = ap.parse_args()
+    global ADAM
+    ADAM = args.adam
     if args.gpus != 1 or int(os.environ.get('WORLD_SIZE', '1')) != 1:
         print('bench.py measures one GPU: the library has no multi-GPU path (rays would shard across GPUs, '
               'with one all-reduce of the parameter gradients per step)', file=sys.stderr)
@@ -919,7 +936,8 @@ def main():
         'vs_baseline': None, 'dtype': 'bf16', 'data': 'synthetic (seeded camera rays, random-init weights)',
         'config': {'workload': 'Simple-NeRF train step: coarse 64 (main + points-aug + views-aug MLPs) + fine '
                                '192 merged samples, fwd+bwd+Adam', 'rays_per_step': args.rays,
-                   'mlp': '8x256 skip@4 + 128 view head, BF16 MFMA / FP32 accumulate', 'parallelism': 'none'},
+                   'mlp': '8x256 skip@4 + 128 view head, BF16 MFMA / FP32 accumulate', 'parallelism': 'none',
+                   'optimizer': 'srf_adam_step (fused HIP Adam)' if ADAM == 'hip' else 'torch fused Adam'},
         'roofline': roof,
         'step_mode': 'hip_graph_replay' if graph_info and graph_info['captured'] else 'eager',
         'eager_rays_per_s': round(eager_value, 1),

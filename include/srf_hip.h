@@ -451,6 +451,18 @@ int srf_color_backward(const srf_color_desc* desc, const void* packed, int n_poi
                        const float* d_rgb, void* workspace, float* d_app, float* d_basis, float* d_w1,
                        float* d_b1, float* d_w2, float* d_b2, float* d_w3, float* d_b3, void* stream);
 
+/* ------------------------------------------------------------------ optimiser (f4) */
+
+/* One Adam step (torch.optim.Adam as src/optimizers/OptimizerFactory02.py:9-21 builds it: amsgrad
+ * and maximize off, weight decay added to the gradient) over n_tensors parameter tensors in one
+ * launch per 32 tensors.  Tensor t: params[t], grads[t], exp_avg[t], exp_avg_sq[t] (FP32, numel[t]
+ * elements in one shared dense layout), steps[t] (device float, torch's state['step'], advanced
+ * by one), lr[t] and weight_decay[t] (its parameter group's, host values).  Same operation order
+ * as torch's single-tensor Adam; the bias corrections in double. */
+int srf_adam_step(int n_tensors, float* const* params, const float* const* grads, float* const* exp_avg,
+                  float* const* exp_avg_sq, float* const* steps, const int64_t* numel, const float* lr,
+                  const float* weight_decay, float beta1, float beta2, float eps, void* stream);
+
 /* ------------------------------------------------------- launch-schedule tuning */
 
 /* Knobs of the launch schedule, picked by an autotuner (srf_hip.autotune) for a workload.  No

@@ -74,6 +74,7 @@ _PROTOS = {
     'srf_compact_workspace_bytes': [_L],
     'srf_threshold_compact': [_L, _P, _F, _P, _P, _P, _P],
     'srf_threshold_compact_noalloc': [_L, _P, _F, _P, _P, _P],
+    'srf_adam_step': [_I, _P, _P, _P, _P, _P, _P, _P, _P, _F, _F, _F, _P],
     'srf_volume_render_fwd': [_I, _I, _P, _P, _P, _I, _P, _P],
     'srf_volume_render_bwd': [_I, _I, _P, _P, _P, _I, _P, _P, _P, _P],
     'srf_aabb_samples': [_I, _I, _P, _P, _P, _F, _F, _F, _P, _P, _P],

@@ -23,6 +23,9 @@ def _opt(kind, model):
     params = [p for p in model.parameters() if p.requires_grad]
     if kind == 'sgd':
         return torch.optim.SGD(params, lr=1e-2, momentum=0.9)
+    if kind == 'adam_hip':
+        from srf_hip.adam import AdamHip
+        return AdamHip(params, lr=1e-3)
     return torch.optim.Adam(params, lr=1e-3, capturable=True, fused=kind == 'adam_fused')
 
 
@@ -30,7 +33,8 @@ def _opt(kind, model):
 # differ at rounding level; SGD passes that through (1e-5 relative).  Adam's m / sqrt(v) turns a
 # rounding-level gradient of either sign into a full +-lr step, so an element may drift by up to
 # lr per step (the loss trajectory above is still held to 1e-4).
-@pytest.mark.parametrize('kind,param_tol', [('sgd', None), ('adam', 1e-3 * 5), ('adam_fused', 1e-3 * 5)])
+@pytest.mark.parametrize('kind,param_tol', [('sgd', None), ('adam', 1e-3 * 5), ('adam_fused', 1e-3 * 5),
+                                            ('adam_hip', 1e-3 * 5)])
 def test_graph_replays_match_eager_steps(cuda, kind, param_tol):
     a = _model(cuda)
     b = copy.deepcopy(a)
