@@ -396,11 +396,21 @@ def render_image_leg(args, dev, precisions=('bf16', 'fp16', 'fp32'), chunk=None)
 PEAK_HBM_BYTES = PEAK_HBM_GBS * 1e9
 
 
-def tensorf_model(dev, seed=0):
+COLOR_PRECISION_NOTE = {
+    'fp32': 'fp32: FP32 MFMA operands (the reference arithmetic)',
+    'bf16_split': 'bf16_split: BF16 MFMAs on operands split into BF16 planes (3 forward, 2 backward), FP32 '
+                  'activations and weight gradients; rgb and gradients as FP32 (tests/test_gpu_color.py)',
+    'bf16': 'bf16: BF16 MFMA operands, FP32 activations, accumulation and weight gradients; rgb within 2e-3 '
+            'of FP64 (tests/test_gpu_color.py)',
+}
+
+
+def tensorf_model(dev, seed=0, color_precision='fp32'):
     """Simple-TensoRF VM model at 300^2: density 3 x 16 components, appearance 3 x 48 (basis 144 -> 27,
     colour MLP on features PE L=2 + view dir: 138 -> 128 -> 128 -> 3), 443 samples per ray, a random
     64^3 occupancy mask with 40 % occupied voxels, and a low-rank points-augmented model (4 density
-    components at 150^2).  Factors N(0, 0.1) as the reference initialises them."""
+    components at 150^2).  Factors N(0, 0.1) as the reference initialises them.  color_precision: the
+    colour network's MFMA operands ('fp32', 'bf16_split' or 'bf16')."""
     import torch
     from models.ModelFactory02 import get_model
     from models.SimpleTensoRFHip09 import AlphaGridMask
@@ -413,6 +423,7 @@ def tensorf_model(dev, seed=0):
                     num_voxels_initial=300 ** 3 + 4, num_samples_max=443, features_positional_encoding_degree=2),
         augmented=dict(num_components_density=(4, 4, 4), bounding_box=box, num_voxels_initial=150 ** 3 + 4,
                        num_samples_max=443))
+    c['model']['color_precision'] = color_precision
     model = get_model(c, model_configs=cfgs.model_configs()).to(dev)
     g = torch.Generator().manual_seed(seed + 7)
     vol = (torch.rand(64, 64, 64, generator=g) < 0.4).float()
@@ -420,11 +431,11 @@ def tensorf_model(dev, seed=0):
     return c, model
 
 
-def tensorf_train_leg(args, dev):
+def tensorf_train_leg(args, dev, color_precision='fp32', tune=True):
     """TensoRF train step (fwd + hand-written bwd + Adam) on args.tensorf_rays rays."""
     import torch
     from srf_hip import _lib, synthetic
-    _, model = tensorf_model(dev)
+    _, model = tensorf_model(dev, color_precision=color_precision)
     model.train()
     groups = model.get_trainable_parameters({'lr_initial_tensor': 2e-2, 'lr_initial_network': 1e-3})
     opt = make_adam(groups, betas=(0.9, 0.99))
@@ -444,7 +455,7 @@ def tensorf_train_leg(args, dev):
 
     for _ in range(args.warmup):
         step()
-    if TUNER is not None:   # gather and colour weight-gradient schedule, measured on this step (untimed)
+    if TUNER is not None and tune:   # gather and colour weight-gradient schedule, measured on this step (untimed)
         for knob in ('gather_run', 'gather_blocks', 'color_dw_chunk'):
             TUNER.tune(f'tensorf_train_{args.tensorf_rays}rays', knob, step, reps=2)
     torch.cuda.synchronize()
@@ -470,17 +481,19 @@ def tensorf_train_leg(args, dev):
             'eager_rays_per_s': round(args.tensorf_rays * n / (t1 - t0), 1),
             'eager_ms_per_step': round((t1 - t0) * 1e3 / n, 3),
             'workload': f'{args.tensorf_rays} rays x 443 samples, VM 300^2 density 3x16 / appearance 3x48, '
-                        '64^3 alpha mask (40 % occupied), low-rank augmented model, fwd+bwd+Adam, FP32',
+                        '64^3 alpha mask (40 % occupied), low-rank augmented model, fwd+bwd+Adam, FP32 grids',
+            'color_precision': COLOR_PRECISION_NOTE[color_precision],
             'kernels': {k: {'calls': v['calls'], 'ms_per_step': round(v['ms'] / n, 3)} for k, v in kern.items()},
             'model': model, 'out': out}
 
 
-def tensorf_ragged_leg(args, dev):
+def tensorf_ragged_leg(args, dev, color_precision='fp32', tune=True):
     """Uneven-batch TensoRF train step at scale: args.ragged_rays rays whose sample counts follow a
     truncated log-normal (median 96, sigma_log 0.5, at most 512), the shape occupancy pruning of a
     128^3 mask leaves; VM grids of 512^2 planes with 24 density / 64 appearance components per
     plane (277 MB of factors), features PE L = 2; metric samples at half-voxel steps, composited
-    ragged (CSR ray offsets); fwd + hand-written bwd + Adam, FP32."""
+    ragged (CSR ray offsets); fwd + hand-written bwd + Adam, FP32 grids, the colour network's MFMA
+    operands in color_precision."""
     import torch
     import torch.nn.functional as F
     from models.ModelFactory02 import get_model
@@ -491,6 +504,7 @@ def tensorf_ragged_leg(args, dev):
         coarse=dict(num_components_density=(24, 24, 24), num_components_color=(64, 64, 64),
                     bounding_box=((-1.0, -1.0, -1.0), (1.0, 1.0, 1.0)), num_voxels_initial=512 ** 3,
                     features_positional_encoding_degree=2))
+    c['model']['color_precision'] = color_precision
     tm = get_model(c, model_configs=cfgs.model_configs()).coarse_model.to(dev).train()
     n = args.ragged_rays
     g = torch.Generator().manual_seed(11)
@@ -522,7 +536,7 @@ def tensorf_ragged_leg(args, dev):
 
     for _ in range(2):
         out = train_step()
-    if TUNER is not None:
+    if TUNER is not None and tune:
         for knob in ('gather_run', 'gather_blocks'):
             TUNER.tune(f'tensorf_ragged_{n}rays', knob, train_step, reps=1)
     torch.cuda.synchronize()
@@ -547,7 +561,8 @@ def tensorf_ragged_leg(args, dev):
             'samples_per_step': n_samples, 'surface_samples': surface,
             'workload': f'{n} rays, ragged counts (truncated log-normal, median 96, max 512; {n_samples:,} '
                         'samples), VM 512^2 density 3x24 / appearance 3x64 (277 MB), features PE L=2, '
-                        'MSE, fwd+bwd+Adam, FP32',
+                        'MSE, fwd+bwd+Adam, FP32 grids',
+            'color_precision': COLOR_PRECISION_NOTE[color_precision],
             'kernels': {k: {'calls': v['calls'], 'ms_per_step': round(v['ms'] / s, 3)} for k, v in kern.items()}}
 
 
@@ -875,6 +890,12 @@ def main():
     ap.add_argument('--ragged-rays', type=int, default=32768)
     ap.add_argument('--ragged-steps', type=int, default=3)
     ap.add_argument('--no-ragged', action='store_true', help='skip the uneven-batch TensoRF leg')
+    ap.add_argument('--tensorf-color', choices=('fp32', 'bf16_split', 'bf16'), default='fp32',
+                    help='colour-network operands of the 300^2 TensoRF leg')
+    ap.add_argument('--ragged-color', choices=('fp32', 'bf16_split', 'bf16'), default='bf16',
+                    help='colour-network operands of the uneven-batch TensoRF leg')
+    ap.add_argument('--color-variants', default='fp32,bf16_split,bf16',
+                    help='colour precisions also timed (value and colour kernels only) beside each TensoRF leg')
     ap.add_argument('--no-graph', action='store_true', help='report the eager step only')
     ap.add_argument('--no-split', action='store_true', help='skip the split-BF16 train leg')
     ap.add_argument('--no-autotune', action='store_true', help="keep the library's built-in launch schedule")
@@ -947,15 +968,38 @@ def main():
         'kernels': per_kernel,
         **({'train_bf16_split': split} if split else {}),
     }
+    variants = [p for p in args.color_variants.split(',') if p]
+
+    def variant_fields(leg):
+        colour = {k: v['ms_per_step'] for k, v in leg['kernels'].items() if k.startswith('srf_color')}
+        return {'value': leg['value'], 'ms_per_step': leg['ms_per_step'], 'eager_ms_per_step': leg['eager_ms_per_step'],
+                'color_kernels_ms_per_step': colour}
+
     if not args.no_tensorf:
-        progress('TensoRF train step (eager, then graph capture and replay)')
-        tf = tensorf_train_leg(args, dev)
+        progress(f'TensoRF train step, colour {args.tensorf_color} (eager, then graph capture and replay)')
+        tf = tensorf_train_leg(args, dev, color_precision=args.tensorf_color)
         result['tensorf_train'] = {k: v for k, v in tf.items() if k not in ('model', 'out')}
         result['hbm_kernels'] = hbm_leg(dev, tf['model'], tf['out'])
         del tf
+        others = {}
+        for p in variants:
+            if p != args.tensorf_color:
+                progress(f'TensoRF train step, colour {p}')
+                leg = tensorf_train_leg(args, dev, color_precision=p, tune=False)
+                others[p] = variant_fields(leg)
+                del leg
+        if others:
+            result['tensorf_train']['other_color_precisions'] = others
     if not args.no_ragged:
-        progress('ragged TensoRF train step (eager, then graph capture and replay)')
-        result['tensorf_ragged_train'] = tensorf_ragged_leg(args, dev)
+        progress(f'ragged TensoRF train step, colour {args.ragged_color} (eager, then graph capture and replay)')
+        result['tensorf_ragged_train'] = tensorf_ragged_leg(args, dev, color_precision=args.ragged_color)
+        others = {}
+        for p in variants:
+            if p != args.ragged_color:
+                progress(f'ragged TensoRF train step, colour {p}')
+                others[p] = variant_fields(tensorf_ragged_leg(args, dev, color_precision=p, tune=False))
+        if others:
+            result['tensorf_ragged_train']['other_color_precisions'] = others
     if TUNER is not None:
         result['autotune'] = TUNER.results
     if not args.no_baselines:

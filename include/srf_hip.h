@@ -25,7 +25,7 @@
 extern "C" {
 #endif
 
-#define SRF_ABI_VERSION 2
+#define SRF_ABI_VERSION 3
 
 enum srf_status {
   SRF_OK = 0,
@@ -405,7 +405,13 @@ int srf_aabb_samples(int n_rays, int n_samples, const float* rays_o, const float
  * (Linear(n_features -> basis_dim), no bias; src/models/SimpleTensoRF09.py:1151) and
  * MlpFeaturesColorPredictor (:1370-1421): [PE_Lf(features), PE_Lv(view dir)] -> hidden ReLU ->
  * hidden ReLU -> 3 sigmoid.  n_features % 4 == 0 and <= 256, basis_dim <= 32, hidden == 128,
- * first-layer width basis_dim (2 features_pe + 1) + 3 (2 views_pe + 1) <= 240. */
+ * first-layer width basis_dim (2 features_pe + 1) + 3 (2 views_pe + 1) <= 240 (<= 160 for
+ * SRF_PREC_BF16_SPLIT, whose backward weight stage is larger).
+ * prec: operands of the MFMA products of the forward and of the backward chain.  SRF_PREC_F32
+ * (FP32 operands); SRF_PREC_BF16_SPLIT (split BF16: three planes in the forward, two in the
+ * backward chain, as srf_mlp_desc's); SRF_PREC_BF16 (BF16 operands rounded from the FP32 values).
+ * Activations, the stored layer inputs, accumulation and the weight-gradient GEMM are FP32 in
+ * every mode; the reference computes in FP32 throughout. */
 typedef struct srf_color_desc {
   int n_features;    /* sum of the appearance components */
   int basis_dim;     /* features_dimension_color (27) */
@@ -413,6 +419,7 @@ typedef struct srf_color_desc {
   int use_view_dirs; /* view_dependent_color */
   int views_pe;      /* views_positional_encoding_degree */
   int hidden;        /* num_units_color_predictor */
+  int prec;          /* SRF_PREC_F32 / SRF_PREC_BF16 / SRF_PREC_BF16_SPLIT */
 } srf_color_desc;
 
 /* Size of the MFMA-fragment copy of the weights written by srf_color_pack (bytes; 0 if the

@@ -1,6 +1,6 @@
-// TensoRF colour network on gfx950 MFMA (FP32 operands): basis matrix, positional encoding of the
-// features and view directions, 128 -> 128 -> 3 MLP with sigmoid, its hand-written backward chain
-// and the weight gradients.
+// TensoRF colour network on gfx950 MFMA: basis matrix, positional encoding of the features and
+// view directions, 128 -> 128 -> 3 MLP with sigmoid, its hand-written backward chain and the
+// weight gradients.
 //
 // Reference semantics: VmDecomposedTensor.get_color after the gather, basis_matrix_color
 // (src/models/SimpleTensoRF09.py:1151, :1262) and MlpFeaturesColorPredictor
@@ -16,6 +16,16 @@
 // and the view directions go through a per-wave LDS tile from which every lane builds its
 // positional-encoding inputs.  Training stores the layer inputs feature-major (row = feature,
 // column = point) for the weight-gradient GEMM, which is the grouped dW kernel of the MLP.
+//
+// Operand precision (desc->prec).  Activations, accumulators, the stored layer inputs and the
+// weight gradients are FP32 in every mode; the products of the forward and of the backward chain
+// multiply
+//   SRF_PREC_F32          FP32 operands (16x16x4 MFMA, one k-unit = 16 features)
+//   SRF_PREC_BF16_SPLIT   split BF16 (16x16x32 MFMA, one k-unit = 32 features): three planes in
+//                         the forward (six plane products, FP32-equivalent operands), two in the
+//                         backward chain (three products), as the NeRF MLP's split mode
+//   SRF_PREC_BF16         BF16 operands rounded from the FP32 activations and weights (one plane)
+// Reduction depths are padded to whole k-units (32 features for the BF16 modes) with zeros.
 #include <string.h>
 
 #include <mutex>
@@ -38,41 +48,75 @@ constexpr int kMaxC = 256;    // appearance components
 constexpr int kMaxB = 32;     // basis outputs (features_dimension_color)
 constexpr int kMaxD1 = 240;   // first-layer inputs (the backward's per-wave LDS tile fits beside the stage)
 constexpr int kST = 8;        // output tiles per weight-stage buffer (the widest layer has kHT = 8)
-constexpr int kStage = 2 * kST * mlp::kKC * 1024;  // double-buffered weight stage (32 KB)
+
+// Forward / backward-chain operand precisions of a descriptor's prec.
+constexpr int fwd_prec_of(int prec) { return prec == SRF_PREC_BF16_SPLIT ? kBFS3 : (prec == SRF_PREC_BF16 ? kBFS1 : kF32); }
+constexpr int bwd_prec_of(int prec) { return prec == SRF_PREC_BF16_SPLIT ? kBFS2 : (prec == SRF_PREC_BF16 ? kBFS1 : kF32); }
+// Double-buffered weight stage of a product in precision P (bytes): kST output tiles x the
+// k-units of one chunk x the BF16 planes, 1 KB each, twice.  F32 and BF16 32 KB, split forward
+// 48 KB, split backward 64 KB.
+template <int P>
+constexpr int stage_bytes() { return 2 * kST * mlp::kc_of<P>() * split_planes<P>() * 1024; }
+constexpr int stage_bytes_of(int P) {
+  return P == kBFS3 ? stage_bytes<kBFS3>() : (P == kBFS2 ? stage_bytes<kBFS2>() : stage_bytes<kF32>());
+}
+// Features per k-unit: 16 (FP32), 32 (BF16 operands).
+template <int P>
+constexpr int kunit() { return P == kF32 ? 16 : 32; }
 
 struct Geo {
-  int C, Cp;     // components, padded to 16
+  int prec;      // SRF_PREC_* of the descriptor
+  int kal;       // reduction padding: 16 (FP32) or 32 (BF16 modes)
+  int fpl, bpl;  // BF16 planes of the forward / backward fragments (1 for FP32)
+  int C, Cp;     // components, padded to kal
   int NB;        // basis outputs (27)
   int Lf, Lv, view;
-  int D1, D1p;   // first-layer inputs, padded to 16
+  int D1, D1p;   // first-layer inputs, padded to kal
   // fragment offsets (floats) in the packed buffer
   size_t fB, fW1, fW2, fW3, fW3T, fW2T, fW1T, fBT, b1, b2, b3, total;
 };
+
+// Floats of the packed fragments of `tiles` output tiles over a reduction of K features: one KB
+// per k-unit and BF16 plane (FP32: per 16 features).
+inline size_t frag_floats(const Geo& g, int tiles, int K, int planes) {
+  return (size_t)tiles * (K / g.kal) * planes * 256;
+}
+
+// Rows of the per-wave LDS tile of the backward: d x1 (D1p rows), later d F (kMaxB rows).
+__host__ __device__ inline int bwd_rows(const Geo& g) { return g.D1p > kMaxB ? g.D1p : kMaxB; }
+constexpr int kBwdLdsRowBytes = kWaves * (kPW + 1) * (int)sizeof(float);  // one row of every wave's tile
 
 bool make_geo(const srf_color_desc* d, Geo* g) {
   if (!d || d->n_features < 4 || d->n_features % 4 != 0 || d->n_features > kMaxC || d->basis_dim < 1 ||
       d->basis_dim > kMaxB || d->features_pe < 0 || d->views_pe < 0 || d->hidden != kHid)
     return false;
+  if (d->prec != SRF_PREC_F32 && d->prec != SRF_PREC_BF16 && d->prec != SRF_PREC_BF16_SPLIT) return false;
   memset(g, 0, sizeof(*g));
+  g->prec = d->prec;
+  g->kal = d->prec == SRF_PREC_F32 ? 16 : 32;
+  g->fpl = d->prec == SRF_PREC_BF16_SPLIT ? 3 : 1;
+  g->bpl = d->prec == SRF_PREC_BF16_SPLIT ? 2 : 1;
   g->C = d->n_features;
-  g->Cp = (g->C + 15) / 16 * 16;
+  g->Cp = (g->C + g->kal - 1) / g->kal * g->kal;
   g->NB = d->basis_dim;
   g->Lf = d->features_pe;
   g->Lv = d->views_pe;
   g->view = d->use_view_dirs != 0;
   g->D1 = g->NB * (2 * g->Lf + 1) + (g->view ? 3 * (2 * g->Lv + 1) : 0);
-  g->D1p = (g->D1 + 15) / 16 * 16;
+  g->D1p = (g->D1 + g->kal - 1) / g->kal * g->kal;
   if (g->D1p > kMaxD1) return false;
+  // the backward's per-wave tile and its weight stage share the CU's 160 KB of LDS
+  if (bwd_rows(*g) * kBwdLdsRowBytes + stage_bytes_of(bwd_prec_of(g->prec)) > 160 * 1024) return false;
   size_t o = 0;
   auto take = [&](size_t n) { size_t r = o; o += n; return r; };
-  g->fB = take((size_t)2 * (g->Cp / 4) * 64);          // [2 tiles][Cp/4 k-steps][64]
-  g->fW1 = take((size_t)kHT * (g->D1p / 4) * 64);
-  g->fW2 = take((size_t)kHT * (kHid / 4) * 64);
-  g->fW3 = take((size_t)1 * (kHid / 4) * 64);
-  g->fW3T = take((size_t)kHT * 4 * 64);                // rows 128, k 16
-  g->fW2T = take((size_t)kHT * (kHid / 4) * 64);
-  g->fW1T = take((size_t)(g->D1p / 16) * (kHid / 4) * 64);
-  g->fBT = take((size_t)(g->Cp / 16) * 8 * 64);        // rows Cp, k 32
+  g->fB = take(frag_floats(*g, 2, g->Cp, g->fpl));
+  g->fW1 = take(frag_floats(*g, kHT, g->D1p, g->fpl));
+  g->fW2 = take(frag_floats(*g, kHT, kHid, g->fpl));
+  g->fW3 = take(frag_floats(*g, 1, kHid, g->fpl));
+  g->fW3T = take(frag_floats(*g, kHT, g->kal, g->bpl));     // rows 128, k: the 3 (16) dz3 rows
+  g->fW2T = take(frag_floats(*g, kHT, kHid, g->bpl));
+  g->fW1T = take(frag_floats(*g, g->D1p / 16, kHid, g->bpl));
+  g->fBT = take(frag_floats(*g, g->Cp / 16, kMaxB, g->bpl));  // rows Cp, k 32
   g->b1 = take(kHid);
   g->b2 = take(kHid);
   g->b3 = take(16);
@@ -107,12 +151,17 @@ Ws make_ws(const Geo& g, int n) {
 }
 
 // ------------------------------------------------------------------------ packing
-// Fragment f of the A operand of a transposed layer: tile T (16 output rows), k-step s (lane group
-// g supplies input feature 16 (s / 4) + 4 g + s % 4), lane l = 16 g + j holds W'[16 T + j][k]
-// where W' = W (rows x cols, leading dimension ld) or its transpose.
+// FP32 fragments (planes 0): fragment f of the A operand of a transposed layer: tile T (16 output
+// rows), k-step s (lane group g supplies input feature 16 (s / 4) + 4 g + s % 4), lane l = 16 g + j
+// holds W'[16 T + j][k] where W' = W (rows x cols, leading dimension ld) or its transpose.
+// BF16 fragments (planes 1-3): per tile T and 32-feature k-unit u, `planes` consecutive 1 KB
+// planes; in each, lane l = 16 g + j holds 8 BF16 values of row 16 T + j (dword d: slots 2 d and
+// 2 d + 1), slot i being feature 32 u + 4 g + i (i < 4) or 32 u + 16 + 4 g + i - 4 (i >= 4), the
+// 16-bit slot map of the MFMA operands; plane p + 1 is the BF16 rounding of what planes 0..p leave.
 struct PackJob {
   const float* src;
-  int rows, cols, ld, trans, tiles, ksteps;
+  int rows, cols, ld, trans, tiles, ksteps;  // ksteps: FP32 k-steps, or k-units for BF16 planes
+  int planes;
   size_t off;
 };
 
@@ -126,22 +175,40 @@ struct PackArgs {
   float* out;
 };
 
+__device__ __forceinline__ float pack_src(const PackJob& jb, int r, int k) {
+  if (!jb.trans) return (r < jb.rows && k < jb.cols) ? jb.src[(size_t)r * jb.ld + k] : 0.f;
+  return (r < jb.cols && k < jb.rows) ? jb.src[(size_t)k * jb.ld + r] : 0.f;
+}
+
 __global__ __launch_bounds__(256) void pack_kernel(PackArgs a) {
   const int jid = blockIdx.y;
   if (jid < a.n_jobs) {
     const PackJob& jb = a.jobs[jid];
-    const int total = jb.tiles * jb.ksteps * 64;
-    for (int e = blockIdx.x * 256 + threadIdx.x; e < total; e += gridDim.x * 256) {
-      const int l = e & 63, s = (e >> 6) % jb.ksteps, T = (e >> 6) / jb.ksteps;
-      const int g = l >> 4, j = l & 15;
-      const int r = 16 * T + j, k = 16 * (s >> 2) + 4 * g + (s & 3);
-      float v = 0.f;
-      if (!jb.trans) {
-        if (r < jb.rows && k < jb.cols) v = jb.src[(size_t)r * jb.ld + k];
-      } else {
-        if (r < jb.cols && k < jb.rows) v = jb.src[(size_t)k * jb.ld + r];
+    if (jb.planes == 0) {
+      const int total = jb.tiles * jb.ksteps * 64;
+      for (int e = blockIdx.x * 256 + threadIdx.x; e < total; e += gridDim.x * 256) {
+        const int l = e & 63, s = (e >> 6) % jb.ksteps, T = (e >> 6) / jb.ksteps;
+        const int g = l >> 4, j = l & 15;
+        a.out[jb.off + e] = pack_src(jb, 16 * T + j, 16 * (s >> 2) + 4 * g + (s & 3));
       }
-      a.out[jb.off + e] = v;
+    } else {
+      // one dword (two BF16 slots) of one lane of one (tile, k-unit) per thread, all its planes
+      uint32_t* out = reinterpret_cast<uint32_t*>(a.out + jb.off);
+      const int total = jb.tiles * jb.ksteps * 256;
+      for (int e = blockIdx.x * 256 + threadIdx.x; e < total; e += gridDim.x * 256) {
+        const int d = e & 3, l = (e >> 2) & 63, u = (e >> 8) % jb.ksteps, T = (e >> 8) / jb.ksteps;
+        const int g = l >> 4, j = l & 15;
+        const int i0 = 2 * d;  // slots i0, i0 + 1 lie on the same side of slot 4
+        const int k0 = 32 * u + (i0 < 4 ? 4 * g + i0 : 16 + 4 * g + i0 - 4);
+        float v0 = pack_src(jb, 16 * T + j, k0), v1 = pack_src(jb, 16 * T + j, k0 + 1);
+        const size_t frag = (size_t)(T * jb.ksteps + u) * jb.planes;
+        for (int pl = 0; pl < jb.planes; ++pl) {
+          const uint32_t h = pack2<kBF16>(v0, v1);
+          out[(frag + pl) * 256 + l * 4 + d] = h;
+          v0 = v0 - __uint_as_float(h << 16);
+          v1 = v1 - __uint_as_float(h & 0xffff0000u);
+        }
+      }
     }
   } else if (blockIdx.x == 0) {
     for (int i = threadIdx.x; i < kHid; i += 256) {
@@ -153,29 +220,32 @@ __global__ __launch_bounds__(256) void pack_kernel(PackArgs a) {
 }
 
 // ------------------------------------------------------------------------ kernels
-// acc[T][cb] += sum over k-units u < nu of A(tile T, u) x fin(u, cb) for T < OUT: mlp::mma_tiles
-// for a reduction depth nu known only at run time.  The A fragments (1 KB per tile and k-unit, upr
-// k-units per packed tile row) are staged through LDS in chunks of kKC k-units, double-buffered;
-// fin may load or compute its operand (it is called once per k-unit and column block, so it must
-// not index register arrays); keep(u, b) then sees the k-unit's operands of all column blocks (the
-// training copies of the layer input are stored there).  Block-collective: every wave calls it with
-// the same arguments.
-template <int OUT, typename FIn, typename Keep>
+// acc[T][cb] += sum over the nt input tiles t of A(tile T, t) x fin(t, cb) for T < OUT:
+// mlp::mma_tiles for a reduction depth known only at run time (nt 16-feature tiles, a whole number
+// of k-units: one tile for FP32 operands, two for BF16).  The A fragments (1 KB per tile, k-unit
+// and BF16 plane, upr k-units per packed tile row) are staged through LDS in chunks of
+// mlp::kc_of<P>() k-units, double-buffered; fin may load or compute its operand (it is called once
+// per input tile and column block, so it must not index register arrays); keep(t, b) then sees
+// tile t's operands of all column blocks (the training copies of the layer input are stored
+// there).  Block-collective: every wave calls it with the same arguments.
+template <int P, int OUT, typename FIn, typename Keep>
 __device__ __forceinline__ void staged_layer_rt(f32x4 (&acc)[OUT][kCB], const float* __restrict__ A, int upr,
-                                                int nu, FIn&& fin, Keep&& keep, uint8_t* stage) {
-  constexpr int kKC = mlp::kKC;
-  constexpr int kHalf = kST * kKC * 1024;
+                                                int nt, FIn&& fin, Keep&& keep, uint8_t* stage) {
+  constexpr int KC = mlp::kc_of<P>(), NPL = split_planes<P>();
+  constexpr int kHalf = kST * KC * NPL * 1024;
   static_assert(OUT <= kST, "the stage buffer holds kST output tiles");
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   const uint8_t* Ab = reinterpret_cast<const uint8_t*>(A);
-  const int nch = (nu + kKC - 1) / kKC;
+  const int nu = P == kF32 ? nt : nt / 2;
+  const int nch = (nu + KC - 1) / KC;
   auto dma = [&](int c, uint8_t* buf) {
-    const int kn = nu - c * kKC < kKC ? nu - c * kKC : kKC;
-    for (int f = w; f < OUT * kn; f += kWaves) {
-      const int to = f / kn, kc = f - (f / kn) * kn;
-      const int frag = __builtin_amdgcn_readfirstlane(to * upr + c * kKC + kc);
-      const int dst = __builtin_amdgcn_readfirstlane((to * kKC + kc) * 1024);
+    const int kn = nu - c * KC < KC ? nu - c * KC : KC;
+    for (int f = w; f < OUT * kn * NPL; f += kWaves) {
+      const int fk = f / NPL, pl = f - fk * NPL;
+      const int to = fk / kn, kc = fk - (fk / kn) * kn;
+      const int frag = __builtin_amdgcn_readfirstlane((to * upr + c * KC + kc) * NPL + pl);
+      const int dst = __builtin_amdgcn_readfirstlane(((to * KC + kc) * NPL + pl) * 1024);
       lds_dma_1k(Ab + (size_t)frag * 1024, buf + dst);
     }
   };
@@ -185,32 +255,79 @@ __device__ __forceinline__ void staged_layer_rt(f32x4 (&acc)[OUT][kCB], const fl
   for (int c = 0; c < nch; ++c) {
     const uint8_t* cur = stage + (c & 1) * kHalf;
     if (c + 1 < nch) dma(c + 1, stage + ((c + 1) & 1) * kHalf);
-    const int kn = nu - c * kKC < kKC ? nu - c * kKC : kKC;
+    const int kn = nu - c * KC < KC ? nu - c * KC : KC;
     for (int kc = 0; kc < kn; ++kc) {
-      f32x4 b[kCB];
+      const int u = c * KC + kc;
+      if constexpr (P == kF32) {
+        f32x4 b[kCB];
 #pragma unroll
-      for (int cb = 0; cb < kCB; ++cb) b[cb] = fin(c * kKC + kc, cb);
-      keep(c * kKC + kc, b);
+        for (int cb = 0; cb < kCB; ++cb) b[cb] = fin(u, cb);
+        keep(u, b);
 #pragma unroll
-      for (int r = 0; r < 4; ++r)
+        for (int r = 0; r < 4; ++r)
 #pragma unroll
-        for (int T = 0; T < OUT; ++T) {
-          const float a = *reinterpret_cast<const float*>(cur + (T * kKC + kc) * 1024 + (r * 64 + lane) * 4);
+          for (int T = 0; T < OUT; ++T) {
+            const float a = *reinterpret_cast<const float*>(cur + (T * KC + kc) * 1024 + (r * 64 + lane) * 4);
 #pragma unroll
-          for (int cb = 0; cb < kCB; ++cb) acc[T][cb] = mfma32(a, b[cb][r], acc[T][cb]);
+            for (int cb = 0; cb < kCB; ++cb) acc[T][cb] = mfma32(a, b[cb][r], acc[T][cb]);
+          }
+      } else {
+        // input tiles 2u, 2u + 1 -> NPL BF16 B operands per column block (mlp::mma_tiles' split)
+        f32x4 x0[kCB], x1[kCB];
+#pragma unroll
+        for (int cb = 0; cb < kCB; ++cb) {
+          x0[cb] = fin(2 * u, cb);
+          x1[cb] = fin(2 * u + 1, cb);
         }
+        keep(2 * u, x0);
+        keep(2 * u + 1, x1);
+        u32x4 b[NPL][kCB];
+#pragma unroll
+        for (int cb = 0; cb < kCB; ++cb) {
+          uint32_t q0[NPL], q1[NPL], q2[NPL], q3[NPL];
+          split_pair<NPL>(x0[cb][0], x0[cb][1], q0);
+          split_pair<NPL>(x0[cb][2], x0[cb][3], q1);
+          split_pair<NPL>(x1[cb][0], x1[cb][1], q2);
+          split_pair<NPL>(x1[cb][2], x1[cb][3], q3);
+#pragma unroll
+          for (int i = 0; i < NPL; ++i) b[i][cb] = u32x4{q0[i], q1[i], q2[i], q3[i]};
+        }
+        // output tiles in pairs (four independent accumulators between dependent MFMAs), the
+        // plane products smallest first
+#pragma unroll
+        for (int T0 = 0; T0 < OUT; T0 += 2) {
+          u32x4 av[2][NPL];
+#pragma unroll
+          for (int q = 0; q < 2; ++q)
+            if (T0 + q < OUT)
+#pragma unroll
+              for (int pl = 0; pl < NPL; ++pl)
+                av[q][pl] = *reinterpret_cast<const u32x4*>(cur + (((T0 + q) * KC + kc) * NPL + pl) * 1024 + lane * 16);
+#pragma unroll
+          for (int sum = NPL - 1; sum >= 0; --sum)
+#pragma unroll
+            for (int i = 0; i <= sum; ++i)
+#pragma unroll
+              for (int q = 0; q < 2; ++q)
+                if (T0 + q < OUT)
+#pragma unroll
+                  for (int cb = 0; cb < kCB; ++cb)
+                    acc[T0 + q][cb] = mfma16<kBF16>(av[q][i], b[sum - i][cb], acc[T0 + q][cb]);
+        }
+      }
     }
     wait_vmem();
     __syncthreads();
   }
 }
 
-// acc += A(tiles of the packed block at A) x in with a compile-time depth of NU k-units, through
-// mlp::mma_tiles on the kST-tile stage.
-template <int OUT, int NU, typename FIn>
+// acc += A(tiles of the packed block at A) x in with a compile-time depth of NT 16-feature input
+// tiles (a whole number of k-units), through mlp::mma_tiles on the kST-tile stage.
+template <int P, int OUT, int NT, typename FIn>
 __device__ __forceinline__ void staged_layer(f32x4 (&acc)[OUT][kCB], const float* __restrict__ A, int upr,
                                              FIn&& fin, uint8_t* stage) {
-  mlp::mma_tiles<kF32, NU, OUT, 0, kST>(acc, reinterpret_cast<const uint8_t*>(A), upr, 0, fin, stage);
+  static_assert(P == kF32 || NT % 2 == 0, "BF16 k-units hold two input tiles");
+  mlp::mma_tiles<P, NT, OUT, 0, kST>(acc, reinterpret_cast<const uint8_t*>(A), upr, 0, fin, stage);
 }
 
 // Positional-encoding input i of the first layer for one point, from its basis outputs F (LDS
@@ -288,8 +405,10 @@ __device__ __forceinline__ void store_fm_row32(uint8_t* base, size_t row_off, in
         o, reinterpret_cast<f32x2_t*>(reinterpret_cast<float*>(base + row_off) + (size_t)row * ld + p0 + 2 * j));
 }
 
+template <int P>
 __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(2, 2))) void color_fwd_kernel(FwdArgs a) {
-  __shared__ __attribute__((aligned(16))) uint8_t stage[kStage];
+  __shared__ __attribute__((aligned(16))) uint8_t stage[stage_bytes<P>()];
+  constexpr int KU = kunit<P>();
   __shared__ float sF[kWaves][kMaxB][kPW + 1];
   __shared__ float sD[kWaves][3][kPW];
   const Geo& g = a.g;
@@ -319,7 +438,7 @@ __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(2, 
 #pragma unroll
     for (int r = 0; r < 4; ++r) store_fm_row32(a.ws, a.w.appT, a.w.ld, c + r, p0, b[0][r], b[1][r], c < g.C);
   };
-  staged_layer_rt<2>(accF, a.pk + g.fB, g.Cp / 16, g.Cp / 16, fin_app, keep_app, stage);
+  staged_layer_rt<P, 2>(accF, a.pk + g.fB, g.Cp / KU, g.Cp / 16, fin_app, keep_app, stage);
   // F and the view directions to LDS (and F feature-major for the backward)
 #pragma unroll
   for (int T = 0; T < 2; ++T)
@@ -372,7 +491,7 @@ __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(2, 
       store_fm_row32(a.ws, a.w.x1, a.w.ld, i, p0, b[0][r], b[1][r], i < g.D1);
     }
   };
-  staged_layer_rt<kHT>(h1, a.pk + g.fW1, g.D1p / 16, g.D1p / 16, fin_pe, keep_pe, stage);
+  staged_layer_rt<P, kHT>(h1, a.pk + g.fW1, g.D1p / KU, g.D1p / 16, fin_pe, keep_pe, stage);
 #pragma unroll
   for (int T = 0; T < kHT; ++T)
 #pragma unroll
@@ -389,7 +508,7 @@ __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(2, 
     for (int cb = 0; cb < kCB; ++cb)
 #pragma unroll
       for (int r = 0; r < 4; ++r) h2[T][cb][r] = a.pk[g.b2 + 16 * T + 4 * gq + r];
-  staged_layer<kHT, kHT>(h2, a.pk + g.fW2, kHid / 16, [&](int u, int cb) { return h1[u][cb]; }, stage);
+  staged_layer<P, kHT, kHT>(h2, a.pk + g.fW2, kHid / KU, [&](int u, int cb) { return h1[u][cb]; }, stage);
 #pragma unroll
   for (int T = 0; T < kHT; ++T)
 #pragma unroll
@@ -404,7 +523,7 @@ __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(2, 
   for (int cb = 0; cb < kCB; ++cb)
 #pragma unroll
     for (int r = 0; r < 4; ++r) o[0][cb][r] = a.pk[g.b3 + 4 * gq + r];
-  staged_layer<1, kHT>(o, a.pk + g.fW3, kHid / 16, [&](int u, int cb) { return h2[u][cb]; }, stage);
+  staged_layer<P, 1, kHT>(o, a.pk + g.fW3, kHid / KU, [&](int u, int cb) { return h2[u][cb]; }, stage);
   if (gq == 0) {
 #pragma unroll
     for (int cb = 0; cb < kCB; ++cb) {
@@ -436,11 +555,11 @@ __device__ __forceinline__ float load_fm(const uint8_t* base, size_t row_off, in
   return reinterpret_cast<const float*>(base + row_off)[(size_t)row * ld + p];
 }
 
-// Rows of the per-wave LDS tile of the backward: d x1 (D1p rows), later d F (kMaxB rows).
-__host__ __device__ inline int bwd_rows(const Geo& g) { return g.D1p > kMaxB ? g.D1p : kMaxB; }
-
+template <int P>
 __global__ __launch_bounds__(64 * kWaves) void color_bwd_kernel(BwdArgs a) {
-  __shared__ __attribute__((aligned(16))) uint8_t stage[kStage];
+  __shared__ __attribute__((aligned(16))) uint8_t stage[stage_bytes<P>()];
+  constexpr int KU = kunit<P>();
+  constexpr int NPL = split_planes<P>();
   extern __shared__ float lds_dyn[];
   const Geo& g = a.g;
   const int n = active_rows(a.n, a.count);
@@ -475,7 +594,10 @@ __global__ __launch_bounds__(64 * kWaves) void color_bwd_kernel(BwdArgs a) {
   for (int T = 0; T < kHT; ++T)
 #pragma unroll
     for (int cb = 0; cb < kCB; ++cb) dz2[T][cb] = f32x4{0.f, 0.f, 0.f, 0.f};
-  staged_layer<kHT, 1>(dz2, a.pk + g.fW3T, 1, [&](int u, int cb) { return dz3[u][cb]; }, stage);
+  // k: the 16 dz3 rows (one FP32 k-unit; BF16: one 32-row k-unit, rows 16-31 zero)
+  staged_layer<P, kHT, KU / 16>(dz2, a.pk + g.fW3T, 1, [&](int u, int cb) {
+    return u == 0 ? dz3[0][cb] : f32x4{0.f, 0.f, 0.f, 0.f};
+  }, stage);
 #pragma unroll
   for (int T = 0; T < kHT; ++T)
 #pragma unroll
@@ -496,7 +618,7 @@ __global__ __launch_bounds__(64 * kWaves) void color_bwd_kernel(BwdArgs a) {
   for (int T = 0; T < kHT; ++T)
 #pragma unroll
     for (int cb = 0; cb < kCB; ++cb) dz1[T][cb] = f32x4{0.f, 0.f, 0.f, 0.f};
-  staged_layer<kHT, kHT>(dz1, a.pk + g.fW2T, kHid / 16, [&](int u, int cb) { return dz2[u][cb]; }, stage);
+  staged_layer<P, kHT, kHT>(dz1, a.pk + g.fW2T, kHid / KU, [&](int u, int cb) { return dz2[u][cb]; }, stage);
 #pragma unroll
   for (int T = 0; T < kHT; ++T)
 #pragma unroll
@@ -520,7 +642,7 @@ __global__ __launch_bounds__(64 * kWaves) void color_bwd_kernel(BwdArgs a) {
     for (int T = 0; T < OUT; ++T)
 #pragma unroll
       for (int cb = 0; cb < kCB; ++cb) dx[T][cb] = f32x4{0.f, 0.f, 0.f, 0.f};
-    staged_layer<OUT, kHT>(dx, a.pk + g.fW1T + (size_t)T0 * (kHid / 4) * 64, kHid / 16, fin_dz1, stage);
+    staged_layer<P, OUT, kHT>(dx, a.pk + g.fW1T + (size_t)T0 * (kHid / KU) * NPL * 256, kHid / KU, fin_dz1, stage);
 #pragma unroll
     for (int T = 0; T < OUT; ++T)
 #pragma unroll
@@ -579,7 +701,7 @@ __global__ __launch_bounds__(64 * kWaves) void color_bwd_kernel(BwdArgs a) {
     for (int T = 0; T < OUT; ++T)
 #pragma unroll
       for (int cb = 0; cb < kCB; ++cb) da[T][cb] = f32x4{0.f, 0.f, 0.f, 0.f};
-    staged_layer<OUT, 2>(da, a.pk + g.fBT + (size_t)T0 * 8 * 64, 2, fin_dF, stage);
+    staged_layer<P, OUT, 2>(da, a.pk + g.fBT + (size_t)T0 * (kMaxB / KU) * NPL * 256, kMaxB / KU, fin_dF, stage);
 #pragma unroll
     for (int T = 0; T < OUT; ++T) {
       const int c = 16 * (T0 + T) + 4 * gq;
@@ -610,6 +732,13 @@ namespace {
 // kernel's limit is raised once per device, to what the largest supported geometry needs, so that
 // every later call (a call under stream capture included, where no such runtime call belongs)
 // launches without touching it.
+template <int P>
+hipError_t raise_bwd_lds(int max_rows) {
+  const int bytes = max_rows * kBwdLdsRowBytes;
+  return hipFuncSetAttribute(reinterpret_cast<const void*>(&color_bwd_kernel<P>),
+                             hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+}
+
 int raise_bwd_lds_limit() {
   constexpr int kMaxDevices = 64;
   static std::mutex mutex;
@@ -618,10 +747,15 @@ int raise_bwd_lds_limit() {
   if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return SRF_EHIP;
   std::lock_guard<std::mutex> lock(mutex);
   if (!raised[dev]) {
-    constexpr int kMaxLds = kWaves * (kMaxD1 > kMaxB ? kMaxD1 : kMaxB) * (kPW + 1) * (int)sizeof(float);
-    static_assert(kMaxLds + kStage <= 160 * 1024, "backward LDS within a CU's 160 KB");
-    if (hipFuncSetAttribute(reinterpret_cast<const void*>(&color_bwd_kernel),
-                            hipFuncAttributeMaxDynamicSharedMemorySize, kMaxLds) != hipSuccess)
+    // per precision: the rows of the largest geometry make_geo accepts beside that stage
+    auto rows = [](int stage) {
+      const int r = (160 * 1024 - stage) / kBwdLdsRowBytes;
+      const int cap = kMaxD1 > kMaxB ? kMaxD1 : kMaxB;
+      return r < cap ? r : cap;
+    };
+    if (raise_bwd_lds<kF32>(rows(stage_bytes<kF32>())) != hipSuccess ||
+        raise_bwd_lds<kBFS2>(rows(stage_bytes<kBFS2>())) != hipSuccess ||
+        raise_bwd_lds<kBFS1>(rows(stage_bytes<kBFS1>())) != hipSuccess)
       return SRF_EHIP;
     raised[dev] = true;
   }
@@ -648,7 +782,8 @@ extern "C" int srf_color_pack(const srf_color_desc* desc, const float* basis, co
   if (!basis || !w1 || !b1 || !w2 || !b2 || !w3 || !b3 || !packed) return SRF_EINVAL;
   PackArgs a;
   memset(&a, 0, sizeof(a));
-  auto job = [&](const float* src, int rows, int cols, int trans, int tiles, int ksteps, size_t off) {
+  // ksteps: FP32 k-steps of 4 features, or the 32-feature k-units of a BF16 image
+  auto job = [&](const float* src, int rows, int cols, int trans, int tiles, int K, int planes, size_t off) {
     PackJob& jb = a.jobs[a.n_jobs++];
     jb.src = src;
     jb.rows = rows;
@@ -656,19 +791,20 @@ extern "C" int srf_color_pack(const srf_color_desc* desc, const float* basis, co
     jb.ld = cols;
     jb.trans = trans;
     jb.tiles = tiles;
-    jb.ksteps = ksteps;
+    jb.ksteps = g.prec == SRF_PREC_F32 ? K / 4 : K / 32;
+    jb.planes = g.prec == SRF_PREC_F32 ? 0 : planes;
     jb.off = off;
   };
   // forward fragments: W [out, in] as A (rows = out)
-  job(basis, g.NB, g.C, 0, 2, g.Cp / 4, g.fB);
-  job(w1, kHid, g.D1, 0, kHT, g.D1p / 4, g.fW1);
-  job(w2, kHid, kHid, 0, kHT, kHid / 4, g.fW2);
-  job(w3, 3, kHid, 0, 1, kHid / 4, g.fW3);
+  job(basis, g.NB, g.C, 0, 2, g.Cp, g.fpl, g.fB);
+  job(w1, kHid, g.D1, 0, kHT, g.D1p, g.fpl, g.fW1);
+  job(w2, kHid, kHid, 0, kHT, kHid, g.fpl, g.fW2);
+  job(w3, 3, kHid, 0, 1, kHid, g.fpl, g.fW3);
   // backward fragments: W^T as A (rows = in)
-  job(w3, 3, kHid, 1, kHT, 4, g.fW3T);
-  job(w2, kHid, kHid, 1, kHT, kHid / 4, g.fW2T);
-  job(w1, kHid, g.D1, 1, g.D1p / 16, kHid / 4, g.fW1T);
-  job(basis, g.NB, g.C, 1, g.Cp / 16, 8, g.fBT);
+  job(w3, 3, kHid, 1, kHT, g.kal, g.bpl, g.fW3T);
+  job(w2, kHid, kHid, 1, kHT, kHid, g.bpl, g.fW2T);
+  job(w1, kHid, g.D1, 1, g.D1p / 16, kHid, g.bpl, g.fW1T);
+  job(basis, g.NB, g.C, 1, g.Cp / 16, kMaxB, g.bpl, g.fBT);
   a.b1 = b1;
   a.b2 = b2;
   a.b3 = b3;
@@ -703,7 +839,11 @@ extern "C" int srf_color_forward(const srf_color_desc* desc, const void* packed,
   a.rgb_at_idx = rgb_at_idx != 0;
   a.ws = reinterpret_cast<uint8_t*>(workspace);
   const int blocks = a.w.cols / (kPW * kWaves);
-  hipLaunchKernelGGL(color_fwd_kernel, dim3(blocks), dim3(64 * kWaves), 0, as_stream(stream), a);
+  switch (fwd_prec_of(g.prec)) {
+    case kBFS3: hipLaunchKernelGGL(color_fwd_kernel<kBFS3>, dim3(blocks), dim3(64 * kWaves), 0, as_stream(stream), a); break;
+    case kBFS1: hipLaunchKernelGGL(color_fwd_kernel<kBFS1>, dim3(blocks), dim3(64 * kWaves), 0, as_stream(stream), a); break;
+    default: hipLaunchKernelGGL(color_fwd_kernel<kF32>, dim3(blocks), dim3(64 * kWaves), 0, as_stream(stream), a); break;
+  }
   return launch_status();
 }
 
@@ -733,7 +873,12 @@ extern "C" int srf_color_backward(const srf_color_desc* desc, const void* packed
   b.ws = reinterpret_cast<uint8_t*>(workspace);
   const size_t lds = (size_t)kWaves * bwd_rows(g) * (kPW + 1) * sizeof(float);
   if (const int st = raise_bwd_lds_limit(); st != SRF_OK) return st;
-  hipLaunchKernelGGL(color_bwd_kernel, dim3(b.w.cols / (kPW * kWaves)), dim3(64 * kWaves), lds, s, b);
+  const dim3 grid(b.w.cols / (kPW * kWaves));
+  switch (bwd_prec_of(g.prec)) {
+    case kBFS2: hipLaunchKernelGGL(color_bwd_kernel<kBFS2>, grid, dim3(64 * kWaves), lds, s, b); break;
+    case kBFS1: hipLaunchKernelGGL(color_bwd_kernel<kBFS1>, grid, dim3(64 * kWaves), lds, s, b); break;
+    default: hipLaunchKernelGGL(color_bwd_kernel<kF32>, grid, dim3(64 * kWaves), lds, s, b); break;
+  }
   int st = launch_status();
   if (st != SRF_OK) return st;
   // weight gradients: the grouped dW GEMM over the stored dZ / input rows

@@ -31,10 +31,14 @@ typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
 // the BF16 rounding of what the previous planes leave, and a product is the sum of the plane
 // products x_i y_j with i + j < planes (16x16x32 BF16 MFMAs into one FP32 accumulator).  Three
 // planes (six products) carry 24 significand bits, as FP32; two planes (three products) 16.
-enum Prec { kF32 = 0, kBF16 = 1, kF16 = 2, kBFS3 = 3, kBFS2 = 4 };
+// kBFS1: one plane, i.e. BF16 operands rounded from FP32 activations and weights (FP32 storage).
+enum Prec { kF32 = 0, kBF16 = 1, kF16 = 2, kBFS3 = 3, kBFS2 = 4, kBFS1 = 5 };
 
 template <int P>
-__host__ __device__ constexpr bool f32_layout() { return P == kF32 || P == kBFS3 || P == kBFS2; }
+__host__ __device__ constexpr bool f32_layout() { return P == kF32 || P == kBFS3 || P == kBFS2 || P == kBFS1; }
+// BF16 operand planes of the FP32-layout precisions
+template <int P>
+__host__ __device__ constexpr bool split_prec() { return P == kBFS3 || P == kBFS2 || P == kBFS1; }
 template <int P>
 __host__ __device__ constexpr int split_planes() { return P == kBFS3 ? 3 : (P == kBFS2 ? 2 : 1); }
 

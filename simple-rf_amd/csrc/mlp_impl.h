@@ -207,6 +207,8 @@ template <>
 struct Tr<kBFS3> : Tr<kF32> {};
 template <>
 struct Tr<kBFS2> : Tr<kF32> {};
+template <>
+struct Tr<kBFS1> : Tr<kF32> {};
 
 template <int P>
 __device__ __forceinline__ typename Tr<P>::Act to_act(const f32x4& v) {
@@ -538,7 +540,7 @@ __device__ __forceinline__ void mma_tiles(f32x4 (&acc)[OUT][Tr<P>::CB], const ui
             for (int cb = 0; cb < CB; ++cb) acc[to][cb] = mfma32(a, b[cb][r], acc[to][cb]);
           }
         }
-      } else if constexpr (NPL > 1) {
+      } else if constexpr (split_prec<P>()) {
         // Split BF16: the FP32 input tiles 2u, 2u + 1 of each column block become NPL BF16 B
         // operands (slot order of the 16-bit map), the weight fragments are already split into
         // NPL planes (stage layout: plane pl of (to, kc) at ((to KC + kc) NPL + pl) KB), and each

@@ -27,6 +27,10 @@ The periodic model modifications (alpha-mask update with its 3x3x3 max pool, shr
 occupied box, factor upsampling, optimiser reconfiguration; :820-944) run on
 srf_dense_grid_points, the density gather, srf_alpha_mask_build and srf_factor_upsample.
 Options outside the path raise NotImplementedError: visibility prediction.
+
+``configs['model']['color_precision']`` in {'fp32', 'bf16_split', 'bf16'} (default 'fp32', the
+reference's arithmetic) selects the operands of the colour network's MFMA products: split BF16
+keeps FP32-level accuracy (three planes forward, two backward), 'bf16' rounds them to BF16.
 """
 from __future__ import annotations
 
@@ -465,7 +469,8 @@ class LowRankTensor(torch.nn.Module):
         self.color_spec = hip_color.ColorSpec(
             n_features=sum(tc['num_components_color']), basis_dim=tc['features_dimension_color'],
             features_pe=tc['features_positional_encoding_degree'], use_view_dirs=bool(tc['view_dependent_color']),
-            views_pe=tc['views_positional_encoding_degree'], hidden=tc['num_units_color_predictor'])
+            views_pe=tc['views_positional_encoding_degree'], hidden=tc['num_units_color_predictor'],
+            prec=self.configs['model'].get('color_precision', 'fp32'))
         if not self.color_spec.supported():
             raise NotImplementedError(f'colour network {self.color_spec} is outside the HIP kernels')
 

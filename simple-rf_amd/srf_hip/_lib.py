@@ -11,7 +11,7 @@ from pathlib import Path
 import torch
 
 LIB_PATH = Path(__file__).resolve().parent / 'libsrf_hip.so'
-ABI_VERSION = 2
+ABI_VERSION = 3
 
 _P = C.c_void_p
 _I = C.c_int

@@ -14,7 +14,12 @@ from ._checks import dev_tensor, zero_grads
 
 class _Desc(C.Structure):
     _fields_ = [('n_features', C.c_int), ('basis_dim', C.c_int), ('features_pe', C.c_int),
-                ('use_view_dirs', C.c_int), ('views_pe', C.c_int), ('hidden', C.c_int)]
+                ('use_view_dirs', C.c_int), ('views_pe', C.c_int), ('hidden', C.c_int), ('prec', C.c_int)]
+
+
+# operand precision of the colour network's products (activations, storage, accumulation and the
+# weight gradients stay FP32): FP32, split BF16 (three planes forward, two backward), plain BF16
+PRECISIONS = {'fp32': 0, 'bf16': 1, 'bf16_split': 3}   # SRF_PREC_F32 / _BF16 / _BF16_SPLIT
 
 
 @dataclass(frozen=True)
@@ -25,10 +30,13 @@ class ColorSpec:
     use_view_dirs: bool = True
     views_pe: int = 0        # views_positional_encoding_degree
     hidden: int = 128        # num_units_color_predictor
+    prec: str = 'fp32'       # a key of PRECISIONS
 
     def desc(self) -> _Desc:
+        if self.prec not in PRECISIONS:
+            raise _lib.SrfError(f'colour precision {self.prec!r} is not one of {sorted(PRECISIONS)}')
         return _Desc(self.n_features, self.basis_dim, self.features_pe, int(self.use_view_dirs), self.views_pe,
-                     self.hidden)
+                     self.hidden, PRECISIONS[self.prec])
 
     def supported(self) -> bool:
         d = self.desc()

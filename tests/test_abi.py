@@ -90,3 +90,31 @@ def test_autotuner_cache_round_trip(tmp_path):
         assert not autotune.Autotuner(path).apply('other')
     finally:
         autotune.reset()
+
+
+def test_color_descriptor_precisions():
+    """Packed-weight sizes of the 300^2 colour network (144 features, features PE 2, first layer
+    138 wide) in each operand precision: FP32 fragments pad reductions to 16 features; the BF16
+    modes pad to 32 and hold one 1 KB fragment per BF16 plane (split: three forward, two
+    backward).  Unsupported precisions and split geometries past the backward's LDS are rejected."""
+    from srf_hip import color
+    base = dict(n_features=144, features_pe=2)
+    nb = lambda **kw: _lib.lib().srf_color_packed_bytes(ctypes.byref(color.ColorSpec(**{**base, **kw}).desc()))
+    kb = 256  # floats per 1 KB fragment
+    bias = 128 + 128 + 16
+    # F32: basis 2 x 9, W1 8 x 9, W2 8 x 8, W3 1 x 8, W3^T 8 x 1, W2^T 8 x 8, W1^T 9 x 8, B^T 9 x 2 fragments
+    f32 = (2 * 9 + 8 * 9 + 8 * 8 + 8 + 8 + 8 * 8 + 9 * 8 + 9 * 2) * kb + bias
+    assert nb(prec='fp32') == 4 * f32
+    # BF16 (units of 32 features): forward basis 2 x 5, W1 8 x 5, W2 8 x 4, W3 1 x 4; backward W3^T 8 x 1,
+    # W2^T 8 x 4, W1^T 10 x 4, B^T 10 x 1
+    fwd, bwd = 2 * 5 + 8 * 5 + 8 * 4 + 4, 8 + 8 * 4 + 10 * 4 + 10
+    assert nb(prec='bf16') == 4 * ((fwd + bwd) * kb + bias)
+    assert nb(prec='bf16_split') == 4 * ((3 * fwd + 2 * bwd) * kb + bias)
+    # first layer 27 x 7 + 3 = 192 wide: within FP32's limit, past split BF16's
+    assert nb(prec='fp32', features_pe=3) > 0 and nb(prec='bf16', features_pe=3) > 0
+    assert nb(prec='bf16_split', features_pe=3) == 0
+    d = color.ColorSpec(**base).desc()
+    d.prec = 2  # SRF_PREC_F16: no colour kernels
+    assert _lib.lib().srf_color_packed_bytes(ctypes.byref(d)) == 0
+    with pytest.raises(_lib.SrfError):
+        color.ColorSpec(**base, prec='fp16').desc()

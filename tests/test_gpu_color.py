@@ -1,5 +1,19 @@
 """GPU parity of the fused TensoRF colour network (basis + PE + MLP, forward and backward) against
-the oracle (FP64 autograd of basis matrix + MlpFeaturesColorPredictor)."""
+the oracle (FP64 autograd of basis matrix + MlpFeaturesColorPredictor), in each operand precision.
+
+FP32 operands and split BF16 (three planes forward, two backward) hold rgb within 1e-5 of FP64 and
+every gradient within 1e-4 of its largest entry (the FP32 bar).  Plain BF16 operands (8 significand
+bits) hold rgb within 2e-3 of FP64, the bar for BF16 renders.  Their gradients are not near FP64's:
+the BF16 forward moves pre-activations near zero across the ReLU, and each such unit sends its whole
+gradient share in one precision and none in the other (measured 3-8 % in norm with random output
+gradients).  They are pinned instead against bf16_model, an FP64 evaluation of the same network
+that rounds the operands of every product to BF16 where the kernels do: rgb within 1e-4 and every
+gradient within 1e-2 in norm.  The kernels' FP32 sums and sines and the model's FP64 ones put a few
+values on opposite sides of a BF16 rounding midpoint; each such value moves by one BF16 step (rgb
+moves by up to ~4e-5) and can move a downstream pre-activation across zero.  Measured: gradients
+within 1e-6 in norm where no ReLU moved, 3e-3 with one moved unit (the features PE L=2 case); a
+wrong fragment or slot layout is off by O(1).
+"""
 import pytest
 import torch
 import torch.nn.functional as F
@@ -36,8 +50,61 @@ def oracle(c, prm, app, dirs):
     return ot.color_mlp_forward(mlp, feats, dirs if c['view'] else None, c['Lf'], c['Lv'])
 
 
+TOL = {'fp32': 1e-5, 'bf16_split': 1e-5, 'bf16': 2e-3}  # rgb against FP64
+
+
+def _bf(t):
+    return t.to(torch.bfloat16).to(torch.float64)
+
+
+def _pe(x, L):
+    out = [x]
+    for k in range(L):
+        out += [torch.sin(x * 2.0 ** k), torch.cos(x * 2.0 ** k)]
+    return torch.cat(out, -1)
+
+
+def bf16_model(c, prm, app, dirs, gout):
+    """FP64 forward and backward of the colour network with the operands of every product rounded to
+    BF16 (round to nearest even) as the BF16 kernels round them: the basis product, the three layers
+    and, in the backward chain, W3^T dz3, W2^T dz2, W1^T dz1 and B^T dF.  Activations, the ReLU
+    decisions, the positional encoding and its backward, and the weight gradients (sums over the
+    points of FP32 dZ x input) are unrounded.  Returns rgb, d_app and the parameter gradients."""
+    basis, w1, b1, w2, b2, w3, b3 = [t.double() for t in prm]
+    app, gout = app.double(), gout.double()
+    nb = basis.shape[0]
+    feats = _bf(app) @ _bf(basis).T
+    x1 = _pe(feats, c['Lf'])
+    if c['view']:
+        x1 = torch.cat([x1, _pe(dirs.double(), c['Lv'])], -1)
+    h1 = (_bf(x1) @ _bf(w1).T + b1).clamp_min(0)
+    h2 = (_bf(h1) @ _bf(w2).T + b2).clamp_min(0)
+    rgb = torch.sigmoid(_bf(h2) @ _bf(w3).T + b3)
+    dz3 = gout * rgb * (1 - rgb)
+    dz2 = (_bf(dz3) @ _bf(w3)) * (h2 > 0)
+    dz1 = (_bf(dz2) @ _bf(w2)) * (h1 > 0)
+    dx1 = _bf(dz1) @ _bf(w1)
+    dF = dx1[:, :nb].clone()
+    for k in range(c['Lf']):
+        fr = 2.0 ** k
+        dF += fr * (dx1[:, (2 * k + 1) * nb:(2 * k + 2) * nb] * torch.cos(feats * fr)
+                    - dx1[:, (2 * k + 2) * nb:(2 * k + 3) * nb] * torch.sin(feats * fr))
+    d_app = _bf(dF) @ _bf(basis)
+    grads = [dF.T @ app, dz1.T @ x1, dz1.sum(0), dz2.T @ h1, dz2.sum(0), dz3.T @ h2, dz3.sum(0)]
+    return rgb, d_app, grads
+
+
+def _rel(a, b):
+    return ((a.cpu().double() - b).abs().max() / b.abs().max()).item()
+
+
+def _rel_norm(a, b):
+    return ((a.cpu().double() - b).norm() / b.norm()).item()
+
+
+@pytest.mark.parametrize('prec', list(TOL))
 @pytest.mark.parametrize('case', list(CASES))
-def test_color_forward_backward(cuda, case):
+def test_color_forward_backward(cuda, case, prec):
     c = CASES[case]
     g = torch.Generator().manual_seed(1)
     n_rays, S = 40, 37
@@ -52,28 +119,42 @@ def test_color_forward_backward(cuda, case):
     a64 = app.double().requires_grad_(True)
     ref = oracle(c, p64, a64, dirs.double())
     (ref * gout.double()).sum().backward()
-    spec = color.ColorSpec(n_features=c['C'], features_pe=c['Lf'], use_view_dirs=c['view'], views_pe=c['Lv'])
+    spec = color.ColorSpec(n_features=c['C'], features_pe=c['Lf'], use_view_dirs=c['view'], views_pe=c['Lv'],
+                           prec=prec)
     pd = [t.to(cuda).requires_grad_(True) for t in prm]
     ad = app.to(cuda).requires_grad_(True)
     rgb = color.color_forward(spec, ad, pd[0], pd[1:], view_dirs=vd.to(cuda), idx=idx.to(cuda), n_samples=S)
     (rgb * gout.to(cuda)).sum().backward()
     torch.cuda.synchronize()
+    names = ['basis', 'w1', 'b1', 'w2', 'b2', 'w3', 'b3']
     err = (rgb.detach().cpu().double() - ref.detach()).abs().max().item()
-    assert err < 1e-5, err
-    ea = ((ad.grad.cpu().double() - a64.grad).abs().max() / a64.grad.abs().max()).item()
-    assert ea < 1e-4, ea
-    for name, a, b in zip(['basis', 'w1', 'b1', 'w2', 'b2', 'w3', 'b3'], pd, p64):
-        e = ((a.grad.cpu().double() - b.grad).abs().max() / b.grad.abs().max()).item()
-        assert e < 1e-4, (name, e)
+    ea = _rel(ad.grad, a64.grad)
+    eg = {k: _rel(a.grad, b.grad) for k, a, b in zip(names, pd, p64)}
+    print(f'colour {case} {prec} vs FP64: rgb {err:.2e}, d_app {ea:.2e}, parameters',
+          ' '.join(f'{k} {v:.1e}' for k, v in eg.items()))
+    assert err < TOL[prec], err
+    tol = 1e-4
+    if prec == 'bf16':
+        rgb_m, d_app_m, grads_m = bf16_model(c, prm, app, dirs, gout)
+        err_m = (rgb.detach().cpu().double() - rgb_m).abs().max().item()
+        ea, eg = _rel_norm(ad.grad, d_app_m), {k: _rel_norm(a.grad, b) for k, a, b in zip(names, pd, grads_m)}
+        print(f'colour {case} bf16 vs the BF16-rounding model: rgb {err_m:.2e}, in norm: d_app {ea:.2e}, '
+              'parameters', ' '.join(f'{k} {v:.1e}' for k, v in eg.items()))
+        assert err_m < 1e-4, err_m
+        tol = 1e-2
+    assert ea < tol, ea
+    for name, e in eg.items():
+        assert e < tol, (name, e)
 
 
-def test_color_inference_matches_training(cuda):
+@pytest.mark.parametrize('prec', list(TOL))
+def test_color_inference_matches_training(cuda, prec):
     c = CASES['vm300']
     g = torch.Generator().manual_seed(3)
     app = torch.randn(700, c['C'], generator=g).to(cuda)
     vd = F.normalize(torch.randn(700, 3, generator=g), dim=-1).to(cuda)
     prm = [t.to(cuda) for t in params(c, 4)]
-    spec = color.ColorSpec(n_features=c['C'], features_pe=c['Lf'])
+    spec = color.ColorSpec(n_features=c['C'], features_pe=c['Lf'], prec=prec)
     with torch.no_grad():
         a = color.color_forward(spec, app, prm[0], prm[1:], view_dirs=vd)
     pr = [t.clone().requires_grad_(True) for t in prm]
