@@ -398,10 +398,10 @@ PEAK_HBM_BYTES = PEAK_HBM_GBS * 1e9
 
 COLOR_PRECISION_NOTE = {
     'fp32': 'fp32: FP32 MFMA operands (the reference arithmetic)',
-    'bf16_split': 'bf16_split: BF16 MFMAs on operands split into BF16 planes (3 forward, 2 backward), FP32 '
-                  'activations and weight gradients; rgb and gradients as FP32 (tests/test_gpu_color.py)',
-    'bf16': 'bf16: BF16 MFMA operands, FP32 activations, accumulation and weight gradients; rgb within 2e-3 '
-            'of FP64 (tests/test_gpu_color.py)',
+    'bf16_split': 'bf16_split: BF16 MFMAs on operands split into BF16 planes (3 forward, 2 backward and '
+                  'weight gradients), FP32 activations; rgb and gradients as FP32 (tests/test_gpu_color.py)',
+    'bf16': 'bf16: BF16 MFMA operands in the forward and backward chain, split BF16 weight gradients, FP32 '
+            'activations and accumulation; rgb within 2e-3 of FP64 (tests/test_gpu_color.py)',
 }
 
 

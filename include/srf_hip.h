@@ -410,8 +410,9 @@ int srf_aabb_samples(int n_rays, int n_samples, const float* rays_o, const float
  * prec: operands of the MFMA products of the forward and of the backward chain.  SRF_PREC_F32
  * (FP32 operands); SRF_PREC_BF16_SPLIT (split BF16: three planes in the forward, two in the
  * backward chain, as srf_mlp_desc's); SRF_PREC_BF16 (BF16 operands rounded from the FP32 values).
- * Activations, the stored layer inputs, accumulation and the weight-gradient GEMM are FP32 in
- * every mode; the reference computes in FP32 throughout. */
+ * Activations, the stored layer inputs and accumulation are FP32 in every mode; the weight
+ * gradients reduce the FP32 rows with FP32 operands (SRF_PREC_F32) or split BF16 (two planes, both
+ * BF16 modes).  The reference computes in FP32 throughout. */
 typedef struct srf_color_desc {
   int n_features;    /* sum of the appearance components */
   int basis_dim;     /* features_dimension_color (27) */

@@ -17,15 +17,16 @@
 // positional-encoding inputs.  Training stores the layer inputs feature-major (row = feature,
 // column = point) for the weight-gradient GEMM, which is the grouped dW kernel of the MLP.
 //
-// Operand precision (desc->prec).  Activations, accumulators, the stored layer inputs and the
-// weight gradients are FP32 in every mode; the products of the forward and of the backward chain
-// multiply
+// Operand precision (desc->prec).  Activations, accumulators and the stored layer inputs are FP32
+// in every mode; the products of the forward and of the backward chain multiply
 //   SRF_PREC_F32          FP32 operands (16x16x4 MFMA, one k-unit = 16 features)
 //   SRF_PREC_BF16_SPLIT   split BF16 (16x16x32 MFMA, one k-unit = 32 features): three planes in
 //                         the forward (six plane products, FP32-equivalent operands), two in the
 //                         backward chain (three products), as the NeRF MLP's split mode
 //   SRF_PREC_BF16         BF16 operands rounded from the FP32 activations and weights (one plane)
-// Reduction depths are padded to whole k-units (32 features for the BF16 modes) with zeros.
+// Reduction depths are padded to whole k-units (32 features for the BF16 modes) with zeros.  The
+// weight gradients reduce the stored FP32 rows with the FP32 GEMM (SRF_PREC_F32) or the split-BF16
+// GEMM, two planes per operand (both BF16 modes).
 #include <string.h>
 
 #include <mutex>
@@ -124,11 +125,20 @@ bool make_geo(const srf_color_desc* d, Geo* g) {
   return true;
 }
 
-// Training workspace (FP32 rows of `ld` points): appT, F, x1, h1, h2, dz3, dz2, dz1, dF.
+// Training workspace: appT, F, x1, h1, h2, dz3, dz2, dz1, dF, each an array of 16-feature tiles of
+// FP32 values in the tile-block layout (store_tile_rows), a tile taking the bytes of 16 rows of `ld`
+// points.  The stores address every array with 32-bit byte offsets, so each stays below 4 GB
+// (ws_addressable).
 struct Ws {
   int cols, ld;
   size_t appT, F, x1, h1, h2, dz3, dz2, dz1, dF, bytes;
 };
+
+inline bool ws_addressable(const Geo& g, const Ws& w) {
+  int rows = g.Cp > g.D1p ? g.Cp : g.D1p;
+  rows = rows > kHid ? rows : kHid;
+  return (uint64_t)rows * (uint64_t)w.ld * 4u < (1ull << 32);
+}
 
 Ws make_ws(const Geo& g, int n) {
   Ws w;
@@ -380,29 +390,17 @@ __device__ __forceinline__ bool block_has_rows(int b, int n) {
   return (int64_t)b * kWaves * kPW < (int64_t)(n + 255) / 256 * 256;
 }
 
-__device__ __forceinline__ void store_fm(uint8_t* base, size_t row_off, int ld, int row, int p, float v) {
-  // streamed past L2 (non-temporal): the rows are read back only by the backward
-  __builtin_nontemporal_store(v, reinterpret_cast<float*>(base + row_off) + (size_t)row * ld + p);
+// Training rows are kept in the tile-block layout of the NeRF MLP's FP32 workspace (mlp::store_tile,
+// 32 points per 2 KB block of a 16-feature tile), so that the weight-gradient GEMM copies every
+// operand fragment as one contiguous 1 KB.  A wave stores tile t of its 32 points (p0) straight from
+// D-tile (or B-operand) registers v[cb] = features 16 t + 4 g + r at point p0 + 16 cb + j; all lanes
+// call it (pair exchange).
+__device__ __forceinline__ void store_tile_rows(uint8_t* ws, size_t arr, int ld, int t, int p0, const f32x4 (&v)[kCB]) {
+  mlp::store_tile<kF32>(mlp::uniform_rsrc(ws + arr), mlp::tile_block<kF32>(t, ld, 4, p0), v);
 }
 
-typedef float f32x2_t __attribute__((ext_vector_type(2)));
-
-// Stores the wave's 32 points (p0 a multiple of 32) of one feature row per 16-lane group: v0 is the
-// lane's value at point p0 + j and v1 at p0 + 16 + j (j = lane & 15, the two column blocks of an
-// MFMA tile).  The group's lanes exchange values (__shfl) so that lane j writes points 2j and
-// 2j + 1 as 8 B: each store then writes every row's 128 B as one whole line, where one store per
-// column block would leave each line half written between the two.  All lanes must call it (the
-// exchange); `keep` only masks the store.
-__device__ __forceinline__ void store_fm_row32(uint8_t* base, size_t row_off, int ld, int row, int p0, float v0,
-                                               float v1, bool keep) {
-  const int lane = threadIdx.x & 63, j = lane & 15, grp = lane & 48;
-  const int s0 = grp | ((2 * j) & 15), s1 = grp | ((2 * j + 1) & 15);
-  const float a0 = __shfl(v0, s0, 64), a1 = __shfl(v1, s0, 64);
-  const float b0 = __shfl(v0, s1, 64), b1 = __shfl(v1, s1, 64);
-  const f32x2_t o = j < 8 ? f32x2_t{a0, b0} : f32x2_t{a1, b1};
-  if (keep)
-    __builtin_nontemporal_store(
-        o, reinterpret_cast<f32x2_t*>(reinterpret_cast<float*>(base + row_off) + (size_t)row * ld + p0 + 2 * j));
+__device__ __forceinline__ void load_tile_rows(const uint8_t* ws, size_t arr, int ld, int t, int p0, f32x4 (&v)[kCB]) {
+  mlp::load_tile_f32(mlp::uniform_rsrc(ws + arr), mlp::tile_block<kF32>(t, ld, 4, p0), v);
 }
 
 template <int P>
@@ -431,12 +429,9 @@ __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(2, 
     if (p < n && c < g.C) v = *reinterpret_cast<const float4*>(a.app + (size_t)p * g.C + c);
     return f32x4{v.x, v.y, v.z, v.w};
   };
-  // training copy of the appearance features, feature-major (rows c .. c + 3 of the lane's group)
+  // training copy of the appearance features (tile u; zero past the components)
   auto keep_app = [&](int u, const f32x4(&b)[kCB]) {
-    if (!train) return;
-    const int c = 16 * u + 4 * gq;
-#pragma unroll
-    for (int r = 0; r < 4; ++r) store_fm_row32(a.ws, a.w.appT, a.w.ld, c + r, p0, b[0][r], b[1][r], c < g.C);
+    if (train) store_tile_rows(a.ws, a.w.appT, a.w.ld, u, p0, b);
   };
   staged_layer_rt<P, 2>(accF, a.pk + g.fB, g.Cp / KU, g.Cp / 16, fin_app, keep_app, stage);
   // F and the view directions to LDS (and F feature-major for the backward)
@@ -447,8 +442,10 @@ __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(2, 
       const int f = 16 * T + 4 * gq + r;
 #pragma unroll
       for (int cb = 0; cb < kCB; ++cb) sF[wv][f][16 * cb + j] = accF[T][cb][r];
-      if (train) store_fm_row32(a.ws, a.w.F, a.w.ld, f, p0, accF[T][0][r], accF[T][1][r], true);
     }
+  if (train)
+#pragma unroll
+    for (int T = 0; T < 2; ++T) store_tile_rows(a.ws, a.w.F, a.w.ld, T, p0, accF[T]);
   if (g.view && lane < kPW) {
     const int p = p0 + lane;
     float d0 = 0.f, d1 = 0.f, d2 = 0.f;
@@ -484,22 +481,17 @@ __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(2, 
     return b;
   };
   auto keep_pe = [&](int u, const f32x4(&b)[kCB]) {
-    if (!train) return;
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int i = 16 * u + 4 * gq + r;
-      store_fm_row32(a.ws, a.w.x1, a.w.ld, i, p0, b[0][r], b[1][r], i < g.D1);
-    }
+    if (train) store_tile_rows(a.ws, a.w.x1, a.w.ld, u, p0, b);
   };
   staged_layer_rt<P, kHT>(h1, a.pk + g.fW1, g.D1p / KU, g.D1p / 16, fin_pe, keep_pe, stage);
 #pragma unroll
-  for (int T = 0; T < kHT; ++T)
+  for (int T = 0; T < kHT; ++T) {
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
+    for (int r = 0; r < 4; ++r)
 #pragma unroll
       for (int cb = 0; cb < kCB; ++cb) h1[T][cb][r] = h1[T][cb][r] > 0.f ? h1[T][cb][r] : 0.f;
-      if (train) store_fm_row32(a.ws, a.w.h1, a.w.ld, 16 * T + 4 * gq + r, p0, h1[T][0][r], h1[T][1][r], true);
-    }
+    if (train) store_tile_rows(a.ws, a.w.h1, a.w.ld, T, p0, h1[T]);
+  }
   // 3) second layer
   f32x4 h2[kHT][kCB];
 #pragma unroll
@@ -510,13 +502,13 @@ __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(2, 
       for (int r = 0; r < 4; ++r) h2[T][cb][r] = a.pk[g.b2 + 16 * T + 4 * gq + r];
   staged_layer<P, kHT, kHT>(h2, a.pk + g.fW2, kHid / KU, [&](int u, int cb) { return h1[u][cb]; }, stage);
 #pragma unroll
-  for (int T = 0; T < kHT; ++T)
+  for (int T = 0; T < kHT; ++T) {
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
+    for (int r = 0; r < 4; ++r)
 #pragma unroll
       for (int cb = 0; cb < kCB; ++cb) h2[T][cb][r] = h2[T][cb][r] > 0.f ? h2[T][cb][r] : 0.f;
-      if (train) store_fm_row32(a.ws, a.w.h2, a.w.ld, 16 * T + 4 * gq + r, p0, h2[T][0][r], h2[T][1][r], true);
-    }
+    if (train) store_tile_rows(a.ws, a.w.h2, a.w.ld, T, p0, h2[T]);
+  }
   // 4) output layer + sigmoid
   f32x4 o[1][kCB];
 #pragma unroll
@@ -551,8 +543,9 @@ struct BwdArgs {
   uint8_t* ws;
 };
 
-__device__ __forceinline__ float load_fm(const uint8_t* base, size_t row_off, int ld, int row, int p) {
-  return reinterpret_cast<const float*>(base + row_off)[(size_t)row * ld + p];
+// One element (feature f, point p) of a tile-block array.
+__device__ __forceinline__ float load_elem(const uint8_t* base, size_t arr, int ld, int f, int p) {
+  return *reinterpret_cast<const float*>(base + arr + mlp::tile_elem_f32(f, p, ld));
 }
 
 template <int P>
@@ -586,8 +579,7 @@ __global__ __launch_bounds__(64 * kWaves) void color_bwd_kernel(BwdArgs a) {
       dz3[0][cb][r] = v;
     }
   }
-#pragma unroll
-  for (int r = 0; r < 4; ++r) store_fm_row32(a.ws, a.w.dz3, a.w.ld, 4 * gq + r, p0, dz3[0][0][r], dz3[0][1][r], true);
+  store_tile_rows(a.ws, a.w.dz3, a.w.ld, 0, p0, dz3[0]);
   // 2) dz2 = (W3^T dz3) * [h2 > 0]
   f32x4 dz2[kHT][kCB];
 #pragma unroll
@@ -599,19 +591,15 @@ __global__ __launch_bounds__(64 * kWaves) void color_bwd_kernel(BwdArgs a) {
     return u == 0 ? dz3[0][cb] : f32x4{0.f, 0.f, 0.f, 0.f};
   }, stage);
 #pragma unroll
-  for (int T = 0; T < kHT; ++T)
+  for (int T = 0; T < kHT; ++T) {
+    f32x4 hv[kCB];
+    load_tile_rows(a.ws, a.w.h2, a.w.ld, T, p0, hv);
 #pragma unroll
     for (int cb = 0; cb < kCB; ++cb)
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int row = 16 * T + 4 * gq + r, p = p0 + 16 * cb + j;
-        dz2[T][cb][r] = load_fm(a.ws, a.w.h2, a.w.ld, row, p) > 0.f ? dz2[T][cb][r] : 0.f;
-      }
-#pragma unroll
-  for (int T = 0; T < kHT; ++T)
-#pragma unroll
-    for (int r = 0; r < 4; ++r)
-      store_fm_row32(a.ws, a.w.dz2, a.w.ld, 16 * T + 4 * gq + r, p0, dz2[T][0][r], dz2[T][1][r], true);
+      for (int r = 0; r < 4; ++r) dz2[T][cb][r] = hv[cb][r] > 0.f ? dz2[T][cb][r] : 0.f;
+    store_tile_rows(a.ws, a.w.dz2, a.w.ld, T, p0, dz2[T]);
+  }
   // 3) dz1 = (W2^T dz2) * [h1 > 0]
   f32x4 dz1[kHT][kCB];
 #pragma unroll
@@ -620,19 +608,15 @@ __global__ __launch_bounds__(64 * kWaves) void color_bwd_kernel(BwdArgs a) {
     for (int cb = 0; cb < kCB; ++cb) dz1[T][cb] = f32x4{0.f, 0.f, 0.f, 0.f};
   staged_layer<P, kHT, kHT>(dz1, a.pk + g.fW2T, kHid / KU, [&](int u, int cb) { return dz2[u][cb]; }, stage);
 #pragma unroll
-  for (int T = 0; T < kHT; ++T)
+  for (int T = 0; T < kHT; ++T) {
+    f32x4 hv[kCB];
+    load_tile_rows(a.ws, a.w.h1, a.w.ld, T, p0, hv);
 #pragma unroll
     for (int cb = 0; cb < kCB; ++cb)
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int row = 16 * T + 4 * gq + r, p = p0 + 16 * cb + j;
-        dz1[T][cb][r] = load_fm(a.ws, a.w.h1, a.w.ld, row, p) > 0.f ? dz1[T][cb][r] : 0.f;
-      }
-#pragma unroll
-  for (int T = 0; T < kHT; ++T)
-#pragma unroll
-    for (int r = 0; r < 4; ++r)
-      store_fm_row32(a.ws, a.w.dz1, a.w.ld, 16 * T + 4 * gq + r, p0, dz1[T][0][r], dz1[T][1][r], true);
+      for (int r = 0; r < 4; ++r) dz1[T][cb][r] = hv[cb][r] > 0.f ? dz1[T][cb][r] : 0.f;
+    store_tile_rows(a.ws, a.w.dz1, a.w.ld, T, p0, dz1[T]);
+  }
   // 4) dx1 = W1^T dz1 into LDS, kST output tiles per staged pass and single tiles for the rest
   auto fin_dz1 = [&](int u, int cb) { return dz1[u][cb]; };
   auto dx_tiles = [&](auto OUTC, int T0) {
@@ -663,7 +647,7 @@ __global__ __launch_bounds__(64 * kWaves) void color_bwd_kernel(BwdArgs a) {
     const int f = e / kPW, col = e - f * kPW;
     float d = 0.f;
     if (f < g.NB) {
-      const float Fv = load_fm(a.ws, a.w.F, a.w.ld, f, p0 + col);
+      const float Fv = load_elem(a.ws, a.w.F, a.w.ld, f, p0 + col);
       d = sX(f, col);
       for (int k = 0; k < g.Lf; ++k) {
         const float fr = (float)(1 << k);
@@ -682,7 +666,6 @@ __global__ __launch_bounds__(64 * kWaves) void color_bwd_kernel(BwdArgs a) {
     const int e = lane + 64 * it;
     const int f = e / kPW, col = e - f * kPW;
     sX(f, col) = dFv[it];
-    store_fm(a.ws, a.w.dF, a.w.ld, f, p0 + col, dFv[it]);
   }
   __syncthreads();
   // 6) d_app = B^T dF (point-major, float4 per lane)
@@ -693,6 +676,8 @@ __global__ __launch_bounds__(64 * kWaves) void color_bwd_kernel(BwdArgs a) {
     for (int cb = 0; cb < kCB; ++cb)
 #pragma unroll
       for (int r = 0; r < 4; ++r) dFt[t][cb][r] = sX(16 * t + 4 * gq + r, 16 * cb + j);
+#pragma unroll
+  for (int t = 0; t < 2; ++t) store_tile_rows(a.ws, a.w.dF, a.w.ld, t, p0, dFt[t]);  // rows past basis_dim: 0
   auto fin_dF = [&](int u, int cb) { return dFt[u][cb]; };
   auto da_tiles = [&](auto OUTC, int T0) {
     constexpr int OUT = decltype(OUTC)::value;
@@ -828,6 +813,7 @@ extern "C" int srf_color_forward(const srf_color_desc* desc, const void* packed,
   FwdArgs a;
   a.g = g;
   a.w = make_ws(g, n_points);
+  if (workspace && !ws_addressable(g, a.w)) return SRF_ETOOLARGE;
   a.pk = reinterpret_cast<const float*>(packed);
   a.n = n_points;
   a.count = point_count;
@@ -862,6 +848,7 @@ extern "C" int srf_color_backward(const srf_color_desc* desc, const void* packed
   BwdArgs b;
   b.g = g;
   b.w = make_ws(g, n_points);
+  if (!ws_addressable(g, b.w)) return SRF_ETOOLARGE;
   b.pk = reinterpret_cast<const float*>(packed);
   b.n = n_points;
   b.count = point_count;
@@ -889,6 +876,10 @@ extern "C" int srf_color_backward(const srf_color_desc* desc, const void* packed
   d.ldP = w.ld;
   d.cols = w.cols;
   d.cols_dev = point_count;
+  d.paired = 1;  // the training rows are tile blocks (store_tile_rows)
+  // weight gradients of the BF16 modes: the split-BF16 GEMM (two planes, three products) on the
+  // FP32 rows; FP32: the FP32 GEMM
+  const bool split_dw = g.prec != SRF_PREC_F32;
   int nj = 0;
   auto job = [&](size_t dz, int dz_rows, size_t x, int x_rows, int M, int N, float* dw, int ld, float* db) {
     DwJob& jb = d.jobs[nj++];
@@ -901,9 +892,11 @@ extern "C" int srf_color_backward(const srf_color_desc* desc, const void* packed
     jb.dw = dw;
     jb.ld_dw = ld;
     jb.db = db;
-    // 128 x 160 tiles (dw_f32_narrow): one tile covers every job at the default sizes
-    jb.mblocks = (M + mlp::kDwNarrowRows - 1) / mlp::kDwNarrowRows;
-    jb.nblocks = (N + mlp::kDwNarrowCols - 1) / mlp::kDwNarrowCols;
+    // FP32: 128 x 160 tiles (dw_f32_narrow), BF16 modes: 256 x 256 tiles (dw_bf16s); one tile
+    // covers every job at the default sizes
+    const int tm = split_dw ? mlp::kDwTile : mlp::kDwNarrowRows, tn = split_dw ? mlp::kDwTile : mlp::kDwNarrowCols;
+    jb.mblocks = (M + tm - 1) / tm;
+    jb.nblocks = (N + tn - 1) / tn;
   };
   job(w.dz1, kHid, w.x1, g.D1p, kHid, g.D1, d_w1, g.D1, d_b1);
   job(w.dz2, kHid, w.h1, kHid, kHid, kHid, d_w2, kHid, d_b2);
@@ -921,5 +914,5 @@ extern "C" int srf_color_backward(const srf_color_desc* desc, const void* packed
     work += d.jobs[i].work_count;
   }
   d.total_work = work;
-  return mlp::dw_f32_narrow(d, s);
+  return split_dw ? mlp::dw_bf16s(d, s) : mlp::dw_f32_narrow(d, s);
 }

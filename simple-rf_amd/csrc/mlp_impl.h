@@ -791,6 +791,32 @@ __device__ __forceinline__ void store_tile(Rsrc dst, uint32_t blk, const typenam
   }
 }
 
+// Inverse of store_tile for FP32 tiles (CB = 2): the lane's D-tile values (v[cb][r] = feature 4 g + r
+// at point 16 cb + j) of one 16-feature tile, read from its 2 KB block.  The lane reads its own
+// record (g, j): chunk c holds features 4 g + 2 (j & 1) + c at points j & ~1, j | 1 and 16 more;
+// the lanes of a pair exchange the two features of the point each one lacks.
+__device__ __forceinline__ void load_tile_f32(Rsrc src, uint32_t blk, f32x4 (&v)[2]) {
+  const uint32_t rec = lane_here() * 16u;
+  const f32x4 c0 = __builtin_bit_cast(f32x4, __builtin_amdgcn_raw_buffer_load_b128(src, rec, blk, 0));
+  const f32x4 c1 = __builtin_bit_cast(f32x4, __builtin_amdgcn_raw_buffer_load_b128(src, rec, blk + 1024, 0));
+  // odd lanes send their first point's values (positions 0, 2), even lanes their second's (1, 3)
+  const float ka = sel_odd(c0[1], c0[0]), kb = sel_odd(c0[3], c0[2]);
+  const float kc = sel_odd(c1[1], c1[0]), kd = sel_odd(c1[3], c1[2]);
+  const float ra = swap_adjacent(sel_odd(c0[0], c0[1])), rb = swap_adjacent(sel_odd(c0[2], c0[3]));
+  const float rc = swap_adjacent(sel_odd(c1[0], c1[1])), rd = swap_adjacent(sel_odd(c1[2], c1[3]));
+  v[0] = f32x4{sel_odd(ra, ka), sel_odd(rc, kc), sel_odd(ka, ra), sel_odd(kc, rc)};
+  v[1] = f32x4{sel_odd(rb, kb), sel_odd(rd, kd), sel_odd(kb, rb), sel_odd(kd, rd)};
+}
+
+// Byte offset of element (feature f, point p) inside an FP32 array of store_tile blocks with row
+// stride ldP points (CB = 2: 32 points per block).
+__host__ __device__ inline size_t tile_elem_f32(int f, int p, int ldP) {
+  const int fl = f & 15, q = p & 31, jj = q & 15;
+  const int rec = 16 * (fl >> 2) + 2 * (jj >> 1) + ((fl >> 1) & 1);
+  return (size_t)(f >> 4) * 16 * ldP * 4 + (size_t)(p >> 5) * 2048 + 1024 * (fl & 1) + 16 * rec +
+         4 * (2 * (q >> 4) + (jj & 1));
+}
+
 __device__ __forceinline__ float sigmoidf_(float x) { return 1.0f / (1.0f + expf(-x)); }
 
 __device__ __forceinline__ float softplusf_(float x) { return x > 20.f ? x : log1pf(expf(x)); }
@@ -1446,8 +1472,9 @@ __global__ __launch_bounds__(64 * kDwWaves) void mlp_dw_kernel(DwArgs a) {
   }
 }
 
-// Split-BF16 weight gradients: the FP32 dZ and X tiles (the FP32 layouts, DMA'd through the same
-// five-slot ring as the FP32 GEMM, 16 points per k-unit) are split into two BF16 planes and reduced
+// Split-BF16 weight gradients: the FP32 dZ and X tiles (the paired tile layout of store_tile or
+// plain feature-major rows, DMA'd through the same five-slot ring as the FP32 GEMM, 16 points per
+// k-unit) are split into two BF16 planes and reduced
 // with 32x32x16 BF16 MFMAs, dW += dZ_hi X_hi^T + dZ_hi X_lo^T + dZ_lo X_hi^T.  One 16-point unit is
 // exactly one K = 16 step: lane (h, i) of an A operand holds row i of a 32-row block at the eight
 // points of lane groups 2h and 2h + 1 of the FP32 fragments (records 16 (2h) + (i mod 16) and
@@ -1485,7 +1512,9 @@ __global__ __launch_bounds__(64 * kDwWaves) void mlp_dw_split_kernel(DwArgs a) {
 #pragma unroll
   for (int b = 0; b < 4; ++b) nv[b] = n0 + 16 * (qn + 2 * b) < jb.x_rows;
   // DMA of one 16-point unit: the 16 dZ and 16 X fragments of the tile, as in mlp_dw_kernel (a row
-  // tile beyond the job's rows re-reads its first tile, so every wave issues four loads per unit)
+  // tile beyond the job's rows re-reads its first tile, so every wave issues four loads per unit).
+  // Either layout gives lane record (g, j) four points of feature j, the same four in the dZ and X
+  // fragments, which is all the plane products below need.
   auto dma_unit = [&](int p, uint8_t* buf) SRF_AI {
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
@@ -1495,9 +1524,14 @@ __global__ __launch_bounds__(64 * kDwWaves) void mlp_dw_split_kernel(DwArgs a) {
       int row0 = (is_a ? m0 : n0) + 16 * rt;
       if (row0 >= (is_a ? jb.dz_rows : jb.x_rows)) row0 = is_a ? m0 : n0;
       const uint8_t* base = is_a ? jb.dz : jb.x;
-      const int wb = p / 32, u = (p % 32) / KPTS;
-      const int rec = 16 * (j >> 2) + 2 * (4 * u + g) + ((j >> 1) & 1);
-      const uint8_t* src = base + (size_t)row0 * ldP * 4 + (size_t)wb * 2048 + (j & 1) * 1024 + rec * 16;
+      const uint8_t* src;
+      if (a.paired) {
+        const int wb = p / 32, u = (p % 32) / KPTS;
+        const int rec = 16 * (j >> 2) + 2 * (4 * u + g) + ((j >> 1) & 1);
+        src = base + (size_t)row0 * ldP * 4 + (size_t)wb * 2048 + (j & 1) * 1024 + rec * 16;
+      } else {
+        src = base + (((size_t)(row0 + j)) * ldP + p + 4 * g) * 4;  // plain feature-major rows
+      }
       __builtin_amdgcn_global_load_lds((const void*)src, (__attribute__((address_space(3))) void*)(buf + f * 1024),
                                        16, 0, 0);
     }
