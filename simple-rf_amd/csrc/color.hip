@@ -37,8 +37,6 @@ namespace srf {
 namespace color {
 namespace {
 
-using mlp::DwArgs;
-using mlp::DwJob;
 
 constexpr int kHid = 128;     // num_units_color_predictor of the shipped configurations
 constexpr int kHT = kHid / 16;
@@ -126,8 +124,8 @@ bool make_geo(const srf_color_desc* d, Geo* g) {
 }
 
 // Training workspace: appT, F, x1, h1, h2, dz3, dz2, dz1, dF, each an array of 16-feature tiles of
-// FP32 values in the tile-block layout (store_tile_rows), a tile taking the bytes of 16 rows of `ld`
-// points.  The stores address every array with 32-bit byte offsets, so each stays below 4 GB
+// FP32 values in the point-block-major tile-block layout (store_tile_rows), an array of R rows taking
+// the bytes of R rows of `ld` points.  The stores address every array with 32-bit byte offsets, so each stays below 4 GB
 // (ws_addressable).
 struct Ws {
   int cols, ld;
@@ -390,17 +388,26 @@ __device__ __forceinline__ bool block_has_rows(int b, int n) {
   return (int64_t)b * kWaves * kPW < (int64_t)(n + 255) / 256 * 256;
 }
 
-// Training rows are kept in the tile-block layout of the NeRF MLP's FP32 workspace (mlp::store_tile,
-// 32 points per 2 KB block of a 16-feature tile), so that the weight-gradient GEMM copies every
-// operand fragment as one contiguous 1 KB.  A wave stores tile t of its 32 points (p0) straight from
-// D-tile (or B-operand) registers v[cb] = features 16 t + 4 g + r at point p0 + 16 cb + j; all lanes
-// call it (pair exchange).
-__device__ __forceinline__ void store_tile_rows(uint8_t* ws, size_t arr, int ld, int t, int p0, const f32x4 (&v)[kCB]) {
-  mlp::store_tile<kF32>(mlp::uniform_rsrc(ws + arr), mlp::tile_block<kF32>(t, ld, 4, p0), v);
+// Training rows are kept as 2 KB tile blocks in the record layout of the NeRF MLP's FP32 workspace
+// (mlp::store_tile: one 16-feature tile of 32 points), so that the weight-gradient GEMM copies every
+// operand fragment as one contiguous 1 KB; the blocks of an array are point-block-major (the n_tiles
+// tiles of points 32 q .. 32 q + 31 adjacent, at byte 2048 (q n_tiles + t)), so that one unit of the
+// GEMM reads one short run per array rather than one run per 16-row tile spread over the whole
+// capacity.  A wave stores tile t of its 32 points (p0) straight from D-tile (or B-operand)
+// registers v[cb] = features 16 t + 4 g + r at point p0 + 16 cb + j; all lanes call it (pair
+// exchange).
+__host__ __device__ inline uint32_t blk_off(int n_tiles, int t, int p0) {
+  return ((uint32_t)(p0 >> 5) * (uint32_t)n_tiles + (uint32_t)t) * 2048u;
 }
 
-__device__ __forceinline__ void load_tile_rows(const uint8_t* ws, size_t arr, int ld, int t, int p0, f32x4 (&v)[kCB]) {
-  mlp::load_tile_f32(mlp::uniform_rsrc(ws + arr), mlp::tile_block<kF32>(t, ld, 4, p0), v);
+__device__ __forceinline__ void store_tile_rows(uint8_t* ws, size_t arr, int n_tiles, int t, int p0,
+                                                const f32x4 (&v)[kCB]) {
+  mlp::store_tile<kF32>(mlp::uniform_rsrc(ws + arr), blk_off(n_tiles, t, p0), v);
+}
+
+__device__ __forceinline__ void load_tile_rows(const uint8_t* ws, size_t arr, int n_tiles, int t, int p0,
+                                               f32x4 (&v)[kCB]) {
+  mlp::load_tile_f32(mlp::uniform_rsrc(ws + arr), blk_off(n_tiles, t, p0), v);
 }
 
 template <int P>
@@ -431,7 +438,7 @@ __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(2, 
   };
   // training copy of the appearance features (tile u; zero past the components)
   auto keep_app = [&](int u, const f32x4(&b)[kCB]) {
-    if (train) store_tile_rows(a.ws, a.w.appT, a.w.ld, u, p0, b);
+    if (train) store_tile_rows(a.ws, a.w.appT, g.Cp / 16, u, p0, b);
   };
   staged_layer_rt<P, 2>(accF, a.pk + g.fB, g.Cp / KU, g.Cp / 16, fin_app, keep_app, stage);
   // F and the view directions to LDS (and F feature-major for the backward)
@@ -445,7 +452,7 @@ __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(2, 
     }
   if (train)
 #pragma unroll
-    for (int T = 0; T < 2; ++T) store_tile_rows(a.ws, a.w.F, a.w.ld, T, p0, accF[T]);
+    for (int T = 0; T < 2; ++T) store_tile_rows(a.ws, a.w.F, kMaxB / 16, T, p0, accF[T]);
   if (g.view && lane < kPW) {
     const int p = p0 + lane;
     float d0 = 0.f, d1 = 0.f, d2 = 0.f;
@@ -481,7 +488,7 @@ __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(2, 
     return b;
   };
   auto keep_pe = [&](int u, const f32x4(&b)[kCB]) {
-    if (train) store_tile_rows(a.ws, a.w.x1, a.w.ld, u, p0, b);
+    if (train) store_tile_rows(a.ws, a.w.x1, g.D1p / 16, u, p0, b);
   };
   staged_layer_rt<P, kHT>(h1, a.pk + g.fW1, g.D1p / KU, g.D1p / 16, fin_pe, keep_pe, stage);
 #pragma unroll
@@ -490,7 +497,7 @@ __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(2, 
     for (int r = 0; r < 4; ++r)
 #pragma unroll
       for (int cb = 0; cb < kCB; ++cb) h1[T][cb][r] = h1[T][cb][r] > 0.f ? h1[T][cb][r] : 0.f;
-    if (train) store_tile_rows(a.ws, a.w.h1, a.w.ld, T, p0, h1[T]);
+    if (train) store_tile_rows(a.ws, a.w.h1, kHT, T, p0, h1[T]);
   }
   // 3) second layer
   f32x4 h2[kHT][kCB];
@@ -507,7 +514,7 @@ __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(2, 
     for (int r = 0; r < 4; ++r)
 #pragma unroll
       for (int cb = 0; cb < kCB; ++cb) h2[T][cb][r] = h2[T][cb][r] > 0.f ? h2[T][cb][r] : 0.f;
-    if (train) store_tile_rows(a.ws, a.w.h2, a.w.ld, T, p0, h2[T]);
+    if (train) store_tile_rows(a.ws, a.w.h2, kHT, T, p0, h2[T]);
   }
   // 4) output layer + sigmoid
   f32x4 o[1][kCB];
@@ -543,9 +550,14 @@ struct BwdArgs {
   uint8_t* ws;
 };
 
-// One element (feature f, point p) of a tile-block array.
-__device__ __forceinline__ float load_elem(const uint8_t* base, size_t arr, int ld, int f, int p) {
-  return *reinterpret_cast<const float*>(base + arr + mlp::tile_elem_f32(f, p, ld));
+// One element (feature f, point p) of a tile-block array of n_tiles tiles (the record of
+// mlp::store_tile holding it: chunk f & 1 of record 16 (fl / 4) + 2 (jj / 2) + (fl / 2) % 2, at
+// position 2 cb + jj % 2, fl = f mod 16, point p = 32 q + 16 cb + jj).
+__device__ __forceinline__ float load_elem(const uint8_t* base, size_t arr, int n_tiles, int f, int p) {
+  const int fl = f & 15, q = p & 31, jj = q & 15;
+  const int rec = 16 * (fl >> 2) + 2 * (jj >> 1) + ((fl >> 1) & 1);
+  const uint32_t off = blk_off(n_tiles, f >> 4, p) + 1024u * (fl & 1) + 16u * rec + 4u * (2 * (q >> 4) + (jj & 1));
+  return *reinterpret_cast<const float*>(base + arr + off);
 }
 
 template <int P>
@@ -579,7 +591,7 @@ __global__ __launch_bounds__(64 * kWaves) void color_bwd_kernel(BwdArgs a) {
       dz3[0][cb][r] = v;
     }
   }
-  store_tile_rows(a.ws, a.w.dz3, a.w.ld, 0, p0, dz3[0]);
+  store_tile_rows(a.ws, a.w.dz3, 1, 0, p0, dz3[0]);
   // 2) dz2 = (W3^T dz3) * [h2 > 0]
   f32x4 dz2[kHT][kCB];
 #pragma unroll
@@ -593,12 +605,12 @@ __global__ __launch_bounds__(64 * kWaves) void color_bwd_kernel(BwdArgs a) {
 #pragma unroll
   for (int T = 0; T < kHT; ++T) {
     f32x4 hv[kCB];
-    load_tile_rows(a.ws, a.w.h2, a.w.ld, T, p0, hv);
+    load_tile_rows(a.ws, a.w.h2, kHT, T, p0, hv);
 #pragma unroll
     for (int cb = 0; cb < kCB; ++cb)
 #pragma unroll
       for (int r = 0; r < 4; ++r) dz2[T][cb][r] = hv[cb][r] > 0.f ? dz2[T][cb][r] : 0.f;
-    store_tile_rows(a.ws, a.w.dz2, a.w.ld, T, p0, dz2[T]);
+    store_tile_rows(a.ws, a.w.dz2, kHT, T, p0, dz2[T]);
   }
   // 3) dz1 = (W2^T dz2) * [h1 > 0]
   f32x4 dz1[kHT][kCB];
@@ -610,12 +622,12 @@ __global__ __launch_bounds__(64 * kWaves) void color_bwd_kernel(BwdArgs a) {
 #pragma unroll
   for (int T = 0; T < kHT; ++T) {
     f32x4 hv[kCB];
-    load_tile_rows(a.ws, a.w.h1, a.w.ld, T, p0, hv);
+    load_tile_rows(a.ws, a.w.h1, kHT, T, p0, hv);
 #pragma unroll
     for (int cb = 0; cb < kCB; ++cb)
 #pragma unroll
       for (int r = 0; r < 4; ++r) dz1[T][cb][r] = hv[cb][r] > 0.f ? dz1[T][cb][r] : 0.f;
-    store_tile_rows(a.ws, a.w.dz1, a.w.ld, T, p0, dz1[T]);
+    store_tile_rows(a.ws, a.w.dz1, kHT, T, p0, dz1[T]);
   }
   // 4) dx1 = W1^T dz1 into LDS, kST output tiles per staged pass and single tiles for the rest
   auto fin_dz1 = [&](int u, int cb) { return dz1[u][cb]; };
@@ -647,7 +659,7 @@ __global__ __launch_bounds__(64 * kWaves) void color_bwd_kernel(BwdArgs a) {
     const int f = e / kPW, col = e - f * kPW;
     float d = 0.f;
     if (f < g.NB) {
-      const float Fv = load_elem(a.ws, a.w.F, a.w.ld, f, p0 + col);
+      const float Fv = load_elem(a.ws, a.w.F, kMaxB / 16, f, p0 + col);
       d = sX(f, col);
       for (int k = 0; k < g.Lf; ++k) {
         const float fr = (float)(1 << k);
@@ -677,7 +689,7 @@ __global__ __launch_bounds__(64 * kWaves) void color_bwd_kernel(BwdArgs a) {
 #pragma unroll
       for (int r = 0; r < 4; ++r) dFt[t][cb][r] = sX(16 * t + 4 * gq + r, 16 * cb + j);
 #pragma unroll
-  for (int t = 0; t < 2; ++t) store_tile_rows(a.ws, a.w.dF, a.w.ld, t, p0, dFt[t]);  // rows past basis_dim: 0
+  for (int t = 0; t < 2; ++t) store_tile_rows(a.ws, a.w.dF, kMaxB / 16, t, p0, dFt[t]);  // rows past basis_dim: 0
   auto fin_dF = [&](int u, int cb) { return dFt[u][cb]; };
   auto da_tiles = [&](auto OUTC, int T0) {
     constexpr int OUT = decltype(OUTC)::value;
@@ -703,6 +715,172 @@ __global__ __launch_bounds__(64 * kWaves) void color_bwd_kernel(BwdArgs a) {
   int Ta = 0;
   for (; Ta + kST <= g.Cp / 16; Ta += kST) da_tiles(std::integral_constant<int, kST>{}, Ta);
   for (; Ta < g.Cp / 16; ++Ta) da_tiles(std::integral_constant<int, 1>{}, Ta);
+}
+
+// ------------------------------------------------------------------- weight gradients
+// dW = dZ X^T (+ db = row sums of dZ) of the four colour jobs (W1, W2, W3, basis) over the stored
+// FP32 tile blocks, in the BF16 modes (FP32: the MLP's FP32 GEMM, mlp_dw_kernel).  A work item is one
+// job over one chunk of points; its block copies, per 16-point unit, only the job's own fragments
+// (nA dZ and nB X row tiles, one contiguous 1 KB each) global -> LDS into a ring whose slot is
+// (nA + nB) KB, so a job of 9-24 fragments keeps 5-15 units in flight in the 160 KB of LDS.  The
+// products are split-BF16 32x32x16 MFMAs (dZ_hi X_hi + dZ_hi X_lo + dZ_lo X_hi, as
+// mlp_dw_split_kernel); wave w multiplies row block w & 3 (32 rows) by column blocks 4 (w >> 2) .. + 3
+// (32 columns each).  Partial sums go to dW with FP32 atomics.
+constexpr int kCdwWaves = 8;
+constexpr int kCdwRingKB = 160;
+constexpr int kCdwMaxSlots = 16;
+constexpr int kCdwMaxA = 8, kCdwMaxB = 16;  // row tiles: dZ <= 128 rows, X <= 256 rows
+
+struct CdwJob {
+  const uint8_t* dz;
+  const uint8_t* x;
+  int nA, nB;  // 16-row tiles of dZ and X (the tiles of their arrays)
+  int M, N;    // valid rows and columns of dW
+  float* dw;
+  int ld_dw;
+  float* db;
+};
+
+struct CdwArgs {
+  CdwJob jobs[4];
+  int n_jobs, cols, chunk;
+  const int32_t* cols_dev;
+};
+
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+// s_waitcnt vmcnt(n) for a wave-uniform n in 0..47.
+__device__ __forceinline__ void wait_vmem_le(int n) {
+  switch (n) {
+#define SRF_WAIT_CASE(k) case k: __builtin_amdgcn_s_waitcnt(mlp::vmcnt_only(k)); break;
+    SRF_WAIT_CASE(0) SRF_WAIT_CASE(1) SRF_WAIT_CASE(2) SRF_WAIT_CASE(3) SRF_WAIT_CASE(4) SRF_WAIT_CASE(5)
+    SRF_WAIT_CASE(6) SRF_WAIT_CASE(7) SRF_WAIT_CASE(8) SRF_WAIT_CASE(9) SRF_WAIT_CASE(10) SRF_WAIT_CASE(11)
+    SRF_WAIT_CASE(12) SRF_WAIT_CASE(13) SRF_WAIT_CASE(14) SRF_WAIT_CASE(15) SRF_WAIT_CASE(16) SRF_WAIT_CASE(17)
+    SRF_WAIT_CASE(18) SRF_WAIT_CASE(19) SRF_WAIT_CASE(20) SRF_WAIT_CASE(21) SRF_WAIT_CASE(22) SRF_WAIT_CASE(23)
+    SRF_WAIT_CASE(24) SRF_WAIT_CASE(25) SRF_WAIT_CASE(26) SRF_WAIT_CASE(27) SRF_WAIT_CASE(28) SRF_WAIT_CASE(29)
+    SRF_WAIT_CASE(30) SRF_WAIT_CASE(31) SRF_WAIT_CASE(32) SRF_WAIT_CASE(33) SRF_WAIT_CASE(34) SRF_WAIT_CASE(35)
+    SRF_WAIT_CASE(36) SRF_WAIT_CASE(37) SRF_WAIT_CASE(38) SRF_WAIT_CASE(39) SRF_WAIT_CASE(40) SRF_WAIT_CASE(41)
+    SRF_WAIT_CASE(42) SRF_WAIT_CASE(43) SRF_WAIT_CASE(44) SRF_WAIT_CASE(45) SRF_WAIT_CASE(46)
+#undef SRF_WAIT_CASE
+    default: __builtin_amdgcn_s_waitcnt(mlp::vmcnt_only(47)); break;
+  }
+}
+
+__global__ __launch_bounds__(64 * kCdwWaves) void color_dw_kernel(CdwArgs a) {
+  __shared__ __attribute__((aligned(16))) uint8_t ring[kCdwRingKB * 1024];
+  const int lane = threadIdx.x & 63, g = lane >> 4, j = lane & 15;
+  const int h = lane >> 5, i32 = lane & 31;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  // items are chunk-major, so that the items with points (the first chunks) of all jobs come first
+  const int job = blockIdx.x % a.n_jobs, chunk = blockIdx.x / a.n_jobs;
+  const CdwJob& jb = a.jobs[job];
+  const int cols = a.cols_dev ? min(a.cols, (*a.cols_dev + 255) / 256 * 256) : a.cols;
+  const int pb = chunk * a.chunk;
+  const int pe = min(pb + a.chunk, cols);
+  if (pb >= pe) return;  // past the device-side count (uniform over the block)
+  const int nA = jb.nA, nB = jb.nB, nf = nA + nB;
+  const int nslots = min(kCdwMaxSlots, kCdwRingKB / nf);
+  const int dma_n = (nf - w + kCdwWaves - 1) / kCdwWaves;  // this wave's fragments per unit (<= 3)
+  // unit at point p: fragment f < nA is dZ row tile f, else X row tile f - nA; lane (g, j) copies
+  // the 16 B of feature j at the four points of lane group g (the record of mlp::store_tile), to
+  // slot byte f KB + 16 lane
+  auto dma_unit = [&](int p, uint8_t* buf) {
+    const int u = (p & 31) >> 4;
+    const int rec = 16 * (j >> 2) + 2 * (4 * u + g) + ((j >> 1) & 1);
+    const uint32_t in_blk = (j & 1) * 1024 + rec * 16;
+    for (int q = 0; q < 3; ++q) {
+      const int f = w + kCdwWaves * q;
+      if (f >= nf) break;
+      const uint8_t* src = f < nA ? jb.dz + blk_off(nA, f, p) : jb.x + blk_off(nB, f - nA, p);
+      __builtin_amdgcn_global_load_lds((const void*)(src + in_blk),
+                                       (__attribute__((address_space(3))) void*)(buf + f * 1024), 16, 0, 0);
+    }
+  };
+  // row block rb (32 rows), column blocks cq .. cq + 3
+  const int rb = w & 3, cq = 4 * (w >> 2);
+  const bool row_ok = 2 * rb < nA;
+  bool col_ok[4];
+  bool any_col = false;
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    col_ok[c] = 2 * (cq + c) < nB;
+    any_col |= col_ok[c];
+  }
+  const bool do_db = jb.db != nullptr && cq == 0;
+  const bool busy = row_ok && (any_col || do_db);
+  // the fragments this lane's half (i32 >> 4) reads: a 32-row block's second tile past the job's
+  // rows repeats the first (rows past M are not stored)
+  const int fa = min(2 * rb + (i32 >> 4), nA - 1);
+  int fb[4];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) fb[c] = nA + min(2 * (cq + c) + (i32 >> 4), nB - 1);
+  f32x16 acc[4];
+#pragma unroll
+  for (int c = 0; c < 4; ++c)
+#pragma unroll
+    for (int r = 0; r < 16; ++r) acc[c][r] = 0.f;
+  float rs = 0.f;
+  // the two BF16 planes of the eight FP32 values lane (h, i) takes from fragment f: records of lane
+  // groups 2h and 2h + 1 of its row i mod 16
+  auto planes = [&](const uint8_t* fr, int f, u32x4& hi, u32x4& lo, bool sum) {
+    const int jj = i32 & 15;
+    const f32x4 v0 = *reinterpret_cast<const f32x4*>(fr + f * 1024 + ((2 * h) * 16 + jj) * 16);
+    const f32x4 v1 = *reinterpret_cast<const f32x4*>(fr + f * 1024 + ((2 * h + 1) * 16 + jj) * 16);
+    uint32_t t[4][2];
+    split_pair<2>(v0[0], v0[1], t[0]);
+    split_pair<2>(v0[2], v0[3], t[1]);
+    split_pair<2>(v1[0], v1[1], t[2]);
+    split_pair<2>(v1[2], v1[3], t[3]);
+    hi = u32x4{t[0][0], t[1][0], t[2][0], t[3][0]};
+    lo = u32x4{t[0][1], t[1][1], t[2][1], t[3][1]};
+    if (sum) rs += ((v0[0] + v0[1]) + (v0[2] + v0[3])) + ((v1[0] + v1[1]) + (v1[2] + v1[3]));
+  };
+  const int n_units = (pe - pb) / 16;
+  const int slot = nf * 1024;
+  for (int u = 0; u < nslots - 1 && u < n_units; ++u) dma_unit(pb + 16 * u, ring + u * slot);
+  for (int u = 0; u < n_units; ++u) {
+    // unit u has landed once at most (units issued after it) x dma_n loads are outstanding
+    wait_vmem_le(min(nslots - 2, n_units - 1 - u) * dma_n);
+    // every wave's part of unit u is in LDS, and slot (u - 1) mod nslots is free (the waves consumed
+    // their reads of it in the MFMAs of iteration u - 1); compiler-only fences around a plain
+    // s_barrier, which would otherwise drain the prefetches (vmcnt(0))
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_s_barrier();
+    asm volatile("" ::: "memory");
+    if (u + nslots - 1 < n_units) dma_unit(pb + 16 * (u + nslots - 1), ring + ((u + nslots - 1) % nslots) * slot);
+    if (!busy) continue;
+    const uint8_t* fr = ring + (u % nslots) * slot;
+    u32x4 ah, al, bh[4], bl[4];
+    planes(fr, fa, ah, al, do_db);
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+      if (col_ok[c]) planes(fr, fb[c], bh[c], bl[c], false);
+#pragma unroll
+    for (int pr = 0; pr < 3; ++pr)
+#pragma unroll
+      for (int c = 0; c < 4; ++c)
+        if (col_ok[c])
+          acc[c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, pr == 0 ? al : ah),
+                                                           __builtin_bit_cast(bf16x8, pr == 1 ? bl[c] : bh[c]),
+                                                           acc[c], 0, 0, 0);
+  }
+  if (!row_ok) return;
+  // 32x32 D layout: lane l, register r holds row 8 (r / 4) + 4 (l / 32) + (r mod 4), column l mod 32
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    if (!col_ok[c]) continue;
+    const int col = 32 * (cq + c) + i32;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const int o = 32 * rb + 8 * (r >> 2) + 4 * h + (r & 3);
+      if (o < jb.M && col < jb.N) unsafeAtomicAdd(jb.dw + (size_t)o * jb.ld_dw + col, acc[c][r]);
+    }
+  }
+  if (do_db) {
+    const float sr = rs + __shfl_xor(rs, 32, 64);
+    const int o = 32 * rb + i32;
+    if (h == 0 && o < jb.M) unsafeAtomicAdd(jb.db + o, sr);
+  }
 }
 
 }  // namespace
@@ -868,51 +1046,75 @@ extern "C" int srf_color_backward(const srf_color_desc* desc, const void* packed
   }
   int st = launch_status();
   if (st != SRF_OK) return st;
-  // weight gradients: the grouped dW GEMM over the stored dZ / input rows
+  // weight gradients over the stored dZ / input tiles (color_dw_kernel)
   const Ws& w = b.w;
   uint8_t* ws = b.ws;
-  DwArgs d;
-  memset(&d, 0, sizeof(d));
-  d.ldP = w.ld;
-  d.cols = w.cols;
-  d.cols_dev = point_count;
-  d.paired = 1;  // the training rows are tile blocks (store_tile_rows)
-  // weight gradients of the BF16 modes: the split-BF16 GEMM (two planes, three products) on the
-  // FP32 rows; FP32: the FP32 GEMM
-  const bool split_dw = g.prec != SRF_PREC_F32;
-  int nj = 0;
-  auto job = [&](size_t dz, int dz_rows, size_t x, int x_rows, int M, int N, float* dw, int ld, float* db) {
-    DwJob& jb = d.jobs[nj++];
+  CdwArgs c;
+  memset(&c, 0, sizeof(c));
+  auto cjob = [&](size_t dz, int dz_rows, size_t x, int x_rows, int M, int N, float* dw, int ld, float* db) {
+    CdwJob& jb = c.jobs[c.n_jobs++];
     jb.dz = ws + dz;
-    jb.dz_rows = dz_rows;
     jb.x = ws + x;
-    jb.x_rows = x_rows;
+    jb.nA = dz_rows / 16;
+    jb.nB = x_rows / 16;
     jb.M = M;
     jb.N = N;
     jb.dw = dw;
     jb.ld_dw = ld;
     jb.db = db;
-    // FP32: 128 x 160 tiles (dw_f32_narrow), BF16 modes: 256 x 256 tiles (dw_bf16s); one tile
-    // covers every job at the default sizes
-    const int tm = split_dw ? mlp::kDwTile : mlp::kDwNarrowRows, tn = split_dw ? mlp::kDwTile : mlp::kDwNarrowCols;
-    jb.mblocks = (M + tm - 1) / tm;
-    jb.nblocks = (N + tn - 1) / tn;
+    return jb.nA <= kCdwMaxA && jb.nB <= kCdwMaxB;
   };
-  job(w.dz1, kHid, w.x1, g.D1p, kHid, g.D1, d_w1, g.D1, d_b1);
-  job(w.dz2, kHid, w.h1, kHid, kHid, kHid, d_w2, kHid, d_b2);
-  job(w.dz3, 16, w.h2, kHid, 3, kHid, d_w3, kHid, d_b3);
-  job(w.dF, kMaxB, w.appT, g.Cp, g.NB, g.C, d_basis, g.C, nullptr);
-  d.n_jobs = nj;
-  long long target = mlp::dw_default_chunk(w.cols, nj, mlp::device_compute_units());
-  if (const int64_t v = tune_value(SRF_TUNE_COLOR_DW_CHUNK)) target = v;  // autotuned schedule
+  if (!cjob(w.dz1, kHid, w.x1, g.D1p, kHid, g.D1, d_w1, g.D1, d_b1) ||
+      !cjob(w.dz2, kHid, w.h1, kHid, kHid, kHid, d_w2, kHid, d_b2) ||
+      !cjob(w.dz3, 16, w.h2, kHid, 3, kHid, d_w3, kHid, d_b3) ||
+      !cjob(w.dF, kMaxB, w.appT, g.Cp, g.NB, g.C, d_basis, g.C, nullptr))
+    return SRF_EINVAL;
+  c.cols = w.cols;
+  c.cols_dev = point_count;
+  // 3840-point chunks by default (the items of the capacity's chunks past the device-side count
+  // exit at once; measured within 2048-8448 on the 300^2 step, whose times differ by their last
+  // partial round of blocks), or the autotuned size
+  const long long tuned = tune_value(SRF_TUNE_COLOR_DW_CHUNK);
+  c.chunk = tuned ? (int)tuned : 3840;
+  if (g.prec != SRF_PREC_F32) {
+    hipLaunchKernelGGL(color_dw_kernel, dim3(c.n_jobs * ((w.cols + c.chunk - 1) / c.chunk)), dim3(64 * kCdwWaves), 0,
+                       s, c);
+    return launch_status();
+  }
+  // FP32: the MLP's grouped FP32 GEMM on 128 x 160 tiles (one tile per job), whose fixed ring of
+  // four k-units in flight keeps its MFMA-bound blocks busy
+  mlp::DwArgs d;
+  memset(&d, 0, sizeof(d));
+  d.ldP = w.ld;
+  d.cols = w.cols;
+  d.cols_dev = point_count;
+  d.paired = 2;
+  d.n_jobs = c.n_jobs;
+  for (int i = 0; i < c.n_jobs; ++i) {
+    mlp::DwJob& jb = d.jobs[i];
+    const CdwJob& cj = c.jobs[i];
+    jb.dz = cj.dz;
+    jb.dz_rows = 16 * cj.nA;
+    jb.x = cj.x;
+    jb.x_rows = 16 * cj.nB;
+    jb.M = cj.M;
+    jb.N = cj.N;
+    jb.dw = cj.dw;
+    jb.ld_dw = cj.ld_dw;
+    jb.db = cj.db;
+    jb.mblocks = (cj.M + mlp::kDwNarrowRows - 1) / mlp::kDwNarrowRows;
+    jb.nblocks = (cj.N + mlp::kDwNarrowCols - 1) / mlp::kDwNarrowCols;
+  }
+  long long target = mlp::dw_default_chunk(w.cols, d.n_jobs, mlp::device_compute_units());
+  if (tuned) target = tuned;
   d.chunk = (int)(target < 2048 ? 2048 : (target > 65536 ? 65536 : target));
   d.n_chunks = (w.cols + d.chunk - 1) / d.chunk;
   int work = 0;
-  for (int i = 0; i < nj; ++i) {
+  for (int i = 0; i < d.n_jobs; ++i) {
     d.jobs[i].work_begin = work;
     d.jobs[i].work_count = d.jobs[i].mblocks * d.jobs[i].nblocks * d.n_chunks;
     work += d.jobs[i].work_count;
   }
   d.total_work = work;
-  return split_dw ? mlp::dw_bf16s(d, s) : mlp::dw_f32_narrow(d, s);
+  return mlp::dw_f32_narrow(d, s);
 }

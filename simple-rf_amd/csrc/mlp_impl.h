@@ -1263,7 +1263,10 @@ inline int device_compute_units() {
 struct DwArgs {
   DwJob jobs[kMaxJobs];
   int n_jobs, ldP, cols, chunk, n_chunks, total_work;
-  int paired;  // 1: dZ / X in the paired tile layout of store_tile; 0: plain feature-major rows
+  // dZ / X layout: 0 plain feature-major rows; 1 the paired tile blocks of store_tile, tile-major
+  // (tile t at 16 t rows of ldP points); 2 the same blocks point-block-major (the dz_rows / 16 or
+  // x_rows / 16 tiles of one stored wave block adjacent; mlp_dw_kernel only)
+  int paired;
   // optional device-side point count: only the columns below it, rounded up to whole 256-point
   // blocks, are reduced (the chunks past it exit); the columns up to that bound must hold finite
   // values with dZ = 0 beyond the count, as the forward and backward kernels leave them
@@ -1365,7 +1368,12 @@ __global__ __launch_bounds__(64 * kDwWaves) void mlp_dw_kernel(DwArgs a) {
         constexpr int PPW = F32L ? 32 : 64;  // points per stored wave block
         const int wb = p / PPW, u = (p % PPW) / KPTS;
         const int rec = 16 * (j >> 2) + 2 * (4 * u + g) + ((j >> 1) & 1);
-        src = base + (size_t)row0 * ldP * ELEM + (size_t)wb * 2048 + (j & 1) * 1024 + rec * 16;
+        // tile-major (paired 1: tile t's blocks span 16 rows of ldP points) or point-block-major
+        // (paired 2: the array's tiles of one wave block adjacent, 2 KB each)
+        const size_t blk = a.paired == 2
+                               ? ((size_t)wb * ((is_a ? jb.dz_rows : jb.x_rows) / 16) + row0 / 16) * 2048
+                               : (size_t)row0 * ldP * ELEM + (size_t)wb * 2048;
+        src = base + blk + (j & 1) * 1024 + rec * 16;
       } else {
         const int pg = F32L ? 4 * g : 8 * g;
         src = base + (((size_t)(row0 + j)) * ldP + p + pg) * ELEM;

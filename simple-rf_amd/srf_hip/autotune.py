@@ -23,7 +23,7 @@ KNOBS = {'dw_chunk': 0, 'color_dw_chunk': 1, 'gather_blocks': 2, 'gather_run': 3
 # candidate values per knob (0 = the library's built-in choice)
 CANDIDATES = {
     'dw_chunk': [0, 2048, 4096, 8192, 16384, 32768],
-    'color_dw_chunk': [0, 2048, 4096, 8192, 16384, 32768, 65536],
+    'color_dw_chunk': [0, 2304, 3072, 4608, 6144, 8192, 16384],
     'gather_blocks': [0, 2048, 4096, 16384, 32768],
     'gather_run': [0, 8, 16, 64, 128],
 }

@@ -114,8 +114,8 @@ def test_weight_grads_under_dw_chunk(cuda, value):
         assert ((a - b).abs().max() / b.abs().max().clamp_min(1e-30)).item() < 1e-5
 
 
-@pytest.mark.parametrize('value', [2048, 65536])
-def test_color_weight_grads_under_chunk(cuda, value):
+@pytest.mark.parametrize('prec,value', [('fp32', 2048), ('fp32', 65536), ('bf16', 2304), ('bf16', 65536)])
+def test_color_weight_grads_under_chunk(cuda, prec, value):
     g = torch.Generator().manual_seed(6)
     m, C = 30000, 72
     app = torch.randn(m, C, generator=g) * 0.3
@@ -124,7 +124,7 @@ def test_color_weight_grads_under_chunk(cuda, value):
     basis = (torch.rand(27, C, generator=g) * 2 - 1) / C ** 0.5
     (w1, b1), (w2, b2), (w3, b3) = lin(128, 30), lin(128, 128), lin(3, 128)
     gout = torch.randn(m, 3, generator=g)
-    spec = color.ColorSpec(n_features=C)
+    spec = color.ColorSpec(n_features=C, prec=prec)
 
     def grads():
         pd = [t.to(cuda).requires_grad_(True) for t in (basis, w1, b1, w2, b2, w3, b3)]
