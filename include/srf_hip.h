@@ -405,8 +405,7 @@ int srf_aabb_samples(int n_rays, int n_samples, const float* rays_o, const float
  * (Linear(n_features -> basis_dim), no bias; src/models/SimpleTensoRF09.py:1151) and
  * MlpFeaturesColorPredictor (:1370-1421): [PE_Lf(features), PE_Lv(view dir)] -> hidden ReLU ->
  * hidden ReLU -> 3 sigmoid.  n_features % 4 == 0 and <= 256, basis_dim <= 32, hidden == 128,
- * first-layer width basis_dim (2 features_pe + 1) + 3 (2 views_pe + 1) <= 240 (<= 160 for
- * SRF_PREC_BF16_SPLIT, whose backward weight stage is larger).
+ * first-layer width basis_dim (2 features_pe + 1) + 3 (2 views_pe + 1) <= 240.
  * prec: operands of the MFMA products of the forward and of the backward chain.  SRF_PREC_F32
  * (FP32 operands); SRF_PREC_BF16_SPLIT (split BF16: three planes in the forward, two in the
  * backward chain, as srf_mlp_desc's); SRF_PREC_BF16 (BF16 operands rounded from the FP32 values).

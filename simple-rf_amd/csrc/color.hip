@@ -29,8 +29,6 @@
 // GEMM, two planes per operand (both BF16 modes).
 #include <string.h>
 
-#include <mutex>
-
 #include "mlp_impl.h"
 
 namespace srf {
@@ -56,9 +54,6 @@ constexpr int bwd_prec_of(int prec) { return prec == SRF_PREC_BF16_SPLIT ? kBFS2
 // 48 KB, split backward 64 KB.
 template <int P>
 constexpr int stage_bytes() { return 2 * kST * mlp::kc_of<P>() * split_planes<P>() * 1024; }
-constexpr int stage_bytes_of(int P) {
-  return P == kBFS3 ? stage_bytes<kBFS3>() : (P == kBFS2 ? stage_bytes<kBFS2>() : stage_bytes<kF32>());
-}
 // Features per k-unit: 16 (FP32), 32 (BF16 operands).
 template <int P>
 constexpr int kunit() { return P == kF32 ? 16 : 32; }
@@ -81,9 +76,6 @@ inline size_t frag_floats(const Geo& g, int tiles, int K, int planes) {
   return (size_t)tiles * (K / g.kal) * planes * 256;
 }
 
-// Rows of the per-wave LDS tile of the backward: d x1 (D1p rows), later d F (kMaxB rows).
-__host__ __device__ inline int bwd_rows(const Geo& g) { return g.D1p > kMaxB ? g.D1p : kMaxB; }
-constexpr int kBwdLdsRowBytes = kWaves * (kPW + 1) * (int)sizeof(float);  // one row of every wave's tile
 
 bool make_geo(const srf_color_desc* d, Geo* g) {
   if (!d || d->n_features < 4 || d->n_features % 4 != 0 || d->n_features > kMaxC || d->basis_dim < 1 ||
@@ -104,8 +96,6 @@ bool make_geo(const srf_color_desc* d, Geo* g) {
   g->D1 = g->NB * (2 * g->Lf + 1) + (g->view ? 3 * (2 * g->Lv + 1) : 0);
   g->D1p = (g->D1 + g->kal - 1) / g->kal * g->kal;
   if (g->D1p > kMaxD1) return false;
-  // the backward's per-wave tile and its weight stage share the CU's 160 KB of LDS
-  if (bwd_rows(*g) * kBwdLdsRowBytes + stage_bytes_of(bwd_prec_of(g->prec)) > 160 * 1024) return false;
   size_t o = 0;
   auto take = [&](size_t n) { size_t r = o; o += n; return r; };
   g->fB = take(frag_floats(*g, 2, g->Cp, g->fpl));
@@ -561,19 +551,18 @@ __device__ __forceinline__ float load_elem(const uint8_t* base, size_t arr, int 
 }
 
 template <int P>
-__global__ __launch_bounds__(64 * kWaves) void color_bwd_kernel(BwdArgs a) {
+__device__ __forceinline__ void color_bwd_body(const BwdArgs& a) {
   __shared__ __attribute__((aligned(16))) uint8_t stage[stage_bytes<P>()];
+  // per wave: the basis outputs F of its 32 points and the gradient dF being accumulated
+  __shared__ float sF[kWaves][kMaxB][kPW + 1];
+  __shared__ float sdF[kWaves][kMaxB][kPW + 1];
   constexpr int KU = kunit<P>();
   constexpr int NPL = split_planes<P>();
-  extern __shared__ float lds_dyn[];
   const Geo& g = a.g;
   const int n = active_rows(a.n, a.count);
   if (!block_has_rows(blockIdx.x, n)) return;  // uniform over the block
   const int lane = threadIdx.x & 63, gq = lane >> 4, j = lane & 15;
   const int wv = threadIdx.x >> 6;
-  const int xrows = bwd_rows(g);
-  float* sXw = lds_dyn + (size_t)wv * xrows * (kPW + 1);
-  auto sX = [&](int row, int col) -> float& { return sXw[row * (kPW + 1) + col]; };
   const int p0 = (blockIdx.x * kWaves + wv) * kPW;
   // 1) dz3 = d_rgb * s (1 - s)
   f32x4 dz3[1][kCB];
@@ -629,7 +618,27 @@ __global__ __launch_bounds__(64 * kWaves) void color_bwd_kernel(BwdArgs a) {
       for (int r = 0; r < 4; ++r) dz1[T][cb][r] = hv[cb][r] > 0.f ? dz1[T][cb][r] : 0.f;
     store_tile_rows(a.ws, a.w.dz1, kHT, T, p0, dz1[T]);
   }
-  // 4) dx1 = W1^T dz1 into LDS, kST output tiles per staged pass and single tiles for the rest
+  // 4) dx1 = W1^T dz1, kST output tiles per staged pass and single tiles for the rest, folded into the
+  // positional-encoding backward as its tiles leave the accumulators:
+  //   dF[f] = dx[f] + sum_k 2^k (dx[sin_k f] cos(2^k F[f]) - dx[cos_k f] sin(2^k F[f]))
+  // accumulated per (f, point) in the wave's LDS rows in increasing input row i = b NB + f (the rows of
+  // one f lie in different tiles, 27 apart, and the tiles come in order), so the sums run in the
+  // reference's order; the view-direction rows (i >= NB (2 Lf + 1)) have no gradient to keep.
+  {
+    f32x4 fv[kCB];
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+      load_tile_rows(a.ws, a.w.F, kMaxB / 16, t, p0, fv);
+#pragma unroll
+      for (int cb = 0; cb < kCB; ++cb)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          sF[wv][16 * t + 4 * gq + r][16 * cb + j] = fv[cb][r];
+          sdF[wv][16 * t + 4 * gq + r][16 * cb + j] = 0.f;  // rows past basis_dim stay 0
+        }
+    }
+  }
+  const int nf = g.NB * (2 * g.Lf + 1);
   auto fin_dz1 = [&](int u, int cb) { return dz1[u][cb]; };
   auto dx_tiles = [&](auto OUTC, int T0) {
     constexpr int OUT = decltype(OUTC)::value;
@@ -642,52 +651,40 @@ __global__ __launch_bounds__(64 * kWaves) void color_bwd_kernel(BwdArgs a) {
 #pragma unroll
     for (int T = 0; T < OUT; ++T)
 #pragma unroll
-      for (int cb = 0; cb < kCB; ++cb)
+      for (int r = 0; r < 4; ++r) {
+        const int i = 16 * (T0 + T) + 4 * gq + r;
+        if (i < nf) {
+          const int b = i / g.NB, f = i - b * g.NB;
 #pragma unroll
-        for (int r = 0; r < 4; ++r) sX(16 * (T0 + T) + 4 * gq + r, 16 * cb + j) = dx[T][cb][r];
+          for (int cb = 0; cb < kCB; ++cb) {
+            const int col = 16 * cb + j;
+            const float v = dx[T][cb][r];
+            float& d = sdF[wv][f][col];
+            if (b == 0) {
+              d = v;
+            } else {
+              const float fr = (float)(1 << ((b - 1) >> 1));
+              float sn, cs;
+              sincosf(__fmul_rn(sF[wv][f][col], fr), &sn, &cs);
+              if (b & 1) d = __fadd_rn(d, __fmul_rn(__fmul_rn(v, cs), fr));
+              else d = __fsub_rn(d, __fmul_rn(__fmul_rn(v, sn), fr));
+            }
+          }
+        }
+      }
   };
   int T0 = 0;
   for (; T0 + kST <= g.D1p / 16; T0 += kST) dx_tiles(std::integral_constant<int, kST>{}, T0);
   for (; T0 < g.D1p / 16; ++T0) dx_tiles(std::integral_constant<int, 1>{}, T0);
-  __syncthreads();
-  // 5) positional-encoding backward: dF[f] = dx[f] + sum_k 2^k (dx[sin_k f] cos(2^k F) - dx[cos_k f] sin(2^k F))
-  constexpr int kIt = kMaxB * kPW / 64;  // (feature, point) pairs per lane
-  float dFv[kIt];
-#pragma unroll
-  for (int it = 0; it < kIt; ++it) {
-    const int e = lane + 64 * it;
-    const int f = e / kPW, col = e - f * kPW;
-    float d = 0.f;
-    if (f < g.NB) {
-      const float Fv = load_elem(a.ws, a.w.F, kMaxB / 16, f, p0 + col);
-      d = sX(f, col);
-      for (int k = 0; k < g.Lf; ++k) {
-        const float fr = (float)(1 << k);
-        const float x = __fmul_rn(Fv, fr);
-        const float gs = sX((2 * k + 1) * g.NB + f, col);
-        const float gc = sX((2 * k + 2) * g.NB + f, col);
-        d = __fadd_rn(d, __fmul_rn(__fmul_rn(gs, cosf(x)), fr));
-        d = __fsub_rn(d, __fmul_rn(__fmul_rn(gc, sinf(x)), fr));
-      }
-    }
-    dFv[it] = d;
-  }
-  __syncthreads();
-#pragma unroll
-  for (int it = 0; it < kIt; ++it) {
-    const int e = lane + 64 * it;
-    const int f = e / kPW, col = e - f * kPW;
-    sX(f, col) = dFv[it];
-  }
-  __syncthreads();
-  // 6) d_app = B^T dF (point-major, float4 per lane)
+  mlp::wave_lds_sync();  // the wave's dF rows, written by its lanes, are read across lanes below
+  // 5) d_app = B^T dF (point-major, float4 per lane)
   f32x4 dFt[2][kCB];
 #pragma unroll
   for (int t = 0; t < 2; ++t)
 #pragma unroll
     for (int cb = 0; cb < kCB; ++cb)
 #pragma unroll
-      for (int r = 0; r < 4; ++r) dFt[t][cb][r] = sX(16 * t + 4 * gq + r, 16 * cb + j);
+      for (int r = 0; r < 4; ++r) dFt[t][cb][r] = sdF[wv][16 * t + 4 * gq + r][16 * cb + j];
 #pragma unroll
   for (int t = 0; t < 2; ++t) store_tile_rows(a.ws, a.w.dF, kMaxB / 16, t, p0, dFt[t]);  // rows past basis_dim: 0
   auto fin_dF = [&](int u, int cb) { return dFt[u][cb]; };
@@ -716,6 +713,15 @@ __global__ __launch_bounds__(64 * kWaves) void color_bwd_kernel(BwdArgs a) {
   for (; Ta + kST <= g.Cp / 16; Ta += kST) da_tiles(std::integral_constant<int, kST>{}, Ta);
   for (; Ta < g.Cp / 16; ++Ta) da_tiles(std::integral_constant<int, 1>{}, Ta);
 }
+
+// The FP32 and BF16 chains fit two waves per SIMD (two blocks per CU: 66 KB of LDS each); the
+// split-BF16 chain, with its 64 KB backward stage and the plane operands, runs one.
+template <int P>
+__global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(2, 2))) void color_bwd_kernel(BwdArgs a) {
+  color_bwd_body<P>(a);
+}
+
+__global__ __launch_bounds__(64 * kWaves) void color_bwd_split_kernel(BwdArgs a) { color_bwd_body<kBFS2>(a); }
 
 // ------------------------------------------------------------------- weight gradients
 // dW = dZ X^T (+ db = row sums of dZ) of the four colour jobs (W1, W2, W3, basis) over the stored
@@ -890,42 +896,6 @@ __global__ __launch_bounds__(64 * kCdwWaves) void color_dw_kernel(CdwArgs a) {
 using namespace srf;
 using namespace srf::color;
 
-namespace {
-// The backward's per-wave LDS tile can take the block past the 64 KB default of dynamic LDS.  The
-// kernel's limit is raised once per device, to what the largest supported geometry needs, so that
-// every later call (a call under stream capture included, where no such runtime call belongs)
-// launches without touching it.
-template <int P>
-hipError_t raise_bwd_lds(int max_rows) {
-  const int bytes = max_rows * kBwdLdsRowBytes;
-  return hipFuncSetAttribute(reinterpret_cast<const void*>(&color_bwd_kernel<P>),
-                             hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
-}
-
-int raise_bwd_lds_limit() {
-  constexpr int kMaxDevices = 64;
-  static std::mutex mutex;
-  static bool raised[kMaxDevices];
-  int dev = -1;
-  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return SRF_EHIP;
-  std::lock_guard<std::mutex> lock(mutex);
-  if (!raised[dev]) {
-    // per precision: the rows of the largest geometry make_geo accepts beside that stage
-    auto rows = [](int stage) {
-      const int r = (160 * 1024 - stage) / kBwdLdsRowBytes;
-      const int cap = kMaxD1 > kMaxB ? kMaxD1 : kMaxB;
-      return r < cap ? r : cap;
-    };
-    if (raise_bwd_lds<kF32>(rows(stage_bytes<kF32>())) != hipSuccess ||
-        raise_bwd_lds<kBFS2>(rows(stage_bytes<kBFS2>())) != hipSuccess ||
-        raise_bwd_lds<kBFS1>(rows(stage_bytes<kBFS1>())) != hipSuccess)
-      return SRF_EHIP;
-    raised[dev] = true;
-  }
-  return SRF_OK;
-}
-}  // namespace
-
 extern "C" size_t srf_color_packed_bytes(const srf_color_desc* desc) {
   Geo g;
   return make_geo(desc, &g) ? g.total * sizeof(float) : 0;
@@ -1036,13 +1006,11 @@ extern "C" int srf_color_backward(const srf_color_desc* desc, const void* packed
   b.d_rgb = d_rgb;
   b.d_app = d_app;
   b.ws = reinterpret_cast<uint8_t*>(workspace);
-  const size_t lds = (size_t)kWaves * bwd_rows(g) * (kPW + 1) * sizeof(float);
-  if (const int st = raise_bwd_lds_limit(); st != SRF_OK) return st;
   const dim3 grid(b.w.cols / (kPW * kWaves));
   switch (bwd_prec_of(g.prec)) {
-    case kBFS2: hipLaunchKernelGGL(color_bwd_kernel<kBFS2>, grid, dim3(64 * kWaves), lds, s, b); break;
-    case kBFS1: hipLaunchKernelGGL(color_bwd_kernel<kBFS1>, grid, dim3(64 * kWaves), lds, s, b); break;
-    default: hipLaunchKernelGGL(color_bwd_kernel<kF32>, grid, dim3(64 * kWaves), lds, s, b); break;
+    case kBFS2: hipLaunchKernelGGL(color_bwd_split_kernel, grid, dim3(64 * kWaves), 0, s, b); break;
+    case kBFS1: hipLaunchKernelGGL(color_bwd_kernel<kBFS1>, grid, dim3(64 * kWaves), 0, s, b); break;
+    default: hipLaunchKernelGGL(color_bwd_kernel<kF32>, grid, dim3(64 * kWaves), 0, s, b); break;
   }
   int st = launch_status();
   if (st != SRF_OK) return st;
@@ -1088,7 +1056,7 @@ extern "C" int srf_color_backward(const srf_color_desc* desc, const void* packed
   d.ldP = w.ld;
   d.cols = w.cols;
   d.cols_dev = point_count;
-  d.paired = 2;
+  d.paired = 1;  // tile blocks (point-block-major, as the narrow tiling reads them)
   d.n_jobs = c.n_jobs;
   for (int i = 0; i < c.n_jobs; ++i) {
     mlp::DwJob& jb = d.jobs[i];

@@ -1263,9 +1263,8 @@ inline int device_compute_units() {
 struct DwArgs {
   DwJob jobs[kMaxJobs];
   int n_jobs, ldP, cols, chunk, n_chunks, total_work;
-  // dZ / X layout: 0 plain feature-major rows; 1 the paired tile blocks of store_tile, tile-major
-  // (tile t at 16 t rows of ldP points); 2 the same blocks point-block-major (the dz_rows / 16 or
-  // x_rows / 16 tiles of one stored wave block adjacent; mlp_dw_kernel only)
+  // dZ / X layout: 0 plain feature-major rows; 1 the paired tile blocks of store_tile (tile-major
+  // for the wide tilings, point-block-major for the narrow one; see mlp_dw_kernel)
   int paired;
   // optional device-side point count: only the columns below it, rounded up to whole 256-point
   // blocks, are reduced (the chunks past it exit); the columns up to that bound must hold finite
@@ -1368,11 +1367,12 @@ __global__ __launch_bounds__(64 * kDwWaves) void mlp_dw_kernel(DwArgs a) {
         constexpr int PPW = F32L ? 32 : 64;  // points per stored wave block
         const int wb = p / PPW, u = (p % PPW) / KPTS;
         const int rec = 16 * (j >> 2) + 2 * (4 * u + g) + ((j >> 1) & 1);
-        // tile-major (paired 1: tile t's blocks span 16 rows of ldP points) or point-block-major
-        // (paired 2: the array's tiles of one wave block adjacent, 2 KB each)
-        const size_t blk = a.paired == 2
-                               ? ((size_t)wb * ((is_a ? jb.dz_rows : jb.x_rows) / 16) + row0 / 16) * 2048
-                               : (size_t)row0 * ldP * ELEM + (size_t)wb * 2048;
+        // the wide tilings read tile-major blocks (tile t's blocks span 16 rows of ldP points, the MLP
+        // workspace); the narrow one point-block-major blocks (the array's tiles of one wave block
+        // adjacent, 2 KB each: the colour network's workspace)
+        size_t blk;
+        if constexpr (kRowWaves) blk = ((size_t)wb * ((is_a ? jb.dz_rows : jb.x_rows) / 16) + row0 / 16) * 2048;
+        else blk = (size_t)row0 * ldP * ELEM + (size_t)wb * 2048;
         src = base + blk + (j & 1) * 1024 + rec * 16;
       } else {
         const int pg = F32L ? 4 * g : 8 * g;
@@ -1480,9 +1480,8 @@ __global__ __launch_bounds__(64 * kDwWaves) void mlp_dw_kernel(DwArgs a) {
   }
 }
 
-// Split-BF16 weight gradients: the FP32 dZ and X tiles (the paired tile layout of store_tile or
-// plain feature-major rows, DMA'd through the same five-slot ring as the FP32 GEMM, 16 points per
-// k-unit) are split into two BF16 planes and reduced
+// Split-BF16 weight gradients: the FP32 dZ and X tiles (the FP32 layouts, DMA'd through the same
+// five-slot ring as the FP32 GEMM, 16 points per k-unit) are split into two BF16 planes and reduced
 // with 32x32x16 BF16 MFMAs, dW += dZ_hi X_hi^T + dZ_hi X_lo^T + dZ_lo X_hi^T.  One 16-point unit is
 // exactly one K = 16 step: lane (h, i) of an A operand holds row i of a 32-row block at the eight
 // points of lane groups 2h and 2h + 1 of the FP32 fragments (records 16 (2h) + (i mod 16) and
@@ -1520,9 +1519,7 @@ __global__ __launch_bounds__(64 * kDwWaves) void mlp_dw_split_kernel(DwArgs a) {
 #pragma unroll
   for (int b = 0; b < 4; ++b) nv[b] = n0 + 16 * (qn + 2 * b) < jb.x_rows;
   // DMA of one 16-point unit: the 16 dZ and 16 X fragments of the tile, as in mlp_dw_kernel (a row
-  // tile beyond the job's rows re-reads its first tile, so every wave issues four loads per unit).
-  // Either layout gives lane record (g, j) four points of feature j, the same four in the dZ and X
-  // fragments, which is all the plane products below need.
+  // tile beyond the job's rows re-reads its first tile, so every wave issues four loads per unit)
   auto dma_unit = [&](int p, uint8_t* buf) SRF_AI {
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
@@ -1532,14 +1529,9 @@ __global__ __launch_bounds__(64 * kDwWaves) void mlp_dw_split_kernel(DwArgs a) {
       int row0 = (is_a ? m0 : n0) + 16 * rt;
       if (row0 >= (is_a ? jb.dz_rows : jb.x_rows)) row0 = is_a ? m0 : n0;
       const uint8_t* base = is_a ? jb.dz : jb.x;
-      const uint8_t* src;
-      if (a.paired) {
-        const int wb = p / 32, u = (p % 32) / KPTS;
-        const int rec = 16 * (j >> 2) + 2 * (4 * u + g) + ((j >> 1) & 1);
-        src = base + (size_t)row0 * ldP * 4 + (size_t)wb * 2048 + (j & 1) * 1024 + rec * 16;
-      } else {
-        src = base + (((size_t)(row0 + j)) * ldP + p + 4 * g) * 4;  // plain feature-major rows
-      }
+      const int wb = p / 32, u = (p % 32) / KPTS;
+      const int rec = 16 * (j >> 2) + 2 * (4 * u + g) + ((j >> 1) & 1);
+      const uint8_t* src = base + (size_t)row0 * ldP * 4 + (size_t)wb * 2048 + (j & 1) * 1024 + rec * 16;
       __builtin_amdgcn_global_load_lds((const void*)src, (__attribute__((address_space(3))) void*)(buf + f * 1024),
                                        16, 0, 0);
     }

@@ -96,7 +96,7 @@ def test_color_descriptor_precisions():
     """Packed-weight sizes of the 300^2 colour network (144 features, features PE 2, first layer
     138 wide) in each operand precision: FP32 fragments pad reductions to 16 features; the BF16
     modes pad to 32 and hold one 1 KB fragment per BF16 plane (split: three forward, two
-    backward).  Unsupported precisions and split geometries past the backward's LDS are rejected."""
+    backward).  Unsupported precisions and first layers past 240 inputs are rejected."""
     from srf_hip import color
     base = dict(n_features=144, features_pe=2)
     nb = lambda **kw: _lib.lib().srf_color_packed_bytes(ctypes.byref(color.ColorSpec(**{**base, **kw}).desc()))
@@ -110,9 +110,9 @@ def test_color_descriptor_precisions():
     fwd, bwd = 2 * 5 + 8 * 5 + 8 * 4 + 4, 8 + 8 * 4 + 10 * 4 + 10
     assert nb(prec='bf16') == 4 * ((fwd + bwd) * kb + bias)
     assert nb(prec='bf16_split') == 4 * ((3 * fwd + 2 * bwd) * kb + bias)
-    # first layer 27 x 7 + 3 = 192 wide: within FP32's limit, past split BF16's
-    assert nb(prec='fp32', features_pe=3) > 0 and nb(prec='bf16', features_pe=3) > 0
-    assert nb(prec='bf16_split', features_pe=3) == 0
+    # first layer 27 x 7 + 3 = 192 wide: supported; 27 x 9 + 3 = 246: past the limit
+    for prec in ('fp32', 'bf16', 'bf16_split'):
+        assert nb(prec=prec, features_pe=3) > 0 and nb(prec=prec, features_pe=4) == 0
     d = color.ColorSpec(**base).desc()
     d.prec = 2  # SRF_PREC_F16: no colour kernels
     assert _lib.lib().srf_color_packed_bytes(ctypes.byref(d)) == 0
