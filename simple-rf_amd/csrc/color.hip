@@ -12,10 +12,10 @@
 // by srf_color_pack) and 16-point column blocks as N; a wave owns 32 points (2 column blocks), so
 // one layer's D tiles are the next layer's B operand without data movement.  The A fragments are
 // streamed global -> LDS by the four waves of a block (double-buffered, as in mlp::mma_tiles) and
-// shared by them.  The basis output F
-// and the view directions go through a per-wave LDS tile from which every lane builds its
-// positional-encoding inputs.  Training stores the layer inputs feature-major (row = feature,
-// column = point) for the weight-gradient GEMM, which is the grouped dW kernel of the MLP.
+// shared by them.  The basis output F and the view directions go through a per-wave LDS tile from
+// which every lane builds its positional-encoding inputs.  Training stores the layer inputs and
+// output gradients as FP32 tile blocks (store_tile_rows) for the weight-gradient GEMMs: the MLP's
+// FP32 grouped GEMM (mlp_dw_kernel, 128 x 160 tiles) or color_dw_kernel.
 //
 // Operand precision (desc->prec).  Activations, accumulators and the stored layer inputs are FP32
 // in every mode; the products of the forward and of the backward chain multiply
@@ -25,8 +25,8 @@
 //                         backward chain (three products), as the NeRF MLP's split mode
 //   SRF_PREC_BF16         BF16 operands rounded from the FP32 activations and weights (one plane)
 // Reduction depths are padded to whole k-units (32 features for the BF16 modes) with zeros.  The
-// weight gradients reduce the stored FP32 rows with the FP32 GEMM (SRF_PREC_F32) or the split-BF16
-// GEMM, two planes per operand (both BF16 modes).
+// weight gradients reduce the stored FP32 tiles with the FP32 GEMM (SRF_PREC_F32) or the split-BF16
+// one, two planes per operand (both BF16 modes).
 #include <string.h>
 
 #include "mlp_impl.h"

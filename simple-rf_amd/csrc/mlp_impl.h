@@ -808,15 +808,6 @@ __device__ __forceinline__ void load_tile_f32(Rsrc src, uint32_t blk, f32x4 (&v)
   v[1] = f32x4{sel_odd(rb, kb), sel_odd(rd, kd), sel_odd(kb, rb), sel_odd(kd, rd)};
 }
 
-// Byte offset of element (feature f, point p) inside an FP32 array of store_tile blocks with row
-// stride ldP points (CB = 2: 32 points per block).
-__host__ __device__ inline size_t tile_elem_f32(int f, int p, int ldP) {
-  const int fl = f & 15, q = p & 31, jj = q & 15;
-  const int rec = 16 * (fl >> 2) + 2 * (jj >> 1) + ((fl >> 1) & 1);
-  return (size_t)(f >> 4) * 16 * ldP * 4 + (size_t)(p >> 5) * 2048 + 1024 * (fl & 1) + 16 * rec +
-         4 * (2 * (q >> 4) + (jj & 1));
-}
-
 __device__ __forceinline__ float sigmoidf_(float x) { return 1.0f / (1.0f + expf(-x)); }
 
 __device__ __forceinline__ float softplusf_(float x) { return x > 20.f ? x : log1pf(expf(x)); }
