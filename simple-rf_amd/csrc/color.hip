@@ -727,14 +727,15 @@ __global__ __launch_bounds__(64 * kWaves) void color_bwd_split_kernel(BwdArgs a)
 // dW = dZ X^T (+ db = row sums of dZ) of the four colour jobs (W1, W2, W3, basis) over the stored
 // FP32 tile blocks, in the BF16 modes (FP32: the MLP's FP32 GEMM, mlp_dw_kernel).  A work item is one
 // job over one chunk of points; its block copies, per 16-point unit, only the job's own fragments
-// (nA dZ and nB X row tiles, one contiguous 1 KB each) global -> LDS into a ring whose slot is
-// (nA + nB) KB, so a job of 9-24 fragments keeps 5-15 units in flight in the 160 KB of LDS.  The
+// (nA dZ and nB X row tiles, one contiguous 1 KB each) global -> LDS into a ring whose slot holds two
+// units, 2 (nA + nB) KB, so a job of 9-24 fragments keeps 4-14 units in flight in the 160 KB of LDS
+// with one barrier per two units (10 % faster than one per unit on the 300^2 step).  The
 // products are split-BF16 32x32x16 MFMAs (dZ_hi X_hi + dZ_hi X_lo + dZ_lo X_hi, as
 // mlp_dw_split_kernel); wave w multiplies row block w & 3 (32 rows) by column blocks 4 (w >> 2) .. + 3
 // (32 columns each).  Partial sums go to dW with FP32 atomics.
 constexpr int kCdwWaves = 8;
 constexpr int kCdwRingKB = 160;
-constexpr int kCdwMaxSlots = 16;
+constexpr int kCdwMaxSlots = 8;
 constexpr int kCdwMaxA = 8, kCdwMaxB = 16;  // row tiles: dZ <= 128 rows, X <= 256 rows
 
 struct CdwJob {
@@ -785,11 +786,14 @@ __global__ __launch_bounds__(64 * kCdwWaves) void color_dw_kernel(CdwArgs a) {
   const int pe = min(pb + a.chunk, cols);
   if (pb >= pe) return;  // past the device-side count (uniform over the block)
   const int nA = jb.nA, nB = jb.nB, nf = nA + nB;
-  const int nslots = min(kCdwMaxSlots, kCdwRingKB / nf);
+  const int nslots = min(kCdwMaxSlots, kCdwRingKB / (2 * nf));  // two-unit slots
   const int dma_n = (nf - w + kCdwWaves - 1) / kCdwWaves;  // this wave's fragments per unit (<= 3)
   // unit at point p: fragment f < nA is dZ row tile f, else X row tile f - nA; lane (g, j) copies
   // the 16 B of feature j at the four points of lane group g (the record of mlp::store_tile), to
   // slot byte f KB + 16 lane
+  // the job's array bases, read from the kernel arguments once (wave-uniform, scalar registers)
+  const uint8_t* const dz_base = jb.dz;
+  const uint8_t* const x_base = jb.x;
   auto dma_unit = [&](int p, uint8_t* buf) {
     const int u = (p & 31) >> 4;
     const int rec = 16 * (j >> 2) + 2 * (4 * u + g) + ((j >> 1) & 1);
@@ -797,7 +801,7 @@ __global__ __launch_bounds__(64 * kCdwWaves) void color_dw_kernel(CdwArgs a) {
     for (int q = 0; q < 3; ++q) {
       const int f = w + kCdwWaves * q;
       if (f >= nf) break;
-      const uint8_t* src = f < nA ? jb.dz + blk_off(nA, f, p) : jb.x + blk_off(nB, f - nA, p);
+      const uint8_t* src = f < nA ? dz_base + blk_off(nA, f, p) : x_base + blk_off(nB, f - nA, p);
       __builtin_amdgcn_global_load_lds((const void*)(src + in_blk),
                                        (__attribute__((address_space(3))) void*)(buf + f * 1024), 16, 0, 0);
     }
@@ -841,21 +845,15 @@ __global__ __launch_bounds__(64 * kCdwWaves) void color_dw_kernel(CdwArgs a) {
     lo = u32x4{t[0][1], t[1][1], t[2][1], t[3][1]};
     if (sum) rs += ((v0[0] + v0[1]) + (v0[2] + v0[3])) + ((v1[0] + v1[1]) + (v1[2] + v1[3]));
   };
-  const int n_units = (pe - pb) / 16;
-  const int slot = nf * 1024;
-  for (int u = 0; u < nslots - 1 && u < n_units; ++u) dma_unit(pb + 16 * u, ring + u * slot);
-  for (int u = 0; u < n_units; ++u) {
-    // unit u has landed once at most (units issued after it) x dma_n loads are outstanding
-    wait_vmem_le(min(nslots - 2, n_units - 1 - u) * dma_n);
-    // every wave's part of unit u is in LDS, and slot (u - 1) mod nslots is free (the waves consumed
-    // their reads of it in the MFMAs of iteration u - 1); compiler-only fences around a plain
-    // s_barrier, which would otherwise drain the prefetches (vmcnt(0))
-    asm volatile("" ::: "memory");
-    __builtin_amdgcn_s_barrier();
-    asm volatile("" ::: "memory");
-    if (u + nslots - 1 < n_units) dma_unit(pb + 16 * (u + nslots - 1), ring + ((u + nslots - 1) % nslots) * slot);
-    if (!busy) continue;
-    const uint8_t* fr = ring + (u % nslots) * slot;
+  // The ring advances by steps of two units (one 32-point tile block, both halves): one barrier and one
+  // wait per step.
+  const int n_steps = (pe - pb) / 32;
+  const int slot = 2 * nf * 1024;
+  auto dma_step = [&](int p, uint8_t* buf) {
+    dma_unit(p, buf);
+    dma_unit(p + 16, buf + nf * 1024);
+  };
+  auto unit_products = [&](const uint8_t* fr) {
     u32x4 ah, al, bh[4], bl[4];
     planes(fr, fa, ah, al, do_db);
 #pragma unroll
@@ -869,6 +867,23 @@ __global__ __launch_bounds__(64 * kCdwWaves) void color_dw_kernel(CdwArgs a) {
           acc[c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, pr == 0 ? al : ah),
                                                            __builtin_bit_cast(bf16x8, pr == 1 ? bl[c] : bh[c]),
                                                            acc[c], 0, 0, 0);
+  };
+  for (int st = 0; st < nslots - 1 && st < n_steps; ++st) dma_step(pb + 32 * st, ring + st * slot);
+  for (int st = 0; st < n_steps; ++st) {
+    // step st has landed once at most (steps issued after it) x 2 dma_n loads are outstanding
+    wait_vmem_le(min(nslots - 2, n_steps - 1 - st) * 2 * dma_n);
+    // every wave's part of step st is in LDS, and slot (st - 1) mod nslots is free (the waves consumed
+    // their reads of it in the MFMAs of iteration st - 1); compiler-only fences around a plain
+    // s_barrier, which would otherwise drain the prefetches (vmcnt(0))
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_s_barrier();
+    asm volatile("" ::: "memory");
+    if (st + nslots - 1 < n_steps)
+      dma_step(pb + 32 * (st + nslots - 1), ring + ((st + nslots - 1) % nslots) * slot);
+    if (!busy) continue;
+    const uint8_t* fr = ring + (st % nslots) * slot;
+    unit_products(fr);
+    unit_products(fr + nf * 1024);
   }
   if (!row_ok) return;
   // 32x32 D layout: lane l, register r holds row 8 (r / 4) + 4 (l / 32) + (r mod 4), column l mod 32
