@@ -16,7 +16,7 @@ pytestmark = pytest.mark.gpu
 CP = dict(decomposition_type='CandecompParafac', num_components_density=(16,), num_components_color=(48,))
 
 
-def _model(cuda, cp=False, features_pe=0):
+def _model(cuda, cp=False, features_pe=0, color_precision='fp32'):
     torch.manual_seed(0)
     coarse = dict(num_voxels_initial=32 ** 3, features_positional_encoding_degree=features_pe)
     if cp:
@@ -24,6 +24,7 @@ def _model(cuda, cp=False, features_pe=0):
     # white background and deterministic sampling: both runs see the same inputs at every step
     c = cfgs.simple_tensorf_configs(white_bkgd=True, perturb=False, coarse=coarse,
                                     augmented=dict(num_voxels_initial=16 ** 3))
+    c['model']['color_precision'] = color_precision
     m = get_model(c, model_configs=cfgs.model_configs()).to(cuda).train()
     if cp:
         with torch.no_grad():
@@ -68,10 +69,11 @@ def test_train_step_has_no_host_synchronisation(cuda):
 # and optimiser state are set to the eager model's before every replay), so that the comparison
 # sees one step's summation-order differences only: over several steps those would grow wherever a
 # density ReLU or the surface threshold flips, and the trajectories would part for that reason alone.
-@pytest.mark.parametrize('kind,cp,features_pe,param_tol', [('sgd', False, 0, None), ('adam', False, 0, 1e-3),
-                                                          ('sgd', True, 0, None), ('sgd', False, 2, None)])
-def test_graph_replays_match_eager_steps(cuda, kind, cp, features_pe, param_tol):
-    a = _model(cuda, cp, features_pe)
+@pytest.mark.parametrize('kind,cp,features_pe,param_tol,color_precision', [
+    ('sgd', False, 0, None, 'fp32'), ('adam', False, 0, 1e-3, 'fp32'), ('sgd', True, 0, None, 'fp32'),
+    ('sgd', False, 2, None, 'fp32'), ('sgd', False, 2, None, 'bf16'), ('adam', False, 2, 1e-3, 'bf16_split')])
+def test_graph_replays_match_eager_steps(cuda, kind, cp, features_pe, param_tol, color_precision):
+    a = _model(cuda, cp, features_pe, color_precision)
     b = copy.deepcopy(a)
     batch, inp = _inputs(cuda)
     opt_a, opt_b = _opt(kind, a), _opt(kind, b)
