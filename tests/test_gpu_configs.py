@@ -361,11 +361,13 @@ def test_vm512_ragged_bf16_colour_rgb_against_fp64(cuda):
     assert err.max().item() < 2e-3
 
 
-def test_vm512_ragged_gradients_against_fp64_finite_differences(cuda):
-    """The kernels' FP32 gradients on a 64-ray uneven batch against FP64 central differences of the
+@pytest.mark.parametrize('color_precision', ['fp32', 'bf16_split'])
+def test_vm512_ragged_gradients_against_fp64_finite_differences(cuda, color_precision):
+    """The kernels' gradients on a 64-ray uneven batch against FP64 central differences of the
     oracle's loss: for each of seven parameter tensors (density and appearance planes and lines, the
-    basis, the colour network's first weight and last bias) the four entries with the largest gradient."""
-    tm = _vm512(cuda)
+    basis, the colour network's first weight and last bias) the four entries with the largest gradient;
+    the colour network on FP32 and on split-BF16 operands."""
+    tm = _vm512(cuda, color_precision)
     b = _ragged(64, float(tm.step_size), 25)
     prm = _vm_params(tm, False)
     sub = _ragged_dev(b, 'cpu', torch.float64)
