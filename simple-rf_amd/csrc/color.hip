@@ -119,7 +119,7 @@ bool make_geo(const srf_color_desc* d, Geo* g) {
 // (ws_addressable).
 struct Ws {
   int cols, ld;
-  size_t appT, F, x1, h1, h2, dz3, dz2, dz1, dF, bytes;
+  size_t appT, F, x1, h1, h2, dz3, dz2, dz1, dF, mask, bytes;
 };
 
 inline bool ws_addressable(const Geo& g, const Ws& w) {
@@ -144,6 +144,7 @@ Ws make_ws(const Geo& g, int n) {
   w.dz2 = take(kHid);
   w.dz1 = take(kHid);
   w.dF = take(kMaxB);
+  w.mask = take(8);  // ReLU bits of h1 and h2: 16 B per lane, 1 KB per wave of 32 points
   w.bytes = o;
   return w;
 }
@@ -481,12 +482,19 @@ __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(2, 
     if (train) store_tile_rows(a.ws, a.w.x1, g.D1p / 16, u, p0, b);
   };
   staged_layer_rt<P, kHT>(h1, a.pk + g.fW1, g.D1p / KU, g.D1p / 16, fin_pe, keep_pe, stage);
+  // ReLU decisions of the two hidden layers, bit 4 T + r of word cb (h1) and 2 + cb (h2), for the
+  // backward chain (it then reads 32 B per point instead of h1 and h2)
+  uint32_t relu_bits[4] = {0u, 0u, 0u, 0u};
 #pragma unroll
   for (int T = 0; T < kHT; ++T) {
 #pragma unroll
     for (int r = 0; r < 4; ++r)
 #pragma unroll
-      for (int cb = 0; cb < kCB; ++cb) h1[T][cb][r] = h1[T][cb][r] > 0.f ? h1[T][cb][r] : 0.f;
+      for (int cb = 0; cb < kCB; ++cb) {
+        const bool pos = h1[T][cb][r] > 0.f;
+        h1[T][cb][r] = pos ? h1[T][cb][r] : 0.f;
+        relu_bits[cb] |= pos ? 1u << (4 * T + r) : 0u;
+      }
     if (train) store_tile_rows(a.ws, a.w.h1, kHT, T, p0, h1[T]);
   }
   // 3) second layer
@@ -503,9 +511,16 @@ __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(2, 
 #pragma unroll
     for (int r = 0; r < 4; ++r)
 #pragma unroll
-      for (int cb = 0; cb < kCB; ++cb) h2[T][cb][r] = h2[T][cb][r] > 0.f ? h2[T][cb][r] : 0.f;
+      for (int cb = 0; cb < kCB; ++cb) {
+        const bool pos = h2[T][cb][r] > 0.f;
+        h2[T][cb][r] = pos ? h2[T][cb][r] : 0.f;
+        relu_bits[2 + cb] |= pos ? 1u << (4 * T + r) : 0u;
+      }
     if (train) store_tile_rows(a.ws, a.w.h2, kHT, T, p0, h2[T]);
   }
+  if (train)
+    mlp::store_nt16(mlp::uniform_rsrc(a.ws + a.w.mask), (uint32_t)lane * 16u, (uint32_t)(p0 >> 5) * 1024u,
+                    u32x4{relu_bits[0], relu_bits[1], relu_bits[2], relu_bits[3]});
   // 4) output layer + sigmoid
   f32x4 o[1][kCB];
 #pragma unroll
@@ -591,14 +606,15 @@ __device__ __forceinline__ void color_bwd_body(const BwdArgs& a) {
   staged_layer<P, kHT, KU / 16>(dz2, a.pk + g.fW3T, 1, [&](int u, int cb) {
     return u == 0 ? dz3[0][cb] : f32x4{0.f, 0.f, 0.f, 0.f};
   }, stage);
+  // the forward's ReLU bits (words 0-1: h1, 2-3: h2, per column block)
+  const u32x4 relu_bits = __builtin_amdgcn_raw_buffer_load_b128(mlp::uniform_rsrc(a.ws + a.w.mask),
+                                                                 (uint32_t)lane * 16u, (uint32_t)(p0 >> 5) * 1024u, 0);
 #pragma unroll
   for (int T = 0; T < kHT; ++T) {
-    f32x4 hv[kCB];
-    load_tile_rows(a.ws, a.w.h2, kHT, T, p0, hv);
 #pragma unroll
     for (int cb = 0; cb < kCB; ++cb)
 #pragma unroll
-      for (int r = 0; r < 4; ++r) dz2[T][cb][r] = hv[cb][r] > 0.f ? dz2[T][cb][r] : 0.f;
+      for (int r = 0; r < 4; ++r) dz2[T][cb][r] = (relu_bits[2 + cb] >> (4 * T + r)) & 1u ? dz2[T][cb][r] : 0.f;
     store_tile_rows(a.ws, a.w.dz2, kHT, T, p0, dz2[T]);
   }
   // 3) dz1 = (W2^T dz2) * [h1 > 0]
@@ -610,12 +626,10 @@ __device__ __forceinline__ void color_bwd_body(const BwdArgs& a) {
   staged_layer<P, kHT, kHT>(dz1, a.pk + g.fW2T, kHid / KU, [&](int u, int cb) { return dz2[u][cb]; }, stage);
 #pragma unroll
   for (int T = 0; T < kHT; ++T) {
-    f32x4 hv[kCB];
-    load_tile_rows(a.ws, a.w.h1, kHT, T, p0, hv);
 #pragma unroll
     for (int cb = 0; cb < kCB; ++cb)
 #pragma unroll
-      for (int r = 0; r < 4; ++r) dz1[T][cb][r] = hv[cb][r] > 0.f ? dz1[T][cb][r] : 0.f;
+      for (int r = 0; r < 4; ++r) dz1[T][cb][r] = (relu_bits[cb] >> (4 * T + r)) & 1u ? dz1[T][cb][r] : 0.f;
     store_tile_rows(a.ws, a.w.dz1, kHT, T, p0, dz1[T]);
   }
   // 4) dx1 = W1^T dz1, kST output tiles per staged pass and single tiles for the rest, folded into the
