@@ -12,8 +12,10 @@
 // and u, searchsorted(right=True) indices and the interpolated samples are bit-identical.
 //
 // Mapping: one wavefront per ray.  The CDF and the bins live in LDS; each lane resolves
-// its u values by a binary search over LDS (upper_bound), and the merge writes every
-// element at its rank in the union (a stable counting merge, exact for sorted output).
+// its u values by a binary search over LDS (upper_bound).  The merge sorts the union of the
+// coarse and fine depths: up to 256 of them with a bitonic network in registers (four slots
+// per lane), more with a stable counting merge (every element written at its rank).  Both
+// only move values, so the output is torch.sort's values bit for bit.
 #include "common.h"
 
 namespace srf {
@@ -103,8 +105,10 @@ __global__ __launch_bounds__(64) void sample_pdf_kernel(int n_rays, int n_bins, 
                                                         float* __restrict__ samples,
                                                         int64_t* __restrict__ inds,
                                                         float* __restrict__ cdf_out) {
-  __shared__ float s_cdf[kMaxBins];
-  __shared__ float s_bins[kMaxBins];
+  // dynamic LDS sized to the call (2 n_bins floats), so that short rows keep many waves resident
+  extern __shared__ float s_dyn[];
+  float* s_cdf = s_dyn;
+  float* s_bins = s_dyn + n_bins;
   const int r = blockIdx.x;
   if (r >= n_rays) return;
   const int lane = lane_id();
@@ -127,20 +131,56 @@ __global__ __launch_bounds__(64) void sample_pdf_kernel(int n_rays, int n_bins, 
   }
 }
 
+// Ascending bitonic sort of the wave's 256 slots, slot 4 lane + q in v[q] (padding: +inf).  Each
+// compare-exchange either keeps or swaps the two values (a select on one comparison, the same
+// decision on both sides of a cross-lane pair), so the result is a permutation of the input:
+// equal values of different bits (+0, -0) and NaN stay in the multiset.
+__device__ __forceinline__ void wave_bitonic_sort256(float (&v)[4]) {
+  const int lane = lane_id();
+#pragma unroll
+  for (int k = 2; k <= 256; k <<= 1) {
+#pragma unroll
+    for (int jj = k >> 1; jj > 0; jj >>= 1) {
+      if (jj >= 4) {
+        // partner slot in lane ^ (jj / 4), same q; the lane's four slots share both bits
+        const bool asc = ((4 * lane) & k) == 0, lower = ((4 * lane) & jj) == 0;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const float o = __shfl_xor(v[q], jj >> 2, SRF_WAVE);
+          const bool take = (lower == asc) ? (o < v[q]) : (v[q] < o);
+          v[q] = take ? o : v[q];
+        }
+      } else {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          if (q & jj) continue;
+          const bool asc = ((4 * lane + q) & k) == 0;
+          const float x = v[q], y = v[q | jj];
+          const bool swap = asc ? (y < x) : (x < y);
+          v[q] = swap ? y : x;
+          v[q | jj] = swap ? x : y;
+        }
+      }
+    }
+  }
+}
+
 __global__ __launch_bounds__(64) void fine_z_kernel(int n_rays, int n_coarse, int n_fine,
                                                     const float* __restrict__ z_coarse,
                                                     const float* __restrict__ w_coarse,
                                                     const float* __restrict__ u,
                                                     float* __restrict__ z_merged,
                                                     float* __restrict__ z_fine_out) {
-  __shared__ float s_cdf[kMaxBins];
-  __shared__ float s_bins[kMaxBins];
-  __shared__ float s_all[kMaxBins];
+  const int n_bins = n_coarse - 1;
+  const int n_all = n_coarse + n_fine;
+  // dynamic LDS sized to the call (2 n_bins + n_all floats)
+  extern __shared__ float s_dyn[];
+  float* s_cdf = s_dyn;
+  float* s_bins = s_dyn + n_bins;
+  float* s_all = s_dyn + 2 * n_bins;
   const int r = blockIdx.x;
   if (r >= n_rays) return;
   const int lane = lane_id();
-  const int n_bins = n_coarse - 1;
-  const int n_all = n_coarse + n_fine;
   const float* zc = z_coarse + (int64_t)r * n_coarse;
   // z_vals_mid = .5 (z[1:] + z[:-1]) are the bins; weights[..., 1:-1] build the CDF
   for (int j = lane; j < n_coarse; j += SRF_WAVE) {
@@ -159,8 +199,21 @@ __global__ __launch_bounds__(64) void fine_z_kernel(int n_rays, int n_coarse, in
     if (z_fine_out) z_fine_out[ub + i] = zf;
   }
   __syncthreads();
-  // stable counting merge: element e goes to #{v < v_e} + #{j < e : v_j == v_e}
   float* out = z_merged + (int64_t)r * n_all;
+  if (n_all <= 4 * SRF_WAVE) {
+    float v[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int e = 4 * lane + q;
+      v[q] = e < n_all ? s_all[e] : __builtin_inff();
+    }
+    wave_bitonic_sort256(v);
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+      if (4 * lane + q < n_all) out[4 * lane + q] = v[q];
+    return;
+  }
+  // stable counting merge: element e goes to #{v < v_e} + #{j < e : v_j == v_e}
   for (int e = lane; e < n_all; e += SRF_WAVE) {
     const float ve = s_all[e];
     int rank = 0;
@@ -228,8 +281,9 @@ extern "C" int srf_sample_pdf(int n_rays, int n_bins, int n_fine, const float* b
   if (n_rays < 0 || n_bins < 2 || n_bins > kMaxBins || n_fine < 1) return SRF_EINVAL;
   if (n_rays == 0) return SRF_OK;
   if (!bins || !samples || (!weights && !cdf_in)) return SRF_EINVAL;
-  hipLaunchKernelGGL(sample_pdf_kernel, dim3(n_rays), dim3(SRF_WAVE), 0, as_stream(stream), n_rays,
-                     n_bins, n_fine, bins, weights, cdf_in, u, samples, inds, cdf_out);
+  const size_t lds = (size_t)2 * n_bins * sizeof(float);
+  hipLaunchKernelGGL(sample_pdf_kernel, dim3(n_rays), dim3(SRF_WAVE), lds, as_stream(stream), n_rays, n_bins,
+                     n_fine, bins, weights, cdf_in, u, samples, inds, cdf_out);
   return launch_status();
 }
 
@@ -239,7 +293,8 @@ extern "C" int srf_fine_z_vals(int n_rays, int n_coarse, int n_fine, const float
   if (n_rays < 0 || n_coarse < 3 || n_fine < 1 || n_coarse + n_fine > kMaxBins) return SRF_EINVAL;
   if (n_rays == 0) return SRF_OK;
   if (!z_coarse || !weights_coarse || !z_merged) return SRF_EINVAL;
-  hipLaunchKernelGGL(fine_z_kernel, dim3(n_rays), dim3(SRF_WAVE), 0, as_stream(stream), n_rays,
+  const size_t lds = (size_t)(2 * (n_coarse - 1) + n_coarse + n_fine) * sizeof(float);
+  hipLaunchKernelGGL(fine_z_kernel, dim3(n_rays), dim3(SRF_WAVE), lds, as_stream(stream), n_rays,
                      n_coarse, n_fine, z_coarse, weights_coarse, u, z_merged, z_fine);
   return launch_status();
 }

@@ -79,7 +79,7 @@ def test_sample_pdf_internal_cdf(cuda):
 
 
 @pytest.mark.parametrize('det', [True, False])
-@pytest.mark.parametrize('n_coarse,n_fine', [(64, 128), (64, 64), (32, 16)])
+@pytest.mark.parametrize('n_coarse,n_fine', [(64, 128), (64, 64), (32, 16), (192, 96)])
 def test_fine_z_vals(cuda, det, n_coarse, n_fine):
     n_rays = 1500
     g = torch.Generator().manual_seed(7)
