@@ -195,6 +195,52 @@ __device__ float patch_rmse(const DcArgs& a, int va, int xa, int ya, int vb, int
   return sqrtf(s / (float)(a.px * a.py * a.c));
 }
 
+// gt_at without branches: the address is clamped into the image and the value zeroed outside it, so
+// that all loads of a patch row can be in flight before the first is used.
+__device__ __forceinline__ float gt_at_sel(const DcArgs& a, int view, int row, int col, int ch, int hpx, int hpy) {
+  row = row < 0 ? row + a.h + hpx : row;
+  col = col < 0 ? col + a.w + hpy : col;
+  const bool in = row >= 0 && row < a.h && col >= 0 && col < a.w;
+  const float v = a.images[in ? (((int64_t)view * a.h + row) * a.w + col) * a.c + ch : 0];
+  return in ? v : 0.f;
+}
+
+// patch_rmse of the patch at (xa, ya) of view va against the patches at (x1, y1) and (x2, y2) of view
+// vb, for a compile-time patch shape (the reference's 5 x 5 x 3): each patch row's values of the three
+// patches are loaded before any is summed, and the sums run in patch_rmse's order (the same bits).
+template <int PX, int PY, int CH>
+__device__ void patch_rmse2(const DcArgs& a, int va, int xa, int ya, int vb, int x1, int y1, int x2, int y2,
+                            float& r1, float& r2) {
+  constexpr int hpx = PX / 2, hpy = PY / 2;
+  float s1 = 0.f, s2 = 0.f;
+#pragma unroll
+  for (int i = -hpy; i <= hpy; ++i) {
+    float ga[PX][CH], g1[PX][CH], g2[PX][CH];
+#pragma unroll
+    for (int j = -hpx; j <= hpx; ++j)
+#pragma unroll
+      for (int ch = 0; ch < CH; ++ch) {
+        ga[j + hpx][ch] = gt_at_sel(a, va, ya + i, xa + j, ch, hpx, hpy);
+        g1[j + hpx][ch] = gt_at_sel(a, vb, y1 + i, x1 + j, ch, hpx, hpy);
+        g2[j + hpx][ch] = gt_at_sel(a, vb, y2 + i, x2 + j, ch, hpx, hpy);
+      }
+#pragma unroll
+    for (int j = 0; j < PX; ++j)
+#pragma unroll
+      for (int ch = 0; ch < CH; ++ch) {
+        const float e1 = __fsub_rn(ga[j][ch], g1[j][ch]);
+        s1 = __fadd_rn(s1, __fmul_rn(e1, e1));
+        const float e2 = __fsub_rn(ga[j][ch], g2[j][ch]);
+        s2 = __fadd_rn(s2, __fmul_rn(e2, e2));
+      }
+  }
+  r1 = sqrtf(s1 / (float)(PX * PY * CH));
+  r2 = sqrtf(s2 / (float)(PX * PY * CH));
+}
+
+// One thread per ray.  PX = 0: the patch shape of the arguments (patch_rmse); otherwise the
+// compile-time shape PX x PY x CH (patch_rmse2).
+template <int PX, int PY, int CH>
 __global__ void depth_consistency_kernel(DcArgs a) {
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= a.n) return;
@@ -228,8 +274,13 @@ __global__ void depth_consistency_kernel(DcArgs a) {
   const bool valid_a = inside(xa, ya), valid_1 = inside(x1, y1), valid_2 = inside(x2, y2);
   const int x1c = min(max(x1, 0), a.w - 1), y1c = min(max(y1, 0), a.h - 1);
   const int x2c = min(max(x2, 0), a.w - 1), y2c = min(max(y2, 0), a.h - 1);
-  const float rmse1 = patch_rmse(a, va, xa, ya, vb, x1c, y1c);
-  const float rmse2 = patch_rmse(a, va, xa, ya, vb, x2c, y2c);
+  float rmse1, rmse2;
+  if constexpr (PX > 0) {
+    patch_rmse2<PX, PY, CH>(a, va, xa, ya, vb, x1c, y1c, x2c, y2c, rmse1, rmse2);
+  } else {
+    rmse1 = patch_rmse(a, va, xa, ya, vb, x1c, y1c);
+    rmse2 = patch_rmse(a, va, xa, ya, vb, x2c, y2c);
+  }
   bool m1 = (rmse1 < rmse2 || !valid_2) && rmse1 < a.rmse_threshold && valid_1 && valid_a;
   bool m2 = (rmse2 < rmse1 || !valid_1) && rmse2 < a.rmse_threshold && valid_2 && valid_a;
   if (a.both_invalid_rule && !valid_1 && !valid_2) {
@@ -360,8 +411,13 @@ extern "C" int srf_depth_consistency_fwd(int n, const float* depth1, const float
   a.flags = flags;
   hipStream_t s = as_stream(stream);
   if (n > 0) {
-    const int threads = 128;
-    hipLaunchKernelGGL(depth_consistency_kernel, dim3((n + threads - 1) / threads), dim3(threads), 0, s, a);
+    const int threads = 64;  // a train batch is a few thousand rays: spread them over many CUs
+    if (patch_x == 5 && patch_y == 5 && channels == 3)
+      hipLaunchKernelGGL((depth_consistency_kernel<5, 5, 3>), dim3((n + threads - 1) / threads), dim3(threads), 0, s,
+                         a);
+    else
+      hipLaunchKernelGGL((depth_consistency_kernel<0, 0, 0>), dim3((n + threads - 1) / threads), dim3(threads), 0, s,
+                         a);
     const int st = launch_status();
     if (st != SRF_OK) return st;
   }

@@ -722,22 +722,33 @@ __global__ __launch_bounds__(kScanThreads) void cell_chunk_apply_kernel(SortWs w
 
 constexpr int kSortedThreads = 512;
 constexpr int kMaxLineLds = 160 * 1024;  // a CU's LDS
+// Component slices of the sorted backward: a line copy above kLineLdsSplit (at most two blocks per
+// CU) is split into slices of at most kLineLdsSlice.  Measured: the 512 x 64 appearance line (128 KB,
+// one block per CU) walks faster in four slices (4.42 -> 4.08 ms per uneven-batch step), while
+// splitting the 57.6 KB and 48 KB copies of the 300^2 appearance and 512^2 density lines cost more
+// (re-walked rows) than the added blocks gained.
+constexpr int kLineLdsSplit = 64 * 1024;
+constexpr int kLineLdsSlice = 32 * 1024;
 
 // Plane p's gradients from its cell-sorted rows: a lane group walks run_len consecutive sorted
 // rows with the corner slots of the run walk (flushing a plane texel when the walk leaves it) and
 // adds the line gradients into g_line_lds (the line copy in LDS, line_lds != 0) or, when the line is
-// too large for LDS, through the line slots of the run walk.
+// too large for LDS, through the line slots of the run walk.  The plane's components are split into
+// gridDim.y slices of whole quads: a block walks its rows for one slice and keeps only that slice's
+// columns of the line in LDS (L x Cs floats), so that more blocks fit in a CU's LDS.
 template <int G, bool kDensity>
 __global__ __launch_bounds__(kSortedThreads) void sorted_bwd_kernel(VmArgs a, BwdRows r, SortWs w, int p,
                                                                     int run_len, int line_lds) {
   const int C = sel3(p, a.comp[0], a.comp[1], a.comp[2]);
   const int L = sel3(p, a.ll[0], a.ll[1], a.ll[2]);
+  const int qs = C / 4 / (int)gridDim.y;           // quads of the slice
+  const int Cs = 4 * qs, cb = Cs * (int)blockIdx.y;  // its components: cb .. cb + Cs - 1
   const int64_t m = w.total[p];
   const int groups = kSortedThreads / G;
   const int64_t b0 = (int64_t)blockIdx.x * groups * run_len;
   if (b0 >= m) return;  // uniform over the block
   if (line_lds) {
-    for (int e = threadIdx.x; e < L * C; e += kSortedThreads) g_line_lds[e] = 0.f;
+    for (int e = threadIdx.x; e < L * Cs; e += kSortedThreads) g_line_lds[e] = 0.f;
     __syncthreads();
   }
   const int grp = threadIdx.x / G, ql = threadIdx.x % G;
@@ -747,7 +758,7 @@ __global__ __launch_bounds__(kSortedThreads) void sorted_bwd_kernel(VmArgs a, Bw
   float* dP = sel3(p, a.dplane[0], a.dplane[1], a.dplane[2]);
   float* dL = sel3(p, a.dline[0], a.dline[1], a.dline[2]);
   const float4* order = sel3(p, w.order[0], w.order[1], w.order[2]);
-  for (int q = q0 + ql; q < q0 + C / 4; q += G) {
+  for (int q = q0 + cb / 4 + ql; q < q0 + cb / 4 + qs; q += G) {
     RunAcc acc;
     run_init(acc);
     int c = 0;
@@ -764,7 +775,7 @@ __global__ __launch_bounds__(kSortedThreads) void sorted_bwd_kernel(VmArgs a, Bw
       // the line slots carry a texel while consecutive rows (one ray's samples within the cell) stay
       // on it, and flush into the LDS copy (or HBM) when the walk leaves it
       if (line_lds)
-        line_accum<true>(acc.lidx, acc.lg, site.lt.idx, site.lt.w, gl, nullptr, C, site.c);
+        line_accum<true>(acc.lidx, acc.lg, site.lt.idx, site.lt.w, gl, nullptr, Cs, site.c - cb);
       else
         line_accum<false>(acc.lidx, acc.lg, site.lt.idx, site.lt.w, gl, dL, C, site.c);
       c = site.c;
@@ -772,16 +783,17 @@ __global__ __launch_bounds__(kSortedThreads) void sorted_bwd_kernel(VmArgs a, Bw
     run_flush_plane(a, p, c, acc);
     for (int t = 0; t < 2; ++t) {
       if (line_lds)
-        flush_slot_lds(acc.lidx[t], acc.lg[t], C, c);
+        flush_slot_lds(acc.lidx[t], acc.lg[t], Cs, c - cb);
       else
         flush_slot(dL, acc.lidx[t], acc.lg[t], C, c);
     }
   }
   if (line_lds) {
     __syncthreads();
-    for (int e = threadIdx.x; e < L * C; e += kSortedThreads) {
+    for (int e = threadIdx.x; e < L * Cs; e += kSortedThreads) {
       const float v = g_line_lds[e];
-      if (v != 0.f) unsafeAtomicAdd(dL + e, v);
+      const int row = e / Cs;
+      if (v != 0.f) unsafeAtomicAdd(dL + (int64_t)row * C + cb + (e - row * Cs), v);
     }
   }
 }
@@ -1265,7 +1277,15 @@ int launch_sorted_bwd(const VmArgs& a, const BwdRows& r, void* ws, hipStream_t s
   hipLaunchKernelGGL((cell_count_kernel<kDensity, true>), dim3(row_blocks), dim3(256), 0, s, a, r, w);
   for (int p = 0; p < kPlanes; ++p) {
     if (a.comp[p] == 0) continue;
-    const int G = next_pow2(a.comp[p] / 4);
+    // component slices: halve the slice (whole quads) while its line copy exceeds kLineLdsSlice,
+    // when the whole line fits in LDS but above kLineLdsSplit
+    const int quads = a.comp[p] / 4;
+    const size_t full_line = (size_t)a.ll[p] * a.comp[p] * sizeof(float);
+    int ns = 1;
+    if (full_line <= (size_t)kMaxLineLds && full_line > (size_t)kLineLdsSplit)
+      while ((size_t)a.ll[p] * (a.comp[p] / ns) * sizeof(float) > (size_t)kLineLdsSlice && quads % (2 * ns) == 0)
+        ns *= 2;
+    const int G = next_pow2(quads / ns);
     const int groups = kSortedThreads / G;
     // about 1024 blocks over the capacity (fewer run when the device count is lower), unless the
     // gather run knob sets the walk length
@@ -1273,9 +1293,9 @@ int launch_sorted_bwd(const VmArgs& a, const BwdRows& r, void* ws, hipStream_t s
     run = run < 16 ? 16 : (run > 256 ? 256 : run);
     if (const int64_t v = tune_value(SRF_TUNE_GATHER_RUN)) run = v;
     const int64_t blocks = (r.n + (int64_t)groups * run - 1) / ((int64_t)groups * run);
-    const size_t line_bytes = (size_t)a.ll[p] * a.comp[p] * sizeof(float);
+    const size_t line_bytes = (size_t)a.ll[p] * (a.comp[p] / ns) * sizeof(float);
     const int line_lds = line_bytes <= (size_t)kMaxLineLds ? 1 : 0;
-    SRF_VM_DISPATCH_KD(G, kDensity, sorted_bwd_kernel, dim3((unsigned)(blocks > 0 ? blocks : 1)),
+    SRF_VM_DISPATCH_KD(G, kDensity, sorted_bwd_kernel, dim3((unsigned)(blocks > 0 ? blocks : 1), ns),
                        dim3(kSortedThreads), line_lds ? line_bytes : 0, s, a, r, w, p, (int)run, line_lds);
   }
   return launch_status();

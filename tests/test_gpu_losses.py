@@ -21,14 +21,14 @@ def scene():
         return {k: torch.from_numpy(z[k]) if z[k].ndim else z[k] for k in z.files}
 
 
-def _gpu_pair(scene, cuda, name, d1=None, d2=None):
+def _gpu_pair(scene, cuda, name, d1=None, d2=None, patch=(5, 5)):
     d1 = (scene['depth1'] if d1 is None else d1).float().to(cuda).requires_grad_(True)
     d2 = (scene['depth2'] if d2 is None else d2).float().to(cuda).requires_grad_(True)
     out = losses.depth_consistency(d1, d2, scene['indices_mask_nerf'].to(cuda), scene['rays_o'].float().to(cuda),
                                    scene['rays_d'].float().to(cuda), scene['extrinsics'].float().to(cuda),
                                    scene['images'].float().to(cuda), scene['pixel_id'].to(cuda),
                                    scene['intrinsics'].float().to(cuda), tuple(int(v) for v in scene['resolution']),
-                                   both_invalid_rule=(name == 'aug_depth'))
+                                   patch_size=patch, both_invalid_rule=(name == 'aug_depth'))
     out[0].backward()
     return out, d1.grad, d2.grad
 
@@ -85,20 +85,21 @@ def _random_scene(n, seed, h=120, w=160, views=5):
                 resolution=np.array([h, w]))
 
 
-@pytest.mark.parametrize('rule', [True, False])
-def test_depth_consistency_matches_oracle_at_batch_size(cuda, rule):
+@pytest.mark.parametrize('rule,patch', [(True, (5, 5)), (False, (5, 5)), (True, (7, 7))])
+def test_depth_consistency_matches_oracle_at_batch_size(cuda, rule, patch):
     """4096 rays: the gate decisions agree with the FP32 oracle except on rays whose reprojection sits
     within FP32 rounding of a half-pixel (the rounding to a pixel) or whose RMSEs tie to rounding; the
-    loss and the gradient agree to FP32 rounding once those rays are set aside."""
+    loss and the gradient agree to FP32 rounding once those rays are set aside.  The 5 x 5 patches run
+    the kernel's compile-time patch path, 7 x 7 the general one."""
     sc = _random_scene(4096, 11)
     name = 'aug_depth' if rule else 'coarse_fine'
-    (loss, m1, m2, flags), g1, g2 = _gpu_pair(sc, cuda, name)
+    (loss, m1, m2, flags), g1, g2 = _gpu_pair(sc, cuda, name, patch=patch)
     d1 = sc['depth1'].clone().requires_grad_(True)
     d2 = sc['depth2'].clone().requires_grad_(True)
     r_loss, r1, r2, k1, k2 = ol.depth_consistency_nerf(d1, d2, sc['indices_mask_nerf'], sc['rays_o'], sc['rays_d'],
                                                        sc['extrinsics'], sc['images'], sc['pixel_id'],
                                                        sc['intrinsics'], tuple(int(v) for v in sc['resolution']),
-                                                       both_invalid_rule=rule)
+                                                       patch_size=patch, both_invalid_rule=rule)
     r_loss.backward()
     sel = sc['indices_mask_nerf']
     f = flags.cpu()[sel]
