@@ -762,11 +762,18 @@ __global__ __launch_bounds__(kSortedThreads) void sorted_bwd_kernel(VmArgs a, Bw
     RunAcc acc;
     run_init(acc);
     int c = 0;
+    // The next row's entry (and its output gradient) are loaded one row ahead, so that a row waits
+    // for its taps only; the look-ahead index is clamped to the last entry (always written).
+    auto grad_of = [&](const float4& e) -> float4 {
+      return kDensity ? make_float4(e.w, e.w, e.w, e.w)
+                      : ld4(r.d_out + (int64_t)__float_as_int(e.w) * a.total_comp + 4 * q);
+    };
+    float4 en = order[s0 < m ? s0 : m - 1];
+    float4 gn = grad_of(en);
     for (int64_t s = s0; s < s1; ++s) {
-      const float4 e = order[s];
+      const float4 e = en, g = gn;
+      en = order[s + 1 < m ? s + 1 : m - 1];
       const float pn[3] = {e.x, e.y, e.z};
-      const float4 g = kDensity ? make_float4(e.w, e.w, e.w, e.w)
-                                : ld4(r.d_out + (int64_t)__float_as_int(e.w) * a.total_comp + 4 * q);
       const QuadSite site = quad_site(a, q, pn);
       float4 pv, lv;
       quad_values(a, site, pv, lv);
@@ -779,6 +786,7 @@ __global__ __launch_bounds__(kSortedThreads) void sorted_bwd_kernel(VmArgs a, Bw
       else
         line_accum<false>(acc.lidx, acc.lg, site.lt.idx, site.lt.w, gl, dL, C, site.c);
       c = site.c;
+      gn = grad_of(en);
     }
     run_flush_plane(a, p, c, acc);
     for (int t = 0; t < 2; ++t) {
