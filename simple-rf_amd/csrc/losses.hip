@@ -47,12 +47,19 @@ struct Terms {
 };
 
 // loss = sum_t weight[t] * mean over selected rays i of (mean over c of (pred_t[i,c] - target[i,c])^2);
-// the count of selected rays goes to *count (the backward reads it).
+// the count of selected rays goes to *count (the backward reads it).  A thread takes rays
+// threadIdx.x + k blockDim.x; the loads of kMseRays of them are issued before their map stores, and
+// CC > 0 fixes the channel count at compile time (CC = 0: c of the arguments).  The sums run in ray
+// order either way.
+constexpr int kMseRays = 4;
+
+template <int CC>
 __global__ __launch_bounds__(kRedThreads) void mse_terms_fwd_kernel(Terms tm, const float* target,
                                                                     const uint8_t* select, int n, int c,
                                                                     float* loss, float* term_values, float* maps,
                                                                     int* count) {
   __shared__ float sh[kRedThreads / 64];
+  if (CC > 0) c = CC;
   float cnt = 0.f;
   for (int i = threadIdx.x; i < n; i += blockDim.x) cnt += (!select || select[i]) ? 1.f : 0.f;
   const float m = block_sum(cnt, sh);
@@ -60,19 +67,37 @@ __global__ __launch_bounds__(kRedThreads) void mse_terms_fwd_kernel(Terms tm, co
   for (int t = 0; t < tm.n_terms; ++t) {
     const float* p = tm.pred[t];
     float s = 0.f;
-    for (int i = threadIdx.x; i < n; i += blockDim.x) {
-      if (select && !select[i]) {
-        if (maps) maps[(int64_t)t * n + i] = 0.f;
-        continue;
+    for (int i0 = threadIdx.x; i0 < n; i0 += kMseRays * blockDim.x) {
+      float r[kMseRays];
+      bool on[kMseRays];
+#pragma unroll
+      for (int u = 0; u < kMseRays; ++u) {
+        const int i = i0 + u * blockDim.x;
+        on[u] = i < n && (!select || select[i]);
+        r[u] = 0.f;
+        if (on[u]) {
+          if constexpr (CC > 0) {
+#pragma unroll
+            for (int k = 0; k < CC; ++k) {
+              const float e = __fsub_rn(p[(int64_t)i * CC + k], target[(int64_t)i * CC + k]);
+              r[u] = __fadd_rn(r[u], __fmul_rn(e, e));
+            }
+          } else {
+            for (int k = 0; k < c; ++k) {
+              const float e = __fsub_rn(p[(int64_t)i * c + k], target[(int64_t)i * c + k]);
+              r[u] = __fadd_rn(r[u], __fmul_rn(e, e));
+            }
+          }
+          r[u] = r[u] / (float)c;
+        }
       }
-      float r = 0.f;
-      for (int k = 0; k < c; ++k) {
-        const float e = __fsub_rn(p[(int64_t)i * c + k], target[(int64_t)i * c + k]);
-        r = __fadd_rn(r, __fmul_rn(e, e));
+#pragma unroll
+      for (int u = 0; u < kMseRays; ++u) {
+        const int i = i0 + u * blockDim.x;
+        if (i >= n) continue;
+        if (maps) maps[(int64_t)t * n + i] = on[u] ? r[u] : 0.f;
+        if (on[u]) s += r[u];
       }
-      r = r / (float)c;
-      if (maps) maps[(int64_t)t * n + i] = r;
-      s += r;
     }
     const float st = block_sum(s, sh);
     const float mean = m > 0.f ? st / m : 0.f;  // an empty selection contributes 0 (MSE14.py:90)
@@ -195,14 +220,18 @@ __device__ float patch_rmse(const DcArgs& a, int va, int xa, int ya, int vb, int
   return sqrtf(s / (float)(a.px * a.py * a.c));
 }
 
-// gt_at without branches: the address is clamped into the image and the value zeroed outside it, so
-// that all loads of a patch row can be in flight before the first is used.
-__device__ __forceinline__ float gt_at_sel(const DcArgs& a, int view, int row, int col, int ch, int hpx, int hpy) {
+// gt_at of all CH channels of a pixel (CH = a.c), without branches: the address is clamped into the
+// image and the values zeroed outside it, so that all loads of a patch row can be in flight before
+// the first is used, and a pixel's channels are one contiguous load.
+template <int CH>
+__device__ __forceinline__ void gt_px_sel(const DcArgs& a, int view, int row, int col, int hpx, int hpy,
+                                          float (&v)[CH]) {
   row = row < 0 ? row + a.h + hpx : row;
   col = col < 0 ? col + a.w + hpy : col;
   const bool in = row >= 0 && row < a.h && col >= 0 && col < a.w;
-  const float v = a.images[in ? (((int64_t)view * a.h + row) * a.w + col) * a.c + ch : 0];
-  return in ? v : 0.f;
+  const float* px = a.images + (in ? (((int64_t)view * a.h + row) * a.w + col) * CH : 0);
+#pragma unroll
+  for (int ch = 0; ch < CH; ++ch) v[ch] = in ? px[ch] : 0.f;
 }
 
 // patch_rmse of the patch at (xa, ya) of view va against the patches at (x1, y1) and (x2, y2) of view
@@ -217,13 +246,11 @@ __device__ void patch_rmse2(const DcArgs& a, int va, int xa, int ya, int vb, int
   for (int i = -hpy; i <= hpy; ++i) {
     float ga[PX][CH], g1[PX][CH], g2[PX][CH];
 #pragma unroll
-    for (int j = -hpx; j <= hpx; ++j)
-#pragma unroll
-      for (int ch = 0; ch < CH; ++ch) {
-        ga[j + hpx][ch] = gt_at_sel(a, va, ya + i, xa + j, ch, hpx, hpy);
-        g1[j + hpx][ch] = gt_at_sel(a, vb, y1 + i, x1 + j, ch, hpx, hpy);
-        g2[j + hpx][ch] = gt_at_sel(a, vb, y2 + i, x2 + j, ch, hpx, hpy);
-      }
+    for (int j = -hpx; j <= hpx; ++j) {
+      gt_px_sel<CH>(a, va, ya + i, xa + j, hpx, hpy, ga[j + hpx]);
+      gt_px_sel<CH>(a, vb, y1 + i, x1 + j, hpx, hpy, g1[j + hpx]);
+      gt_px_sel<CH>(a, vb, y2 + i, x2 + j, hpx, hpy, g2[j + hpx]);
+    }
 #pragma unroll
     for (int j = 0; j < PX; ++j)
 #pragma unroll
@@ -240,8 +267,10 @@ __device__ void patch_rmse2(const DcArgs& a, int va, int xa, int ya, int vb, int
 
 // One thread per ray.  PX = 0: the patch shape of the arguments (patch_rmse); otherwise the
 // compile-time shape PX x PY x CH (patch_rmse2).
+constexpr int kDcThreads = 64;  // a train batch is a few thousand rays: spread them over many CUs
+
 template <int PX, int PY, int CH>
-__global__ void depth_consistency_kernel(DcArgs a) {
+__global__ __launch_bounds__(kDcThreads) void depth_consistency_kernel(DcArgs a) {
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= a.n) return;
   if (a.select && !a.select[r]) {
@@ -355,8 +384,12 @@ extern "C" int srf_mse_terms_fwd(int n_terms, const float* const* preds, const f
   Terms tm;
   if (!make_terms(n_terms, preds, weights, nullptr, &tm) || !target || n < 0 || channels < 1 || !loss || !count)
     return SRF_EINVAL;
-  hipLaunchKernelGGL(mse_terms_fwd_kernel, dim3(1), dim3(kRedThreads), 0, as_stream(stream), tm, target, select, n,
-                     channels, loss, term_values, maps, count);
+  if (channels == 3)
+    hipLaunchKernelGGL(mse_terms_fwd_kernel<3>, dim3(1), dim3(kRedThreads), 0, as_stream(stream), tm, target,
+                       select, n, channels, loss, term_values, maps, count);
+  else
+    hipLaunchKernelGGL(mse_terms_fwd_kernel<0>, dim3(1), dim3(kRedThreads), 0, as_stream(stream), tm, target,
+                       select, n, channels, loss, term_values, maps, count);
   return launch_status();
 }
 
@@ -411,7 +444,7 @@ extern "C" int srf_depth_consistency_fwd(int n, const float* depth1, const float
   a.flags = flags;
   hipStream_t s = as_stream(stream);
   if (n > 0) {
-    const int threads = 64;  // a train batch is a few thousand rays: spread them over many CUs
+    const int threads = kDcThreads;
     if (patch_x == 5 && patch_y == 5 && channels == 3)
       hipLaunchKernelGGL((depth_consistency_kernel<5, 5, 3>), dim3((n + threads - 1) / threads), dim3(threads), 0, s,
                          a);
