@@ -1,13 +1,16 @@
 """Ragged (occupancy-pruned) TensoRF batches on the HIP path: LowRankTensor.forward_ragged against
 the oracle rendering each ray on its own in FP64 (outputs and factor / network gradients), for
-the VM and CP decompositions, on 64 rays with log-normal sample counts."""
+the VM and CP decompositions, on 64 rays with log-normal sample counts; and the ragged train step
+replayed from a captured HIP graph against eager steps."""
+import copy
+
 import pytest
 import torch
 import torch.nn.functional as F
 
 import oracle.tensorf as ot
 from models.ModelFactory02 import get_model
-from srf_hip import configs as cfgs
+from srf_hip import configs as cfgs, losses
 
 pytestmark = pytest.mark.gpu
 
@@ -101,3 +104,67 @@ def test_ragged_render_and_grads_match_oracle(cuda, cp):
     for x, y in nets:
         err = ((x.grad.cpu().double() - y.grad).abs().max() / y.grad.abs().max().clamp_min(1e-30)).item()
         assert err < 1e-3, err
+
+
+@pytest.mark.parametrize('color_precision', ['fp32', 'bf16'])
+def test_ragged_graph_replays_match_eager_steps(cuda, color_precision):
+    """The ragged train step (forward_ragged, the MSE kernel, backward, Adam) captured once in a HIP
+    graph: each replay matches an eager step of a copy of the model from the same state (parameters
+    and optimiser state copied in before every replay), with eager work and host synchronisation
+    between replays."""
+    torch.manual_seed(0)
+    c = cfgs.simple_tensorf_configs(augmentations=False, ndc=False, white_bkgd=True, perturb=False,
+                                    coarse=dict(num_voxels_initial=32 ** 3, features_positional_encoding_degree=2))
+    c['model']['color_precision'] = color_precision
+    a = get_model(c, model_configs=cfgs.model_configs()).to(cuda).train().coarse_model
+    with torch.no_grad():
+        for p in list(a.vectors_density) + list(a.matrices_density):
+            p.mul_(3.0)
+    b = copy.deepcopy(a)
+    batch = ragged_batch(256, 24, 96, a.bounding_box.cpu(), seed=5)
+    inp = {k: (v.to(cuda) if torch.is_tensor(v) else v) for k, v in batch.items()}
+    target = torch.rand(256, 3, generator=torch.Generator().manual_seed(6)).to(cuda)
+    pa = [p for p in a.parameters() if p.requires_grad]
+    pb = [p for p in b.parameters() if p.requires_grad]
+    opt_a = torch.optim.Adam(pa, lr=1e-3, capturable=True)
+    opt_b = torch.optim.Adam(pb, lr=1e-3, capturable=True)
+
+    def step(model, opt):
+        out = model.forward_ragged(inp)
+        loss = losses.mse_terms([out['rgb']], target)
+        loss.backward()
+        opt.step()
+        return loss
+
+    for _ in range(2):
+        opt_a.zero_grad(set_to_none=True)
+        step(a, opt_a)
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        for _ in range(2):
+            opt_b.zero_grad(set_to_none=True)
+            step(b, opt_b)
+    torch.cuda.current_stream().wait_stream(side)
+    graph = torch.cuda.CUDAGraph()
+    opt_b.zero_grad(set_to_none=True)
+    with torch.cuda.graph(graph):
+        loss_b = step(b, opt_b)
+    for k in range(3):
+        with torch.no_grad():
+            for x, y in zip(pb, pa):
+                x.copy_(y)
+            for x, y in zip(pb, pa):
+                for key, v in opt_a.state.get(y, {}).items():
+                    if isinstance(v, torch.Tensor):
+                        opt_b.state[x][key].copy_(v)
+        opt_a.zero_grad(set_to_none=True)
+        la = step(a, opt_a).item()
+        graph.replay()
+        torch.cuda.synchronize()
+        lb = loss_b.item()
+        _ = sum(float(p.detach().abs().sum()) for p in pb)   # eager work between replays
+        assert la > 0 and abs(la - lb) <= 1e-5 * max(1.0, abs(la)), (k, la, lb)
+        for (na, xa), xb in zip(a.named_parameters(), b.parameters()):
+            err = (xa.detach() - xb.detach()).abs().max().item()
+            assert err <= 1e-3, (k, na, err)
