@@ -84,10 +84,15 @@ __device__ void pack_element(const PackArgs& a, int l, int bwd, long idx) {
   const int ks = f32 ? 4 * k_tiles : k_tiles / 2;
   const long total = (long)rows_t * ks * 64 * kpl;
   if (idx >= total) return;
-  const int slot = (int)(idx % kpl);
-  const int lane = (int)((idx / kpl) % 64);
-  const int s = (int)((idx / (kpl * 64)) % ks);
-  const int T = (int)(idx / ((long)kpl * 64 * ks));
+  // an image holds at most 2^31 elements (the launch's block count is an int): 32-bit index
+  // arithmetic, shifts for the power-of-two factors and one 32-bit division by ks
+  const unsigned ui = (unsigned)idx;
+  const unsigned slot = f32 ? 0u : (ui & 7u);
+  const unsigned rest = f32 ? ui : (ui >> 3);  // idx / kpl
+  const int lane = (int)(rest & 63u);
+  const unsigned fs = rest >> 6;               // idx / (kpl * 64)
+  const int T = (int)(fs / (unsigned)ks);
+  const int s = (int)(fs - (unsigned)T * (unsigned)ks);
   const int g = lane >> 4, jj = lane & 15;
   const int kp = f32 ? slot_k<true>(s, g, slot) : slot_k<false>(s, g, slot);
   float v;
