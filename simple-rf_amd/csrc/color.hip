@@ -867,20 +867,28 @@ __global__ __launch_bounds__(64 * kCdwWaves) void color_dw_kernel(CdwArgs a) {
     dma_unit(p, buf);
     dma_unit(p + 16, buf + nf * 1024);
   };
+  // the valid column blocks are a prefix (ncol of them, uniform over the wave): the products run as
+  // straight-line code for that count, one uniform choice per unit instead of a branch per MFMA
+  int ncol = 0;
+#pragma unroll
+  for (int c = 0; c < 4; ++c) ncol += col_ok[c] ? 1 : 0;
+  ncol = __builtin_amdgcn_readfirstlane(ncol);
   auto unit_products = [&](const uint8_t* fr) {
     u32x4 ah, al, bh[4], bl[4];
     planes(fr, fa, ah, al, do_db);
+    mlp::static_for<1, 5>([&](auto NI) {
+      constexpr int NCOL = decltype(NI)::value;
+      if (ncol != NCOL) return;
 #pragma unroll
-    for (int c = 0; c < 4; ++c)
-      if (col_ok[c]) planes(fr, fb[c], bh[c], bl[c], false);
+      for (int c = 0; c < NCOL; ++c) planes(fr, fb[c], bh[c], bl[c], false);
 #pragma unroll
-    for (int pr = 0; pr < 3; ++pr)
+      for (int pr = 0; pr < 3; ++pr)
 #pragma unroll
-      for (int c = 0; c < 4; ++c)
-        if (col_ok[c])
+        for (int c = 0; c < NCOL; ++c)
           acc[c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, pr == 0 ? al : ah),
                                                            __builtin_bit_cast(bf16x8, pr == 1 ? bl[c] : bh[c]),
                                                            acc[c], 0, 0, 0);
+    });
   };
   for (int st = 0; st < nslots - 1 && st < n_steps; ++st) dma_step(pb + 32 * st, ring + st * slot);
   for (int st = 0; st < n_steps; ++st) {

@@ -1328,6 +1328,14 @@ __global__ __launch_bounds__(64 * kDwWaves) void mlp_dw_kernel(DwArgs a) {
   for (int i = 0; i < RM; ++i) any_m |= mv[i];
 #pragma unroll
   for (int k = 0; k < NC; ++k) any_n |= nv[k];
+  // valid row and column tiles of this wave (prefixes of the tile lists)
+  int nmv = 0, nnv = 0;
+#pragma unroll
+  for (int i = 0; i < RM; ++i) nmv += mv[i] ? 1 : 0;
+#pragma unroll
+  for (int k = 0; k < NC; ++k) nnv += nv[k] ? 1 : 0;
+  nmv = __builtin_amdgcn_readfirstlane(nmv);
+  nnv = __builtin_amdgcn_readfirstlane(nnv);
   // DMA of the 32 fragments of the k-unit at point p into a ring slot: fragment f < 16 is dZ row
   // tile f, f >= 16 is X row tile f - 16; wave w copies f = w, w + 8, w + 16, w + 24.  Every wave
   // issues exactly four loads per k-unit (a tile beyond the job's rows re-reads its first row
@@ -1413,19 +1421,40 @@ __global__ __launch_bounds__(64 * kDwWaves) void mlp_dw_kernel(DwArgs a) {
       dma_unit(pb + (u + kDwSlots - 1) * KPTS, ring + ((u + kDwSlots - 1) % kDwSlots) * kDwSlot);
     if (busy) {
       const uint8_t* fr = ring + (u % kDwSlots) * kDwSlot;
+      // Narrow tiling: the products of the valid tiles in straight-line code.  The valid row and
+      // column tiles are prefixes (nmv, nnv, uniform over the wave), so the guard is one uniform
+      // choice per k-unit among unguarded blocks, one per count of valid columns, instead of a
+      // branch after every FP32 MFMA (which doubled the products' time).  The wide tilings keep the
+      // per-tile guards: a second, unguarded copy of their products spills.
       if constexpr (P == kF32) {
         f32x4 av[RM], bv[NC];
 #pragma unroll
         for (int i = 0; i < RM; ++i) av[i] = *reinterpret_cast<const f32x4*>(fr + (qm + i) * 1024 + lane * 16);
 #pragma unroll
         for (int k = 0; k < NC; ++k) bv[k] = *reinterpret_cast<const f32x4*>(fr + (16 + qn + k) * 1024 + lane * 16);
+        auto prod = [&](auto NRI, auto NVI) SRF_AI {
+          constexpr int NR = decltype(NRI)::value, NV = decltype(NVI)::value;
 #pragma unroll
-        for (int q = 0; q < 4; ++q)
+          for (int q = 0; q < 4; ++q)
 #pragma unroll
-          for (int i = 0; i < RM; ++i)
+            for (int i = 0; i < NR; ++i)
 #pragma unroll
-            for (int k = 0; k < NC; ++k)
-              if (mv[i] && nv[k]) acc[i][k] = mfma32(av[i][q], bv[k][q], acc[i][k]);
+              for (int k = 0; k < NV; ++k) acc[i][k] = mfma32(av[i][q], bv[k][q], acc[i][k]);
+        };
+        if constexpr (kRowWaves) {
+          if (nmv == RM)
+            static_for<1, NC + 1>([&](auto VI) SRF_AI {
+              if (nnv == decltype(VI)::value) prod(std::integral_constant<int, RM>{}, VI);
+            });
+        } else {
+#pragma unroll
+          for (int q = 0; q < 4; ++q)
+#pragma unroll
+            for (int i = 0; i < RM; ++i)
+#pragma unroll
+              for (int k = 0; k < NC; ++k)
+                if (mv[i] && nv[k]) acc[i][k] = mfma32(av[i][q], bv[k][q], acc[i][k]);
+        }
         if (do_db)
 #pragma unroll
           for (int i = 0; i < RM; ++i) rs[i] += (av[i][0] + av[i][1]) + (av[i][2] + av[i][3]);
