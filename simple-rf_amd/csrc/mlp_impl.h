@@ -1464,11 +1464,12 @@ __global__ __launch_bounds__(64 * kDwWaves) void mlp_dw_kernel(DwArgs a) {
         for (int i = 0; i < RM; ++i) av[i] = *reinterpret_cast<const u32x4*>(fr + (qm + i) * 1024 + lane * 16);
 #pragma unroll
         for (int k = 0; k < NC; ++k) bv[k] = *reinterpret_cast<const u32x4*>(fr + (16 + qn + k) * 1024 + lane * 16);
+        // every tile's product, valid or not (the epilogue stores the valid ones; the slot holds a
+        // whole k-unit either way): no branch between the MFMAs
 #pragma unroll
         for (int i = 0; i < RM; ++i)
 #pragma unroll
-          for (int k = 0; k < NC; ++k)
-            if (mv[i] && nv[k]) acc[i][k] = mfma16<P>(av[i], bv[k], acc[i][k]);
+          for (int k = 0; k < NC; ++k) acc[i][k] = mfma16<P>(av[i], bv[k], acc[i][k]);
         if (do_db)
 #pragma unroll
           for (int i = 0; i < RM; ++i)
