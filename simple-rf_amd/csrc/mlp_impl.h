@@ -1421,11 +1421,12 @@ __global__ __launch_bounds__(64 * kDwWaves) void mlp_dw_kernel(DwArgs a) {
       dma_unit(pb + (u + kDwSlots - 1) * KPTS, ring + ((u + kDwSlots - 1) % kDwSlots) * kDwSlot);
     if (busy) {
       const uint8_t* fr = ring + (u % kDwSlots) * kDwSlot;
-      // Narrow tiling: the products of the valid tiles in straight-line code.  The valid row and
-      // column tiles are prefixes (nmv, nnv, uniform over the wave), so the guard is one uniform
-      // choice per k-unit among unguarded blocks, one per count of valid columns, instead of a
-      // branch after every FP32 MFMA (which doubled the products' time).  The wide tilings keep the
-      // per-tile guards: a second, unguarded copy of their products spills.
+      // No branch between the MFMAs (a branch after every FP32 MFMA doubled the products' time).
+      // Narrow tiling: the valid row and column tiles are prefixes (nmv, nnv, uniform over the
+      // wave), so the products run as one unguarded block per count of valid columns, chosen once
+      // per k-unit.  Wide tilings: every tile's product runs (a tile past the job's rows multiplies
+      // the re-read first row tile; the epilogue stores only the valid tiles), since a second,
+      // unguarded copy of their products spills.
       if constexpr (P == kF32) {
         f32x4 av[RM], bv[NC];
 #pragma unroll
@@ -1447,13 +1448,7 @@ __global__ __launch_bounds__(64 * kDwWaves) void mlp_dw_kernel(DwArgs a) {
               if (nnv == decltype(VI)::value) prod(std::integral_constant<int, RM>{}, VI);
             });
         } else {
-#pragma unroll
-          for (int q = 0; q < 4; ++q)
-#pragma unroll
-            for (int i = 0; i < RM; ++i)
-#pragma unroll
-              for (int k = 0; k < NC; ++k)
-                if (mv[i] && nv[k]) acc[i][k] = mfma32(av[i][q], bv[k][q], acc[i][k]);
+          prod(std::integral_constant<int, RM>{}, std::integral_constant<int, NC>{});  // as the 16-bit tiling
         }
         if (do_db)
 #pragma unroll
@@ -1610,16 +1605,17 @@ __global__ __launch_bounds__(64 * kDwWaves) void mlp_dw_split_kernel(DwArgs a) {
         planes(fr, qm + 2 * rb + (i32 >> 4), ah[rb], al[rb], do_db ? &rs[rb] : nullptr);
 #pragma unroll
       for (int cb = 0; cb < 4; ++cb) planes(fr, 16 + qn + 2 * cb + (i32 >> 4), bh[cb], bl[cb], nullptr);
+      // every block's products, valid or not (as in mlp_dw_kernel: the slot holds a whole unit, the
+      // epilogue stores the valid blocks), so that no branch separates the MFMAs
 #pragma unroll
       for (int pr = 0; pr < 3; ++pr)
 #pragma unroll
         for (int rb = 0; rb < 2; ++rb)
 #pragma unroll
           for (int cb = 0; cb < 4; ++cb)
-            if (mv[rb] && nv[cb])
-              acc[rb][cb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
-                  __builtin_bit_cast(bf16x8, pr == 0 ? al[rb] : ah[rb]),
-                  __builtin_bit_cast(bf16x8, pr == 1 ? bl[cb] : bh[cb]), acc[rb][cb], 0, 0, 0);
+            acc[rb][cb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
+                __builtin_bit_cast(bf16x8, pr == 0 ? al[rb] : ah[rb]),
+                __builtin_bit_cast(bf16x8, pr == 1 ? bl[cb] : bh[cb]), acc[rb][cb], 0, 0, 0);
     }
   }
 #pragma unroll
